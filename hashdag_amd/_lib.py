@@ -1,0 +1,157 @@
+"""ctypes binding of libhashdag_hip.so (the C ABI declared in include/hashdag_hip.h).
+
+The library is loaded from this package directory. There is no fallback: if the HIP
+library is missing, importing the binding raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhashdag_hip.so")
+
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+u8p = C.POINTER(C.c_uint8)
+dblp = C.POINTER(C.c_double)
+
+
+class HashDAGError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+HD_OK = 0
+HD_ERR_INVALID = -1
+HD_ERR_HIP = -2
+HD_ERR_BUCKET_FULL = -3
+HD_ERR_POOL_FULL = -4
+HD_ERR_OOM = -5
+HD_ERR_COLOR_FULL = -6
+HD_ERR_HEADER_OVERFLOW = -7
+
+
+class hd_config(C.Structure):
+    _fields_ = [
+        ("levels", C.c_uint32),
+        ("page_size", C.c_uint32),
+        ("num_top_levels", C.c_uint32),
+        ("bucket_bits_top", C.c_uint32),
+        ("bucket_bits_low", C.c_uint32),
+        ("bucket_size_top", C.c_uint32),
+        ("bucket_size_low", C.c_uint32),
+        ("color_tree_levels", C.c_uint32),
+        ("pool_pages", C.c_uint32),
+        ("device", C.c_uint32),
+        ("color_arena_words", C.c_uint64),
+        ("color_nodes_cap", C.c_uint32),
+        ("color_leaves_cap", C.c_uint32),
+    ]
+
+
+class hd_ray_params(C.Structure):
+    _fields_ = [
+        ("camera_position", C.c_double * 3),
+        ("ray_min", C.c_double * 3),
+        ("ray_ddx", C.c_double * 3),
+        ("ray_ddy", C.c_double * 3),
+    ]
+
+    def as_list(self) -> list[float]:
+        return [*self.camera_position, *self.ray_min, *self.ray_ddx, *self.ray_ddy]
+
+
+class hd_image_info(C.Structure):
+    _fields_ = [
+        ("root", C.c_uint32),
+        ("pool_top", C.c_uint32),
+        ("pool_pages", C.c_uint32),
+        ("levels", C.c_uint32),
+        ("total_pages", C.c_uint64),
+        ("total_buckets", C.c_uint64),
+        ("total_virtual", C.c_uint64),
+    ]
+
+
+class hd_terrain_params(C.Structure):
+    _fields_ = [
+        ("levels", C.c_uint32),
+        ("footprint", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("octaves", C.c_uint32),
+        ("amplitude", C.c_double),
+        ("frequency", C.c_double),
+        ("color_tree_levels", C.c_uint32),
+        ("colors", C.c_uint32),
+        ("color_seed", C.c_uint32),
+        ("threads", C.c_uint32),
+    ]
+
+
+_SIGS = {
+    "hd_default_config": (None, [C.c_uint32, C.POINTER(hd_config)]),
+    "hd_create": (C.c_int32, [C.POINTER(hd_config), C.POINTER(C.c_void_p)]),
+    "hd_destroy": (C.c_int32, [C.c_void_p]),
+    "hd_last_error": (C.c_char_p, [C.c_void_p]),
+    "hd_load_basic_dag": (C.c_int32, [C.c_void_p, u32p, C.c_uint64]),
+    "hd_load_colors": (C.c_int32, [C.c_void_p, u32p, C.c_uint64, u64p, C.c_uint64, u64p, C.c_uint64, u32p,
+                                   C.c_uint64, u64p, C.c_uint64]),
+    "hd_image_info_get": (C.c_int32, [C.c_void_p, C.POINTER(hd_image_info)]),
+    "hd_download_image": (C.c_int32, [C.c_void_p, u32p, u32p, u32p, u32p]),
+    "hd_upload_image": (C.c_int32, [C.c_void_p, u32p, u32p, u32p, u32p, C.c_uint32, C.c_uint32]),
+    "hd_get_root": (C.c_int32, [C.c_void_p, u32p]),
+    "hd_set_root": (C.c_int32, [C.c_void_p, C.c_uint32]),
+    "hd_colors_info": (C.c_int32, [C.c_void_p, u64p, u64p, u64p]),
+    "hd_download_color_nodes": (C.c_int32, [C.c_void_p, u32p]),
+    "hd_color_leaf_sizes": (C.c_int32, [C.c_void_p, C.c_uint32, u64p]),
+    "hd_download_color_leaf": (C.c_int32, [C.c_void_p, C.c_uint32, u64p, u32p, u64p]),
+    "hd_ray_params_from_camera": (C.c_int32, [dblp, dblp, dblp, dblp, C.c_uint32, C.c_uint32, C.c_uint32,
+                                              C.POINTER(hd_ray_params)]),
+    "hd_trace_paths": (C.c_int32, [C.c_void_p, C.POINTER(hd_ray_params), C.c_uint32, C.c_uint32, C.c_void_p,
+                                   C.c_void_p]),
+    "hd_trace_colors": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hd_trace_shadows": (C.c_int32, [C.c_void_p, C.POINTER(hd_ray_params), C.c_uint32, C.c_uint32, C.c_void_p,
+                                     C.c_void_p, C.c_float, C.c_float, C.c_void_p]),
+    "hd_set_trace_tile": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "hd_trace_count_bytes": (C.c_int32, [C.c_void_p, C.POINTER(hd_ray_params), C.c_uint32, C.c_uint32, C.c_void_p,
+                                         C.c_void_p, C.c_float, C.c_float, u64p]),
+    "hd_scene_generate": (C.c_int32, [C.POINTER(hd_terrain_params), C.POINTER(C.c_void_p)]),
+    "hd_scene_free": (None, [C.c_void_p]),
+    "hd_scene_sizes": (C.c_int32, [C.c_void_p, u64p]),
+    "hd_scene_copy": (C.c_int32, [C.c_void_p, u32p, u32p, u64p, u64p, u32p, u64p]),
+    "hd_scene_level_nodes": (C.c_int32, [C.c_void_p, u64p]),
+    "hd_terrain_height": (C.c_int32, [C.POINTER(hd_terrain_params), C.c_uint32, C.c_uint32]),
+}
+
+EXPORTED = sorted(_SIGS)
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(the HIP library has no fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(status: int, ctx=None) -> None:
+    if status != HD_OK:
+        msg = lib.hd_last_error(ctx)
+        raise HashDAGError(status, msg.decode() if msg else "error")
+
+
+def ptr(arr, ctype=C.c_uint32):
+    """ctypes pointer to a contiguous numpy array (None passes NULL)."""
+    if arr is None:
+        return None
+    return arr.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,60 @@
+// Context object behind the C ABI (one per device-resident HashDAG).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/hashdag_hip.h"
+#include "hd_internal.h"
+
+struct hd_ctx {
+    hd::Geom g{};
+    hd_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // hash table
+    uint32_t* vpool = nullptr;          // total_virtual + kVpoolPad words
+    uint32_t* page_table = nullptr;     // total_pages
+    uint32_t* bucket_sizes = nullptr;   // total_buckets
+    uint32_t* full_nodes = nullptr;     // levels
+    uint32_t* counters = nullptr;       // [0] pool_top, [1] error flags, [2..] scratch counters
+    uint32_t root = 0;
+    bool loaded = false;
+
+    // colours
+    uint32_t* cnodes = nullptr;
+    uint64_t n_cnodes = 0;
+    uint64_t* coffsets = nullptr;
+    uint64_t n_coffsets = 0;
+    hd::LeafDesc* cleaves = nullptr;
+    uint64_t n_cleaves = 0;
+    std::vector<hd::LeafDesc> h_leaves;  // host mirror of the leaf descriptors
+    uint64_t* arena = nullptr;          // colour arena (64-bit words)
+    uint64_t arena_top = 0;
+    hd::LeafDesc main{};
+    bool colors_loaded = false;
+
+    // tracer
+    uint32_t tile_w = 8, tile_h = 8;
+    unsigned long long* byte_counters = nullptr;  // 3 counters for instrumented launches
+};
+
+namespace hd {
+constexpr uint64_t kVpoolPad = 64;  // words after the virtual space (vector reads past a node end)
+
+hd_status fail(hd_ctx* ctx, hd_status code, const std::string& msg);
+hd_status hip_fail(hd_ctx* ctx, hipError_t e, const char* what);
+DevDag dev_dag(const hd_ctx* ctx);
+DevColors dev_colors(const hd_ctx* ctx);
+}  // namespace hd
+
+#define HD_HIP(ctx, call)                                          \
+    do {                                                           \
+        hipError_t e_ = (call);                                    \
+        if (e_ != hipSuccess) return hd::hip_fail(ctx, e_, #call); \
+    } while (0)

@@ -1,0 +1,612 @@
+// Tracer kernels for gfx950: primary paths, colours and shadows over the device-resident
+// HashDAG. Semantics follow Tracer::trace_paths / trace_colors / trace_shadows
+// (tracer.cu:7-697) bit for bit: FP64 ray setup and FP32 slab math in the reference's
+// operation order, compiled without FP contraction.
+//
+// Mapping: one lane per pixel; a 64-lane wavefront covers a tile_w x tile_h pixel tile
+// (8 x 8 by default) so neighbouring rays share their upper-level nodes; a 256-thread block
+// holds 2 x 2 wave tiles. The DFS stack (node index, child mask, visit mask per level) lives
+// in LDS, laid out [level][thread] so a wavefront's accesses are bank-conflict free. Node
+// words are read straight from the virtual-address pool (no page-table indirection).
+#include <hip/hip_runtime.h>
+
+#include "hd_ctx.h"
+
+namespace hd {
+namespace {
+
+constexpr uint32_t kMaxLevels = 17;
+constexpr uint32_t kBlock = 256;
+
+struct RayP {
+    double cam[3], rmin[3], ddx[3], ddy[3];
+};
+
+struct F3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ uint32_t child_off(uint32_t mask, uint32_t child) {
+    return __popc(mask & ((1u << child) - 1u)) + 1u;
+}
+
+// base_dag.h:16-58
+__device__ __forceinline__ uint32_t leaf_first_mask(uint64_t leaf) {
+    const uint32_t lo = (uint32_t)leaf, hi = (uint32_t)(leaf >> 32);
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        m |= ((lo >> (8 * i)) & 0xFFu) ? (1u << i) : 0u;
+        m |= ((hi >> (8 * i)) & 0xFFu) ? (1u << (4 + i)) : 0u;
+    }
+    return m;
+}
+__device__ __forceinline__ uint32_t leaf_second_mask(uint64_t leaf, uint32_t c) {
+    const uint32_t w = (c & 4) ? (uint32_t)(leaf >> 32) : (uint32_t)leaf;
+    return (w >> ((c & 3) * 8)) & 0xFFu;
+}
+
+// tracer.cu:7-17: first child c ^ order with its bit set, via a bit permutation
+__device__ __forceinline__ uint32_t next_child(uint32_t order, uint32_t mask) {
+    uint32_t m = mask;
+    if (order & 4) m = ((m & 0x0Fu) << 4) | ((m & 0xF0u) >> 4);
+    if (order & 2) m = ((m & 0x33u) << 2) | ((m & 0xCCu) >> 2);
+    if (order & 1) m = ((m & 0x55u) << 1) | ((m & 0xAAu) >> 1);
+    return (uint32_t)__builtin_ctz(m) ^ order;
+}
+
+__device__ __forceinline__ float tmax3(float x, float y, float z) {
+    const float m = y > z ? y : z;
+    return x > m ? x : m;
+}
+__device__ __forceinline__ float tmin3(float x, float y, float z) {
+    const float m = y < z ? y : z;
+    return x < m ? x : m;
+}
+
+// tracer.cu:19-136
+template <bool isRoot>
+__device__ __forceinline__ uint32_t intersection_mask(uint32_t levels, uint32_t level, uint32_t px, uint32_t py,
+                                                      uint32_t pz, F3 o, F3 d, F3 inv) {
+    const uint32_t shift = levels - level;
+    const float radius = (float)(1u << (shift - 1));
+    const float cx = radius + (float)(px << shift);
+    const float cy = radius + (float)(py << shift);
+    const float cz = radius + (float)(pz << shift);
+    const float rx = cx - o.x, ry = cy - o.y, rz = cz - o.z;
+    const float tx = rx * inv.x, ty = ry * inv.y, tz = rz * inv.z;
+    const float sx = radius * fabsf(inv.x), sy = radius * fabsf(inv.y), sz = radius * fabsf(inv.z);
+    const float tmin = fmaxf(tmax3(tx - sx, ty - sy, tz - sz), 0.0f);
+    const float tmax = tmin3(tx + sx, ty + sy, tz + sz);
+    if (isRoot && (tmin >= tmax)) return 0;
+    uint32_t mask;
+    {
+        const float h = 0.5f * (tmin + tmax);
+        const float qx = h * d.x, qy = h * d.y, qz = h * d.z;
+        const uint32_t first = (qx >= rx ? 4u : 0u) + (qy >= ry ? 2u : 0u) + (qz >= rz ? 1u : 0u);
+        mask = 1u << first;
+    }
+    const float eps = 1e-4f;
+    if (tmin <= tx && tx <= tmax) {
+        const float qy = tx * d.y, qz = tx * d.z;
+        uint32_t A = 0, B = 0;
+        if (qy >= ry - eps) A |= 0xCC;
+        if (qy <= ry + eps) A |= 0x33;
+        if (qz >= rz - eps) B |= 0xAA;
+        if (qz <= rz + eps) B |= 0x55;
+        mask |= A & B;
+    }
+    if (tmin <= ty && ty <= tmax) {
+        const float qx = ty * d.x, qz = ty * d.z;
+        uint32_t C = 0, D = 0;
+        if (qx >= rx - eps) C |= 0xF0;
+        if (qx <= rx + eps) C |= 0x0F;
+        if (qz >= rz - eps) D |= 0xAA;
+        if (qz <= rz + eps) D |= 0x55;
+        mask |= C & D;
+    }
+    if (tmin <= tz && tz <= tmax) {
+        const float qx = tz * d.x, qy = tz * d.y;
+        uint32_t E = 0, F = 0;
+        if (qx >= rx - eps) E |= 0xF0;
+        if (qx <= rx + eps) E |= 0x0F;
+        if (qy >= ry - eps) F |= 0xCC;
+        if (qy <= ry + eps) F |= 0x33;
+        mask |= E & F;
+    }
+    return mask;
+}
+
+// DFS over the DAG (tracer.cu:166-249 ordered first hit; tracer.cu:458-542 any hit).
+// Returns true on a hit; (px, py, pz) is the voxel path on a hit, (0,0,0) on a miss.
+template <bool kAnyHit, bool kCount>
+__device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, uint32_t order, uint32_t& px,
+                                        uint32_t& py, uint32_t& pz, uint32_t* __restrict__ sIdx,
+                                        uint16_t* __restrict__ sMask, uint32_t tid, uint64_t& bytes) {
+    const uint32_t* __restrict__ vpool = dag.vpool;
+    const uint32_t levels = dag.levels, leafLevel = dag.leaf_level;
+    uint32_t level = 0;
+    px = py = pz = 0;
+    uint32_t index = dag.root;
+    uint32_t childMask = vpool[index] & 0xFFu;
+    if (kCount) bytes += 4;
+    uint32_t visit = childMask & intersection_mask<true>(levels, 0, 0, 0, 0, o, d, inv);
+    uint64_t leaf = 0;
+    for (;;) {
+        {
+            uint32_t newLevel = level;
+            while (newLevel > 0 && !visit) {
+                newLevel--;
+                index = sIdx[newLevel * kBlock + tid];
+                const uint32_t m = sMask[newLevel * kBlock + tid];
+                childMask = m & 0xFFu;
+                visit = m >> 8;
+            }
+            if (newLevel == 0 && !visit) {
+                px = py = pz = 0;
+                return false;
+            }
+            const uint32_t up = level - newLevel;
+            px >>= up;
+            py >>= up;
+            pz >>= up;
+            level = newLevel;
+        }
+        const uint32_t next = kAnyHit ? (31u - (uint32_t)__builtin_clz(visit)) : next_child(order, visit);
+        visit &= ~(1u << next);
+        px = (px << 1) | ((next >> 2) & 1u);
+        py = (py << 1) | ((next >> 1) & 1u);
+        pz = (pz << 1) | (next & 1u);
+        sIdx[level * kBlock + tid] = index;
+        sMask[level * kBlock + tid] = (uint16_t)(childMask | (visit << 8));
+        level++;
+        if (level == levels) return true;
+        if (level < leafLevel) {
+            index = vpool[index + child_off(childMask, next)];
+            childMask = vpool[index] & 0xFFu;
+            if (kCount) bytes += 8;
+        } else {
+            if (level == leafLevel) {
+                const uint32_t addr = vpool[index + child_off(childMask, next)];
+                leaf = *reinterpret_cast<const uint64_t*>(vpool + addr);
+                childMask = leaf_first_mask(leaf);
+                if (kCount) bytes += 12;
+            } else {
+                childMask = leaf_second_mask(leaf, next);
+            }
+        }
+        visit = childMask & intersection_mask<false>(levels, level, px, py, pz, o, d, inv);
+    }
+}
+
+__device__ __forceinline__ void dnormalize(double& x, double& y, double& z) {
+    const double len = __dsqrt_rn(x * x + y * y + z * z);
+    const double s = 1.0 / len;
+    x = s * x;
+    y = s * y;
+    z = s * z;
+}
+
+__device__ __forceinline__ void pixel_of(uint32_t tileW, uint32_t tileH, uint32_t& x, uint32_t& y) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    x = blockIdx.x * (2 * tileW) + (wave & 1u) * tileW + lane % tileW;
+    y = blockIdx.y * (2 * tileH) + (wave >> 1) * tileH + lane / tileW;
+}
+
+template <bool kCount>
+__device__ __forceinline__ void flush_count(uint64_t v, unsigned long long* counter) {
+    if (!kCount) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor((unsigned long long)v, off, 64);
+    if ((threadIdx.x & 63u) == 0) atomicAdd(counter, (unsigned long long)v);
+}
+
+// tracer.cu:145-252
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, uint32_t W, uint32_t H,
+                                                        uint4* __restrict__ out, uint32_t tileW, uint32_t tileH,
+                                                        unsigned long long* counter) {
+    __shared__ uint32_t sIdx[kMaxLevels * kBlock];
+    __shared__ uint16_t sMask[kMaxLevels * kBlock];
+    uint32_t x, y;
+    pixel_of(tileW, tileH, x, y);
+    uint64_t bytes = 0;
+    if (x < W && y < H) {
+        double dx = p.rmin[0] + (double)x * p.ddx[0] + (double)y * p.ddy[0] - p.cam[0];
+        double dy = p.rmin[1] + (double)x * p.ddx[1] + (double)y * p.ddy[1] - p.cam[1];
+        double dz = p.rmin[2] + (double)x * p.ddx[2] + (double)y * p.ddy[2] - p.cam[2];
+        dnormalize(dx, dy, dz);
+        const F3 o{(float)p.cam[0], (float)p.cam[1], (float)p.cam[2]};
+        const F3 d{(float)dx, (float)dy, (float)dz};
+        const F3 inv{(float)(1.0 / (double)d.x), (float)(1.0 / (double)d.y), (float)(1.0 / (double)d.z)};
+        const uint32_t order = (d.x < 0.f ? 4u : 0u) + (d.y < 0.f ? 2u : 0u) + (d.z < 0.f ? 1u : 0u);
+        uint32_t vx, vy, vz;
+        descend<false, kCount>(dag, o, d, inv, order, vx, vy, vz, sIdx, sMask, threadIdx.x, bytes);
+        out[(uint64_t)(H - 1 - y) * W + x] = make_uint4(vx, vy, vz, 0u);
+        if (kCount) bytes += 16;
+    }
+    flush_count<kCount>(bytes, counter);
+}
+
+// ---- colours ----
+
+__device__ __forceinline__ float clamp01(float v) { return v < 0.f ? 0.f : (v > 1.f ? 1.f : v); }
+
+// color_utils.h:28-37
+__device__ __forceinline__ uint32_t to_rgb888(float r, float g, float b) {
+    r = clamp01(r);
+    g = clamp01(g);
+    b = clamp01(b);
+    return ((uint32_t)(r * 255.0f)) | ((uint32_t)(g * 255.0f) << 8) | ((uint32_t)(b * 255.0f) << 16) | 0xff000000u;
+}
+
+struct Leaf {
+    const uint64_t* blocks;
+    const uint64_t* macro;
+    const uint32_t* weights;
+    uint64_t n_blocks, n_macro, offset;
+};
+
+__device__ __forceinline__ uint32_t hdr_index(uint32_t h) { return h & 16383u; }
+__device__ __forceinline__ uint32_t hdr_bpw(uint32_t h) { return (h >> 16) == 0xFFFFu ? 0u : ((h >> 14) & 3u) + 1u; }
+
+// color_utils.h:112-148 (big-endian 16-bit window over the byte-swapped weight words)
+template <bool kCount>
+__device__ __forceinline__ uint32_t extract_bits(uint32_t bits, const uint32_t* __restrict__ w, uint64_t bitPtr,
+                                                 uint64_t& bytes) {
+    const uint64_t byte = bitPtr >> 3;
+    const uint32_t w0 = w[byte >> 2];
+    const uint32_t sh = (uint32_t)(byte & 3u) * 8u;
+    uint32_t b0 = (w0 >> sh) & 0xFFu, b1;
+    if ((byte & 3u) == 3u) {
+        b1 = w[(byte >> 2) + 1] & 0xFFu;
+        if (kCount) bytes += 4;
+    } else {
+        b1 = (w0 >> (sh + 8)) & 0xFFu;
+    }
+    if (kCount) bytes += 4;
+    const uint32_t dst = (b0 << 8) | b1;
+    const uint32_t shift = 16u - bits - (uint32_t)(bitPtr & 7u);
+    return (dst >> shift) & ((1u << bits) - 1u);
+}
+
+// variable_weight_size_colors.h:343-413 + CompressedColor::get_color (:144-154)
+template <bool kCount>
+__device__ __forceinline__ void leaf_color(const Leaf& L, uint64_t colorIndex, float& r, float& g, float& b,
+                                           uint64_t& bytes) {
+    if (L.offset != ~0ull) colorIndex += L.offset;
+    const uint32_t local = (uint32_t)(colorIndex & 16383u);
+    const uint64_t m = colorIndex >> 14;
+    uint32_t lo = (uint32_t)L.macro[2 * m];
+    uint32_t hi = 2 * (m + 1) < L.n_macro ? (uint32_t)L.macro[2 * (m + 1)] - 1u : (uint32_t)L.n_blocks - 1u;
+    if (kCount) bytes += 16;
+    uint32_t pos = (lo + hi) / 2;
+    uint64_t blk = L.blocks[pos];
+    if (kCount) bytes += 8;
+    while (hdr_index((uint32_t)blk) != local && lo <= hi) {
+        if (hdr_index((uint32_t)blk) > local) hi = pos - 1;
+        else lo = pos + 1;
+        pos = (lo + hi) / 2;
+        blk = L.blocks[pos];
+        if (kCount) bytes += 8;
+    }
+    const uint32_t h = (uint32_t)blk, bits = (uint32_t)(blk >> 32);
+    const uint32_t bpw = hdr_bpw(h);
+    if (bpw == 0) {
+        r = (float)((bits >> 0) & 0x3FFu) / 1023.0f;
+        g = (float)((bits >> 10) & 0xFFFu) / 4095.0f;
+        b = (float)((bits >> 22) & 0x3FFu) / 1023.0f;
+        return;
+    }
+    const uint64_t macroWeight = L.macro[2 * m + 1];
+    if (kCount) bytes += 8;
+    const uint64_t bit = macroWeight + (h >> 16) + (uint64_t)((local - hdr_index(h)) * bpw);
+    const uint32_t wq = extract_bits<kCount>(bpw, L.weights, bit, bytes);
+    const float f = (float)wq / (float)((1u << bpw) - 1u);
+    const float r0 = (float)(bits & 0x1Fu) / 31.0f, g0 = (float)((bits >> 5) & 0x3Fu) / 63.0f,
+                b0 = (float)((bits >> 11) & 0x1Fu) / 31.0f;
+    const float r1 = (float)((bits >> 16) & 0x1Fu) / 31.0f, g1 = (float)((bits >> 21) & 0x3Fu) / 63.0f,
+                b1 = (float)((bits >> 27) & 0x1Fu) / 31.0f;
+    const float omf = 1 - f;
+    r = r0 * omf + r1 * f;
+    g = g0 * omf + g1 * f;
+    b = b0 * omf + b1 * f;
+}
+
+__device__ __forceinline__ Leaf make_leaf(const DevColors& c, const LeafDesc& d, uint64_t offset) {
+    Leaf L;
+    L.blocks = c.arena + d.blocks;
+    L.macro = c.arena + d.macro;
+    L.weights = reinterpret_cast<const uint32_t*>(c.arena) + d.weights32;
+    L.n_blocks = d.n_blocks;
+    L.n_macro = d.n_macro;
+    L.offset = offset;
+    return L;
+}
+
+// tracer.cu:254-451 (debugColors = None, TOOL_OVERLAY = 0)
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_colors(DevDag dag, DevColors colors, uint32_t W, uint32_t H,
+                                                         const uint4* __restrict__ paths, uint32_t* __restrict__ out,
+                                                         uint32_t tileW, uint32_t tileH, unsigned long long* counter) {
+    uint32_t x, y;
+    pixel_of(tileW, tileH, x, y);
+    uint64_t bytes = 0;
+    if (x < W && y < H) {
+        const uint64_t pix = (uint64_t)y * W + x;
+        const uint4 pp = paths[pix];
+        if (kCount) bytes += 20;  // path read + colour write
+        uint32_t result;
+        const uint32_t* __restrict__ vpool = dag.vpool;
+        if (pp.x == 0 && pp.y == 0 && pp.z == 0) {
+            result = to_rgb888(187.0f / 255.f, 242.0f / 255.f, 250.0f / 255.f);
+        } else {
+            const uint32_t cb = (pp.x ^ pp.y ^ pp.z) & 1u;
+            const uint32_t invalid = to_rgb888(1.0f, (float)cb, 1.f - (float)cb);
+            const uint32_t ctl = dag.color_tree_levels, L = dag.leaf_level, levels = dag.levels;
+            uint64_t nof = 0;
+            uint32_t colorNode = 0;
+            bool haveLeaf = false;
+            Leaf leaf{};
+            uint32_t level = 0;
+            uint32_t nodeIndex = dag.root;
+            result = 0;
+            bool done = false;
+            while (level < L) {
+                level++;
+                const uint32_t node = vpool[nodeIndex];
+                const uint32_t mask = node & 0xFFu;
+                const uint32_t s = levels - level;
+                const uint32_t child = (((pp.x >> s) & 1u) << 2) | (((pp.y >> s) & 1u) << 1) | ((pp.z >> s) & 1u);
+                if (kCount) bytes += 4;
+                if (!(mask & (1u << child))) {
+                    result = 0xFF00FFu;
+                    done = true;
+                    break;
+                }
+                if (level - 1 < ctl) {
+                    colorNode = colors.nodes[colorNode + child];
+                    if (kCount) bytes += 4;
+                    if (level == ctl) {
+                        if (colorNode & 0x80000000u) {
+                            const LeafDesc dsc = colors.leaves[colorNode & 0x7FFFFFFFu];
+                            leaf = make_leaf(colors, dsc, dsc.offset);
+                            haveLeaf = dsc.n_blocks > 0;
+                        } else {
+                            leaf = make_leaf(colors, colors.main, colors.offsets[colorNode]);
+                            haveLeaf = colors.main.n_blocks > 0;
+                        }
+                        if (kCount) bytes += 8;
+                    } else if (!colorNode) {
+                        result = invalid;
+                        done = true;
+                        break;
+                    }
+                }
+                if (level == L) {
+                    for (uint32_t c = 0; c < child; ++c) {
+                        if (mask & (1u << c)) {
+                            const uint32_t ci = vpool[nodeIndex + child_off(mask, c)];
+                            nof += __popcll(*reinterpret_cast<const uint64_t*>(vpool + ci));
+                            if (kCount) bytes += 12;
+                        }
+                    }
+                    const uint32_t ci = vpool[nodeIndex + child_off(mask, child)];
+                    const uint64_t lf = *reinterpret_cast<const uint64_t*>(vpool + ci);
+                    if (kCount) bytes += 12;
+                    const uint32_t bit = ((pp.x & 1u) ? 4u : 0u) | ((pp.y & 1u) ? 2u : 0u) | ((pp.z & 1u) ? 1u : 0u) |
+                                         ((pp.x & 2u) ? 32u : 0u) | ((pp.y & 2u) ? 16u : 0u) | ((pp.z & 2u) ? 8u : 0u);
+                    nof += __popcll(lf & ((1ull << bit) - 1ull));
+                    break;
+                } else {
+                    if (level > ctl) {
+                        for (uint32_t c = 0; c < child; ++c) {
+                            if (mask & (1u << c)) {
+                                nof += vpool[vpool[nodeIndex + child_off(mask, c)]] >> 8;
+                                if (kCount) bytes += 8;
+                            }
+                        }
+                    }
+                    nodeIndex = vpool[nodeIndex + child_off(mask, child)];
+                    if (kCount) bytes += 4;
+                }
+            }
+            if (!done) {
+                // is_valid / is_valid_index (variable_weight_size_colors.h:247-262)
+                if (!haveLeaf || !(2 * (nof + (leaf.offset != ~0ull ? leaf.offset : 0ull)) / 16384u < leaf.n_macro)) {
+                    result = invalid;
+                } else {
+                    float r, g, b;
+                    leaf_color<kCount>(leaf, nof, r, g, b, bytes);
+                    result = to_rgb888(r, g, b);
+                }
+            }
+        }
+        out[pix] = result;
+    }
+    flush_count<kCount>(bytes, counter);
+}
+
+// ---- shadows ----
+
+__device__ __forceinline__ void sun_direction(float& x, float& y, float& z) {
+    // normalize(make_float3(0.3f, 1.f, 0.5f)) with (1 / length) * v (cuda_math.h:51-52)
+    const float len = __fsqrt_rn(0.3f * 0.3f + 1.f * 1.f + 0.5f * 0.5f);
+    const float s = __fdiv_rn(1.0f, len);
+    x = s * 0.3f;
+    y = s * 1.f;
+    z = s * 0.5f;
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, uint32_t W, uint32_t H,
+                                                          const uint4* __restrict__ paths, uint32_t* __restrict__ colors,
+                                                          float bias, float fog, uint32_t tileW, uint32_t tileH,
+                                                          unsigned long long* counter) {
+    __shared__ uint32_t sIdx[kMaxLevels * kBlock];
+    __shared__ uint16_t sMask[kMaxLevels * kBlock];
+    uint32_t x, y;
+    pixel_of(tileW, tileH, x, y);
+    uint64_t bytes = 0;
+    if (x < W && y < H) {
+        const uint64_t pix = (uint64_t)y * W + x;
+        const uint4 pp = paths[pix];
+        const uint32_t colorIn = colors[pix];
+        if (kCount) bytes += 24;
+        const float ox = (float)pp.x, oy = (float)pp.y, oz = (float)pp.z;
+        double cx = p.rmin[0] + (double)x * p.ddx[0] + (double)(H - 1 - y) * p.ddy[0] - p.cam[0];
+        double cy = p.rmin[1] + (double)x * p.ddx[1] + (double)(H - 1 - y) * p.ddy[1] - p.cam[1];
+        double cz = p.rmin[2] + (double)x * p.ddx[2] + (double)(H - 1 - y) * p.ddy[2] - p.cam[2];
+        dnormalize(cx, cy, cz);
+        float sx, sy, sz;
+        sun_direction(sx, sy, sz);
+        float light;
+        double distance, nx, ny, nz;
+        if (pp.x == 0 && pp.y == 0 && pp.z == 0) {  // length(rayOrigin) == 0 (tracer.cu:644)
+            light = 1.f;
+            distance = 1e9;
+            nx = cx;
+            ny = cy;
+            nz = cz;
+        } else {
+            // ray_box_intersection (tracer.cu:574-587)
+            const double bx = (double)ox, by = (double)oy, bz = (double)oz;
+            const double t0x = (bx - p.cam[0]) / cx, t0y = (by - p.cam[1]) / cy, t0z = (bz - p.cam[2]) / cz;
+            const double t1x = (bx + 1 - p.cam[0]) / cx, t1y = (by + 1 - p.cam[1]) / cy, t1z = (bz + 1 - p.cam[2]) / cz;
+            const double mx = t0x < t1x ? t0x : t1x, my = t0y < t1y ? t0y : t1y, mz = t0z < t1z ? t0z : t1z;
+            const double maxmin = fmax(fmax(mx, my), mz);
+            const float hx = (float)(p.cam[0] + cx * maxmin);
+            const float hy = (float)(p.cam[1] + cy * maxmin);
+            const float hz = (float)(p.cam[2] + cz * maxmin);
+            const F3 o{hx + bias * sx, hy + bias * sy, hz + bias * sz};
+            const F3 d{sx, sy, sz};
+            const F3 inv{(float)(1.0 / (double)sx), (float)(1.0 / (double)sy), (float)(1.0 / (double)sz)};
+            uint32_t vx, vy, vz;
+            const bool shadowed = descend<true, kCount>(dag, o, d, inv, 0, vx, vy, vz, sIdx, sMask, threadIdx.x, bytes);
+            light = shadowed ? 0.f : 1.f;
+            const double vxd = bx - p.cam[0], vyd = by - p.cam[1], vzd = bz - p.cam[2];
+            distance = __dsqrt_rn(vxd * vxd + vyd * vyd + vzd * vzd);
+            nx = vxd / distance;
+            ny = vyd / distance;
+            nz = vzd / distance;
+        }
+        // shading (tracer.cu:604-619) and applyFog (tracer.cu:551-572)
+        const float k = clamp01(0.5f + light);
+        float r = (float)((colorIn >> 0) & 0xFFu) / 255.0f * k;
+        float g = (float)((colorIn >> 8) & 0xFFu) / 255.0f * k;
+        float b = (float)((colorIn >> 16) & 0xFFu) / 255.0f * k;
+        const float fd = fog * 0.00001f;
+        const double fogAmount = 1.0 - exp(-distance * (double)fd);
+        const double sunAmount = (double)1.01f * fmax(nx * (double)sx + ny * (double)sy + nz * (double)sz, 0.0);
+        const float sa = (float)pow(sunAmount, 30.0);
+        const float oms = 1 - sa;
+        const float fr = 187.0f / 255.f * oms + 1.0f * sa;
+        const float fgc = 242.0f / 255.f * oms + 1.0f * sa;
+        const float fb = 250.0f / 255.f * oms + 1.0f * sa;
+        const float fa = clamp01((float)fogAmount);
+        const float omf = 1 - fa;
+        r = r * omf + fr * fa;
+        g = g * omf + fgc * fa;
+        b = b * omf + fb * fa;
+        colors[pix] = to_rgb888(r, g, b);
+    }
+    flush_count<kCount>(bytes, counter);
+}
+
+inline RayP to_rayp(const hd_ray_params* p) {
+    RayP r;
+    for (int i = 0; i < 3; ++i) {
+        r.cam[i] = p->camera_position[i];
+        r.rmin[i] = p->ray_min[i];
+        r.ddx[i] = p->ray_ddx[i];
+        r.ddy[i] = p->ray_ddy[i];
+    }
+    return r;
+}
+
+inline dim3 grid_of(const hd_ctx* ctx, uint32_t W, uint32_t H) {
+    return dim3((W + 2 * ctx->tile_w - 1) / (2 * ctx->tile_w), (H + 2 * ctx->tile_h - 1) / (2 * ctx->tile_h));
+}
+
+inline hipStream_t stream_of(hd_ctx* ctx, void* s) { return s ? reinterpret_cast<hipStream_t>(s) : ctx->stream; }
+
+}  // namespace
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" {
+
+hd_status hd_set_trace_tile(hd_ctx* ctx, uint32_t tw, uint32_t th) {
+    if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
+    if (tw == 0 && th == 0) {
+        tw = 8;
+        th = 8;
+    }
+    if (tw * th != 64) return fail(ctx, HD_ERR_INVALID, "tile must cover 64 pixels (one wavefront)");
+    ctx->tile_w = tw;
+    ctx->tile_h = th;
+    return HD_OK;
+}
+
+hd_status hd_trace_paths(hd_ctx* ctx, const hd_ray_params* params, uint32_t W, uint32_t H, uint32_t* d_paths,
+                         void* stream) {
+    if (!ctx || !params || !d_paths || W == 0 || H == 0) return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
+    if (ctx->g.levels > kMaxLevels) return fail(ctx, HD_ERR_INVALID, "too many levels");
+    k_trace_paths<false><<<grid_of(ctx, W, H), kBlock, 0, stream_of(ctx, stream)>>>(
+        dev_dag(ctx), to_rayp(params), W, H, reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h, nullptr);
+    HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+
+hd_status hd_trace_colors(hd_ctx* ctx, uint32_t W, uint32_t H, const uint32_t* d_paths, uint32_t* d_colors,
+                          void* stream) {
+    if (!ctx || !d_paths || !d_colors || W == 0 || H == 0) return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    if (!ctx->loaded || !ctx->colors_loaded) return fail(ctx, HD_ERR_INVALID, "no DAG or colours loaded");
+    k_trace_colors<false><<<grid_of(ctx, W, H), kBlock, 0, stream_of(ctx, stream)>>>(
+        dev_dag(ctx), dev_colors(ctx), W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, ctx->tile_w,
+        ctx->tile_h, nullptr);
+    HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+
+hd_status hd_trace_shadows(hd_ctx* ctx, const hd_ray_params* params, uint32_t W, uint32_t H, const uint32_t* d_paths,
+                           uint32_t* d_colors, float bias, float fog, void* stream) {
+    if (!ctx || !params || !d_paths || !d_colors || W == 0 || H == 0) return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
+    k_trace_shadows<false><<<grid_of(ctx, W, H), kBlock, 0, stream_of(ctx, stream)>>>(
+        dev_dag(ctx), to_rayp(params), W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
+        ctx->tile_w, ctx->tile_h, nullptr);
+    HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+
+hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_t W, uint32_t H, uint32_t* d_paths,
+                               uint32_t* d_colors, float bias, float fog, uint64_t bytes[3]) {
+    if (!ctx || !params || !d_paths || !d_colors || !bytes) return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
+    const dim3 grid = grid_of(ctx, W, H);
+    HD_HIP(ctx, hipMemsetAsync(ctx->byte_counters, 0, 3 * sizeof(unsigned long long), ctx->stream));
+    k_trace_paths<true><<<grid, kBlock, 0, ctx->stream>>>(dev_dag(ctx), to_rayp(params), W, H,
+                                                          reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h,
+                                                          ctx->byte_counters + 0);
+    HD_HIP(ctx, hipGetLastError());
+    if (ctx->colors_loaded) {
+        k_trace_colors<true><<<grid, kBlock, 0, ctx->stream>>>(dev_dag(ctx), dev_colors(ctx), W, H,
+                                                               reinterpret_cast<const uint4*>(d_paths), d_colors,
+                                                               ctx->tile_w, ctx->tile_h, ctx->byte_counters + 1);
+        HD_HIP(ctx, hipGetLastError());
+    }
+    k_trace_shadows<true><<<grid, kBlock, 0, ctx->stream>>>(dev_dag(ctx), to_rayp(params), W, H,
+                                                            reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
+                                                            ctx->tile_w, ctx->tile_h, ctx->byte_counters + 2);
+    HD_HIP(ctx, hipGetLastError());
+    unsigned long long b[3];
+    HD_HIP(ctx, hipMemcpyAsync(b, ctx->byte_counters, sizeof(b), hipMemcpyDeviceToHost, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < 3; ++i) bytes[i] = b[i];
+    return HD_OK;
+}
+
+}  // extern "C"

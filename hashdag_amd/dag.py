@@ -1,0 +1,172 @@
+"""Device-resident HashDAG: the hash table (pool, page table, buckets), the root and the
+colour tree, behind the C ABI. Mirrors the reference's HashDAG / HashTable / HashDAGColors
+surface (hash_dag.h:214-636, hash_table.h:137-898, hash_dag_colors.h:9-198).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import check, hd_config, hd_image_info, lib, ptr
+from .scene import Scene
+
+
+@dataclass
+class Geometry:
+    """HashTable geometry (hash_dag_globals.h:7-40; defaults of typedefs.h:202-238)."""
+    levels: int = 16
+    page_size: int = 512
+    num_top_levels: int = 9
+    bucket_bits_top: int = 10
+    bucket_bits_low: int = 16
+    bucket_size_top: int = 1024
+    bucket_size_low: int = 4096
+    color_tree_levels: int = 10
+
+    @property
+    def leaf_level(self) -> int:
+        return self.levels - 2
+
+    def top(self) -> int:
+        return min(self.num_top_levels, self.levels)
+
+    def total_buckets(self) -> int:
+        return self.top() * (1 << self.bucket_bits_top) + (self.levels - self.top()) * (1 << self.bucket_bits_low)
+
+    def total_pages(self) -> int:
+        return (self.top() * (1 << self.bucket_bits_top) * (self.bucket_size_top // self.page_size) +
+                (self.levels - self.top()) * (1 << self.bucket_bits_low) * (self.bucket_size_low // self.page_size))
+
+    def total_virtual(self) -> int:
+        return self.total_pages() * self.page_size
+
+
+@dataclass
+class HostImage:
+    """A HashTable in the reference's CPU layout (hash_table.h:829-843)."""
+    geometry: Geometry
+    pool: np.ndarray          # uint32 physical pages
+    page_table: np.ndarray    # uint32
+    bucket_sizes: np.ndarray  # uint32
+    full_nodes: np.ndarray    # uint32
+    pool_top: int
+    root: int
+
+
+class HashDAG:
+    """One device-resident HashDAG (+ colours) on one GPU."""
+
+    def __init__(self, geometry: Geometry, pool_pages: int | None = None, device: int = 0,
+                 color_leaves_cap: int = 1 << 20):
+        self.geometry = geometry
+        cfg = hd_config()
+        lib.hd_default_config(geometry.levels, C.byref(cfg))
+        cfg.page_size = geometry.page_size
+        cfg.num_top_levels = geometry.num_top_levels
+        cfg.bucket_bits_top = geometry.bucket_bits_top
+        cfg.bucket_bits_low = geometry.bucket_bits_low
+        cfg.bucket_size_top = geometry.bucket_size_top
+        cfg.bucket_size_low = geometry.bucket_size_low
+        cfg.color_tree_levels = geometry.color_tree_levels
+        if pool_pages is not None:
+            cfg.pool_pages = pool_pages
+        cfg.device = device
+        cfg.color_leaves_cap = color_leaves_cap
+        self.pool_pages = cfg.pool_pages
+        h = C.c_void_p()
+        check(lib.hd_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.has_colors = False
+
+    # -- lifetime --
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.hd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- construction: HashDAGFactory::load_from_DAG / load_colors_from_DAG --
+    def load_basic_dag(self, words: np.ndarray) -> None:
+        words = np.ascontiguousarray(words, np.uint32)
+        check(lib.hd_load_basic_dag(self._h, ptr(words), words.size), self._h)
+
+    def load_colors(self, nodes, offsets, blocks, weights, macro) -> None:
+        nodes = np.ascontiguousarray(nodes, np.uint32)
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        blocks = np.ascontiguousarray(blocks, np.uint64)
+        weights = np.ascontiguousarray(weights, np.uint32)
+        macro = np.ascontiguousarray(macro, np.uint64)
+        check(lib.hd_load_colors(self._h, ptr(nodes), nodes.size, ptr(offsets, C.c_uint64), offsets.size,
+                                 ptr(blocks, C.c_uint64), blocks.size, ptr(weights), weights.size,
+                                 ptr(macro, C.c_uint64), macro.size), self._h)
+        self.has_colors = True
+
+    def load_scene(self, scene: Scene) -> None:
+        self.load_basic_dag(scene.dag)
+        self.load_colors(scene.color_nodes, scene.color_offsets, scene.blocks, scene.weights, scene.macro)
+
+    # -- root (firstNodeIndex) --
+    @property
+    def root(self) -> int:
+        r = C.c_uint32()
+        check(lib.hd_get_root(self._h, C.byref(r)), self._h)
+        return r.value
+
+    @root.setter
+    def root(self, value: int) -> None:
+        check(lib.hd_set_root(self._h, value), self._h)
+
+    # -- image exchange --
+    def info(self) -> hd_image_info:
+        i = hd_image_info()
+        check(lib.hd_image_info_get(self._h, C.byref(i)), self._h)
+        return i
+
+    def download(self) -> HostImage:
+        i = self.info()
+        g = self.geometry
+        pool = np.zeros(self.pool_pages * g.page_size, np.uint32)
+        pt = np.zeros(i.total_pages, np.uint32)
+        bs = np.zeros(i.total_buckets, np.uint32)
+        full = np.zeros(g.levels, np.uint32)
+        check(lib.hd_download_image(self._h, ptr(pool), ptr(pt), ptr(bs), ptr(full)), self._h)
+        return HostImage(g, pool, pt, bs, full, i.pool_top, i.root)
+
+    def upload(self, img: HostImage) -> None:
+        pool = np.ascontiguousarray(img.pool, np.uint32)
+        check(lib.hd_upload_image(self._h, ptr(pool), ptr(np.ascontiguousarray(img.page_table, np.uint32)),
+                                  ptr(np.ascontiguousarray(img.bucket_sizes, np.uint32)),
+                                  ptr(np.ascontiguousarray(img.full_nodes, np.uint32)), img.pool_top, img.root),
+              self._h)
+
+    # -- colours --
+    def color_nodes(self) -> np.ndarray:
+        n = C.c_uint64()
+        check(lib.hd_colors_info(self._h, C.byref(n), None, None), self._h)
+        out = np.zeros(n.value, np.uint32)
+        check(lib.hd_download_color_nodes(self._h, ptr(out)), self._h)
+        return out
+
+    def color_leaf(self, handle: int):
+        sizes = np.zeros(4, np.uint64)
+        check(lib.hd_color_leaf_sizes(self._h, handle, ptr(sizes, C.c_uint64)), self._h)
+        blocks = np.zeros(int(sizes[0]), np.uint64)
+        weights = np.zeros(int(sizes[1]), np.uint32)
+        macro = np.zeros(int(sizes[2]), np.uint64)
+        check(lib.hd_download_color_leaf(self._h, handle, ptr(blocks, C.c_uint64), ptr(weights),
+                                         ptr(macro, C.c_uint64)), self._h)
+        return blocks, weights, macro, int(sizes[3])

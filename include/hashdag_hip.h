@@ -1,0 +1,170 @@
+/* hashdag_hip.h - C ABI of the MI355X HashDAG hot path (tracing, edits, hash table, GC).
+ *
+ * Plain C types only: device buffers are passed as raw device pointers, streams as the
+ * opaque hipStream_t handle cast to void* (NULL = the context's own stream). Every entry
+ * point returns an hd_status (0 = OK, < 0 = error); hd_last_error() returns the message.
+ * Contexts are independent; calls on one context are stream-ordered and must not race.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * reference's src/ directory).
+ */
+#ifndef HASHDAG_HIP_H
+#define HASHDAG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t hd_status;
+#define HD_OK 0
+#define HD_ERR_INVALID (-1)          /* bad argument or state */
+#define HD_ERR_HIP (-2)              /* HIP runtime error */
+#define HD_ERR_BUCKET_FULL (-3)      /* a hash bucket overflowed (the reference does not check: hash_table.h:393,461) */
+#define HD_ERR_POOL_FULL (-4)        /* the physical page budget is exhausted (hash_table.h:794-809) */
+#define HD_ERR_OOM (-5)              /* device allocation failed */
+#define HD_ERR_COLOR_FULL (-6)       /* colour arena / colour tree capacity exhausted */
+#define HD_ERR_HEADER_OVERFLOW (-7)  /* voxel count >= 2^24 in a node header (hash_dag_edits.h:605) */
+
+typedef struct hd_ctx hd_ctx;
+
+/* Hash table geometry and capacities. Replaces the compile-time knobs of typedefs.h:202-238
+ * and hash_dag_globals.h:7-40 (SCENE_DEPTH, PAGE_SIZE, BUCKETS_*, C_colorTreeLevels) and the
+ * pool size given to HashTable::create (hash_table.cpp:5, main.cpp:46). */
+typedef struct hd_config {
+    uint32_t levels;              /* scene depth D: extent 2^D, leaf level D-2 */
+    uint32_t page_size;           /* words per page (512) */
+    uint32_t num_top_levels;      /* levels using the top-level bucket geometry (9) */
+    uint32_t bucket_bits_top;     /* log2 buckets per top level (10) */
+    uint32_t bucket_bits_low;     /* log2 buckets per low level (16) */
+    uint32_t bucket_size_top;     /* words per top-level bucket (1024) */
+    uint32_t bucket_size_low;     /* words per low-level bucket (4096) */
+    uint32_t color_tree_levels;   /* levels covered by the colour tree (10) */
+    uint32_t pool_pages;          /* physical page budget */
+    uint32_t device;              /* HIP device ordinal */
+    uint64_t color_arena_words;   /* 64-bit words reserved for colour leaves */
+    uint32_t color_nodes_cap;     /* colour tree entries capacity */
+    uint32_t color_leaves_cap;    /* unique colour leaf slots capacity */
+} hd_config;
+
+/* Fills the reference defaults for depth `levels`. */
+void hd_default_config(uint32_t levels, hd_config* out);
+
+/* HashTable::create (hash_table.cpp:5-71) + DAGTracer setup (dag_tracer.cu:9-45). */
+hd_status hd_create(const hd_config* cfg, hd_ctx** out);
+/* HashTable::destroy (hash_table.cpp:73-109), HashDAGColors::free (hash_dag_colors.h:122-134). */
+hd_status hd_destroy(hd_ctx* ctx);
+/* Last error message of ctx (or of the calling thread when ctx is NULL). */
+const char* hd_last_error(hd_ctx* ctx);
+
+/* HashDAGFactory::load_from_DAG (hash_dag_factory.cpp:152-184): BasicDAG words (root at 0,
+ * basic_dag.h:11-45) -> full nodes first, then every node appended, uploaded to the device. */
+hd_status hd_load_basic_dag(hd_ctx* ctx, const uint32_t* data, uint64_t n_words);
+
+/* HashDAGFactory::load_colors_from_DAG (hash_dag_factory.cpp:186-198) after
+ * HashColorsBuilder::build (hash_dag_colors.h:215-256): the colour tree `nodes` (8 entries
+ * per node; level color_tree_levels-1 entries are leaf handles), the shared-leaf `offsets`
+ * and the main compressed leaf (blocks / byte-swapped weights / macro pairs). */
+hd_status hd_load_colors(hd_ctx* ctx, const uint32_t* nodes, uint64_t n_nodes, const uint64_t* offsets,
+                         uint64_t n_offsets, const uint64_t* blocks, uint64_t n_blocks, const uint32_t* weights,
+                         uint64_t n_weights, const uint64_t* macro, uint64_t n_macro);
+
+/* Sizes of the reference-layout image (HashTable members, hash_table.h:818-861). */
+typedef struct hd_image_info {
+    uint32_t root;           /* HashDAG::firstNodeIndex (hash_dag.h:221) */
+    uint32_t pool_top;       /* HashTable::poolTop */
+    uint32_t pool_pages;     /* CpuData::poolMaxSize */
+    uint32_t levels;
+    uint64_t total_pages;    /* C_totalPages (page table length) */
+    uint64_t total_buckets;  /* C_totalNumberOfBuckets */
+    uint64_t total_virtual;  /* C_totalVirtualAddresses */
+} hd_image_info;
+hd_status hd_image_info_get(hd_ctx* ctx, hd_image_info* out);
+
+/* Device -> host copy in the reference's CPU layout: pool of physical pages (pool_pages *
+ * page_size words), page table, bucket sizes, full node indices (hash_table.h:829-843). */
+hd_status hd_download_image(hd_ctx* ctx, uint32_t* pool, uint32_t* page_table, uint32_t* bucket_sizes,
+                            uint32_t* full_nodes);
+/* HashTable::full_upload_to_gpu (hash_table.cpp:186-206) of a reference-layout image. */
+hd_status hd_upload_image(hd_ctx* ctx, const uint32_t* pool, const uint32_t* page_table, const uint32_t* bucket_sizes,
+                          const uint32_t* full_nodes, uint32_t pool_top, uint32_t root);
+
+/* HashDAG::firstNodeIndex get/set (undo/redo = root history, hash_dag.h:68-120). */
+hd_status hd_get_root(hd_ctx* ctx, uint32_t* root);
+hd_status hd_set_root(hd_ctx* ctx, uint32_t root);
+
+/* Colour state download (HashDAGColors members, hash_dag_colors.h:65-73). */
+hd_status hd_colors_info(hd_ctx* ctx, uint64_t* n_nodes, uint64_t* n_leaves, uint64_t* n_offsets);
+hd_status hd_download_color_nodes(hd_ctx* ctx, uint32_t* nodes);
+/* sizes[0..3] = n_blocks, n_weights, n_macro, offset (UINT64_MAX = unique) of the leaf a
+ * handle resolves to (HashDAGColors::get_leaf, hash_dag_colors.h:37-50) */
+hd_status hd_color_leaf_sizes(hd_ctx* ctx, uint32_t handle, uint64_t sizes[4]);
+hd_status hd_download_color_leaf(hd_ctx* ctx, uint32_t handle, uint64_t* blocks, uint32_t* weights, uint64_t* macro);
+
+/* Camera ray parameters: get_trace_params (dag_tracer.cu:71-113), FP64 on the host.
+ * rotation = CameraView::rotation rows (camera_view.h:8-27); 60 degree vertical fov. */
+typedef struct hd_ray_params {
+    double camera_position[3];
+    double ray_min[3];
+    double ray_ddx[3];
+    double ray_ddy[3];
+} hd_ray_params;
+hd_status hd_ray_params_from_camera(const double position[3], const double rotation[9], const double bounds_min[3],
+                                    const double bounds_max[3], uint32_t levels, uint32_t width, uint32_t height,
+                                    hd_ray_params* out);
+
+/* DAGTracer::resolve_paths -> Tracer::trace_paths (dag_tracer.cu:116-143, tracer.cu:145-252).
+ * d_paths: width*height uint4 (x, y, z, 0); row r holds the ray of pixel y = height-1-r. */
+hd_status hd_trace_paths(hd_ctx* ctx, const hd_ray_params* params, uint32_t width, uint32_t height, uint32_t* d_paths,
+                         void* stream);
+/* DAGTracer::resolve_colors -> Tracer::trace_colors (dag_tracer.cu:145-179, tracer.cu:254-451),
+ * debug colours off. d_colors: width*height RGBA8 words. */
+hd_status hd_trace_colors(hd_ctx* ctx, uint32_t width, uint32_t height, const uint32_t* d_paths, uint32_t* d_colors,
+                          void* stream);
+/* DAGTracer::resolve_shadows -> Tracer::trace_shadows (dag_tracer.cu:181-219, tracer.cu:589-697),
+ * EXACT_SHADOWS; read-modify-write of d_colors. */
+hd_status hd_trace_shadows(hd_ctx* ctx, const hd_ray_params* params, uint32_t width, uint32_t height,
+                           const uint32_t* d_paths, uint32_t* d_colors, float shadow_bias, float fog_density,
+                           void* stream);
+
+/* Tracer tuning (tile shape); 0 = default. Not in the reference (its launch shape is fixed
+ * at (4,64) blocks, dag_tracer.cu:121-122). */
+hd_status hd_set_trace_tile(hd_ctx* ctx, uint32_t tile_w, uint32_t tile_h);
+
+/* Algorithmic HBM bytes of the last traced frame per kernel (paths, colors, shadows), filled
+ * by an instrumented launch: hd_trace_count_bytes runs the three kernels once with counters. */
+hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_t width, uint32_t height,
+                               uint32_t* d_paths, uint32_t* d_colors, float shadow_bias, float fog_density,
+                               uint64_t bytes[3]);
+
+/* Synthetic terrain scene (not in the reference, whose scenes are offline .bin files,
+ * basic_dag.cpp:55-117): generates a BasicDAG plus colour tree and main leaf on the host. */
+typedef struct hd_terrain_params {
+    uint32_t levels;
+    uint32_t footprint;        /* 0 = 2^levels */
+    uint64_t seed;
+    uint32_t octaves;
+    double amplitude;
+    double frequency;
+    uint32_t color_tree_levels;
+    uint32_t colors;           /* 0/1 */
+    uint32_t color_seed;
+    uint32_t threads;          /* 0 = all cores */
+} hd_terrain_params;
+typedef struct hd_scene hd_scene;
+hd_status hd_scene_generate(const hd_terrain_params* p, hd_scene** out);
+void hd_scene_free(hd_scene* s);
+/* sizes[0..5] = dag words, color nodes, color offsets, blocks, weights, macro; sizes[6] = voxels */
+hd_status hd_scene_sizes(const hd_scene* s, uint64_t sizes[7]);
+hd_status hd_scene_copy(const hd_scene* s, uint32_t* dag, uint32_t* color_nodes, uint64_t* color_offsets,
+                        uint64_t* blocks, uint32_t* weights, uint64_t* macro);
+/* unique node counts per level (levels-1 entries) */
+hd_status hd_scene_level_nodes(const hd_scene* s, uint64_t* out);
+int32_t hd_terrain_height(const hd_terrain_params* p, uint32_t x, uint32_t z);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HASHDAG_HIP_H */

@@ -1,0 +1,370 @@
+// Oracle: editors (hash_dag_editors.h) and the two-pass edit recursion
+// (hash_dag_edits.h:106-643, hash_dag.h:255-405), single-threaded and deterministic.
+// TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
+#include "oracle_core.hpp"
+
+#include <map>
+#include <stdexcept>
+#include <tuple>
+
+namespace orc {
+
+namespace {
+
+static float sq(float f) { return f * f; }
+static float tmaxf(float a, float b) { return a > b ? a : b; }  // host ::max (cuda_math.h:7-8)
+
+struct Editor {
+    Edit e;
+    F3 bmin{}, bmax{};  // BoxEditorBase (hash_dag_editors.h:57-74)
+    float r2 = 0;       // SphereEditorBase (hash_dag_editors.h:99-110)
+
+    explicit Editor(const Edit& ed) : e(ed) {
+        bmin = e.center - f3(e.radius);
+        bmax = e.center + f3(e.radius);
+        r2 = e.radius * e.radius;
+    }
+    bool is_box() const { return e.kind == BOX_ADD || e.kind == BOX_CARVE; }
+    bool adding() const { return e.kind == SPHERE_ADD || e.kind == BOX_ADD; }
+
+    // hash_dag_editors.h:112-126 / 76-91
+    bool should_edit_impl(F3 mn, F3 mx) const {
+        if (is_box()) {
+            return !(bmin.x >= mx.x || bmin.y >= mx.y || bmin.z >= mx.z || bmax.x <= mn.x || bmax.y <= mn.y ||
+                     bmax.z <= mn.z || mn.x >= bmax.x || mn.y >= bmax.y || mn.z >= bmax.z || mx.x <= bmin.x ||
+                     mx.y <= bmin.y || mx.z <= bmin.z);
+        }
+        const F3 c = e.center;
+        float dist = r2;
+        if (c.x < mn.x) dist -= sq(c.x - mn.x);
+        else if (c.x > mx.x) dist -= sq(c.x - mx.x);
+        if (c.y < mn.y) dist -= sq(c.y - mn.y);
+        else if (c.y > mx.y) dist -= sq(c.y - mx.y);
+        if (c.z < mn.z) dist -= sq(c.z - mn.z);
+        else if (c.z > mx.z) dist -= sq(c.z - mx.z);
+        return dist > 0;
+    }
+    bool contained_box(F3 mn, F3 mx) const {  // hash_dag_editors.h:233-252
+        return bmin.x <= mn.x && bmin.y <= mn.y && bmin.z <= mn.z && bmax.x >= mx.x && bmax.y >= mx.y && bmax.z >= mx.z;
+    }
+    bool inside_sphere(F3 mn, F3 mx) const {  // hash_dag_editors.h:280-295
+        const F3 c = e.center;
+        return tmaxf(sq(mn.x - c.x), sq(mx.x - c.x)) + tmaxf(sq(mn.y - c.y), sq(mx.y - c.y)) +
+                   tmaxf(sq(mn.z - c.z), sq(mx.z - c.z)) <
+               r2;
+    }
+    bool is_full_impl(F3 mn, F3 mx) const {
+        if (e.kind == BOX_ADD) return contained_box(mn, mx);
+        if (e.kind == SPHERE_ADD) return inside_sphere(mn, mx);
+        return false;
+    }
+    bool is_empty_impl(F3 mn, F3 mx) const {
+        if (e.kind == BOX_CARVE) return contained_box(mn, mx);
+        if (e.kind == SPHERE_CARVE) return inside_sphere(mn, mx);
+        return false;
+    }
+    // Editor::should_edit/is_full/is_empty (hash_dag_editors.h:12-27)
+    bool should_edit(const Path& p, u32 depth) const {
+        const F3 s = p.as_position(depth);
+        return should_edit_impl(s, s + f3((float)(1u << depth)));
+    }
+    bool is_full(const Path& p, u32 depth) const {
+        const F3 s = p.as_position(depth);
+        return is_full_impl(s, s + f3((float)(1u << depth)));
+    }
+    bool is_empty(const Path& p, u32 depth) const {
+        const F3 s = p.as_position(depth);
+        return is_empty_impl(s, s + f3((float)(1u << depth)));
+    }
+    F3 single_color() const { return is_box() ? F3{1, 0, 1} : F3{1, 0, 0}; }
+    bool new_value(bool old) const { return e.kind == SPHERE_PAINT ? old : adding(); }
+    // hash_dag_editors.h:264-269, 307-312, 323-329
+    template <typename Fn>
+    CompressedColor new_color(F3 pos, Fn oldColor) const {
+        CompressedColor c;
+        if (e.kind == SPHERE_PAINT) {
+            const CompressedColor prev = oldColor();
+            const float f = tclamp(1 - lengthf(pos - e.center) / e.radius, 0.f, .1f);
+            const F3 a = prev.get_color();
+            const F3 b{0, 0, 1};
+            c.set_single_color(a * (1 - f) + b * f);
+        } else {
+            c.set_single_color(single_color());
+        }
+        return c;
+    }
+};
+
+struct LeafResult { u32 nodeIndex; u32 count; };
+
+struct LeafCtx {
+    HashTable& t;
+    LeafView leaf;
+    ColorLeafBuilder* builder;  // null when colours are disabled
+    u64 oldLeavesCount = 0;
+    EditStats& stats;
+};
+
+// hash_dag_edits.h:335-643
+LeafResult edit_leaf(u32 level, u32 nodeIndex, Path path, const Editor& ed, LeafCtx& ctx) {
+    HashTable& t = ctx.t;
+    const Geometry& g = t.g;
+    const u32 levels = g.levels, leafLevel = g.leaf_level();
+    ctx.stats.numNodes++;
+    auto voxelCount = [&]() -> u32 {
+        const u32* p = t.ptr(nodeIndex);
+        return level < leafLevel ? (p[0] >> 8) : popc(p[0]) + popc(p[1]);
+    };
+    const u32 fullCount = 1u << (3 * (levels - level));
+    if (!ed.should_edit(path, levels - level)) {
+        u32 count = 0;
+        if (nodeIndex != C_InvalidNodeIndex) {
+            count = voxelCount();
+            if (ctx.builder) {
+                if (!ctx.leaf.is_valid()) throw std::runtime_error("oracle: copy from missing colour leaf");
+                u64 start = ctx.oldLeavesCount;
+                if (ctx.leaf.is_shared()) start += ctx.leaf.offset;
+                ctx.leaf.copy_colors(*ctx.builder, start, count);
+                ctx.oldLeavesCount += count;
+            }
+        }
+        return {nodeIndex, count};
+    }
+    if (ed.is_empty(path, levels - level)) {
+        if (ctx.builder && nodeIndex != C_InvalidNodeIndex) ctx.oldLeavesCount += voxelCount();
+        ctx.stats.numVoxels += fullCount;
+        return {C_InvalidNodeIndex, 0};
+    }
+    if (ed.is_full(path, levels - level)) {
+        if (ctx.builder) {
+            if (nodeIndex != C_InvalidNodeIndex) ctx.oldLeavesCount += voxelCount();
+            ctx.builder->add_large_single_color(ed.single_color(), fullCount);
+        }
+        ctx.stats.numVoxels += fullCount;
+        return {t.full_nodes[level], fullCount};
+    }
+    if (level == leafLevel) {
+        u64 mask = 0;
+        if (nodeIndex != C_InvalidNodeIndex) {
+            const u32* p = t.ptr(nodeIndex);
+            mask = (u64)p[0] | ((u64)p[1] << 32);
+        }
+        const u64 initial = mask;
+        for (u8 c1 = 0; c1 < 8; ++c1) {
+            Path p1 = path;
+            p1.descend(c1);
+            for (u8 c2 = 0; c2 < 8; ++c2) {
+                Path p2 = p1;
+                p2.descend(c2);
+                const F3 pos = p2.as_position(0);
+                const u64 bit = 1ull << (c1 * 8u + c2);
+                const bool shouldEdit = ed.should_edit_impl(pos, pos + 1.0f);
+                const bool prev = (mask & bit) != 0;
+                const bool nv = shouldEdit ? ed.new_value(prev) : prev;
+                ctx.stats.numVoxels += shouldEdit;
+                if (ctx.builder) {
+                    if (nv) {
+                        auto prevColor = [&]() -> CompressedColor {
+                            if (prev && ctx.leaf.is_valid() && ctx.leaf.is_valid_index(ctx.oldLeavesCount))
+                                return ctx.leaf.get_color(ctx.oldLeavesCount);
+                            return CompressedColor{};
+                        };
+                        const CompressedColor col = shouldEdit ? ed.new_color(pos, prevColor) : prevColor();
+                        ctx.builder->add(col);
+                    }
+                    ctx.oldLeavesCount += prev;
+                }
+                if (prev != nv) {
+                    if (nv) mask |= bit;
+                    else mask &= ~bit;
+                }
+            }
+        }
+        if (mask == initial) return {nodeIndex, popcll(mask)};
+        if (mask == 0) return {C_InvalidNodeIndex, 0};
+        return {t.find_or_add_leaf(mask), popcll(mask)};
+    }
+    const u32* np = nullptr;
+    u8 cmask = 0;
+    if (nodeIndex != C_InvalidNodeIndex) {
+        np = t.ptr(nodeIndex);
+        cmask = child_mask(np[0]);
+    }
+    u32 off = 1;
+    u32 newChildren[8];
+    u32 newCount = 0;
+    bool changed = false;
+    for (u8 c = 0; c < 8; ++c) {
+        Path cp = path;
+        cp.descend(c);
+        const u32 childIndex = (cmask & (1u << c)) ? np[off++] : C_InvalidNodeIndex;
+        const LeafResult r = edit_leaf(level + 1, childIndex, cp, ed, ctx);
+        np = nodeIndex != C_InvalidNodeIndex ? t.ptr(nodeIndex) : nullptr;  // table storage is stable
+        if (r.nodeIndex != C_InvalidNodeIndex) newCount += r.count;
+        changed |= childIndex != r.nodeIndex;
+        newChildren[c] = r.nodeIndex;
+    }
+    if (newCount >= (1u << 24)) throw std::runtime_error("oracle: voxel count overflows the node header");
+    if (!changed) return {nodeIndex, newCount};
+    u32 node[9];
+    u32 n = 0;
+    node[n++] = newCount << 8;
+    for (u8 c = 0; c < 8; ++c) {
+        if (newChildren[c] != C_InvalidNodeIndex) {
+            node[n++] = newChildren[c];
+            node[0] |= 1u << c;
+        }
+    }
+    if (child_mask(node[0]) == 0) return {C_InvalidNodeIndex, 0};
+    return {t.find_or_add_interior(level, n, node), newCount};
+}
+
+struct Task {
+    u32 childNodeIndex;
+    Path path;
+    LeafView existing;
+    std::unique_ptr<ColorLeafBuilder> builder;
+    EditStats stats;
+    u32 finalIndex = 0;
+};
+
+struct ColorToBuild {
+    std::unique_ptr<ColorLeafBuilder> builder;
+    LeafView oldLeaf;
+    u32 newLeafIndex;
+};
+
+struct EditCtx {
+    HashTable& t;
+    HashDAGColors* colors;
+    const Editor& ed;
+    std::map<std::tuple<u32, u32, u32>, std::unique_ptr<Task>> tasks;
+    std::vector<Task*> taskOrder;
+    std::vector<ColorToBuild> toBuild;
+    EditStats stats;
+};
+
+struct EditIndex { u32 nodeIndex; u32 colorIndex; };
+
+// hash_dag_edits.h:106-333
+EditIndex edit_threaded(bool queue, u32 level, u32 nodeIndex, u32 colorNodeIndex, Path path, EditCtx& ctx) {
+    HashTable& t = ctx.t;
+    const Geometry& g = t.g;
+    const u32 levels = g.levels, ctl = g.color_tree_levels;
+    ctx.stats.numNodes++;
+    if (!ctx.ed.should_edit(path, levels - level)) return {queue ? C_InvalidNodeIndex : nodeIndex, C_IgnoreColorIndex};
+    auto colorNode = [&](u32 idx) -> u32 { return ctx.colors ? ctx.colors->nodes[idx] : 0u; };
+    auto allocateColorNode = [&]() {
+        colorNodeIndex = ctx.colors->allocate_interior_node();
+        for (int i = 0; i < 7; ++i) ctx.colors->allocate_interior_node();
+    };
+    u8 cmask = 0;
+    if (nodeIndex != C_InvalidNodeIndex) cmask = child_mask(*t.ptr(nodeIndex));
+    u32 off = 1;
+    u32 newChildren[8];
+    bool changed = false;
+    for (u8 c = 0; c < 8; ++c) {
+        Path np = path;
+        np.descend(c);
+        const u32 childNodeIndex = (cmask & (1u << c)) ? t.ptr(nodeIndex)[off++] : C_InvalidNodeIndex;
+        u32 newChild;
+        u32 childColor = 0;
+        if (level == 0 || colorNodeIndex > 0) childColor = colorNode(colorNodeIndex + c);
+        if (level == ctl - 1) {
+            if (ctx.ed.should_edit(np, levels - level - 1)) {
+                LeafView existing;
+                u32 existingIndex = (u32)-1;
+                if (childColor > 0 && ctx.colors) {
+                    existing = ctx.colors->get_leaf(childColor);
+                    existingIndex = childColor;
+                }
+                const auto key = std::make_tuple(np.p.x, np.p.y, np.p.z);
+                if (queue) {
+                    auto task = std::make_unique<Task>();
+                    task->childNodeIndex = childNodeIndex;
+                    task->path = np;
+                    task->existing = existing;
+                    if (ctx.colors && g.leaf_level() > ctl) task->builder = std::make_unique<ColorLeafBuilder>();
+                    ctx.taskOrder.push_back(task.get());
+                    ctx.tasks[key] = std::move(task);
+                    newChild = C_InvalidNodeIndex;
+                } else {
+                    Task& task = *ctx.tasks.at(key);
+                    newChild = task.finalIndex;
+                    if (task.builder && task.builder->colorIndex > 0) {
+                        if (existingIndex == (u32)-1 || !(existingIndex & 0x80000000u)) {
+                            if (colorNodeIndex == 0) allocateColorNode();
+                            childColor = ctx.colors->allocate_leaf();
+                            ctx.colors->nodes[colorNodeIndex + c] = childColor;
+                            existingIndex = childColor;
+                        }
+                        ctx.toBuild.push_back({std::move(task.builder), existing, existingIndex});
+                    }
+                }
+            } else {
+                newChild = queue ? C_InvalidNodeIndex : childNodeIndex;
+            }
+        } else {
+            const EditIndex r = edit_threaded(queue, level + 1, childNodeIndex, childColor, np, ctx);
+            newChild = r.nodeIndex;
+            if (!queue && ctx.colors && r.colorIndex != childColor && r.colorIndex != C_IgnoreColorIndex &&
+                r.colorIndex != C_RemoveColorIndex) {
+                if (level > 0 && colorNodeIndex == 0) allocateColorNode();
+                ctx.colors->nodes[colorNodeIndex + c] = r.colorIndex;
+            }
+        }
+        if (!queue) {
+            changed |= childNodeIndex != newChild;
+            newChildren[c] = newChild;
+        }
+    }
+    if (queue) return {C_InvalidNodeIndex, C_IgnoreColorIndex};
+    if (!changed) return {nodeIndex, colorNodeIndex};
+    u32 node[9];
+    u32 n = 0;
+    node[n++] = 0;
+    for (u8 c = 0; c < 8; ++c) {
+        if (newChildren[c] != C_InvalidNodeIndex) {
+            node[n++] = newChildren[c];
+            node[0] |= 1u << c;
+        }
+    }
+    if (child_mask(node[0]) == 0) return {C_InvalidNodeIndex, C_IgnoreColorIndex};
+    return {t.find_or_add_interior(level, n, node), colorNodeIndex};
+}
+
+}  // namespace
+
+// hash_dag.h:255-405 (tasks run in creation order; undo frame kept when requested)
+EditStats apply_edit(DAG& dag, HashDAGColors* colors, const Edit& e, UndoFrame* undo) {
+    Editor ed(e);
+    EditCtx ctx{dag.t, colors, ed, {}, {}, {}, {}};
+    if (undo) {
+        undo->root = dag.root;
+        undo->leaves.clear();
+    }
+    edit_threaded(true, 0, dag.root, 0, Path(0, 0, 0), ctx);
+    for (Task* task : ctx.taskOrder) {
+        LeafCtx lc{dag.t, task->existing, task->builder.get(), 0, task->stats};
+        task->finalIndex = edit_leaf(dag.g().color_tree_levels, task->childNodeIndex, task->path, ed, lc).nodeIndex;
+        ctx.stats.numNodes += task->stats.numNodes;
+        ctx.stats.numVoxels += task->stats.numVoxels;
+    }
+    dag.root = edit_threaded(false, 0, dag.root, 0, Path(0, 0, 0), ctx).nodeIndex;
+    ctx.stats.leavesRebuilt = ctx.toBuild.size();
+    for (auto& tb : ctx.toBuild) {
+        ColorLeaf& ref = colors->leaves[tb.newLeafIndex & 0x7FFFFFFFu];
+        // The undo frame keeps the leaf the subtree used before the edit: the shared main
+        // leaf (with its offset) when the slot was freshly allocated, else the slot itself.
+        ColorLeaf previous = ref;
+        if (tb.oldLeaf.d == &colors->mainLeaf) {
+            previous = colors->mainLeaf;
+            previous.offset = tb.oldLeaf.offset;
+        }
+        tb.builder->build(ref);
+        if (undo) undo->leaves.push_back({tb.newLeafIndex, {std::move(previous), ref}});
+    }
+    return ctx.stats;
+}
+
+}  // namespace orc

@@ -1,0 +1,104 @@
+/* Oracle C API (ctypes) over the CPU restatement in oracle_core.hpp.
+ *
+ * TEST INFRASTRUCTURE ONLY: called from tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg, never from the product.
+ *
+ * Parity pinning of the oracle (which independent check fixes each part):
+ *   hashes            MurmurHash3_x86_32 / x64_128 published vectors (fmix32, fmix64, the
+ *                     word-count finaliser relation)            tests/test_oracle_hashes.py
+ *   leaf masks, Path  brute force over all bytes / random masks  tests/test_oracle_basics.py
+ *   colour codec      decode(encode(seq)) == seq, bit-stream layout vs a direct big-endian
+ *                     bit-stream model                          tests/test_oracle_colors.py
+ *   hash table        invariants: no page straddle, find after add, bucket/page accounting,
+ *                     the interior-find page-end quirk          tests/test_oracle_table.py
+ *   load_from_DAG     decoded voxel set == dense generator grid tests/test_oracle_scene.py
+ *   trace_paths       brute-force DDA nearest hit agrees except where the reference's DFS
+ *                     order/epsilon differ (counted), mask properties, root rejection
+ *                                                                tests/test_oracle_tracer.py
+ *   trace_colors      colour index == Morton rank of the voxel in its level-10 subtree
+ *                     (dense-grid brute force)                   tests/test_oracle_tracer.py
+ *   trace_shadows     any-hit result == brute-force segment/voxel test on a dense grid
+ *   edits             dense-grid brute force of sphere/box/paint voxel sets and colours
+ *                                                                tests/test_oracle_edits.py
+ *   GC                canonical form unchanged, no unreachable nodes tests/test_oracle_gc.py
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_geometry {
+    uint32_t levels, page_size, num_top_levels, bucket_bits_top, bucket_bits_low, bucket_size_top, bucket_size_low,
+        color_tree_levels;
+} orc_geometry;
+
+const char* orc_last_error(void);
+
+/* hashes */
+uint32_t orc_murmurhash32(uint32_t h);
+uint64_t orc_murmurhash64(uint64_t h);
+uint32_t orc_murmurhash32xN(const uint32_t* words, uint64_t n, uint32_t seed);
+
+/* DAG */
+void* orc_dag_from_basic(const orc_geometry* g, uint32_t pool_pages, const uint32_t* data, uint64_t n);
+void* orc_dag_from_image(const orc_geometry* g, uint32_t pool_pages, const uint32_t* pool, const uint32_t* page_table,
+                         const uint32_t* bucket_sizes, const uint32_t* full_nodes, uint32_t pool_top, uint32_t root);
+void orc_dag_free(void* dag);
+uint32_t orc_dag_root(void* dag);
+uint32_t orc_dag_pool_top(void* dag);
+/* copies the reference-layout image out; arrays sized per geometry / pool_pages */
+int orc_dag_export(void* dag, uint32_t* pool, uint32_t* page_table, uint32_t* bucket_sizes, uint32_t* full_nodes);
+uint64_t orc_canonical(void* dag, uint32_t* out, uint64_t cap); /* returns required length */
+int orc_get_value(void* dag, uint32_t x, uint32_t y, uint32_t z);
+/* dense occupancy of the whole 2^levels cube (small depths only), x-major [x][y][z] */
+int orc_dense(void* dag, uint8_t* out);
+
+/* colours: tree nodes, shared offsets and the main leaf */
+void* orc_colors_new(const uint32_t* nodes, uint64_t n_nodes, const uint64_t* offsets, uint64_t n_offsets,
+                     const uint64_t* blocks, uint64_t n_blocks, const uint32_t* weights, uint64_t n_weights,
+                     const uint64_t* macro, uint64_t n_macro);
+/* appends a unique leaf (weights in stored byte-swapped order); returns its handle (bit 31 set) */
+uint32_t orc_colors_add_unique(void* colors, const uint64_t* blocks, uint64_t n_blocks, const uint32_t* weights,
+                               uint64_t n_weights, const uint64_t* macro, uint64_t n_macro);
+void orc_colors_free(void* colors);
+uint64_t orc_colors_num_nodes(void* colors);
+uint64_t orc_colors_num_leaves(void* colors);
+int orc_colors_nodes(void* colors, uint32_t* out);
+/* sizes / copy of unique leaf i: sizes[0..2] = n_blocks, n_weights, n_macro */
+int orc_colors_leaf_sizes(void* colors, uint32_t i, uint64_t sizes[3]);
+int orc_colors_leaf_copy(void* colors, uint32_t i, uint64_t* blocks, uint32_t* weights, uint64_t* macro);
+/* decoded colour sequence per level-(color_tree_levels) subtree in DFS order: for each subtree
+ * [x, y, z, n, then n x (colorBits, weight | bpw << 8)]; returns required length */
+uint64_t orc_color_canonical(void* dag, void* colors, uint32_t* out, uint64_t cap);
+
+/* colour codec round trip: encode n colours (colorBits[i], weight[i], bpw[i]) with the
+ * reference encoder; sizes out = n_blocks, n_weights, n_macro */
+int orc_encode_colors(const uint32_t* colorBits, const uint8_t* weight, const uint8_t* bpw, uint64_t n,
+                      uint64_t* blocks, uint32_t* weights, uint64_t* macro, uint64_t sizes[3], uint64_t cap_blocks,
+                      uint64_t cap_weights, uint64_t cap_macro);
+/* decode colour index i of a leaf given as arrays; out = colorBits, weight | bpw << 8 */
+int orc_decode_color(const uint64_t* blocks, uint64_t n_blocks, const uint32_t* weights, uint64_t n_weights,
+                     const uint64_t* macro, uint64_t n_macro, uint64_t offset, uint64_t index, uint32_t out[2]);
+
+/* tracer; params = 12 doubles: cameraPosition, rayMin, rayDDx, rayDDy */
+void orc_ray_params(const double position[3], const double rotation[9], const double bmin[3], const double bmax[3],
+                    uint32_t levels, uint32_t W, uint32_t H, double out[12]);
+/* rows [y0, y1) of the W x H paths image (uint4 per pixel, row r = pixel y H-1-r) */
+int orc_trace_paths(void* dag, const double params[12], uint32_t W, uint32_t H, uint32_t y0, uint32_t y1,
+                    uint32_t* paths, uint64_t* bytes, int threads);
+int orc_trace_colors(void* dag, void* colors, uint32_t W, uint32_t H, uint32_t y0, uint32_t y1, const uint32_t* paths,
+                     uint32_t* colors_out, int threads);
+int orc_trace_shadows(void* dag, const double params[12], uint32_t W, uint32_t H, uint32_t y0, uint32_t y1,
+                      const uint32_t* paths, uint32_t* colors_inout, float bias, float fog, uint64_t* bytes,
+                      int threads);
+
+/* edits: kind 0 sphere add, 1 sphere carve, 2 box add, 3 box carve, 4 sphere paint.
+ * stats out: num nodes, num voxels, colour leaves rebuilt. colors may be NULL. */
+int orc_edit(void* dag, void* colors, uint32_t kind, const float center[3], float radius, uint64_t stats[3]);
+int orc_gc(void* dag, uint32_t max_level);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,120 @@
+"""GPU parity of the tracer kernels against the oracle: per-pixel paths, colours and
+shadowed colours must be bit-exact; algorithmic byte counters must agree."""
+import numpy as np
+import pytest
+
+from tests.scenes import POOL_PAGES, scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(name):
+    import torch  # noqa: F401
+    from hashdag_amd import HashDAG
+    from oracle import OracleColors, OracleDAG
+
+    p, sc, g = scene(name)
+    dag = HashDAG(g, pool_pages=POOL_PAGES)
+    dag.load_scene(sc)
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    return p, sc, g, dag, od, oc
+
+
+@pytest.mark.parametrize("name", ["d7", "d8", "d10", "d13"])
+def test_image_matches_oracle_layout(name):
+    """load_from_DAG on the host + upload/download round trip == the oracle's table, word for word."""
+    p, sc, g, dag, od, oc = _setup(name)
+    img = dag.download()
+    pool, pt, bs, full, top = od.export()
+    assert img.root == od.root
+    assert img.pool_top == top
+    assert np.array_equal(img.page_table, pt)
+    assert np.array_equal(img.bucket_sizes, bs)
+    assert np.array_equal(img.full_nodes, full)
+    assert np.array_equal(img.pool[: top * g.page_size], pool[: top * g.page_size])
+
+
+@pytest.mark.parametrize("name,W,H,ncam", [("d7", 64, 48, 4), ("d8", 96, 64, 4), ("d10", 128, 72, 3),
+                                           ("d13", 160, 90, 3)])
+def test_trace_parity(name, W, H, ncam):
+    import torch
+    from hashdag_amd import DAGInfo, DAGTracer, ray_params
+    from hashdag_amd.workload import cameras
+    from oracle import Oracle
+
+    p, sc, g, dag, od, oc = _setup(name)
+    info = DAGInfo.unit(g.levels)
+    tracer = DAGTracer(W, H)
+    for cam in cameras(p, ncam, seed=11):
+        prm = ray_params(cam, info, g.levels, W, H)
+        oparams = Oracle.ray_params(cam, info, g.levels, W, H)
+        assert np.array_equal(np.array(prm.as_list()), oparams), "ray setup differs"
+        tracer.resolve_paths(cam, info, dag, params=prm)
+        tracer.resolve_colors(dag)
+        torch.cuda.synchronize()
+        gpaths = tracer.paths.cpu().numpy().view(np.uint32)
+        gcols = tracer.colors.cpu().numpy().view(np.uint32)
+        tracer.resolve_shadows(cam, info, dag, 1.0, 0.0, params=prm)
+        torch.cuda.synchronize()
+        gshad = tracer.colors.cpu().numpy().view(np.uint32)
+
+        opaths = Oracle.trace_paths(od, oparams, W, H)
+        assert np.array_equal(gpaths, opaths), f"paths differ at {np.argwhere((gpaths != opaths).any(-1))[:5]}"
+        hits = (opaths[..., :3] != 0).any(-1).mean()
+        assert hits > 0.05, "camera sees almost nothing"
+        ocols = Oracle.trace_colors(od, oc, opaths, W, H)
+        assert np.array_equal(gcols, ocols), f"colours differ at {np.argwhere(gcols != ocols)[:5]}"
+        oshad = Oracle.trace_shadows(od, oparams, opaths, ocols, W, H, 1.0, 0.0)
+        assert np.array_equal(gshad, oshad), f"shadows differ at {np.argwhere(gshad != oshad)[:5]}"
+
+
+def test_byte_counters_match_oracle():
+    import torch
+    from hashdag_amd import DAGInfo, DAGTracer, ray_params
+    from hashdag_amd.workload import cameras
+    from oracle import Oracle
+
+    p, sc, g, dag, od, oc = _setup("d10")
+    W, H = 128, 72
+    info = DAGInfo.unit(g.levels)
+    tracer = DAGTracer(W, H)
+    cam = cameras(p, 1, seed=5)[0]
+    prm = ray_params(cam, info, g.levels, W, H)
+    tracer.resolve_paths(cam, info, dag, params=prm)
+    tracer.resolve_colors(dag)
+    b = tracer.count_bytes(dag, prm)
+    torch.cuda.synchronize()
+    oparams = Oracle.ray_params(cam, info, g.levels, W, H)
+    opaths, ob = Oracle.trace_paths(od, oparams, W, H, count=True)
+    ocols = Oracle.trace_colors(od, oc, opaths, W, H)
+    _, osb = Oracle.trace_shadows(od, oparams, opaths, ocols, W, H, 1.0, 0.0, count=True)
+    assert b[0] == ob
+    assert b[2] == osb
+    assert b[1] > 0
+
+
+def test_fog_close_to_oracle():
+    """With fog the shading uses exp/pow; device and host libm may differ in the last ulp,
+    so RGB8 bytes are allowed to differ by at most 1."""
+    import torch
+    from hashdag_amd import DAGInfo, DAGTracer, ray_params
+    from hashdag_amd.workload import cameras
+    from oracle import Oracle
+
+    p, sc, g, dag, od, oc = _setup("d8")
+    W, H = 96, 64
+    info = DAGInfo.unit(g.levels)
+    tracer = DAGTracer(W, H)
+    cam = cameras(p, 1, seed=3)[0]
+    prm = ray_params(cam, info, g.levels, W, H)
+    tracer.resolve_paths(cam, info, dag, params=prm)
+    tracer.resolve_colors(dag)
+    tracer.resolve_shadows(cam, info, dag, 1.0, 2000.0, params=prm)
+    torch.cuda.synchronize()
+    g8 = tracer.colors.cpu().numpy().view(np.uint8).reshape(H, W, 4).astype(int)
+    oparams = Oracle.ray_params(cam, info, g.levels, W, H)
+    opaths = Oracle.trace_paths(od, oparams, W, H)
+    ocols = Oracle.trace_colors(od, oc, opaths, W, H)
+    o8 = Oracle.trace_shadows(od, oparams, opaths, ocols, W, H, 1.0, 2000.0).view(np.uint8).reshape(H, W, 4).astype(int)
+    assert np.abs(g8 - o8).max() <= 1
