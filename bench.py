@@ -1,0 +1,222 @@
+"""Benchmark: primary Mrays/s @1080p on the 2^15 coloured terrain (one GPU).
+
+Workload (one step = one frame for one camera): trace primary paths, resolve colours and
+one shadow ray per pixel at 1920x1080, cameras cycling over 64 seeded views placed 5-20 %
+of the extent above the terrain. The scene (seeded simplex heightfield shell, per-voxel
+RGB8 colours in compressed leaves) is generated and loaded before timing; the timed region
+runs with everything resident in HBM.
+
+Prints one JSON line with the metric, a roofline object for the dominant kernel (algorithmic
+bytes counted by an instrumented launch of the same kernels, divided by the average kernel
+time measured with events on the launch stream) and a CPU baseline (the oracle's host
+traversal on a bounded sample of the same frames, all host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--levels", type=int, default=15)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cameras", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=1337)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile", type=str, default="8x8")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print("bench.py measures a single MI355X; multi-GPU runs are not part of this build", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from hashdag_amd import DAGInfo, DAGTracer, Geometry, HashDAG, TerrainParams, generate_terrain, ray_params
+    from hashdag_amd._lib import lib
+    from hashdag_amd.workload import cameras
+
+    W, H, D = args.width, args.height, args.levels
+    t0 = time.time()
+    params = TerrainParams(levels=D, seed=args.seed, colors=True)
+    scene = generate_terrain(params)
+    log(f"scene 2^{D}: {scene.dag.size} BasicDAG words, {scene.num_voxels} voxels, "
+        f"{scene.blocks.size} colour blocks, generated in {time.time() - t0:.1f}s")
+
+    geom = Geometry(levels=D)
+    t0 = time.time()
+    dag = HashDAG(geom)
+    dag.load_scene(scene)
+    info = dag.info()
+    log(f"loaded into the device hash table in {time.time() - t0:.1f}s (pool_top {info.pool_top} pages)")
+    dinfo = DAGInfo.unit(D)
+    cams = cameras(params, args.cameras, seed=args.seed)
+    tracer = DAGTracer(W, H)
+    tw, th = (int(v) for v in args.tile.split("x"))
+    tracer.set_tile(dag, tw, th)
+    rparams = [ray_params(c, dinfo, D, W, H) for c in cams]
+
+    stream = torch.cuda.current_stream()
+
+    def frame(i: int, ev=None):
+        p = rparams[i % len(rparams)]
+        if ev is not None:
+            ev[0].record(stream)
+        tracer.resolve_paths(None, dinfo, dag, params=p)
+        if ev is not None:
+            ev[1].record(stream)
+        tracer.resolve_colors(dag)
+        if ev is not None:
+            ev[2].record(stream)
+        tracer.resolve_shadows(None, dinfo, dag, 1.0, 0.0, params=p)
+        if ev is not None:
+            ev[3].record(stream)
+
+    for i in range(args.warmup):
+        frame(i)
+    torch.cuda.synchronize()
+
+    # algorithmic bytes of the frames the timed region runs (instrumented launches, untimed)
+    alg = np.zeros(3, np.float64)
+    ncount = min(args.steps, len(rparams))
+    for i in range(ncount):
+        alg += np.array(tracer.count_bytes(dag, rparams[i % len(rparams)]), np.float64)
+    alg /= ncount
+    torch.cuda.synchronize()
+
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        frame(i, events[i])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+
+    k_ms = np.zeros(3)
+    for ev in events:
+        k_ms += [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3])]
+    k_ms /= args.steps
+    names = ["trace_paths", "trace_colors", "trace_shadows"]
+    dom = int(np.argmax(k_ms))
+    achieved = alg[dom] / (k_ms[dom] * 1e-3) / 1e9
+
+    # achievable streaming bandwidth on this device (read + write of 2 GiB)
+    nbytes = 1 << 31
+    src = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    import ctypes as C
+    for _ in range(3):
+        lib.hd_stream_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), nbytes, C.c_void_p(stream.cuda_stream))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    reps = 10
+    for _ in range(reps):
+        lib.hd_stream_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), nbytes, C.c_void_p(stream.cuda_stream))
+    e1.record(stream)
+    torch.cuda.synchronize()
+    stream_gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+
+    value = W * H * args.steps / elapsed / 1e6
+    result = {
+        "metric": "primary Mrays/s @1080p",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2^{D} coloured simplex terrain, {W}x{H} primary paths + colours + 1 shadow ray per pixel, "
+                        f"{len(cams)} seeded cameras (one camera per step)",
+            "levels": D, "width": W, "height": H, "cameras": len(cams), "seed": args.seed,
+            "voxels": int(scene.num_voxels), "tile": args.tile,
+        },
+        "kernels_ms": {n: round(float(v), 4) for n, v in zip(names, k_ms)},
+        "primary_only_mrays_s": round(W * H / (k_ms[0] * 1e-3) / 1e6, 2),
+        "algorithmic_bytes_per_launch": {n: int(v) for n, v in zip(names, alg)},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": names[dom],
+            "achieved": round(float(achieved), 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(float(achieved) / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "achievable_stream_gbs": round(stream_gbs, 1),
+        },
+    }
+
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(scene, geom, rparams, W, H, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+    dag.close()
+    return 0
+
+
+def cpu_baseline(scene, geom, rparams, W, H, budget_s):
+    """The oracle's host traversal (paths + colours + shadows) on a bounded sample of the same
+    frames: whole rows of the 1080p frames, all host cores, until the time budget is used."""
+    import numpy as np
+
+    from oracle import Oracle, OracleColors, OracleDAG
+
+    threads = len(os.sched_getaffinity(0))
+    t0 = time.time()
+    pool_pages = 0x8FFFFFFF // 512 // 4
+    od = OracleDAG.from_basic(geom, scene.dag, pool_pages)
+    oc = OracleColors.from_scene(scene)
+    log(f"oracle loaded the scene in {time.time() - t0:.1f}s")
+    rays = 0
+    spent = 0.0
+    frames = 0
+    rows_per_chunk = 64
+    while spent < budget_s and frames < len(rparams):
+        p = np.array(rparams[frames].as_list())
+        for y0 in range(0, H, rows_per_chunk):
+            if spent >= budget_s:
+                break
+            y1 = min(H, y0 + rows_per_chunk)
+            t = time.perf_counter()
+            paths = Oracle.trace_paths(od, p, W, H, rows=(H - y1, H - y0), threads=threads)
+            cols = Oracle.trace_colors(od, oc, paths, W, H, rows=(y0, y1), threads=threads)
+            Oracle.trace_shadows(od, p, paths, cols, W, H, rows=(y0, y1), threads=threads)
+            spent += time.perf_counter() - t
+            rays += W * (y1 - y0)
+        frames += 1
+    return {
+        "value": round(rays / spent / 1e6, 3),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{rays} pixels ({rays / (W * H):.2f} frames of {W}x{H}, paths+colours+shadows) in {spent:.1f}s",
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

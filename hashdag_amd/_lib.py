@@ -117,6 +117,7 @@ _SIGS = {
     "hd_set_trace_tile": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "hd_trace_count_bytes": (C.c_int32, [C.c_void_p, C.POINTER(hd_ray_params), C.c_uint32, C.c_uint32, C.c_void_p,
                                          C.c_void_p, C.c_float, C.c_float, u64p]),
+    "hd_stream_copy": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "hd_scene_generate": (C.c_int32, [C.POINTER(hd_terrain_params), C.POINTER(C.c_void_p)]),
     "hd_scene_free": (None, [C.c_void_p]),
     "hd_scene_sizes": (C.c_int32, [C.c_void_p, u64p]),
@@ -129,6 +130,11 @@ EXPORTED = sorted(_SIGS)
 
 
 def _load() -> C.CDLL:
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 / libhsa-runtime64.so.1;
+    # loading torch first makes the library's dependencies resolve to those (same sonames)
+    # instead of pulling a second runtime from /opt/rocm, which cannot share the device.
+    import torch  # noqa: F401
+
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "

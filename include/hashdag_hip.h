@@ -138,6 +138,10 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
                                uint32_t* d_paths, uint32_t* d_colors, float shadow_bias, float fog_density,
                                uint64_t bytes[3]);
 
+/* Streaming device copy of `bytes` (multiple of 16) from d_src to d_dst: measures the
+ * achievable HBM bandwidth next to the tracer (not in the reference). */
+hd_status hd_stream_copy(const void* d_src, void* d_dst, uint64_t bytes, void* stream);
+
 /* Synthetic terrain scene (not in the reference, whose scenes are offline .bin files,
  * basic_dag.cpp:55-117): generates a BasicDAG plus colour tree and main leaf on the host. */
 typedef struct hd_terrain_params {
