@@ -63,7 +63,9 @@ def main() -> int:
     log(f"scene 2^{D}: {scene.dag.size} BasicDAG words, {scene.num_voxels} voxels, "
         f"{scene.blocks.size} colour blocks, generated in {time.time() - t0:.1f}s")
 
-    geom = Geometry(levels=D)
+    geom = Geometry.fit(D, scene.level_words, headroom=1.5)
+    log(f"bucket geometry: {geom.num_top_levels} top-geometry levels, {geom.total_virtual()} virtual words; "
+        f"level words {[int(w) for w in scene.level_words]}")
     t0 = time.time()
     dag = HashDAG(geom)
     dag.load_scene(scene)
@@ -156,6 +158,8 @@ def main() -> int:
                         f"{len(cams)} seeded cameras (one camera per step)",
             "levels": D, "width": W, "height": H, "cameras": len(cams), "seed": args.seed,
             "voxels": int(scene.num_voxels), "tile": args.tile,
+            "bucket_geometry": {"num_top_levels": geom.num_top_levels, "top": "1024 x 1024 words",
+                                "low": "65536 x 4096 words", "page_words": geom.page_size},
         },
         "kernels_ms": {n: round(float(v), 4) for n, v in zip(names, k_ms)},
         "primary_only_mrays_s": round(W * H / (k_ms[0] * 1e-3) / 1e6, 2),

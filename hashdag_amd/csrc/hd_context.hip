@@ -479,6 +479,7 @@ hd_status hd_scene_copy(const hd_scene* s, uint32_t* dag, uint32_t* color_nodes,
 hd_status hd_scene_level_nodes(const hd_scene* s, uint64_t* out) {
     if (!s) return fail(nullptr, HD_ERR_INVALID, "null scene");
     std::copy(s->s.level_nodes.begin(), s->s.level_nodes.end(), out);
+    std::copy(s->s.level_words.begin(), s->s.level_words.end(), out + s->s.level_nodes.size());
     return HD_OK;
 }
 

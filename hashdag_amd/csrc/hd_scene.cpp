@@ -197,9 +197,11 @@ Scene generate_terrain(const TerrainParams& prm) {
     sc.color_tree_levels = ctl;
     sc.num_voxels = voxels.load();
     sc.level_nodes.assign(L + 1, 0);
+    sc.level_words.assign(L + 1, 0);
     std::vector<std::array<std::vector<uint32_t>, kShards>> pos(L + 1);
     uint64_t total = 0;
     for (uint32_t l = 0; l <= L; ++l) {
+        const uint64_t levelStart = total;
         for (uint32_t s = 0; s < kShards; ++s) {
             Shard& sh = B.sets[l]->shards[s];
             pos[l][s].resize(sh.count);
@@ -210,6 +212,7 @@ Scene generate_terrain(const TerrainParams& prm) {
             }
             sc.level_nodes[l] += sh.count;
         }
+        sc.level_words[l] = total - levelStart;
         if (total >= 0xFFFFFFF0ull) throw std::runtime_error("scene: BasicDAG exceeds 32-bit indices");
     }
     if (sc.level_nodes[0] != 1) throw std::runtime_error("scene: root level must hold one node");

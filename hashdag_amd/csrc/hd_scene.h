@@ -28,6 +28,7 @@ struct Scene {
     std::vector<uint32_t> dag;       // BasicDAG words; root at 0; header = mask | count << 8 at
                                      // levels >= color_tree_levels, mask only above
     std::vector<uint64_t> level_nodes;  // unique nodes per level
+    std::vector<uint64_t> level_words;  // BasicDAG words per level
     uint64_t num_voxels = 0;
     // colour tree (empty when colours are off: a single all-zero root node is used)
     std::vector<uint32_t> color_nodes;

@@ -42,6 +42,31 @@ class Geometry:
     def total_virtual(self) -> int:
         return self.total_pages() * self.page_size
 
+    def capacity(self, level: int) -> int:
+        if level < self.top():
+            return (1 << self.bucket_bits_top) * self.bucket_size_top
+        return (1 << self.bucket_bits_low) * self.bucket_size_low
+
+    @staticmethod
+    def fit(levels: int, level_words, color_tree_levels: int = 10, top_fill: float = 0.4,
+            low_fill: float = 0.6, headroom: float = 1.0) -> "Geometry":
+        """Reference bucket geometry with as many top-geometry levels as the measured per-level
+        node words allow (a level is 'top' while its words fit `top_fill` of the top capacity;
+        the reference's 9 when they all do). `headroom` scales the words for later growth."""
+        g = Geometry(levels=levels, color_tree_levels=color_tree_levels)
+        top_cap = (1 << g.bucket_bits_top) * g.bucket_size_top
+        n_top = 0
+        while n_top < min(9, levels) and n_top < len(level_words) and \
+                level_words[n_top] * headroom <= top_fill * top_cap:
+            n_top += 1
+        g.num_top_levels = n_top
+        for lvl, w in enumerate(level_words):
+            if w * headroom > (low_fill if lvl >= n_top else top_fill) * g.capacity(lvl):
+                raise ValueError(f"level {lvl}: {int(w)} words do not fit the bucket geometry")
+        if g.total_virtual() >= 1 << 32:
+            raise ValueError("virtual address space exceeds 32 bits")
+        return g
+
 
 @dataclass
 class HostImage:

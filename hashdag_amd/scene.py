@@ -46,6 +46,7 @@ class Scene:
     num_voxels: int
     level_nodes: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
     params: TerrainParams | None = None
+    level_words: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
 
 
 def generate_terrain(params: TerrainParams) -> Scene:
@@ -63,12 +64,13 @@ def generate_terrain(params: TerrainParams) -> Scene:
         macro = np.zeros(int(sizes[5]), np.uint64)
         check(lib.hd_scene_copy(handle, ptr(dag), ptr(nodes), ptr(offsets, C.c_uint64), ptr(blocks, C.c_uint64),
                                 ptr(weights), ptr(macro, C.c_uint64)))
-        level_nodes = np.zeros(params.levels - 1, np.uint64)
-        check(lib.hd_scene_level_nodes(handle, ptr(level_nodes, C.c_uint64)))
+        per_level = np.zeros(2 * (params.levels - 1), np.uint64)
+        check(lib.hd_scene_level_nodes(handle, ptr(per_level, C.c_uint64)))
     finally:
         lib.hd_scene_free(handle)
+    n = params.levels - 1
     return Scene(params.levels, params.color_tree_levels, dag, nodes, offsets, blocks, weights, macro,
-                 int(sizes[6]), level_nodes, params)
+                 int(sizes[6]), per_level[:n].copy(), params, per_level[n:].copy())
 
 
 def terrain_height(params: TerrainParams, x: int, z: int) -> int:

@@ -163,7 +163,7 @@ void hd_scene_free(hd_scene* s);
 hd_status hd_scene_sizes(const hd_scene* s, uint64_t sizes[7]);
 hd_status hd_scene_copy(const hd_scene* s, uint32_t* dag, uint32_t* color_nodes, uint64_t* color_offsets,
                         uint64_t* blocks, uint32_t* weights, uint64_t* macro);
-/* unique node counts per level (levels-1 entries) */
+/* unique node counts per level (levels-1 entries), then BasicDAG words per level (levels-1) */
 hd_status hd_scene_level_nodes(const hd_scene* s, uint64_t* out);
 int32_t hd_terrain_height(const hd_terrain_params* p, uint32_t x, uint32_t z);
 
