@@ -528,7 +528,8 @@ inline dim3 grid_of(const hd_ctx* ctx, uint32_t W, uint32_t H) {
     return dim3((W + 2 * ctx->tile_w - 1) / (2 * ctx->tile_w), (H + 2 * ctx->tile_h - 1) / (2 * ctx->tile_h));
 }
 
-inline hipStream_t stream_of(hd_ctx* ctx, void* s) { return s ? reinterpret_cast<hipStream_t>(s) : ctx->stream; }
+// The caller's stream as given; NULL is the HIP default stream (torch's default stream).
+inline hipStream_t stream_of(hd_ctx*, void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 }  // namespace
 }  // namespace hd

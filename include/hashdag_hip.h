@@ -1,7 +1,7 @@
 /* hashdag_hip.h - C ABI of the MI355X HashDAG hot path (tracing, edits, hash table, GC).
  *
  * Plain C types only: device buffers are passed as raw device pointers, streams as the
- * opaque hipStream_t handle cast to void* (NULL = the context's own stream). Every entry
+ * opaque hipStream_t handle cast to void* (NULL = the HIP default stream). Every entry
  * point returns an hd_status (0 = OK, < 0 = error); hd_last_error() returns the message.
  * Contexts are independent; calls on one context are stream-ordered and must not race.
  *

