@@ -171,7 +171,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(float(achieved) / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(names[dom], D, W, H),
+            "traffic_source": "profiles/trace_kernels_2p15_1080p.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "achievable_stream_gbs": round(stream_gbs, 1),
         },
     }
@@ -181,6 +182,20 @@ def main() -> int:
     print(json.dumps(result), flush=True)
     dag.close()
     return 0
+
+
+def measured_traffic(kernel: str, levels: int, W: int, H: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (scaled by the copy-kernel calibration), when it covers this workload."""
+    path = os.path.join(ROOT, "profiles", "trace_kernels_2p15_1080p.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes "):
+            kv = dict(tok.split("=") for tok in line.split()[1:])
+            if (int(kv.get("levels", -1)), int(kv.get("width", -1)), int(kv.get("height", -1))) == (levels, W, H):
+                return int(kv[kernel]) if kernel in kv else None
+    return None
 
 
 def cpu_baseline(scene, geom, rparams, W, H, budget_s):

@@ -89,6 +89,7 @@ static hd_status upload_image_impl(hd_ctx* ctx, const uint32_t* pool, const uint
     }
     const uint32_t counters[2] = {pool_top, 0u};
     if (e == hipSuccess) e = hipMemcpyAsync(ctx->counters, counters, sizeof(counters), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_root, &root, 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(dphys);
     if (e != hipSuccess) return hip_fail(ctx, e, "upload image");
@@ -153,6 +154,7 @@ hd_status hd_create(const hd_config* cfg, hd_ctx** out) {
     if ((e = hipMalloc(&ctx->counters, 64 * 4)) != hipSuccess) return cleanup(hip_fail(nullptr, e, "counters"));
     if ((e = hipMalloc(&ctx->byte_counters, 8 * sizeof(unsigned long long))) != hipSuccess)
         return cleanup(hip_fail(nullptr, e, "byte counters"));
+    if ((e = hipMalloc(&ctx->d_root, 64)) != hipSuccess) return cleanup(hip_fail(nullptr, e, "root"));
     hipMemsetAsync(ctx->vpool, 0, vwords * 4, ctx->stream);
     hipMemsetAsync(ctx->page_table, 0, g.total_pages() * 4, ctx->stream);
     hipMemsetAsync(ctx->bucket_sizes, 0, g.total_buckets() * 4, ctx->stream);
@@ -166,6 +168,9 @@ hd_status hd_create(const hd_config* cfg, hd_ctx** out) {
 hd_status hd_destroy(hd_ctx* ctx) {
     if (!ctx) return HD_OK;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    hipDeviceSynchronize();
+    edit_free(ctx);
+    hipFree(ctx->d_root);
     hipFree(ctx->vpool);
     hipFree(ctx->page_table);
     hipFree(ctx->bucket_sizes);
@@ -262,6 +267,7 @@ hd_status hd_set_root(hd_ctx* ctx, uint32_t root) {
     if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
     if ((uint64_t)root >= ctx->g.total_virtual()) return fail(ctx, HD_ERR_INVALID, "root outside the virtual space");
     ctx->root = root;
+    HD_HIP(ctx, hipMemcpy(ctx->d_root, &root, 4, hipMemcpyHostToDevice));
     return HD_OK;
 }
 
@@ -308,6 +314,9 @@ hd_status hd_load_colors(hd_ctx* ctx, const uint32_t* nodes, uint64_t n_nodes, c
     if (n_weights)
         HD_HIP(ctx, hipMemcpyAsync(reinterpret_cast<uint32_t*>(ctx->arena) + m.weights32, weights, n_weights * 4,
                                    hipMemcpyHostToDevice, ctx->stream));
+    // colour-tree allocation tops used by edits: entries in use, unique leaves in use
+    const uint32_t tops[2] = {(uint32_t)n_nodes, 0u};
+    HD_HIP(ctx, hipMemcpyAsync(ctx->counters + 2, tops, sizeof(tops), hipMemcpyHostToDevice, ctx->stream));
     HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->main = m;
     ctx->arena_top = mainWords;
@@ -338,8 +347,9 @@ static hd_status resolve_leaf(hd_ctx* ctx, uint32_t handle, LeafDesc& d) {
     if (!ctx || !ctx->colors_loaded) return fail(ctx, HD_ERR_INVALID, "no colours");
     if (handle & 0x80000000u) {
         const uint32_t i = handle & 0x7FFFFFFFu;
-        if (i >= ctx->h_leaves.size()) return fail(ctx, HD_ERR_INVALID, "leaf handle out of range");
-        d = ctx->h_leaves[i];
+        if (i >= ctx->n_cleaves) return fail(ctx, HD_ERR_INVALID, "leaf handle out of range");
+        HD_HIP(ctx, hipMemcpyAsync(&d, ctx->cleaves + i, sizeof(d), hipMemcpyDeviceToHost, ctx->stream));
+        HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
         return HD_OK;
     }
     if (handle >= ctx->n_coffsets) return fail(ctx, HD_ERR_INVALID, "shared handle out of range");

@@ -8,7 +8,18 @@
 #include <vector>
 
 #include "../../include/hashdag_hip.h"
+#include "hd_edit.h"
 #include "hd_internal.h"
+
+namespace hd {
+// One undo step (HashDAGUndoRedo::Frame, hash_dag.h:29-45): the roots before / after and the
+// colour-leaf descriptors the pass replaced (old leaf data stays in the colour arena).
+struct UndoFrame {
+    uint32_t prev_root = 0, new_root = 0;
+    std::vector<uint32_t> handles;
+    std::vector<LeafDesc> prev, next;
+};
+}  // namespace hd
 
 struct hd_ctx {
     hd::Geom g{};
@@ -42,11 +53,17 @@ struct hd_ctx {
     // tracer
     uint32_t tile_w = 8, tile_h = 8;
     unsigned long long* byte_counters = nullptr;  // 3 counters for instrumented launches
+
+    // edits
+    uint32_t* d_root = nullptr;  // device copy of the root (edit passes chain through it)
+    hd::EditScratch edit;
+    std::vector<hd::UndoFrame> undo, redo;
 };
 
 namespace hd {
 constexpr uint64_t kVpoolPad = 64;  // words after the virtual space (vector reads past a node end)
 
+void edit_free(hd_ctx* ctx);
 hd_status fail(hd_ctx* ctx, hd_status code, const std::string& msg);
 hd_status hip_fail(hd_ctx* ctx, hipError_t e, const char* what);
 DevDag dev_dag(const hd_ctx* ctx);

@@ -1,0 +1,114 @@
+// Device edit pipeline: shared structures (see hd_edit.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "hd_internal.h"
+
+struct hd_ctx;
+
+namespace hd {
+
+constexpr uint32_t EDIT_SPHERE_ADD = 0, EDIT_SPHERE_CARVE = 1, EDIT_BOX_ADD = 2, EDIT_BOX_CARVE = 3,
+                   EDIT_SPHERE_PAINT = 4;
+
+// error bits reported by a pass
+constexpr uint32_t ERR_SCRATCH = 1, ERR_BUCKET = 2, ERR_POOL = 4, ERR_HEADER = 8, ERR_COLOR_SCRATCH = 16,
+                   ERR_COLOR_ARENA = 32, ERR_COLOR_TREE = 64, ERR_UNDO = 128, ERR_EMPTY = 256;
+
+constexpr uint32_t kMaxEditLevels = 17;
+constexpr uint32_t kUndoCap = 1u << 20;
+
+// An editor with its derived bounds (hash_dag_editors.h:57-127): box min/max = c -/+ r,
+// r2 = r * r, `single` = rgb101210 colour of add editors (sphere red, box magenta).
+struct DevEdit {
+    uint32_t kind;
+    float c[3];
+    float radius;
+    float r2;
+    float bmin[3], bmax[3];
+    uint32_t single;
+    uint32_t pad[3];
+};
+
+// Work items of one level (structure of arrays, capacity `cap`; child arrays are 8 * cap).
+struct Items {
+    uint32_t *px, *py, *pz;   // path of the item's node
+    uint32_t* old_;           // effective old node (may be a full node after a full cover)
+    uint32_t* orig;           // the node the parent actually points to
+    uint32_t* parent;
+    uint8_t* slot;
+    uint32_t *estart, *ecount;  // edit sub-list in elist[level]
+    uint32_t* aux;              // levels < ctl: colour node; levels >= ctl: task (level-ctl item)
+    uint32_t* handle;           // tasks: colour leaf handle before the pass
+    uint32_t* newhandle;        // tasks: unique leaf handle written by the pass
+    uint32_t *cmode, *cmodeval;  // old colours: 0 = old leaf, 1 = single colour cmodeval, 2 = none
+    uint8_t* ckind;             // per child slot: keep / empty / full / item
+    uint32_t *cval, *corig, *cold, *cnew, *ccolor, *cstart, *costart;
+    uint32_t *nidx, *ncount;    // result: new node index and voxel count
+    uint32_t *cont, *clen, *hash, *rep, *state;  // pending node content (9 words) for the table
+    uint32_t *nstart, *ostart;  // colour starts relative to the task
+    uint64_t* cbase;            // tasks: base of the task's colour array
+};
+
+struct EditPass {
+    Geom g;
+    uint32_t levels, ctl;
+    uint32_t* vpool;
+    uint32_t* page_table;
+    uint32_t* bucket_sizes;
+    const uint32_t* full_nodes;
+    uint32_t* pool_top;
+    uint32_t* errors;
+    uint32_t* n_cnodes;
+    uint32_t* n_leaves;
+    uint32_t* undo_count;
+    uint32_t pool_pages;
+    uint32_t* count;   // items per level
+    uint32_t* ecount;  // edit-list words per level
+    Items items[kMaxEditLevels];
+    uint32_t* elist[kMaxEditLevels];
+    uint32_t cap, ecap;
+    uint32_t* hset;
+    uint32_t hcap;
+    const DevEdit* edits;
+    uint32_t single_edit;
+    unsigned long long* stat_voxels;
+    unsigned long long* stat_new_words;
+    unsigned long long* color_top;
+    unsigned long long* arena_top;
+    uint64_t* colors;
+    uint64_t color_cap;
+    uint32_t* cnodes;
+    uint32_t cnodes_cap;
+    const uint64_t* offsets;
+    LeafDesc* leaves;
+    uint32_t leaves_cap;
+    LeafDesc main;
+    uint64_t* arena;
+    uint64_t arena_cap;
+    uint32_t* undo_handles;
+    LeafDesc* undo_descs;
+    uint32_t undo_cap;
+};
+
+struct EditScratch {
+    uint32_t levels = 0, item_cap = 0, elist_cap = 0, hcap = 0, edits_cap = 0;
+    uint64_t color_cap = 0;
+    Items items[kMaxEditLevels]{};
+    uint32_t* elist[kMaxEditLevels]{};
+    uint32_t* hset = nullptr;
+    uint64_t* colors = nullptr;
+    uint32_t* dcount = nullptr;
+    uint32_t* decount = nullptr;
+    uint64_t* dstats = nullptr;
+    uint32_t* undo_handles = nullptr;
+    LeafDesc* undo_descs = nullptr;
+    DevEdit* edits_dev = nullptr;
+    std::vector<void*> allocs;
+};
+
+}  // namespace hd

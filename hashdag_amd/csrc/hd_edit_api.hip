@@ -1,0 +1,319 @@
+// C ABI of the device edits: HashDAG::edit_threads for one edit per pass (plain) or a
+// whole batch fused into one pass, edited-voxel counts, undo / redo over root history and
+// colour-leaf descriptors (HashDAGUndoRedo, hash_dag.h:20-212).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hd_ctx.h"
+
+namespace hd {
+
+hd_status edit_reserve(hd_ctx* ctx, uint32_t item_cap, uint32_t elist_cap, uint64_t color_cap);
+void edit_free(hd_ctx* ctx);
+hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s);
+
+namespace {
+
+// edited-voxel count of one editor: voxels of [0, E)^3 whose unit box the editor touches
+// (the reference's "num voxels", hash_dag.h:374-378; touch = should_edit_impl on the voxel).
+__device__ __forceinline__ bool touch_voxel(const DevEdit& e, float x, float y, float z) {
+    const float x1 = x + 1.0f, y1 = y + 1.0f, z1 = z + 1.0f;
+    if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE)
+        return !(e.bmin[0] >= x1 || e.bmin[1] >= y1 || e.bmin[2] >= z1 || e.bmax[0] <= x || e.bmax[1] <= y ||
+                 e.bmax[2] <= z || x >= e.bmax[0] || y >= e.bmax[1] || z >= e.bmax[2] || x1 <= e.bmin[0] ||
+                 y1 <= e.bmin[1] || z1 <= e.bmin[2]);
+    float dist = e.r2;
+    if (e.c[0] < x) dist -= (e.c[0] - x) * (e.c[0] - x);
+    else if (e.c[0] > x1) dist -= (e.c[0] - x1) * (e.c[0] - x1);
+    if (e.c[1] < y) dist -= (e.c[1] - y) * (e.c[1] - y);
+    else if (e.c[1] > y1) dist -= (e.c[1] - y1) * (e.c[1] - y1);
+    if (e.c[2] < z) dist -= (e.c[2] - z) * (e.c[2] - z);
+    else if (e.c[2] > z1) dist -= (e.c[2] - z1) * (e.c[2] - z1);
+    return dist > 0;
+}
+
+// One block per editor; threads stride over the (x, z) columns of its bounding square; the
+// touched y values of a column form an interval (the predicate is monotone in |y - c.y|),
+// found by two binary searches around the centre row.
+__global__ void k_count_voxels(const DevEdit* __restrict__ edits, uint32_t n, uint32_t E,
+                               unsigned long long* __restrict__ out) {
+    for (uint32_t ei = blockIdx.x; ei < n; ei += gridDim.x) {
+        const DevEdit e = edits[ei];
+        const float r = e.radius;
+        const int64_t x0 = std::max<int64_t>(0, (int64_t)floorf(e.c[0] - r) - 1);
+        const int64_t x1 = std::min<int64_t>((int64_t)E - 1, (int64_t)ceilf(e.c[0] + r) + 1);
+        const int64_t z0 = std::max<int64_t>(0, (int64_t)floorf(e.c[2] - r) - 1);
+        const int64_t z1 = std::min<int64_t>((int64_t)E - 1, (int64_t)ceilf(e.c[2] + r) + 1);
+        const int64_t ylo = std::max<int64_t>(0, (int64_t)floorf(e.c[1] - r) - 1);
+        const int64_t yhi = std::min<int64_t>((int64_t)E - 1, (int64_t)ceilf(e.c[1] + r) + 1);
+        if (x1 < x0 || z1 < z0 || yhi < ylo) continue;
+        const int64_t nx = x1 - x0 + 1, nz = z1 - z0 + 1;
+        unsigned long long acc = 0;
+        const int64_t ymid = std::min(yhi, std::max(ylo, (int64_t)floorf(e.c[1])));
+        for (int64_t col = threadIdx.x; col < nx * nz; col += blockDim.x) {
+            const float x = (float)(x0 + col % nx), z = (float)(z0 + col / nx);
+            if (!touch_voxel(e, x, (float)ymid, z)) continue;
+            // first touched y in [ylo, ymid] and last touched y in [ymid, yhi]
+            int64_t a = ylo, b = ymid;
+            while (a < b) {
+                const int64_t m = (a + b) / 2;
+                if (touch_voxel(e, x, (float)m, z)) b = m;
+                else a = m + 1;
+            }
+            const int64_t first = a;
+            a = ymid;
+            b = yhi;
+            while (a < b) {
+                const int64_t m = (a + b + 1) / 2;
+                if (touch_voxel(e, x, (float)m, z)) a = m;
+                else b = m - 1;
+            }
+            acc += (unsigned long long)(a - first + 1);
+        }
+        atomicAdd(out, acc);
+    }
+}
+
+__global__ void k_reset(uint32_t* counters, uint64_t* stats) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        counters[1] = 0;  // errors
+        counters[4] = 0;  // undo log
+        for (int i = 0; i < 8; ++i) stats[i] = 0;
+    }
+}
+
+DevEdit make_edit(const hd_edit& e) {
+    DevEdit d{};
+    d.kind = e.kind;
+    for (int i = 0; i < 3; ++i) d.c[i] = e.center[i];
+    d.radius = e.radius;
+    d.r2 = e.radius * e.radius;
+    for (int i = 0; i < 3; ++i) {
+        d.bmin[i] = e.center[i] - e.radius;
+        d.bmax[i] = e.center[i] + e.radius;
+    }
+    auto c101210 = [](float r, float g, float b) {
+        return (uint32_t)(r * 1023.0f) | ((uint32_t)(g * 4095.0f) << 10) | ((uint32_t)(b * 1023.0f) << 22);
+    };
+    d.single = (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE) ? c101210(1, 0, 1) : c101210(1, 0, 0);
+    return d;
+}
+
+struct Readback {
+    uint32_t counters[8];
+    uint64_t stats[8];
+    uint32_t root;
+    uint32_t count_ctl;
+};
+
+}  // namespace
+
+static hd_status ensure_color_tree(hd_ctx* ctx) {
+    if (ctx->colors_loaded) return HD_OK;
+    const uint32_t zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint64_t off0 = 0;
+    return hd_load_colors(ctx, zeros, 8, &off0, 1, nullptr, 0, nullptr, 0, nullptr, 0);
+}
+
+static hd_status grow_arena(hd_ctx* ctx, uint64_t need) {
+    const uint64_t cap = ctx->cfg.color_arena_words;
+    if (ctx->arena_top + need <= cap) return HD_OK;
+    const uint64_t ncap = std::max<uint64_t>(cap * 2, ctx->arena_top + need + (1u << 24));
+    uint64_t* na = nullptr;
+    HD_HIP(ctx, hipMalloc(&na, ncap * 8));
+    HD_HIP(ctx, hipMemsetAsync(na, 0, ncap * 8, ctx->stream));
+    HD_HIP(ctx, hipMemcpyAsync(na, ctx->arena, ctx->arena_top * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(ctx->arena);
+    ctx->arena = na;
+    ctx->cfg.color_arena_words = ncap;
+    return HD_OK;
+}
+
+static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s, Readback& rb, UndoFrame* frame) {
+    const bool colors = ctx->colors_loaded && ctx->g.leaf_level() > ctx->g.color_tree_levels;
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        hd_status st = grow_arena(ctx, 1u << 22);
+        if (st != HD_OK) return st;
+        EditScratch& S = ctx->edit;
+        k_reset<<<1, 1, 0, s>>>(ctx->counters, S.dstats);
+        HD_HIP(ctx, hipMemcpyAsync(S.dstats + 3, &ctx->arena_top, 8, hipMemcpyHostToDevice, s));
+        st = enqueue_pass(ctx, first, n, colors, s);
+        if (st != HD_OK) return st;
+        if (n > 1) {
+            k_count_voxels<<<std::min<uint32_t>(n, 4096), 256, 0, s>>>(S.edits_dev + first, n, 1u << ctx->g.levels,
+                                                                      reinterpret_cast<unsigned long long*>(S.dstats + 0));
+        }
+        HD_HIP(ctx, hipMemcpyAsync(rb.counters, ctx->counters, sizeof(rb.counters), hipMemcpyDeviceToHost, s));
+        HD_HIP(ctx, hipMemcpyAsync(rb.stats, S.dstats, sizeof(rb.stats), hipMemcpyDeviceToHost, s));
+        HD_HIP(ctx, hipMemcpyAsync(&rb.root, ctx->d_root, 4, hipMemcpyDeviceToHost, s));
+        HD_HIP(ctx, hipMemcpyAsync(&rb.count_ctl, S.dcount + ctx->g.color_tree_levels, 4, hipMemcpyDeviceToHost, s));
+        HD_HIP(ctx, hipStreamSynchronize(s));
+        const uint32_t err = rb.counters[1];
+        if (err & (ERR_SCRATCH | ERR_COLOR_SCRATCH)) {
+            // nothing was written to the table or the colours: grow the scratch and retry
+            st = edit_reserve(ctx, S.item_cap * 4, S.elist_cap * 4, (err & ERR_COLOR_SCRATCH) ? S.color_cap * 4 : S.color_cap);
+            if (st != HD_OK) return st;
+            continue;
+        }
+        if (err & ERR_BUCKET) return fail(ctx, HD_ERR_BUCKET_FULL, "hash bucket full during edit (old root kept)");
+        if (err & ERR_POOL) return fail(ctx, HD_ERR_POOL_FULL, "page pool exhausted during edit (old root kept)");
+        if (err & ERR_HEADER) return fail(ctx, HD_ERR_HEADER_OVERFLOW, "voxel count >= 2^24 in a node header");
+        if (err & (ERR_COLOR_ARENA | ERR_COLOR_TREE | ERR_UNDO))
+            return fail(ctx, HD_ERR_COLOR_FULL, "colour capacity exhausted during edit");
+        if (err & ERR_EMPTY) return fail(ctx, HD_ERR_INVALID, "edit would leave an empty scene (old root kept)");
+        ctx->arena_top = rb.stats[3];
+        ctx->n_cnodes = rb.counters[2];
+        ctx->n_cleaves = rb.counters[3];
+        if (frame) {
+            const uint32_t nu = rb.counters[4];
+            frame->handles.resize(nu);
+            frame->prev.resize(nu);
+            frame->next.resize(nu);
+            if (nu) {
+                HD_HIP(ctx, hipMemcpyAsync(frame->handles.data(), S.undo_handles, nu * 4, hipMemcpyDeviceToHost, s));
+                HD_HIP(ctx, hipMemcpyAsync(frame->prev.data(), S.undo_descs, nu * sizeof(LeafDesc), hipMemcpyDeviceToHost, s));
+                HD_HIP(ctx, hipStreamSynchronize(s));
+                for (uint32_t k = 0; k < nu; ++k) {
+                    LeafDesc d;
+                    HD_HIP(ctx, hipMemcpy(&d, ctx->cleaves + (frame->handles[k] & 0x7FFFFFFFu), sizeof(d), hipMemcpyDeviceToHost));
+                    frame->next[k] = d;
+                }
+            }
+        }
+        return HD_OK;
+    }
+    return fail(ctx, HD_ERR_OOM, "edit scratch could not be grown enough");
+}
+
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" {
+
+hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t fused, hd_edit_stats* stats,
+                        void* stream) {
+    if (!ctx || (!edits && n) ) return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
+    for (uint32_t i = 0; i < n; ++i)
+        if (edits[i].kind > EDIT_SPHERE_PAINT || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
+            return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");
+    hd_status st = ensure_color_tree(ctx);
+    if (st != HD_OK) return st;
+    if (!ctx->edit.item_cap) {
+        st = edit_reserve(ctx, 1u << 16, 1u << 18, 1u << 22);
+        if (st != HD_OK) return st;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // editors to the device
+    std::vector<DevEdit> dev(n);
+    for (uint32_t i = 0; i < n; ++i) dev[i] = make_edit(edits[i]);
+    if (n > ctx->edit.edits_cap) {
+        EditScratch& S = ctx->edit;
+        DevEdit* ne = nullptr;
+        HD_HIP(ctx, hipMalloc(&ne, (uint64_t)n * sizeof(DevEdit)));
+        S.allocs.push_back(ne);
+        S.edits_dev = ne;
+        S.edits_cap = n;
+    }
+    if (n) HD_HIP(ctx, hipMemcpyAsync(ctx->edit.edits_dev, dev.data(), (uint64_t)n * sizeof(DevEdit), hipMemcpyHostToDevice, s));
+    HD_HIP(ctx, hipMemcpyAsync(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice, s));
+    hd_edit_stats out{};
+    if (fused) {
+        if (n > ctx->edit.elist_cap) {
+            st = edit_reserve(ctx, ctx->edit.item_cap, std::max<uint32_t>(n * 2, ctx->edit.elist_cap), ctx->edit.color_cap);
+            if (st != HD_OK) return st;
+        }
+        UndoFrame frame;
+        frame.prev_root = ctx->root;
+        Readback rb{};
+        st = run_pass(ctx, 0, n, s, rb, &frame);
+        if (st != HD_OK) return st;
+        ctx->root = rb.root;
+        frame.new_root = rb.root;
+        ctx->undo.push_back(std::move(frame));
+        ctx->redo.clear();
+        out.num_voxels += rb.stats[0];
+        out.new_words += rb.stats[1];
+        out.leaves_rebuilt += rb.counters[4];
+        out.passes = 1;
+    } else {
+        for (uint32_t i = 0; i < n; ++i) {
+            UndoFrame frame;
+            frame.prev_root = ctx->root;
+            Readback rb{};
+            st = run_pass(ctx, i, 1, s, rb, &frame);
+            if (st != HD_OK) return st;
+            ctx->root = rb.root;
+            frame.new_root = rb.root;
+            ctx->undo.push_back(std::move(frame));
+            ctx->redo.clear();
+            out.num_voxels += rb.stats[0];
+            out.new_words += rb.stats[1];
+            out.leaves_rebuilt += rb.counters[4];
+            out.passes++;
+        }
+    }
+    out.root = ctx->root;
+    if (stats) *stats = out;
+    return HD_OK;
+}
+
+static hd_status apply_frame(hd_ctx* ctx, const UndoFrame& f, bool undo) {
+    for (size_t k = 0; k < f.handles.size(); ++k) {
+        const size_t idx = undo ? f.handles.size() - 1 - k : k;
+        const LeafDesc& d = undo ? f.prev[idx] : f.next[idx];
+        HD_HIP(ctx, hipMemcpy(ctx->cleaves + (f.handles[idx] & 0x7FFFFFFFu), &d, sizeof(d), hipMemcpyHostToDevice));
+    }
+    ctx->root = undo ? f.prev_root : f.new_root;
+    HD_HIP(ctx, hipMemcpy(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice));
+    return HD_OK;
+}
+
+hd_status hd_undo(hd_ctx* ctx) {
+    if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
+    if (ctx->undo.empty()) return HD_OK;
+    UndoFrame f = std::move(ctx->undo.back());
+    ctx->undo.pop_back();
+    hd_status st = apply_frame(ctx, f, true);
+    if (st != HD_OK) return st;
+    ctx->redo.push_back(std::move(f));
+    return HD_OK;
+}
+
+hd_status hd_redo(hd_ctx* ctx) {
+    if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
+    if (ctx->redo.empty()) return HD_OK;
+    UndoFrame f = std::move(ctx->redo.back());
+    ctx->redo.pop_back();
+    hd_status st = apply_frame(ctx, f, false);
+    if (st != HD_OK) return st;
+    ctx->undo.push_back(std::move(f));
+    return HD_OK;
+}
+
+hd_status hd_undo_clear(hd_ctx* ctx) {
+    if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
+    ctx->undo.clear();
+    ctx->redo.clear();
+    return HD_OK;
+}
+
+hd_status hd_undo_depth(hd_ctx* ctx, uint32_t* undo, uint32_t* redo) {
+    if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
+    if (undo) *undo = (uint32_t)ctx->undo.size();
+    if (redo) *redo = (uint32_t)ctx->redo.size();
+    return HD_OK;
+}
+
+hd_status hd_reserve_edit_scratch(hd_ctx* ctx, uint32_t items_per_level, uint32_t edit_list_words, uint64_t colors) {
+    if (!ctx || !items_per_level) return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    return edit_reserve(ctx, items_per_level, edit_list_words, colors);
+}
+
+}  // extern "C"

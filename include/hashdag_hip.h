@@ -138,6 +138,42 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
                                uint32_t* d_paths, uint32_t* d_colors, float shadow_bias, float fog_density,
                                uint64_t bytes[3]);
 
+/* Editors (hash_dag_editors.h): SphereEditor<true/false> (272-313), BoxEditor<true/false>
+ * (223-270), SpherePaintEditor (315-330); centre and radius in voxels. */
+#define HD_EDIT_SPHERE_ADD 0u
+#define HD_EDIT_SPHERE_CARVE 1u
+#define HD_EDIT_BOX_ADD 2u
+#define HD_EDIT_BOX_CARVE 3u
+#define HD_EDIT_SPHERE_PAINT 4u
+typedef struct hd_edit {
+    uint32_t kind;
+    float center[3];
+    float radius;
+} hd_edit;
+
+typedef struct hd_edit_stats {
+    uint64_t num_voxels;     /* "num voxels" (hash_dag.h:374-378): voxels the editors touched */
+    uint64_t new_words;      /* words appended to the hash table */
+    uint64_t leaves_rebuilt; /* "num color leaves rebuilt" (hash_dag.h:379) */
+    uint32_t root;           /* new firstNodeIndex */
+    uint32_t passes;         /* device passes run */
+} hd_edit_stats;
+
+/* Engine::edit<T> -> HashDAG::edit_threads<T> (engine.h:77-103, hash_dag.h:255-405) for each
+ * edit in order. fused = 0: one device pass per edit (the plain form); fused = 1: the whole
+ * batch in one pass, canonically identical to applying the edits one by one. On error the
+ * previous root is kept. One undo frame per pass. */
+hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t fused, hd_edit_stats* stats,
+                        void* stream);
+/* HashDAGUndoRedo::undo / redo (hash_dag.h:68-120) and free (133-145). */
+hd_status hd_undo(hd_ctx* ctx);
+hd_status hd_redo(hd_ctx* ctx);
+hd_status hd_undo_clear(hd_ctx* ctx);
+hd_status hd_undo_depth(hd_ctx* ctx, uint32_t* undo, uint32_t* redo);
+/* Pre-sizes the edit work-item scratch (items per level, edit-list words per level,
+ * colours per pass); grown automatically when a pass overflows. Not in the reference. */
+hd_status hd_reserve_edit_scratch(hd_ctx* ctx, uint32_t items_per_level, uint32_t edit_list_words, uint64_t colors);
+
 /* Streaming device copy of `bytes` (multiple of 16) from d_src to d_dst: measures the
  * achievable HBM bandwidth next to the tracer (not in the reference). */
 hd_status hd_stream_copy(const void* d_src, void* d_dst, uint64_t bytes, void* stream);
