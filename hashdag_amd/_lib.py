@@ -90,7 +90,23 @@ class hd_terrain_params(C.Structure):
     ]
 
 
+class hd_edit(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("center", C.c_float * 3), ("radius", C.c_float)]
+
+
+class hd_edit_stats(C.Structure):
+    _fields_ = [("num_voxels", C.c_uint64), ("new_words", C.c_uint64), ("leaves_rebuilt", C.c_uint64),
+                ("root", C.c_uint32), ("passes", C.c_uint32)]
+
+
 _SIGS = {
+    "hd_edit_batch": (C.c_int32, [C.c_void_p, C.POINTER(hd_edit), C.c_uint32, C.c_uint32, C.POINTER(hd_edit_stats),
+                                  C.c_void_p]),
+    "hd_undo": (C.c_int32, [C.c_void_p]),
+    "hd_redo": (C.c_int32, [C.c_void_p]),
+    "hd_undo_clear": (C.c_int32, [C.c_void_p]),
+    "hd_undo_depth": (C.c_int32, [C.c_void_p, u32p, u32p]),
+    "hd_reserve_edit_scratch": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64]),
     "hd_default_config": (None, [C.c_uint32, C.POINTER(hd_config)]),
     "hd_create": (C.c_int32, [C.POINTER(hd_config), C.POINTER(C.c_void_p)]),
     "hd_destroy": (C.c_int32, [C.c_void_p]),

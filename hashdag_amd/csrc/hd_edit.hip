@@ -170,7 +170,7 @@ __device__ __forceinline__ void set_error(uint32_t* flags, uint32_t bit) { atomi
 // expand: items of level l -> child classification and items of level l+1
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
-    const uint32_t n = P.count[l];
+    const uint32_t n = min(P.count[l], P.cap);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
     const uint32_t L = P.levels - 2;
-    const uint32_t n = P.count[L];
+    const uint32_t n = min(P.count[L], P.cap);
     if (*P.errors) return;
     const Items I = P.items[L];
     const uint32_t lane = threadIdx.x & 63u;
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
 // build interior items of level l from their children's results
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_build(EditPass P, uint32_t l) {
-    const uint32_t n = P.count[l];
+    const uint32_t n = min(P.count[l], P.cap);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
@@ -431,7 +431,7 @@ __device__ __forceinline__ bool same_content(const Items& I, uint32_t a, uint32_
 }
 
 __global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
-    const uint32_t n = P.count[l];
+    const uint32_t n = min(P.count[l], P.cap);
     if (*P.errors) return;
     const Items I = P.items[l];
     const uint32_t hmask = P.hcap - 1;
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
 // position inside the bucket's used size whose words equal the node and that does not
 // cross a page boundary holds the same node content).
 __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
-    const uint32_t n = P.count[l];
+    const uint32_t n = min(P.count[l], P.cap);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Geom g = P.g;
@@ -516,7 +516,7 @@ __device__ bool alloc_page(EditPass& P, uint32_t page) {
 }
 
 __global__ __launch_bounds__(256) void k_add(EditPass P, uint32_t l) {
-    const uint32_t n = P.count[l];
+    const uint32_t n = min(P.count[l], P.cap);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Geom g = P.g;
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(256) void k_add(EditPass P, uint32_t l) {
 }
 
 __global__ __launch_bounds__(256) void k_resolve(EditPass P, uint32_t l) {
-    const uint32_t n = P.count[l];
+    const uint32_t n = min(P.count[l], P.cap);
     if (*P.errors) return;
     const Items I = P.items[l];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(256) void k_resolve(EditPass P, uint32_t l) {
 // ---------------------------------------------------------------------------------------------
 // new / old colour starts of the child items (relative to the task's colour array)
 __global__ __launch_bounds__(256) void k_color_starts(EditPass P, uint32_t l) {
-    const uint32_t n = P.count[l];
+    const uint32_t n = min(P.count[l], P.cap);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(256) void k_color_starts(EditPass P, uint32_t l) {
 // colour array of every task with colours: base offset in the colour scratch
 __global__ void k_color_alloc(EditPass P) {
     const uint32_t ctl = P.ctl;
-    const uint32_t n = P.count[ctl];
+    const uint32_t n = min(P.count[ctl], P.cap);
     if (*P.errors) return;
     const Items T = P.items[ctl];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -626,7 +626,7 @@ __device__ __forceinline__ uint64_t old_color(const EditPass& P, uint32_t mode, 
 
 // keep / full slots of interior items at levels >= ctl: one wavefront per slot
 __global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
-    const uint32_t n = P.count[l];
+    const uint32_t n = min(P.count[l], P.cap);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items T = P.items[P.ctl];
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
 // leaf items: fold each voxel's value and colour through the edit list
 __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
     const uint32_t L = P.levels - 2;
-    const uint32_t n = P.count[L];
+    const uint32_t n = min(P.count[L], P.cap);
     if (*P.errors) return;
     const Items I = P.items[L];
     const Items T = P.items[P.ctl];
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
 __global__ void k_color_tree(EditPass P) {
     if (blockIdx.x != 0 || threadIdx.x != 0 || *P.errors) return;
     const uint32_t ctl = P.ctl, D = P.levels;
-    const uint32_t n = P.count[ctl];
+    const uint32_t n = min(P.count[ctl], P.cap);
     const Items T = P.items[ctl];
     for (uint32_t i = 0; i < n; ++i) {
         if (T.ncount[i] == 0) {
@@ -762,7 +762,7 @@ constexpr uint32_t kEncBlock = 256;
 
 __global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
     const uint32_t ctl = P.ctl;
-    const uint32_t n = P.count[ctl];
+    const uint32_t n = min(P.count[ctl], P.cap);
     if (*P.errors) return;
     const Items T = P.items[ctl];
     __shared__ uint32_t sScanA[kEncBlock], sScanB[kEncBlock];

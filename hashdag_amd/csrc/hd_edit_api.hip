@@ -200,6 +200,10 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
                         void* stream) {
     if (!ctx || (!edits && n) ) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
+    if (n == 0) {
+        if (stats) *stats = hd_edit_stats{0, 0, 0, ctx->root, 0};
+        return HD_OK;
+    }
     for (uint32_t i = 0; i < n; ++i)
         if (edits[i].kind > EDIT_SPHERE_PAINT || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
             return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");

@@ -9,7 +9,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import check, hd_config, hd_image_info, lib, ptr
+from ._lib import check, hd_config, hd_edit, hd_edit_stats, hd_image_info, lib, ptr
 from .scene import Scene
 
 
@@ -177,6 +177,38 @@ class HashDAG:
                                   ptr(np.ascontiguousarray(img.bucket_sizes, np.uint32)),
                                   ptr(np.ascontiguousarray(img.full_nodes, np.uint32)), img.pool_top, img.root),
               self._h)
+
+    # -- edits: HashDAG::edit_threads<Editor> (hash_dag.h:255-405) --
+    def edit_batch(self, edits, fused: bool = False, stream=None) -> hd_edit_stats:
+        """Applies [(kind, (x, y, z), radius), ...] in order; fused=True runs them in one pass."""
+        arr = (hd_edit * len(edits))()
+        for i, (kind, center, radius) in enumerate(edits):
+            arr[i].kind = int(kind)
+            arr[i].center[:] = [float(c) for c in center]
+            arr[i].radius = float(radius)
+        st = hd_edit_stats()
+        check(lib.hd_edit_batch(self._h, arr, len(edits), 1 if fused else 0, C.byref(st), stream), self._h)
+        return st
+
+    def edit(self, kind: int, center, radius: float) -> hd_edit_stats:
+        return self.edit_batch([(kind, center, radius)])
+
+    def undo(self) -> None:
+        check(lib.hd_undo(self._h), self._h)
+
+    def redo(self) -> None:
+        check(lib.hd_redo(self._h), self._h)
+
+    def clear_undo(self) -> None:
+        check(lib.hd_undo_clear(self._h), self._h)
+
+    def undo_depth(self):
+        u, r = C.c_uint32(), C.c_uint32()
+        check(lib.hd_undo_depth(self._h, C.byref(u), C.byref(r)), self._h)
+        return u.value, r.value
+
+    def reserve_edit_scratch(self, items_per_level: int, edit_list_words: int, colors: int) -> None:
+        check(lib.hd_reserve_edit_scratch(self._h, items_per_level, edit_list_words, colors), self._h)
 
     # -- colours --
     def color_nodes(self) -> np.ndarray:

@@ -1,0 +1,140 @@
+"""GPU edits vs the oracle: after every edit (plain form) and after a fused batch, the device
+DAG must be canonically identical to the oracle's (same unique node set, same root) and every
+colour subtree must decode to the same colour sequence; the edited-voxel counts must agree.
+"""
+import numpy as np
+import pytest
+
+from tests.scenes import POOL_PAGES, scene
+
+pytestmark = pytest.mark.gpu
+
+SPHERE_ADD, SPHERE_CARVE, BOX_ADD, BOX_CARVE, SPHERE_PAINT = range(5)
+
+
+def random_edits(p, n, seed, rmax=None):
+    rng = np.random.default_rng(seed)
+    F = p.footprint or (1 << p.levels)
+    rmax = rmax or max(4, F // 8)
+    from hashdag_amd import terrain_height
+    out = []
+    for _ in range(n):
+        x, z = (int(v) for v in rng.integers(0, F, size=2))
+        y = terrain_height(p, x, z)
+        kind = int(rng.choice([SPHERE_ADD, SPHERE_CARVE, BOX_ADD, BOX_CARVE, SPHERE_PAINT],
+                              p=[0.3, 0.3, 0.1, 0.1, 0.2]))
+        r = float(np.exp(rng.uniform(np.log(2.0), np.log(rmax))))
+        out.append((kind, (float(x), float(y), float(z)), float(round(r))))
+    return out
+
+
+def gpu_state_in_oracle(dag, sc):
+    """Downloads the device table and colour state into oracle objects (checker only)."""
+    from oracle import OracleColors, OracleDAG
+
+    img = dag.download()
+    od = OracleDAG.from_image(img, dag.pool_pages)
+    oc = OracleColors(dag.color_nodes(), sc.color_offsets, sc.blocks, sc.weights, sc.macro)
+    import ctypes as C
+
+    from hashdag_amd._lib import lib
+    n = C.c_uint64()
+    lib.hd_colors_info(dag.handle, None, C.byref(n), None)
+    from oracle.binding import _p, orc
+    for i in range(n.value):
+        b, w, m, off = dag.color_leaf(i | 0x80000000)
+        h = orc.orc_colors_add_unique(oc.h, _p(b, C.c_uint64), b.size, _p(w), w.size, _p(m, C.c_uint64), m.size)
+        assert h == (i | 0x80000000)
+        if off != (1 << 64) - 1:
+            raise AssertionError("device leaf slots hold unique leaves only")
+    return od, oc
+
+
+def make(name):
+    from hashdag_amd import HashDAG
+    from oracle import OracleColors, OracleDAG
+
+    p, sc, g = scene(name)
+    dag = HashDAG(g, pool_pages=POOL_PAGES)
+    dag.load_scene(sc)
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    return p, sc, g, dag, od, oc
+
+
+def assert_same(dag, sc, od, oc, what):
+    god, goc = gpu_state_in_oracle(dag, sc)
+    assert np.array_equal(god.canonical(), od.canonical()), f"{what}: DAG differs"
+    assert np.array_equal(goc.canonical(god), oc.canonical(od)), f"{what}: colours differ"
+
+
+@pytest.mark.parametrize("name", ["d8", "d13"])
+def test_plain_edits_match_oracle(name):
+    p, sc, g, dag, od, oc = make(name)
+    for k, (kind, center, r) in enumerate(random_edits(p, 10, seed=7)):
+        st = dag.edit(kind, center, r)
+        ost = od.edit(kind, center, r, oc)
+        assert st.num_voxels == ost[1], f"edit {k}: voxel count {st.num_voxels} != {ost[1]}"
+        assert_same(dag, sc, od, oc, f"edit {k} {(kind, center, r)}")
+
+
+@pytest.mark.parametrize("name", ["d8", "d13"])
+def test_fused_batch_matches_sequential_oracle(name):
+    p, sc, g, dag, od, oc = make(name)
+    edits = random_edits(p, 40, seed=11)
+    st = dag.edit_batch(edits, fused=True)
+    total = 0
+    for kind, center, r in edits:
+        total += od.edit(kind, center, r, oc)[1]
+    assert st.passes == 1
+    assert st.num_voxels == total
+    assert_same(dag, sc, od, oc, "fused batch")
+
+
+def test_undo_redo_restore_roots_and_colours():
+    p, sc, g, dag, od, oc = make("d13")
+    from oracle import OracleColors, OracleDAG
+    base_dag = od.canonical()
+    base_col = oc.canonical(od)
+    edits = random_edits(p, 5, seed=3)
+    for kind, center, r in edits:
+        dag.edit(kind, center, r)
+        od.edit(kind, center, r, oc)
+    final_dag, final_col = od.canonical(), oc.canonical(od)
+    for _ in edits:
+        dag.undo()
+    god, goc = gpu_state_in_oracle(dag, sc)
+    assert np.array_equal(god.canonical(), base_dag)
+    assert np.array_equal(goc.canonical(god), base_col)
+    for _ in edits:
+        dag.redo()
+    god, goc = gpu_state_in_oracle(dag, sc)
+    assert np.array_equal(god.canonical(), final_dag)
+    assert np.array_equal(goc.canonical(god), final_col)
+
+
+def test_trace_after_edits_bit_exact():
+    import torch
+    from hashdag_amd import DAGInfo, DAGTracer, ray_params
+    from hashdag_amd.workload import cameras
+    from oracle import Oracle
+
+    p, sc, g, dag, od, oc = make("d13")
+    for kind, center, r in random_edits(p, 8, seed=5):
+        dag.edit(kind, center, r)
+        od.edit(kind, center, r, oc)
+    W, H = 128, 72
+    info = DAGInfo.unit(g.levels)
+    tr = DAGTracer(W, H)
+    for cam in cameras(p, 2, seed=13):
+        prm = ray_params(cam, info, g.levels, W, H)
+        tr.resolve_paths(cam, info, dag, params=prm)
+        tr.resolve_colors(dag)
+        tr.resolve_shadows(cam, info, dag, 1.0, 0.0, params=prm)
+        torch.cuda.synchronize()
+        gpu = tr.colors.cpu().numpy().view(np.uint32)
+        op = Oracle.ray_params(cam, info, g.levels, W, H)
+        paths = Oracle.trace_paths(od, op, W, H)
+        cols = Oracle.trace_colors(od, oc, paths, W, H)
+        ref = Oracle.trace_shadows(od, op, paths, cols, W, H, 1.0, 0.0)
+        assert np.array_equal(gpu, ref), f"{int((gpu != ref).sum())} pixels differ"
