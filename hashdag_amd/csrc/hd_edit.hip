@@ -480,18 +480,32 @@ __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
                 if (b) found = p0 + 2 * (uint32_t)__ffsll((long long)b) - 2;
             }
         } else {
+            // The reference walks node by node from each page start and compares only nodes
+            // with start + n < min(bucketSize, pageEnd). The wave compares all positions at
+            // once, then confirms the first candidate lies on that walk.
             const uint32_t h0 = I.cont[i * 9];
-            for (uint32_t p0 = 0; p0 + nw <= bs && found == ~0u; p0 += 64) {
+            const uint32_t ps = g.page_size;
+            for (uint32_t p0 = 0; p0 + nw < bs && found == ~0u; p0 += 64) {
                 const uint32_t pos = p0 + lane;
-                bool hit = pos + nw <= bs && (pos % g.page_size) + nw <= g.page_size && P.vpool[base + pos] == h0;
+                const uint32_t pageEnd = min(bs, pos - pos % ps + ps);
+                bool hit = pos + nw < pageEnd && P.vpool[base + pos] == h0;
                 if (hit)
                     for (uint32_t k = 1; k < nw; ++k)
                         if (P.vpool[base + pos + k] != I.cont[i * 9 + k]) {
                             hit = false;
                             break;
                         }
-                const uint64_t b = __ballot(hit);
-                if (b) found = p0 + (uint32_t)__ffsll((long long)b) - 1;
+                uint64_t b = __ballot(hit);
+                while (b) {
+                    const uint32_t c = p0 + (uint32_t)__ffsll((long long)b) - 1;  // wave-uniform
+                    uint32_t w = c - c % ps;
+                    while (w < c) w += 1u + __popc(P.vpool[base + w] & 0xFFu);  // advances >= 1 word
+                    if (w == c) {
+                        found = c;
+                        break;
+                    }
+                    b &= b - 1;
+                }
             }
         }
         if (lane == 0) {

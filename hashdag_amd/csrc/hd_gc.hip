@@ -1,0 +1,299 @@
+// Device garbage collection: HashDAG::remove_stale_nodes(maxLevel) (hash_dag.h:407-547).
+//
+//   mark     level by level from the root: every child pointer of a live node sets its bit in
+//            a bitmap over the virtual address space; the first setter appends the child to
+//            the next level's frontier (the reference's per-level unordered_sets)
+//   rebuild  levels maxLevel .. 0: gather every live node with its children remapped through
+//            the old -> new map of the level below, reset the level's bucket sizes, append the
+//            full node first (ADD_FULL_NODES_FIRST), append every live node, record old -> new
+//
+// Pages stay mapped (poolTop never shrinks), as in the reference. The result is canonically
+// identical to the DAG before GC; node order within buckets is the append order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "hd_ctx.h"
+#include "hd_table_dev.h"
+
+namespace hd {
+namespace {
+
+constexpr uint32_t kGcGrid = 2048;
+constexpr uint32_t GC_ERR_CAP = 1, GC_ERR_BUCKET = 2, GC_ERR_POOL = 4, GC_ERR_MAP = 8;
+
+__device__ __forceinline__ uint32_t child_off(uint32_t mask, uint32_t c) { return __popc(mask & ((1u << c) - 1u)) + 1u; }
+
+__global__ void k_gc_fill(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_gc_seed(uint32_t* frontier0, uint32_t* counts, const uint32_t* d_root, uint32_t levels) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (uint32_t l = 0; l <= levels; ++l) counts[l] = 0;
+        frontier0[0] = *d_root;
+        counts[0] = 1;
+    }
+}
+
+// mark children of the live nodes of `level` (an interior level)
+__global__ __launch_bounds__(256) void k_gc_mark(const uint32_t* __restrict__ vpool, uint32_t* __restrict__ bitmap,
+                                                 const uint32_t* __restrict__ frontier, uint32_t* __restrict__ next,
+                                                 uint32_t* counts, uint32_t level, uint32_t capCur, uint32_t cap,
+                                                 uint64_t numVirtual, uint32_t* errors) {
+    const uint32_t n = min(counts[level], capCur);
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * 8; t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t va = frontier[t >> 3], c = (uint32_t)(t & 7u);
+        const uint32_t mask = vpool[va] & 0xFFu;
+        if (!(mask & (1u << c))) continue;
+        const uint32_t child = vpool[va + child_off(mask, c)];
+        if (child >= numVirtual) {  // not an address of this table
+            atomicOr(errors, GC_ERR_MAP);
+            continue;
+        }
+        const uint32_t bit = 1u << (child & 31u);
+        if (atomicOr(&bitmap[child >> 5], bit) & bit) continue;
+        const uint32_t slot = atomicAdd(&counts[level + 1], 1u);
+        if (slot >= cap) {
+            atomicOr(errors, GC_ERR_CAP);
+            continue;
+        }
+        next[slot] = child;
+    }
+}
+
+__device__ __forceinline__ uint32_t map_hash(uint32_t k) {
+    k ^= k >> 16;
+    k *= 0x7feb352du;
+    k ^= k >> 15;
+    k *= 0x846ca68bu;
+    return k ^ (k >> 16);
+}
+
+__device__ __forceinline__ uint32_t map_find(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                             uint32_t mask, uint32_t key) {
+    uint32_t s = map_hash(key) & mask;
+    for (uint32_t probes = 0; probes <= mask; ++probes) {
+        const uint32_t k = keys[s];
+        if (k == key + 1) return vals[s];
+        if (k == 0) return kInvalidNode;
+        s = (s + 1) & mask;
+    }
+    return kInvalidNode;
+}
+
+// gather live nodes of `level` with remapped children into tmp (9 words per node)
+__global__ __launch_bounds__(256) void k_gc_gather(const uint32_t* __restrict__ vpool,
+                                                   const uint32_t* __restrict__ frontier, const uint32_t* counts,
+                                                   uint32_t level, uint32_t leafLevel, uint32_t maxLevel, uint32_t cap,
+                                                   const uint32_t* __restrict__ mkeys, const uint32_t* __restrict__ mvals,
+                                                   uint32_t mmask, uint32_t* __restrict__ tmp, uint32_t* __restrict__ tlen,
+                                                   uint32_t* errors) {
+    const uint32_t n = min(counts[level], cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t va = frontier[i];
+        uint32_t* out = tmp + (uint64_t)i * 9;
+        if (level == leafLevel) {
+            out[0] = vpool[va];
+            out[1] = vpool[va + 1];
+            tlen[i] = 2;
+            continue;
+        }
+        const uint32_t hdr = vpool[va];
+        const uint32_t nk = __popc(hdr & 0xFFu);
+        out[0] = hdr;
+        for (uint32_t k = 1; k <= nk; ++k) {
+            uint32_t c = vpool[va + k];
+            if (level != maxLevel) {
+                c = map_find(mkeys, mvals, mmask, c);
+                if (c == kInvalidNode) atomicOr(errors, GC_ERR_MAP);
+            }
+            out[k] = c;
+        }
+        tlen[i] = nk + 1;
+    }
+}
+
+__global__ void k_gc_full(TableRef T, uint32_t level, uint32_t* full_nodes, uint32_t* errors) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const Geom& g = T.g;
+    uint32_t node[9];
+    uint32_t n, h;
+    if (level == g.leaf_level()) {
+        node[0] = node[1] = 0xFFFFFFFFu;
+        n = 2;
+        h = hash_leaf(~0ull);
+    } else {
+        const uint32_t count = level < g.color_tree_levels ? 0u : (1u << (3 * (g.levels - level)));
+        node[0] = (count << 8) | 0xFFu;
+        for (int c = 0; c < 8; ++c) node[1 + c] = full_nodes[level + 1];
+        n = 9;
+        h = hash_words(node, 9);
+    }
+    const uint32_t va = table_append(T, level, g.bucket_from_hash(level, h), node, n);
+    if (va == kAppendBucketFull) atomicOr(errors, GC_ERR_BUCKET);
+    else if (va == kAppendPoolFull) atomicOr(errors, GC_ERR_POOL);
+    else full_nodes[level] = va;
+}
+
+__global__ __launch_bounds__(256) void k_gc_add(TableRef T, uint32_t level, const uint32_t* counts, uint32_t cap,
+                                                const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ tmp,
+                                                const uint32_t* __restrict__ tlen, uint32_t* __restrict__ mkeys,
+                                                uint32_t* __restrict__ mvals, uint32_t mmask, uint32_t* errors) {
+    const uint32_t n = min(counts[level], cap);
+    const bool leaf = level == T.g.leaf_level();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t* node = tmp + (uint64_t)i * 9;
+        const uint32_t len = tlen[i];
+        const uint32_t h = leaf ? hash_leaf((uint64_t)node[0] | ((uint64_t)node[1] << 32)) : hash_words(node, len);
+        const uint32_t va = table_append(T, level, T.g.bucket_from_hash(level, h), node, len);
+        if (va == kAppendBucketFull) {
+            atomicOr(errors, GC_ERR_BUCKET);
+            continue;
+        }
+        if (va == kAppendPoolFull) {
+            atomicOr(errors, GC_ERR_POOL);
+            continue;
+        }
+        // old address -> new address
+        const uint32_t key = frontier[i];
+        uint32_t s = map_hash(key) & mmask;
+        for (;;) {
+            const uint32_t prev = atomicCAS(&mkeys[s], 0u, key + 1);
+            if (prev == 0) {
+                mvals[s] = va;
+                break;
+            }
+            s = (s + 1) & mmask;
+        }
+    }
+}
+
+__global__ void k_gc_root(const uint32_t* mkeys, const uint32_t* mvals, uint32_t mmask, uint32_t* d_root,
+                          uint32_t* errors) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *errors == 0) {
+        const uint32_t r = map_find(mkeys, mvals, mmask, *d_root);
+        if (r == kInvalidNode) atomicOr(errors, GC_ERR_MAP);
+        else *d_root = r;
+    }
+}
+
+}  // namespace
+}  // namespace hd
+
+using namespace hd;
+
+extern "C" hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], void* stream) {
+    if (!ctx || !ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
+    const Geom& g = ctx->g;
+    const uint32_t L = g.leaf_level();
+    if (max_level > L) return fail(ctx, HD_ERR_INVALID, "max_level above the leaf level");
+    if (ctx->root >= g.total_virtual()) return fail(ctx, HD_ERR_INVALID, "root is not an address of the table");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // capacities: used words per level bound the number of nodes
+    std::vector<uint32_t> bs(g.total_buckets());
+    HD_HIP(ctx, hipMemcpyAsync(bs.data(), ctx->bucket_sizes, bs.size() * 4, hipMemcpyDeviceToHost, s));
+    HD_HIP(ctx, hipStreamSynchronize(s));
+    std::vector<uint64_t> used(g.levels, 0);
+    for (uint32_t l = 0; l < g.levels; ++l)
+        for (uint32_t b = 0; b < g.buckets_per_level(l); ++b) used[l] += bs[g.bucket_global(l, b)];
+    uint64_t capMax = 16;
+    // every node takes at least 2 words, so a level holds at most used / 2 nodes
+    for (uint32_t l = 0; l <= max_level; ++l) capMax = std::max<uint64_t>(capMax, used[l] / 2 + 16);
+    if (capMax >= (1u << 30)) return fail(ctx, HD_ERR_INVALID, "level too large for GC scratch");
+    const uint32_t cap = (uint32_t)capMax;
+    uint32_t mcap = 1;
+    while (mcap < 2 * cap) mcap <<= 1;
+
+    std::vector<void*> allocs;
+    auto alloc = [&](void** p, uint64_t bytes) -> hipError_t {
+        const hipError_t e = hipMalloc(p, std::max<uint64_t>(bytes, 16));
+        if (e == hipSuccess) allocs.push_back(*p);
+        return e;
+    };
+    auto release = [&]() {
+        hipStreamSynchronize(s);
+        for (void* p : allocs) hipFree(p);
+    };
+    const uint64_t bitmapWords = (g.total_virtual() + 31) / 32;
+    uint32_t *bitmap, *counts, *errors, *tmp, *tlen, *mk[2], *mv[2];
+    std::vector<uint32_t*> frontier(max_level + 1, nullptr);
+    hipError_t e = alloc((void**)&bitmap, bitmapWords * 4);
+    if (e == hipSuccess) e = alloc((void**)&counts, 64 * 4);
+    if (e == hipSuccess) e = alloc((void**)&errors, 4);
+    for (uint32_t l = 0; l <= max_level && e == hipSuccess; ++l)
+        e = alloc((void**)&frontier[l], (uint64_t)std::max<uint64_t>(used[l] / 2 + 16, 16) * 4);
+    if (e == hipSuccess) e = alloc((void**)&tmp, (uint64_t)cap * 36);
+    if (e == hipSuccess) e = alloc((void**)&tlen, (uint64_t)cap * 4);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = alloc((void**)&mk[k], (uint64_t)mcap * 4);
+        if (e == hipSuccess) e = alloc((void**)&mv[k], (uint64_t)mcap * 4);
+    }
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(ctx, e, "GC scratch");
+    }
+    k_gc_fill<<<1024, 256, 0, s>>>(bitmap, bitmapWords, 0u);
+    k_gc_fill<<<1, 64, 0, s>>>(errors, 1, 0u);
+    HD_HIP(ctx, hipMemcpyAsync(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice, s));
+    k_gc_seed<<<1, 1, 0, s>>>(frontier[0], counts, ctx->d_root, g.levels);
+    auto fcap = [&](uint32_t l) { return (uint32_t)std::max<uint64_t>(used[l] / 2 + 16, 16); };
+    for (uint32_t l = 0; l < max_level; ++l)
+        k_gc_mark<<<kGcGrid, 256, 0, s>>>(ctx->vpool, bitmap, frontier[l], frontier[l + 1], counts, l, fcap(l),
+                                          fcap(l + 1), g.total_virtual(), errors);
+    {  // nothing has been modified yet: stop here if the mark overflowed or met a bad pointer
+        uint32_t markErr = 0;
+        HD_HIP(ctx, hipMemcpyAsync(&markErr, errors, 4, hipMemcpyDeviceToHost, s));
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess || markErr) {
+            release();
+            if (e != hipSuccess) return hip_fail(ctx, e, "GC mark");
+            return fail(ctx, HD_ERR_INVALID, "GC mark: a child pointer is outside the table or a level overflowed");
+        }
+    }
+    TableRef T{g, ctx->vpool, ctx->page_table, ctx->bucket_sizes, ctx->counters + 0, ctx->cfg.pool_pages};
+    int cur = 0;
+    for (uint32_t l = max_level; l != (uint32_t)-1; --l) {
+        const int prev = cur ^ 1;  // map of level l + 1
+        k_gc_gather<<<kGcGrid, 256, 0, s>>>(ctx->vpool, frontier[l], counts, l, L, max_level, cap, mk[prev], mv[prev],
+                                            mcap - 1, tmp, tlen, errors);
+        // reset the level's buckets, then the full node first, then the live nodes
+        const uint32_t b0 = g.bucket_global(l, 0);
+        k_gc_fill<<<256, 256, 0, s>>>(ctx->bucket_sizes + b0, g.buckets_per_level(l), 0u);
+        k_gc_full<<<1, 1, 0, s>>>(T, l, ctx->full_nodes, errors);
+        k_gc_fill<<<256, 256, 0, s>>>(mk[cur], mcap, 0u);
+        k_gc_add<<<kGcGrid, 256, 0, s>>>(T, l, counts, cap, frontier[l], tmp, tlen, mk[cur], mv[cur], mcap - 1, errors);
+        cur ^= 1;
+    }
+    k_gc_root<<<1, 1, 0, s>>>(mk[cur ^ 1], mv[cur ^ 1], mcap - 1, ctx->d_root, errors);
+    uint32_t err = 0, root = 0;
+    std::vector<uint32_t> cnt(max_level + 1);
+    HD_HIP(ctx, hipMemcpyAsync(&err, errors, 4, hipMemcpyDeviceToHost, s));
+    HD_HIP(ctx, hipMemcpyAsync(&root, ctx->d_root, 4, hipMemcpyDeviceToHost, s));
+    HD_HIP(ctx, hipMemcpyAsync(cnt.data(), counts, cnt.size() * 4, hipMemcpyDeviceToHost, s));
+    e = hipStreamSynchronize(s);
+    release();
+    if (e != hipSuccess) return hip_fail(ctx, e, "GC");
+    if (err) return fail(ctx, err & GC_ERR_POOL ? HD_ERR_POOL_FULL : HD_ERR_BUCKET_FULL,
+                         "GC failed (capacity); the table is no longer consistent");
+    ctx->root = root;
+    ctx->undo.clear();  // old roots point at collected nodes (engine.cpp:178-185 frees undo)
+    ctx->redo.clear();
+    if (stats) {
+        uint64_t live = 0;
+        for (uint32_t v : cnt) live += v;
+        uint64_t before = 0;
+        for (uint32_t l = 0; l <= max_level; ++l) before += used[l];
+        stats[0] = live;
+        stats[1] = before;
+        std::vector<uint32_t> bs2(g.total_buckets());
+        HD_HIP(ctx, hipMemcpy(bs2.data(), ctx->bucket_sizes, bs2.size() * 4, hipMemcpyDeviceToHost));
+        uint64_t after = 0;
+        for (uint32_t l = 0; l <= max_level; ++l)
+            for (uint32_t b = 0; b < g.buckets_per_level(l); ++b) after += bs2[g.bucket_global(l, b)];
+        stats[2] = after;
+        stats[3] = 0;
+    }
+    return HD_OK;
+}

@@ -193,6 +193,13 @@ class HashDAG:
     def edit(self, kind: int, center, radius: float) -> hd_edit_stats:
         return self.edit_batch([(kind, center, radius)])
 
+    def remove_stale_nodes(self, max_level: int | None = None, stream=None) -> dict:
+        """GC (hash_dag.h:407-547); the engine calls it with levels - 2 (engine.cpp:182)."""
+        st = np.zeros(4, np.uint64)
+        lvl = self.geometry.leaf_level if max_level is None else max_level
+        check(lib.hd_gc(self._h, lvl, ptr(st, C.c_uint64), stream), self._h)
+        return {"live_nodes": int(st[0]), "words_before": int(st[1]), "words_after": int(st[2])}
+
     def undo(self) -> None:
         check(lib.hd_undo(self._h), self._h)
 

@@ -133,7 +133,9 @@ hd_status hd_trace_shadows(hd_ctx* ctx, const hd_ray_params* params, uint32_t wi
 hd_status hd_set_trace_tile(hd_ctx* ctx, uint32_t tile_w, uint32_t tile_h);
 
 /* Algorithmic HBM bytes of the last traced frame per kernel (paths, colors, shadows), filled
- * by an instrumented launch: hd_trace_count_bytes runs the three kernels once with counters. */
+ * by an instrumented launch: hd_trace_count_bytes runs the three kernels once with counters.
+ * No counterpart in the reference (a measurement hook for the roofline of the three tracer
+ * kernels, tracer.cu:165-423). */
 hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_t width, uint32_t height,
                                uint32_t* d_paths, uint32_t* d_colors, float shadow_bias, float fog_density,
                                uint64_t bytes[3]);
@@ -173,6 +175,12 @@ hd_status hd_undo_depth(hd_ctx* ctx, uint32_t* undo, uint32_t* redo);
 /* Pre-sizes the edit work-item scratch (items per level, edit-list words per level,
  * colours per pass); grown automatically when a pass overflows. Not in the reference. */
 hd_status hd_reserve_edit_scratch(hd_ctx* ctx, uint32_t items_per_level, uint32_t edit_list_words, uint64_t colors);
+
+/* HashDAG::remove_stale_nodes(maxLevel) (hash_dag.h:407-547), called with levels - 2 by the
+ * engine (engine.cpp:178-185): keeps only nodes reachable from the root at levels 0..max_level,
+ * re-appended level by level bottom-up with their children remapped, full nodes first.
+ * Clears the undo history. stats (may be NULL): live nodes, words before, words after, 0. */
+hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], void* stream);
 
 /* Streaming device copy of `bytes` (multiple of 16) from d_src to d_dst: measures the
  * achievable HBM bandwidth next to the tracer (not in the reference). */

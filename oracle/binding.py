@@ -41,6 +41,7 @@ _SIGS = {
     "orc_colors_new": (C.c_void_p, [u32p, C.c_uint64, u64p, C.c_uint64, u64p, C.c_uint64, u32p, C.c_uint64, u64p,
                                     C.c_uint64]),
     "orc_colors_add_unique": (C.c_uint32, [C.c_void_p, u64p, C.c_uint64, u32p, C.c_uint64, u64p, C.c_uint64]),
+    "orc_colors_add_shared": (C.c_uint32, [C.c_void_p, C.c_uint64]),
     "orc_colors_free": (None, [C.c_void_p]),
     "orc_colors_num_nodes": (C.c_uint64, [C.c_void_p]),
     "orc_colors_num_leaves": (C.c_uint64, [C.c_void_p]),
@@ -61,6 +62,7 @@ _SIGS = {
                                     C.c_float, C.c_float, u64p, C.c_int]),
     "orc_edit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, fltp, C.c_float, u64p]),
     "orc_gc": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "orc_dag_reachable": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
 }
 
 
@@ -171,6 +173,12 @@ class OracleDAG:
     def gc(self, max_level: int) -> None:
         if orc.orc_gc(self.h, max_level) != 0:
             raise RuntimeError(_err())
+
+    def reachable(self, max_level: int) -> np.ndarray:
+        out = np.zeros(max_level + 1, np.uint64)
+        if orc.orc_dag_reachable(self.h, max_level, out.ctypes.data) != 0:
+            raise RuntimeError(_err())
+        return out
 
 
 class OracleColors:

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <exception>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -179,6 +180,15 @@ uint32_t orc_colors_add_unique(void* cp, const uint64_t* blocks, uint64_t n_bloc
     l.weights.assign(weights, weights + n_weights);
     l.macro.assign(macro, macro + n_macro);
     l.valid = n_blocks > 0;
+    return h;
+}
+
+uint32_t orc_colors_add_shared(void* cp, uint64_t offset) {
+    auto* c = static_cast<HashDAGColors*>(cp);
+    const uint32_t h = c->allocate_leaf();
+    ColorLeaf& l = c->leaves[h & 0x7FFFFFFFu];
+    l = c->mainLeaf;
+    l.offset = offset;
     return h;
 }
 
@@ -373,6 +383,23 @@ int orc_edit(void* d, void* c, uint32_t kind, const float center[3], float radiu
 int orc_gc(void* d, uint32_t max_level) {
     GUARD_BEGIN
     remove_stale_nodes(*static_cast<DAG*>(d), max_level);
+    return 0;
+    GUARD_END(-1)
+}
+
+// distinct node addresses reachable from the root per level 0..max_level (the valid-node
+// sets of hash_dag.h:423-437)
+int orc_dag_reachable(void* d, uint32_t max_level, uint64_t* out) {
+    GUARD_BEGIN
+    DAG& dag = *static_cast<DAG*>(d);
+    std::vector<std::set<uint32_t>> valid(max_level + 1);
+    valid[0].insert(dag.root);
+    for (uint32_t level = 0; level < max_level; ++level)
+        for (uint32_t idx : valid[level]) {
+            const uint32_t* p = dag.t.ptr(idx);
+            for (uint32_t o = 1; o < total_size(p[0]); ++o) valid[level + 1].insert(p[o]);
+        }
+    for (uint32_t l = 0; l <= max_level; ++l) out[l] = valid[l].size();
     return 0;
     GUARD_END(-1)
 }

@@ -62,6 +62,9 @@ void* orc_colors_new(const uint32_t* nodes, uint64_t n_nodes, const uint64_t* of
 /* appends a unique leaf (weights in stored byte-swapped order); returns its handle (bit 31 set) */
 uint32_t orc_colors_add_unique(void* colors, const uint64_t* blocks, uint64_t n_blocks, const uint32_t* weights,
                                uint64_t n_weights, const uint64_t* macro, uint64_t n_macro);
+/* appends a leaf slot holding the shared main leaf at `offset` (what undo restores into a
+ * slot that was shared before the edit, hash_dag.h:81-88); returns its handle */
+uint32_t orc_colors_add_shared(void* colors, uint64_t offset);
 void orc_colors_free(void* colors);
 uint64_t orc_colors_num_nodes(void* colors);
 uint64_t orc_colors_num_leaves(void* colors);
@@ -98,6 +101,8 @@ int orc_trace_shadows(void* dag, const double params[12], uint32_t W, uint32_t H
  * stats out: num nodes, num voxels, colour leaves rebuilt. colors may be NULL. */
 int orc_edit(void* dag, void* colors, uint32_t kind, const float center[3], float radius, uint64_t stats[3]);
 int orc_gc(void* dag, uint32_t max_level);
+/* distinct node addresses reachable from the root, per level 0..max_level */
+int orc_dag_reachable(void* dag, uint32_t max_level, uint64_t* out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,105 @@
+"""Device GC (HashDAG::remove_stale_nodes, hash_dag.h:407-547) vs the oracle: after edits
+leave stale nodes behind, GC on both sides must keep the DAG and colours canonically
+identical, keep exactly the node addresses reachable from the root (counted by the oracle
+on the pre-GC device image), reclaim words, and leave the traced frame bit-exact.
+
+Word counts per bucket are not compared: they include page padding, which depends on the
+order nodes are appended to a bucket. That order is not fixed by the reference either
+(edit_threads appends from several threads under per-bucket locks; remove_stale_nodes
+iterates unordered_sets).
+"""
+import numpy as np
+import pytest
+
+from tests.test_edit_gpu import assert_same, make, random_edits
+
+pytestmark = pytest.mark.gpu
+
+
+def reachable_on_device(dag, max_level):
+    from oracle import OracleDAG
+    return int(OracleDAG.from_image(dag.download(), dag.pool_pages).reachable(max_level).sum())
+
+
+def level_words(g, bucket_sizes):
+    out = []
+    b0 = 0
+    for lvl in range(g.levels):
+        nb = 1 << (g.bucket_bits_top if lvl < g.top() else g.bucket_bits_low)
+        out.append(int(bucket_sizes[b0:b0 + nb].astype(np.uint64).sum()))
+        b0 += nb
+    return out
+
+
+def device_words(dag):
+    return level_words(dag.geometry, dag.download().bucket_sizes)
+
+
+@pytest.mark.parametrize("name", ["d8", "d13"])
+def test_gc_after_edits_matches_oracle(name):
+    p, sc, g, dag, od, oc = make(name)
+    for kind, center, r in random_edits(p, 12, seed=21):
+        dag.edit(kind, center, r)
+        od.edit(kind, center, r, oc)
+    before = device_words(dag)
+    live = reachable_on_device(dag, g.leaf_level)
+    st = dag.remove_stale_nodes()
+    od.gc(g.leaf_level)
+    assert st["live_nodes"] == live
+    assert st["words_after"] < st["words_before"]
+    after = device_words(dag)
+    assert sum(after) < sum(before)
+    assert reachable_on_device(dag, g.leaf_level) == live
+    assert dag.undo_depth() == (0, 0)
+    assert_same(dag, sc, od, oc, "after GC")
+    # the table stays editable after GC
+    for kind, center, r in random_edits(p, 4, seed=22):
+        dag.edit(kind, center, r)
+        od.edit(kind, center, r, oc)
+    assert_same(dag, sc, od, oc, "edits after GC")
+
+
+def test_gc_partial_max_level_matches_oracle():
+    p, sc, g, dag, od, oc = make("d13")
+    for kind, center, r in random_edits(p, 6, seed=31):
+        dag.edit(kind, center, r)
+        od.edit(kind, center, r, oc)
+    lvl = g.leaf_level - 3
+    live = reachable_on_device(dag, lvl)
+    before = device_words(dag)
+    st = dag.remove_stale_nodes(lvl)
+    od.gc(lvl)
+    assert st["live_nodes"] == live
+    after = device_words(dag)
+    assert after[lvl + 1:] == before[lvl + 1:]  # levels below max_level untouched
+    assert sum(after[:lvl + 1]) < sum(before[:lvl + 1])
+    assert_same(dag, sc, od, oc, "after partial GC")
+
+
+def test_gc_keeps_trace_bit_exact():
+    import torch
+
+    from hashdag_amd import DAGInfo, DAGTracer, ray_params
+    from hashdag_amd.workload import cameras
+    p, sc, g, dag, od, oc = make("d13")
+    for kind, center, r in random_edits(p, 8, seed=41):
+        dag.edit(kind, center, r)
+    W, H = 128, 72
+    info = DAGInfo.unit(g.levels)
+    tr = DAGTracer(W, H)
+    frames = []
+    cams = cameras(p, 2, seed=17)
+    for stage in range(2):
+        if stage == 1:
+            dag.remove_stale_nodes()
+        out = []
+        for cam in cams:
+            prm = ray_params(cam, info, g.levels, W, H)
+            tr.resolve_paths(cam, info, dag, params=prm)
+            tr.resolve_colors(dag)
+            tr.resolve_shadows(cam, info, dag, 1.0, 0.0, params=prm)
+            torch.cuda.synchronize()
+            out.append(tr.colors.cpu().numpy().copy())
+        frames.append(out)
+    for a, b in zip(*frames):
+        assert np.array_equal(a, b)
