@@ -6,12 +6,14 @@
  * Parity pinning of the oracle (which independent check fixes each part):
  *   hashes            MurmurHash3_x86_32 / x64_128 published vectors (fmix32, fmix64, the
  *                     word-count finaliser relation)            tests/test_oracle_hashes.py
- *   leaf masks, Path  brute force over all bytes / random masks  tests/test_oracle_basics.py
+ *   leaf masks, Path  decoded grid and get_value == the grid the terrain heights define
+ *                                                                tests/test_oracle_table.py
  *   colour codec      decode(encode(seq)) == seq, bit-stream layout vs a direct big-endian
  *                     bit-stream model                          tests/test_oracle_colors.py
- *   hash table        invariants: no page straddle, find after add, bucket/page accounting,
- *                     the interior-find page-end quirk          tests/test_oracle_table.py
- *   load_from_DAG     decoded voxel set == dense generator grid tests/test_oracle_scene.py
+ *   hash table        every node in the bucket its (published-vector) hash selects, no page
+ *                     straddle, children in the child level's range, full nodes
+ *                                                                tests/test_oracle_table.py
+ *   load_from_DAG     decoded voxel set == dense generator grid tests/test_oracle_table.py
  *   trace_paths       brute-force DDA nearest hit agrees except where the reference's DFS
  *                     order/epsilon differ (counted), mask properties, root rejection
  *                                                                tests/test_oracle_tracer.py
@@ -20,7 +22,10 @@
  *   trace_shadows     any-hit result == brute-force segment/voxel test on a dense grid
  *   edits             dense-grid brute force of sphere/box/paint voxel sets and colours
  *                                                                tests/test_oracle_edits.py
- *   GC                canonical form unchanged, no unreachable nodes tests/test_oracle_gc.py
+ *   GC                canonical form unchanged, idempotent, used words == reachable words +
+ *                     full node + padding        tests/test_oracle_table.py, test_oracle_edits.py
+ *   interior find     the reference's walk rule (start + n < min(bucket size, page end)) is
+ *                     pinned only by reading hash_table.h:276-357: parity unpinned
  */
 #pragma once
 #include <stdint.h>
