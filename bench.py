@@ -42,6 +42,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=str, default="8x8")
+    ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
+    ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
+    ap.add_argument("--no-edits", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -141,7 +144,7 @@ def main() -> int:
 
     value = W * H * args.steps / elapsed / 1e6
     result = {
-        "metric": "primary Mrays/s @1080p",
+        "metric": "primary Mrays/s @1080p; edited voxels/s batched vs plain; p99 request latency",
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -177,11 +180,95 @@ def main() -> int:
         },
     }
 
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(scene, geom, rparams, W, H, args.cpu_seconds)
-    print(json.dumps(result), flush=True)
+    if not args.no_edits:
+        result["edits"] = edit_bench(scene, geom, params, dag, dinfo, rparams[:2], W, H, args)
     dag.close()
+    if not args.no_cpu_baseline:
+        od, oc, result["cpu_baseline"] = cpu_baseline(scene, geom, rparams, W, H, args.cpu_seconds)
+        if "edits" in result:
+            result["edits"]["cpu_baseline"] = cpu_edit_baseline(od, oc, params, args)
+    print(json.dumps(result), flush=True)
     return 0
+
+
+def edit_bench(scene, geom, params, dag, dinfo, rparams, W, H, args):
+    """Edited voxels/s of the same seeded mixed edit list applied one pass per edit (plain)
+    and as one fused pass (batched), each on a freshly loaded copy of the scene with the edit
+    scratch already sized by an untimed run (undone and garbage-collected before timing).
+    The two results must trace to identical frames."""
+    import numpy as np
+    import torch
+
+    from hashdag_amd import DAGTracer, HashDAG
+    from hashdag_amd.workload import mixed_edits
+
+    edits = mixed_edits(params, args.edits, seed=args.seed + 1)
+    log(f"edits: {len(edits)} mixed edits (70% sphere, 20% box, 10% paint; radius log-uniform 2-256)")
+
+    def frames(d):
+        tr = DAGTracer(W, H)
+        out = []
+        for p in rparams:
+            tr.resolve_paths(None, dinfo, d, params=p)
+            tr.resolve_colors(d)
+            tr.resolve_shadows(None, dinfo, d, 1.0, 0.0, params=p)
+            torch.cuda.synchronize()
+            out.append(tr.colors.cpu().numpy().copy())
+        return out
+
+    def timed(d, fused, warm_edits):
+        d.edit_batch(warm_edits, fused=fused)
+        for _ in range(d.undo_depth()[0]):
+            d.undo()
+        d.remove_stale_nodes()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = d.edit_batch(edits, fused=fused)
+        torch.cuda.synchronize()
+        return st, time.perf_counter() - t0
+
+    st_f, t_f = timed(dag, True, edits)
+    log(f"fused: {st_f.num_voxels} voxels in {t_f * 1e3:.1f} ms ({st_f.passes} pass)")
+    img_f = frames(dag)
+    plain = HashDAG(geom)
+    plain.load_scene(scene)
+    st_p, t_p = timed(plain, False, edits[:16])
+    log(f"plain: {st_p.num_voxels} voxels in {t_p * 1e3:.1f} ms ({st_p.passes} passes)")
+    img_p = frames(plain)
+    plain.close()
+    same = st_p.num_voxels == st_f.num_voxels and all(np.array_equal(a, b) for a, b in zip(img_f, img_p))
+    if not same:
+        raise RuntimeError("fused and plain edit results differ")
+    return {
+        "workload": f"{len(edits)} mixed edits on the 2^{params.levels} coloured terrain (70% sphere add/carve, "
+                    f"20% box add/carve, 10% sphere paint; radius log-uniform 2-256; seed {args.seed + 1})",
+        "edited_voxels": int(st_f.num_voxels),
+        "plain_voxels_per_s": round(st_p.num_voxels / t_p, 1),
+        "batched_voxels_per_s": round(st_f.num_voxels / t_f, 1),
+        "batched_over_plain": round(t_p / t_f, 2),
+        "plain_ms": round(t_p * 1e3, 2),
+        "batched_ms": round(t_f * 1e3, 2),
+        "plain_ms_per_edit": round(t_p * 1e3 / len(edits), 3),
+        "results_identical": same,
+    }
+
+
+def cpu_edit_baseline(od, oc, params, args):
+    """The oracle's single-threaded edit path (a C++ restatement of edit_threads with colour
+    rebuild) on the first edits of the same list, until the time budget is used."""
+    from hashdag_amd.workload import mixed_edits
+
+    edits = mixed_edits(params, args.edits, seed=args.seed + 1)
+    voxels, spent, n = 0, 0.0, 0
+    for kind, center, r in edits:
+        if spent >= args.edit_cpu_seconds:
+            break
+        t = time.perf_counter()
+        voxels += od.edit(kind, center, r, oc)[1]
+        spent += time.perf_counter() - t
+        n += 1
+    return {"value": round(voxels / spent, 1), "unit": "edited voxels/s", "cores": 1, "kind": "port",
+            "sample": f"first {n} edits of the list ({voxels} voxels) in {spent:.1f}s"}
 
 
 def measured_traffic(kernel: str, levels: int, W: int, H: int):
@@ -228,7 +315,7 @@ def cpu_baseline(scene, geom, rparams, W, H, budget_s):
             spent += time.perf_counter() - t
             rays += W * (y1 - y0)
         frames += 1
-    return {
+    return od, oc, {
         "value": round(rays / spent / 1e6, 3),
         "unit": "Mrays/s",
         "cores": threads,

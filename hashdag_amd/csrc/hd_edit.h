@@ -17,7 +17,8 @@ constexpr uint32_t EDIT_SPHERE_ADD = 0, EDIT_SPHERE_CARVE = 1, EDIT_BOX_ADD = 2,
 
 // error bits reported by a pass
 constexpr uint32_t ERR_SCRATCH = 1, ERR_BUCKET = 2, ERR_POOL = 4, ERR_HEADER = 8, ERR_COLOR_SCRATCH = 16,
-                   ERR_COLOR_ARENA = 32, ERR_COLOR_TREE = 64, ERR_UNDO = 128, ERR_EMPTY = 256;
+                   ERR_COLOR_ARENA = 32, ERR_COLOR_TREE = 64, ERR_UNDO = 128, ERR_EMPTY = 256,
+                   ERR_LEAF_SIZE = 512;
 
 constexpr uint32_t kMaxEditLevels = 17;
 constexpr uint32_t kUndoCap = 1u << 20;
@@ -71,7 +72,7 @@ struct EditPass {
     uint32_t* ecount;  // edit-list words per level
     Items items[kMaxEditLevels];
     uint32_t* elist[kMaxEditLevels];
-    uint32_t cap, ecap;
+    uint32_t cap[kMaxEditLevels], ecap[kMaxEditLevels];  // item / edit-list capacity per level
     uint32_t* hset;
     uint32_t hcap;
     const DevEdit* edits;
@@ -95,11 +96,15 @@ struct EditPass {
     uint32_t undo_cap;
 };
 
+// Per-pass scratch. Item arrays and edit lists are sized per level (one allocation each) from
+// what passes ask for; the uploaded edits and the undo log survive re-reserves.
 struct EditScratch {
-    uint32_t levels = 0, item_cap = 0, elist_cap = 0, hcap = 0, edits_cap = 0;
+    uint32_t levels = 0, hcap = 0, edits_cap = 0, undo_cap = 0;
+    uint32_t item_cap[kMaxEditLevels]{}, elist_cap[kMaxEditLevels]{};
     uint64_t color_cap = 0;
     Items items[kMaxEditLevels]{};
     uint32_t* elist[kMaxEditLevels]{};
+    void* level_mem[kMaxEditLevels]{};
     uint32_t* hset = nullptr;
     uint64_t* colors = nullptr;
     uint32_t* dcount = nullptr;
@@ -107,6 +112,7 @@ struct EditScratch {
     uint64_t* dstats = nullptr;
     uint32_t* undo_handles = nullptr;
     LeafDesc* undo_descs = nullptr;
+    LeafDesc* undo_next = nullptr;
     DevEdit* edits_dev = nullptr;
     std::vector<void*> allocs;
 };

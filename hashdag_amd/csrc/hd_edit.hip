@@ -41,6 +41,53 @@ namespace {
 constexpr uint32_t KIND_KEEP = 0, KIND_EMPTY = 1, KIND_FULL = 2, KIND_ITEM = 3;
 constexpr uint32_t kGrid = 2048;  // blocks of grid-stride kernels
 
+// Wave-strided loops (one work item per wavefront): the index is made wave-uniform so the
+// compiler keeps it in a scalar register and loads the item's fields through the scalar cache.
+__device__ __forceinline__ uint32_t wave_index() {
+    return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+}
+__device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) >> 6; }
+
+// Wave-aggregated counters (every lane of the wavefront must call these): one atomic per
+// wavefront instead of one per lane, so millions of items do not queue on one address.
+__device__ __forceinline__ uint32_t wave_scan32(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_scan64(uint64_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t lo = __shfl_up((uint32_t)v, off, 64), hi = __shfl_up((uint32_t)(v >> 32), off, 64);
+        if (lane >= off) v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+// exclusive offset of this lane's v in a block reserved from *ctr
+__device__ __forceinline__ uint32_t wave_alloc32(uint32_t* ctr, uint32_t v) {
+    const uint32_t incl = wave_scan32(v);
+    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t base = 0;
+    if (total != 0 && (threadIdx.x & 63u) == 63u) base = atomicAdd(ctr, total);
+    base = __shfl(base, 63, 64);
+    return base + incl - v;
+}
+__device__ __forceinline__ uint64_t wave_alloc64(unsigned long long* ctr, uint64_t v) {
+    const uint64_t incl = wave_scan64(v);
+    const uint32_t lo = __shfl((uint32_t)incl, 63, 64), hi = __shfl((uint32_t)(incl >> 32), 63, 64);
+    const uint64_t total = ((uint64_t)hi << 32) | lo;
+    unsigned long long base = 0;
+    if (total != 0 && (threadIdx.x & 63u) == 63u) base = atomicAdd(ctr, (unsigned long long)total);
+    const uint64_t b = ((uint64_t)__shfl((uint32_t)(base >> 32), 63, 64) << 32) | __shfl((uint32_t)base, 63, 64);
+    return b + incl - v;
+}
+__device__ __forceinline__ void wave_add64(unsigned long long* ctr, uint64_t v) { (void)wave_alloc64(ctr, v); }
+
 // ---------------------------------------------------------------------------------------------
 // Editors (hash_dag_editors.h:9-330). FP32, contraction off.
 // ---------------------------------------------------------------------------------------------
@@ -170,103 +217,119 @@ __device__ __forceinline__ void set_error(uint32_t* flags, uint32_t bit) { atomi
 // expand: items of level l -> child classification and items of level l+1
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
-    const uint32_t n = min(P.count[l], P.cap);
+    const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
     const uint32_t D = P.levels, L = D - 2, ctl = P.ctl;
     const uint32_t depth = D - (l + 1);  // child cube side = 2^depth
     const float side = (float)(1u << depth);
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * 8; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t total = (uint64_t)n * 8;
+    // wave-uniform loop: child items are allocated with one atomic per wavefront
+    for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < total;
+         t0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = t0 + (threadIdx.x & 63u);
+        const bool active = t < total;
         const uint32_t i = (uint32_t)(t >> 3), c = (uint32_t)(t & 7u);
-        const uint32_t cx = (I.px[i] << 1) | ((c >> 2) & 1u), cy = (I.py[i] << 1) | ((c >> 1) & 1u),
-                       cz = (I.pz[i] << 1) | (c & 1u);
-        // child of the effective old node, and of the original node
-        uint32_t childOld = kInvalidNode, childOrig = kInvalidNode;
-        const uint32_t old = I.old_[i], orig = I.orig[i];
-        if (old != kInvalidNode) {
-            const uint32_t m = P.vpool[old] & 0xFFu;
-            if (m & (1u << c)) childOld = P.vpool[old + child_off(m, c)];
-        }
-        if (orig == old) {
-            childOrig = childOld;
-        } else if (orig != kInvalidNode) {
-            const uint32_t m = P.vpool[orig] & 0xFFu;
-            if (m & (1u << c)) childOrig = P.vpool[orig + child_off(m, c)];
-        }
-        const float x0 = (float)(cx << depth), y0 = (float)(cy << depth), z0 = (float)(cz << depth);
-        const float x1 = x0 + side, y1 = y0 + side, z1 = z0 + side;
-        const uint32_t es = I.estart[i], ec = I.ecount[i];
-        // filter the edit list; remember the last cover at levels >= ctl
-        uint32_t m = 0, lastCover = ~0u, coverKind = 0;
-        for (uint32_t k = 0; k < ec; ++k) {
-            const DevEdit& e = P.edits[P.elist[l][es + k]];
-            if (!e_touch(e, x0, y0, z0, x1, y1, z1)) continue;
-            if (l + 1 >= ctl) {
-                const uint32_t cv = e_cover(e, x0, y0, z0, x1, y1, z1);
-                if (cv) {
-                    lastCover = m;
-                    coverKind = cv;
-                }
-            }
-            m++;
-        }
-        // old voxel count of the effective old child (colour offsets, "keep" counts)
-        uint32_t oldCount = 0;
-        if (l + 1 >= ctl && childOld != kInvalidNode) {
-            if (l + 1 == L) {
-                const uint64_t lf = *reinterpret_cast<const uint64_t*>(P.vpool + childOld);
-                oldCount = __popcll(lf);
-            } else {
-                oldCount = P.vpool[childOld] >> 8;
-            }
-        }
         const uint32_t slot = i * 8 + c;
-        I.corig[slot] = childOrig;
-        I.cold[slot] = oldCount;
-        uint32_t kind;
-        const bool covered = lastCover != ~0u;
-        if (m == 0) {
-            kind = KIND_KEEP;
-            I.cval[slot] = childOld;
-            I.cnew[slot] = oldCount;
-        } else {
-            const uint32_t rest = covered ? m - lastCover - 1 : m;
-            // a covered colour subtree root stays an item (its colour leaf is rebuilt: the
-            // reference's task runs edit_leaf<color_tree_levels>, which takes the early exit)
-            if (covered && rest == 0 && l + 1 > ctl) {
-                kind = coverKind == 1 ? KIND_FULL : KIND_EMPTY;
-                I.cval[slot] = coverKind == 1 ? P.full_nodes[l + 1] : kInvalidNode;
-                I.cnew[slot] = coverKind == 1 ? (1u << (3 * depth)) : 0u;
-            } else {
-                kind = KIND_ITEM;
+        uint32_t kind = KIND_KEEP, m = 0, lastCover = ~0u, coverKind = 0, es = 0, ec = 0;
+        uint32_t cx = 0, cy = 0, cz = 0, childOld = kInvalidNode, childOrig = kInvalidNode;
+        float x0 = 0, y0 = 0, z0 = 0, x1 = 0, y1 = 0, z1 = 0;
+        unsigned long long coveredVoxels = 0;
+        if (active) {
+            cx = (I.px[i] << 1) | ((c >> 2) & 1u);
+            cy = (I.py[i] << 1) | ((c >> 1) & 1u);
+            cz = (I.pz[i] << 1) | (c & 1u);
+            // child of the effective old node, and of the original node
+            const uint32_t old = I.old_[i], orig = I.orig[i];
+            if (old != kInvalidNode) {
+                const uint32_t cm = P.vpool[old] & 0xFFu;
+                if (cm & (1u << c)) childOld = P.vpool[old + child_off(cm, c)];
             }
-            // numVoxels of a plain edit: a full / empty early exit counts the whole node
-            if (covered && rest == 0 && P.single_edit)
-                atomicAdd(P.stat_voxels, (unsigned long long)(1ull << (3 * depth)));
-            if (covered) {
-                // colour of a full cover (add_large_single_color of the covering editor)
-                uint32_t q = 0;
-                for (uint32_t k = 0; k < ec; ++k) {
-                    const DevEdit& e = P.edits[P.elist[l][es + k]];
-                    if (!e_touch(e, x0, y0, z0, x1, y1, z1)) continue;
-                    if (q == lastCover) {
-                        I.ccolor[slot] = e.single;
-                        break;
+            if (orig == old) {
+                childOrig = childOld;
+            } else if (orig != kInvalidNode) {
+                const uint32_t cm = P.vpool[orig] & 0xFFu;
+                if (cm & (1u << c)) childOrig = P.vpool[orig + child_off(cm, c)];
+            }
+            x0 = (float)(cx << depth);
+            y0 = (float)(cy << depth);
+            z0 = (float)(cz << depth);
+            x1 = x0 + side;
+            y1 = y0 + side;
+            z1 = z0 + side;
+            es = I.estart[i];
+            ec = I.ecount[i];
+            // filter the edit list; remember the last cover at levels >= ctl
+            for (uint32_t k = 0; k < ec; ++k) {
+                const DevEdit& e = P.edits[P.elist[l][es + k]];
+                if (!e_touch(e, x0, y0, z0, x1, y1, z1)) continue;
+                if (l + 1 >= ctl) {
+                    const uint32_t cv = e_cover(e, x0, y0, z0, x1, y1, z1);
+                    if (cv) {
+                        lastCover = m;
+                        coverKind = cv;
                     }
-                    q++;
+                }
+                m++;
+            }
+            // old voxel count of the effective old child (colour offsets, "keep" counts)
+            uint32_t oldCount = 0;
+            if (l + 1 >= ctl && childOld != kInvalidNode) {
+                if (l + 1 == L) {
+                    const uint64_t lf = *reinterpret_cast<const uint64_t*>(P.vpool + childOld);
+                    oldCount = __popcll(lf);
+                } else {
+                    oldCount = P.vpool[childOld] >> 8;
+                }
+            }
+            I.corig[slot] = childOrig;
+            I.cold[slot] = oldCount;
+            const bool covered = lastCover != ~0u;
+            if (m == 0) {
+                kind = KIND_KEEP;
+                I.cval[slot] = childOld;
+                I.cnew[slot] = oldCount;
+            } else {
+                const uint32_t rest = covered ? m - lastCover - 1 : m;
+                // a covered colour subtree root stays an item (its colour leaf is rebuilt: the
+                // reference's task runs edit_leaf<color_tree_levels>, which takes the early exit)
+                if (covered && rest == 0 && l + 1 > ctl) {
+                    kind = coverKind == 1 ? KIND_FULL : KIND_EMPTY;
+                    I.cval[slot] = coverKind == 1 ? P.full_nodes[l + 1] : kInvalidNode;
+                    I.cnew[slot] = coverKind == 1 ? (1u << (3 * depth)) : 0u;
+                } else {
+                    kind = KIND_ITEM;
+                }
+                // numVoxels of a plain edit: a full / empty early exit counts the whole node
+                if (covered && rest == 0) coveredVoxels = 1ull << (3 * depth);
+                if (covered) {
+                    // colour of a full cover (add_large_single_color of the covering editor)
+                    uint32_t q = 0;
+                    for (uint32_t k = 0; k < ec; ++k) {
+                        const DevEdit& e = P.edits[P.elist[l][es + k]];
+                        if (!e_touch(e, x0, y0, z0, x1, y1, z1)) continue;
+                        if (q == lastCover) {
+                            I.ccolor[slot] = e.single;
+                            break;
+                        }
+                        q++;
+                    }
                 }
             }
         }
-        if (kind == KIND_ITEM) {
-            const uint32_t ci = atomicAdd(&P.count[l + 1], 1u);
-            const uint32_t rest = lastCover == ~0u ? m : m - lastCover - 1;
-            const uint32_t eo = atomicAdd(&P.ecount[l + 1], rest);
-            if (ci >= P.cap || eo + rest > P.ecap) {
-                set_error(P.errors, ERR_SCRATCH);
-                I.ckind[slot] = (uint8_t)kind;
-                continue;
-            }
+        const bool isItem = active && kind == KIND_ITEM;
+        const uint32_t rest = isItem ? (lastCover == ~0u ? m : m - lastCover - 1) : 0u;
+        if (P.single_edit) wave_add64(P.stat_voxels, coveredVoxels);
+        const uint32_t ci = wave_alloc32(&P.count[l + 1], isItem ? 1u : 0u);
+        const uint32_t eo = wave_alloc32(&P.ecount[l + 1], rest);
+        if (!active) continue;
+        if (isItem && (ci >= P.cap[l + 1] || eo + rest > P.ecap[l + 1])) {
+            set_error(P.errors, ERR_SCRATCH);
+            I.ckind[slot] = (uint8_t)kind;
+            continue;
+        }
+        if (isItem) {
             C.px[ci] = cx;
             C.py[ci] = cy;
             C.pz[ci] = cz;
@@ -310,53 +373,60 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// leaf level: one wavefront per leaf item, one lane per voxel (bit = child1 * 8 + child2)
+// leaf level: one thread per leaf item, its 64 voxels as bit masks (bit = child1 * 8 + child2).
+// Thread-per-item keeps many items' loads in flight; the per-voxel editor tests are cheap.
 // ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t leaf_touch_mask(const DevEdit& e, uint32_t bx, uint32_t by, uint32_t bz) {
+    uint64_t tm = 0;
+#pragma unroll 8
+    for (uint32_t b = 0; b < 64; ++b) {
+        const uint32_t c1 = b >> 3, c2 = b & 7u;
+        const float fx = (float)(bx | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u));
+        const float fy = (float)(by | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u));
+        const float fz = (float)(bz | ((c1 & 1u) << 1) | (c2 & 1u));
+        if (e_touch(e, fx, fy, fz, fx + 1.0f, fy + 1.0f, fz + 1.0f)) tm |= 1ull << b;
+    }
+    return tm;
+}
+
 __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
     const uint32_t L = P.levels - 2;
-    const uint32_t n = min(P.count[L], P.cap);
+    const uint32_t n = min(P.count[L], P.cap[L]);
     if (*P.errors) return;
     const Items I = P.items[L];
-    const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n; i += (gridDim.x * blockDim.x) >> 6) {
-        const uint32_t c1 = lane >> 3, c2 = lane & 7u;
-        const uint32_t x = (I.px[i] << 2) | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u);
-        const uint32_t y = (I.py[i] << 2) | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u);
-        const uint32_t z = (I.pz[i] << 2) | ((c1 & 1u) << 1) | (c2 & 1u);
-        uint64_t oldMask = 0, origMask = 0;
-        if (I.old_[i] != kInvalidNode) oldMask = *reinterpret_cast<const uint64_t*>(P.vpool + I.old_[i]);
-        if (I.orig[i] != kInvalidNode) origMask = *reinterpret_cast<const uint64_t*>(P.vpool + I.orig[i]);
-        bool v = (oldMask >> lane) & 1u;
-        const float fx = (float)x, fy = (float)y, fz = (float)z;
-        uint32_t touched = 0;
-        for (uint32_t k = 0; k < I.ecount[i]; ++k) {
-            const DevEdit& e = P.edits[P.elist[L][I.estart[i] + k]];
-            if (e_touch(e, fx, fy, fz, fx + 1.0f, fy + 1.0f, fz + 1.0f)) {
-                v = e_value(e, v);
-                touched = 1;
+    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += gridDim.x * blockDim.x) {
+        const uint32_t i = t0 + (threadIdx.x & 63u);
+        const bool active = i < n;
+        uint64_t touched = 0;
+        if (active) {
+            const uint32_t bx = I.px[i] << 2, by = I.py[i] << 2, bz = I.pz[i] << 2;
+            const uint32_t old = I.old_[i], orig = I.orig[i];
+            uint64_t v = old != kInvalidNode ? *reinterpret_cast<const uint64_t*>(P.vpool + old) : 0ull;
+            const uint64_t origMask = orig != kInvalidNode ? *reinterpret_cast<const uint64_t*>(P.vpool + orig) : 0ull;
+            const uint32_t es = I.estart[i], ec = I.ecount[i];
+            for (uint32_t k = 0; k < ec; ++k) {
+                const DevEdit& e = P.edits[P.elist[L][es + k]];
+                const uint64_t tm = leaf_touch_mask(e, bx, by, bz);
+                if (e.kind == EDIT_SPHERE_ADD || e.kind == EDIT_BOX_ADD) v |= tm;
+                else if (e.kind != EDIT_SPHERE_PAINT) v &= ~tm;
+                touched |= tm;
             }
-        }
-        const uint64_t newMask = __ballot(v);
-        if (P.single_edit) {
-            const uint64_t t = __ballot(touched);
-            if (lane == 0 && t) atomicAdd(P.stat_voxels, (unsigned long long)__popcll(t));
-        }
-        if (lane == 0) {
-            I.ncount[i] = __popcll(newMask);
-            if (newMask == origMask) {
-                I.nidx[i] = I.orig[i];
+            I.ncount[i] = __popcll(v);
+            if (v == origMask) {
+                I.nidx[i] = orig;
                 I.state[i] = 0;
-            } else if (newMask == 0) {
+            } else if (v == 0) {
                 I.nidx[i] = kInvalidNode;
                 I.state[i] = 0;
             } else {
                 I.clen[i] = 2;
-                I.cont[i * 9 + 0] = (uint32_t)newMask;
-                I.cont[i * 9 + 1] = (uint32_t)(newMask >> 32);
-                I.hash[i] = hash_leaf(newMask);
+                I.cont[i * 9 + 0] = (uint32_t)v;
+                I.cont[i * 9 + 1] = (uint32_t)(v >> 32);
+                I.hash[i] = hash_leaf(v);
                 I.state[i] = 1;
             }
         }
+        if (P.single_edit) wave_add64(P.stat_voxels, (uint64_t)__popcll(touched));
     }
 }
 
@@ -364,7 +434,7 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
 // build interior items of level l from their children's results
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_build(EditPass P, uint32_t l) {
-    const uint32_t n = min(P.count[l], P.cap);
+    const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
@@ -431,7 +501,7 @@ __device__ __forceinline__ bool same_content(const Items& I, uint32_t a, uint32_
 }
 
 __global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
-    const uint32_t n = min(P.count[l], P.cap);
+    const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const uint32_t hmask = P.hcap - 1;
@@ -439,10 +509,15 @@ __global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
         if (I.state[i] != 1) continue;
         uint32_t s = I.hash[i] & hmask;
         for (;;) {
-            const uint32_t cur = atomicCAS(&P.hset[s], 0u, i + 1);
+            // read first: many items can share one content (and so one slot), and a CAS per
+            // item on one address would serialise them all
+            uint32_t cur = __atomic_load_n(&P.hset[s], __ATOMIC_RELAXED);
             if (cur == 0) {
-                I.rep[i] = i;
-                break;
+                cur = atomicCAS(&P.hset[s], 0u, i + 1);
+                if (cur == 0) {
+                    I.rep[i] = i;
+                    break;
+                }
             }
             const uint32_t j = cur - 1;
             if (I.hash[j] == I.hash[i] && same_content(I, i, j)) {
@@ -458,13 +533,13 @@ __global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
 // position inside the bucket's used size whose words equal the node and that does not
 // cross a page boundary holds the same node content).
 __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
-    const uint32_t n = min(P.count[l], P.cap);
+    const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Geom g = P.g;
     const uint32_t lane = threadIdx.x & 63u;
     const bool leafLevel = l == g.leaf_level();
-    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n; i += (gridDim.x * blockDim.x) >> 6) {
+    for (uint32_t i = wave_index(); i < n; i += wave_count()) {
         if (I.state[i] != 1 || I.rep[i] != i) continue;
         const uint32_t bucket = g.bucket_from_hash(l, I.hash[i]);
         const uint32_t base = g.make_ptr(l, bucket, 0);
@@ -530,45 +605,50 @@ __device__ bool alloc_page(EditPass& P, uint32_t page) {
 }
 
 __global__ __launch_bounds__(256) void k_add(EditPass P, uint32_t l) {
-    const uint32_t n = min(P.count[l], P.cap);
+    const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Geom g = P.g;
     const uint32_t ps = g.page_size;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (I.state[i] != 1 || I.rep[i] != i) continue;
-        const uint32_t nw = I.clen[i];
-        const uint32_t bucket = g.bucket_from_hash(l, I.hash[i]);
-        uint32_t* bsPtr = &P.bucket_sizes[g.bucket_global(l, bucket)];
-        const uint32_t cap = g.bucket_size(l);
-        uint32_t start;
-        bool ok = true;
-        uint32_t cur = *reinterpret_cast<volatile uint32_t*>(bsPtr);
-        for (;;) {
-            const uint32_t left = ps - cur % ps;
-            start = (left == ps || left < nw) ? (left == ps ? cur : cur + left) : cur;
-            if (start + nw > cap) {
-                ok = false;
-                break;
+    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += gridDim.x * blockDim.x) {
+        const uint32_t i = t0 + (threadIdx.x & 63u);
+        uint32_t added = 0;
+        if (i < n && I.state[i] == 1 && I.rep[i] == i) {
+            const uint32_t nw = I.clen[i];
+            const uint32_t bucket = g.bucket_from_hash(l, I.hash[i]);
+            uint32_t* bsPtr = &P.bucket_sizes[g.bucket_global(l, bucket)];
+            const uint32_t cap = g.bucket_size(l);
+            uint32_t start;
+            bool ok = true;
+            uint32_t cur = *reinterpret_cast<volatile uint32_t*>(bsPtr);
+            for (;;) {
+                const uint32_t left = ps - cur % ps;
+                start = (left == ps || left < nw) ? (left == ps ? cur : cur + left) : cur;
+                if (start + nw > cap) {
+                    ok = false;
+                    break;
+                }
+                const uint32_t prev = atomicCAS(bsPtr, cur, start + nw);
+                if (prev == cur) break;
+                cur = prev;
             }
-            const uint32_t prev = atomicCAS(bsPtr, cur, start + nw);
-            if (prev == cur) break;
-            cur = prev;
+            if (!ok) {
+                set_error(P.errors, ERR_BUCKET);
+            } else {
+                const uint32_t va = g.make_ptr(l, bucket, start);
+                if (start % ps != 0 || alloc_page(P, va / ps)) {
+                    for (uint32_t k = 0; k < nw; ++k) P.vpool[va + k] = I.cont[i * 9 + k];
+                    I.nidx[i] = va;
+                    added = nw;
+                }
+            }
         }
-        if (!ok) {
-            set_error(P.errors, ERR_BUCKET);
-            continue;
-        }
-        const uint32_t va = g.make_ptr(l, bucket, start);
-        if (start % ps == 0 && !alloc_page(P, va / ps)) continue;
-        for (uint32_t k = 0; k < nw; ++k) P.vpool[va + k] = I.cont[i * 9 + k];
-        I.nidx[i] = va;
-        atomicAdd(P.stat_new_words, (unsigned long long)nw);
+        wave_add64(P.stat_new_words, added);
     }
 }
 
 __global__ __launch_bounds__(256) void k_resolve(EditPass P, uint32_t l) {
-    const uint32_t n = min(P.count[l], P.cap);
+    const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -580,7 +660,7 @@ __global__ __launch_bounds__(256) void k_resolve(EditPass P, uint32_t l) {
 // ---------------------------------------------------------------------------------------------
 // new / old colour starts of the child items (relative to the task's colour array)
 __global__ __launch_bounds__(256) void k_color_starts(EditPass P, uint32_t l) {
-    const uint32_t n = min(P.count[l], P.cap);
+    const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
@@ -606,19 +686,19 @@ __global__ __launch_bounds__(256) void k_color_starts(EditPass P, uint32_t l) {
 }
 
 // colour array of every task with colours: base offset in the colour scratch
-__global__ void k_color_alloc(EditPass P) {
+__global__ __launch_bounds__(256) void k_color_alloc(EditPass P) {
     const uint32_t ctl = P.ctl;
-    const uint32_t n = min(P.count[ctl], P.cap);
+    const uint32_t n = min(P.count[ctl], P.cap[ctl]);
     if (*P.errors) return;
     const Items T = P.items[ctl];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t cnt = T.ncount[i];
-        uint64_t base = 0;
-        if (cnt) {
-            base = atomicAdd(P.color_top, (unsigned long long)cnt);
-            if (base + cnt > P.color_cap) set_error(P.errors, ERR_COLOR_SCRATCH);
+    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += gridDim.x * blockDim.x) {
+        const uint32_t i = t0 + (threadIdx.x & 63u);
+        const uint32_t cnt = i < n ? T.ncount[i] : 0u;
+        const uint64_t base = wave_alloc64(P.color_top, cnt);
+        if (i < n) {
+            if (cnt && base + cnt > P.color_cap) set_error(P.errors, ERR_COLOR_SCRATCH);
+            T.cbase[i] = base;
         }
-        T.cbase[i] = base;
     }
 }
 
@@ -640,12 +720,12 @@ __device__ __forceinline__ uint64_t old_color(const EditPass& P, uint32_t mode, 
 
 // keep / full slots of interior items at levels >= ctl: one wavefront per slot
 __global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
-    const uint32_t n = min(P.count[l], P.cap);
+    const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items T = P.items[P.ctl];
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n * 8; w += (gridDim.x * blockDim.x) >> 6) {
+    for (uint32_t w = wave_index(); w < n * 8; w += wave_count()) {
         const uint32_t i = w >> 3, slot = w;
         const uint32_t kind = I.ckind[slot];
         if (kind != KIND_KEEP && kind != KIND_FULL) continue;
@@ -668,12 +748,12 @@ __global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
 // leaf items: fold each voxel's value and colour through the edit list
 __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
     const uint32_t L = P.levels - 2;
-    const uint32_t n = min(P.count[L], P.cap);
+    const uint32_t n = min(P.count[L], P.cap[L]);
     if (*P.errors) return;
     const Items I = P.items[L];
     const Items T = P.items[P.ctl];
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n; i += (gridDim.x * blockDim.x) >> 6) {
+    for (uint32_t i = wave_index(); i < n; i += wave_count()) {
         const uint32_t c1 = lane >> 3, c2 = lane & 7u;
         const uint32_t x = (I.px[i] << 2) | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u);
         const uint32_t y = (I.py[i] << 2) | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u);
@@ -714,203 +794,263 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
 
 // colour tree: make sure every task with colours has colour-tree nodes down to its leaf slot
 // and a unique leaf handle (hash_dag_edits.h:225-249); one thread, tasks in index order.
-__global__ void k_color_tree(EditPass P) {
-    if (blockIdx.x != 0 || threadIdx.x != 0 || *P.errors) return;
-    const uint32_t ctl = P.ctl, D = P.levels;
-    const uint32_t n = min(P.count[ctl], P.cap);
+// One thread per task: walk the colour tree to the task's leaf slot, creating missing colour
+// nodes (8 zeroed entries; installed with a CAS, a loser's block stays unused), give the task a
+// unique leaf slot (a shared or empty slot gets a new one) and log the slot's previous leaf
+// for undo. Colour-node entries only ever change from 0 to a fixed value, so a stale read is
+// either that value or 0, and a 0 goes through the CAS. Leaf slots are per task (one task per
+// level-ctl node position), so they are written without atomics.
+__global__ __launch_bounds__(256) void k_color_tree(EditPass P) {
+    if (*P.errors) return;
+    const uint32_t ctl = P.ctl;
+    const uint32_t n = min(P.count[ctl], P.cap[ctl]);
     const Items T = P.items[ctl];
-    for (uint32_t i = 0; i < n; ++i) {
-        if (T.ncount[i] == 0) {
-            T.newhandle[i] = 0;
-            continue;
-        }
-        uint32_t idx = 0;
-        const uint32_t x = T.px[i], y = T.py[i], z = T.pz[i];
-        bool ok = true;
-        for (uint32_t lvl = 1; lvl < ctl; ++lvl) {
-            const uint32_t s = ctl - lvl;
-            const uint32_t c = (((x >> s) & 1u) << 2) | (((y >> s) & 1u) << 1) | ((z >> s) & 1u);
-            uint32_t e = P.cnodes[idx + c];
-            if (e == 0) {
-                e = *P.n_cnodes;
-                if (e + 8 > P.cnodes_cap) {
-                    set_error(P.errors, ERR_COLOR_TREE);
-                    ok = false;
-                    break;
+    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += gridDim.x * blockDim.x) {
+        const uint32_t i = t0 + (threadIdx.x & 63u);
+        bool ok = i < n && T.ncount[i] != 0;
+        if (i < n && !ok) T.newhandle[i] = 0;
+        uint32_t idx = 0, c = 0, h = 0;
+        if (ok) {
+            const uint32_t x = T.px[i], y = T.py[i], z = T.pz[i];
+            for (uint32_t lvl = 1; lvl < ctl; ++lvl) {
+                const uint32_t s = ctl - lvl;
+                const uint32_t cc = (((x >> s) & 1u) << 2) | (((y >> s) & 1u) << 1) | ((z >> s) & 1u);
+                uint32_t e = __atomic_load_n(&P.cnodes[idx + cc], __ATOMIC_RELAXED);
+                if (e == 0) {
+                    const uint32_t ne = atomicAdd(P.n_cnodes, 8u);  // rare: new colour-tree nodes
+                    if (ne + 8 > P.cnodes_cap) {
+                        set_error(P.errors, ERR_COLOR_TREE);
+                        ok = false;
+                        break;
+                    }
+                    const uint32_t prev = atomicCAS(&P.cnodes[idx + cc], 0u, ne);
+                    e = prev == 0 ? ne : prev;
                 }
-                *P.n_cnodes = e + 8;
-                P.cnodes[idx + c] = e;
+                idx = e;
             }
-            idx = e;
+            c = ((x & 1u) << 2) | ((y & 1u) << 1) | (z & 1u);
+            if (ok) h = __atomic_load_n(&P.cnodes[idx + c], __ATOMIC_RELAXED);
+            if (!ok) T.newhandle[i] = 0;
         }
-        if (!ok) break;
-        const uint32_t c = ((x & 1u) << 2) | ((y & 1u) << 1) | (z & 1u);
-        uint32_t h = P.cnodes[idx + c];
-        const LeafDesc prevDesc = task_leaf(P, h);
-        if (h == 0 || !(h & 0x80000000u)) {
-            const uint32_t li = *P.n_leaves;
-            if (li >= P.leaves_cap) {
-                set_error(P.errors, ERR_COLOR_TREE);
-                break;
-            }
-            *P.n_leaves = li + 1;
+        const LeafDesc prevDesc = ok ? task_leaf(P, h) : LeafDesc{};
+        const bool needSlot = ok && (h == 0 || !(h & 0x80000000u));
+        const uint32_t li = wave_alloc32(P.n_leaves, needSlot ? 1u : 0u);
+        if (needSlot && li >= P.leaves_cap) {
+            set_error(P.errors, ERR_COLOR_TREE);
+            T.newhandle[i] = 0;
+            ok = false;
+        }
+        // every undo-log entry handed out below is written (a failed pass restores from it)
+        const uint32_t u = wave_alloc32(P.undo_count, ok ? 1u : 0u);
+        if (!ok) continue;
+        if (needSlot) {
             h = li | 0x80000000u;
-            P.cnodes[idx + c] = h;
+            // the slot keeps resolving to the previous colours until the pass commits
+            P.leaves[li] = prevDesc;
+            __atomic_store_n(&P.cnodes[idx + c], h, __ATOMIC_RELAXED);
         }
         T.newhandle[i] = h;
         // undo log: handle + the leaf it resolved to before this pass
-        const uint32_t u = *P.undo_count;
         if (u < P.undo_cap) {
             P.undo_handles[u] = h;
             P.undo_descs[u] = prevDesc;
-            *P.undo_count = u + 1;
         } else {
             set_error(P.errors, ERR_UNDO);
         }
-        (void)D;
     }
 }
 
-// One workgroup per task: encode its colour array into a new unique leaf in the arena.
-constexpr uint32_t kEncBlock = 256;
+// A failed pass: every slot in the undo log gets back the leaf it resolved to before.
+__global__ __launch_bounds__(256) void k_restore_leaves(LeafDesc* leaves, const uint32_t* handles,
+                                                        const LeafDesc* descs, const uint32_t* count, uint32_t cap) {
+    const uint32_t n = min(*count, cap);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        leaves[handles[k] & 0x7FFFFFFFu] = descs[k];
+}
 
+// leaves[handles[k]] = descs[k] (undo / redo of a frame; a frame names each slot once).
+__global__ __launch_bounds__(256) void k_scatter_leaves(LeafDesc* leaves, const uint32_t* handles,
+                                                        const LeafDesc* descs, uint32_t n) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        leaves[handles[k] & 0x7FFFFFFFu] = descs[k];
+}
+
+// The leaves a pass wrote, in undo-log order (the redo half of an undo frame).
+__global__ __launch_bounds__(256) void k_gather_leaves(const LeafDesc* leaves, const uint32_t* handles,
+                                                       LeafDesc* out, uint32_t n) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        out[k] = leaves[handles[k] & 0x7FFFFFFFu];
+}
+
+// One wavefront per task: encode its colour array into a new unique leaf in the arena.
+// Pass 1 counts blocks and weight bits; pass 2 walks the colours in chunks of 64 with wave
+// scans carried across chunks. Weight bits of a chunk are gathered in LDS and each weight
+// word is written once (the word a chunk leaves partial is carried to the next chunk).
+constexpr uint32_t kEncBlock = 256;
+constexpr uint32_t kEncWaves = kEncBlock / 64;
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (uint32_t off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void color_at(const uint64_t* cols, uint32_t i, uint32_t cnt, uint64_t& c, uint32_t& bpw,
+                                         uint32_t& start) {
+    c = 0;
+    bpw = 0;
+    start = 0;
+    if (i >= cnt) return;
+    c = cols[i];
+    bpw = (uint32_t)(c >> 40) & 0xFFu;
+    start = (i % 16384u) == 0;
+    if (!start) {
+        const uint64_t p = cols[i - 1];
+        start = (uint32_t)p != (uint32_t)c || ((p >> 40) & 0xFFu) != bpw;
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
+    return ((uint64_t)__shfl((uint32_t)(v >> 32), src, 64) << 32) | __shfl((uint32_t)v, src, 64);
+}
+
+// Tasks are taken in rounds of 64 per wavefront (lane j records task j's sizes in pass 1), so
+// the arena is reserved with one atomic per round.
 __global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
     const uint32_t ctl = P.ctl;
-    const uint32_t n = min(P.count[ctl], P.cap);
+    const uint32_t n = min(P.count[ctl], P.cap[ctl]);
     if (*P.errors) return;
     const Items T = P.items[ctl];
-    __shared__ uint32_t sScanA[kEncBlock], sScanB[kEncBlock];
-    __shared__ uint32_t sMacroBlock[160], sMacroBit[160];
-    __shared__ uint64_t sBase;
-    __shared__ uint32_t sCarryBlocks, sCarryBits;
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
-        const uint32_t cnt = T.ncount[t];
-        if (cnt == 0) continue;
-        const uint64_t* cols = P.colors + T.cbase[t];
-        const uint32_t nMacro = (cnt + 16383u) / 16384u;
-        if (nMacro > 160) {
-            if (tid == 0) set_error(P.errors, ERR_COLOR_SCRATCH);
-            continue;
-        }
-        // pass 1: number of blocks and weight bits
-        uint32_t myBlocks = 0, myBits = 0;
-        for (uint32_t i = tid; i < cnt; i += kEncBlock) {
-            const uint64_t c = cols[i];
-            const uint32_t bpw = (uint32_t)(c >> 40) & 0xFFu;
-            bool start = (i % 16384u) == 0;
-            if (!start) {
-                const uint64_t p = cols[i - 1];
-                start = (uint32_t)p != (uint32_t)c || ((p >> 40) & 0xFFu) != bpw;
+    __shared__ uint32_t sBits[kEncWaves][12];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint32_t* buf = sBits[wv];
+    const uint32_t nW = gridDim.x * kEncWaves;
+    for (uint32_t r = blockIdx.x * kEncWaves + wv; r < n; r += 64 * nW) {
+        // pass 1: blocks and weight bits of each task of the round
+        uint32_t myBlocks = 0, myBits = 0, myCnt = 0;
+        for (uint32_t j = 0; j < 64; ++j) {
+            const uint32_t t = r + j * nW;
+            if (t >= n) break;
+            const uint32_t cnt = T.ncount[t];
+            if (cnt == 0) continue;
+            if ((cnt + 16383u) / 16384u > 160) {  // more than 160 * 16384 colours in one subtree
+                if (lane == 0) set_error(P.errors, ERR_LEAF_SIZE);
+                continue;
             }
-            myBlocks += start;
-            myBits += bpw;
-        }
-        sScanA[tid] = myBlocks;
-        sScanB[tid] = myBits;
-        __syncthreads();
-        for (uint32_t s = kEncBlock / 2; s > 0; s >>= 1) {
-            if (tid < s) {
-                sScanA[tid] += sScanA[tid + s];
-                sScanB[tid] += sScanB[tid + s];
+            const uint64_t* cols = P.colors + T.cbase[t];
+            uint32_t nb = 0, bits = 0;
+            for (uint32_t i = lane; i < cnt; i += 64) {
+                uint64_t c;
+                uint32_t bpw, start;
+                color_at(cols, i, cnt, c, bpw, start);
+                nb += start;
+                bits += bpw;
             }
-            __syncthreads();
+            nb = wave_sum(nb);
+            bits = wave_sum(bits);
+            if (lane == j) {
+                myBlocks = nb;
+                myBits = bits;
+                myCnt = cnt;
+            }
         }
-        const uint32_t nBlocks = sScanA[0], nBits = sScanB[0];
-        const uint32_t nWeights = (nBits + 31) / 32;
-        const uint64_t words = (uint64_t)nBlocks + 2ull * nMacro + (nWeights + 1) / 2 + 1;
-        if (tid == 0) {
-            sBase = atomicAdd(P.arena_top, (unsigned long long)words);
-            sCarryBlocks = 0;
-            sCarryBits = 0;
-        }
-        __syncthreads();
-        const uint64_t base = sBase;
-        if (base + words > P.arena_cap) {
-            if (tid == 0) set_error(P.errors, ERR_COLOR_ARENA);
-            __syncthreads();
-            continue;
-        }
-        uint64_t* blocks = P.arena + base;
-        uint64_t* macro = blocks + nBlocks;
-        uint32_t* weights = reinterpret_cast<uint32_t*>(macro + 2 * nMacro);
-        for (uint32_t k = tid; k < nWeights + 2; k += kEncBlock) weights[k] = 0u;
-        __syncthreads();
-        // pass 2: chunks of kEncBlock colours in order, exclusive scans carried across chunks
-        for (uint32_t c0 = 0; c0 < cnt; c0 += kEncBlock) {
-            const uint32_t i = c0 + tid;
-            uint64_t c = 0;
-            uint32_t bpw = 0, start = 0;
-            if (i < cnt) {
-                c = cols[i];
-                bpw = (uint32_t)(c >> 40) & 0xFFu;
-                start = (i % 16384u) == 0;
-                if (!start) {
-                    const uint64_t p = cols[i - 1];
-                    start = (uint32_t)p != (uint32_t)c || ((p >> 40) & 0xFFu) != bpw;
+        const uint32_t myMacro = (myCnt + 16383u) / 16384u;
+        const uint64_t myWords = myCnt ? (uint64_t)myBlocks + 2ull * myMacro + ((myBits + 31) / 32 + 1) / 2 + 1 : 0ull;
+        const uint64_t myBase = wave_alloc64(P.arena_top, myWords);
+        if (myCnt && myBase + myWords > P.arena_cap) set_error(P.errors, ERR_COLOR_ARENA);
+        // pass 2
+        for (uint32_t j = 0; j < 64; ++j) {
+            const uint32_t t = r + j * nW;
+            if (t >= n) break;
+            const uint32_t cnt = __shfl(myCnt, j, 64);
+            if (cnt == 0) continue;
+            const uint64_t base = shfl64(myBase, j), words = shfl64(myWords, j);
+            if (base + words > P.arena_cap) continue;
+            const uint32_t nBlocks = __shfl(myBlocks, j, 64), nBits = __shfl(myBits, j, 64);
+            const uint32_t nMacro = (cnt + 16383u) / 16384u;
+            const uint32_t nWeights = (nBits + 31) / 32;
+            const uint64_t* cols = P.colors + T.cbase[t];
+            uint64_t* blocks = P.arena + base;
+            uint64_t* macro = blocks + nBlocks;
+            uint32_t* weights = reinterpret_cast<uint32_t*>(macro + 2 * nMacro);
+            uint32_t carryBlocks = 0, carryBits = 0, macroBit = 0, carryWord = 0;
+            for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
+                const uint32_t i = c0 + lane;
+                uint64_t c;
+                uint32_t bpw, start;
+                color_at(cols, i, cnt, c, bpw, start);
+                const uint32_t sA = wave_incl_scan(start, lane), sB = wave_incl_scan(bpw, lane);
+                const uint32_t blockIdxIncl = carryBlocks + sA;  // block index + 1 if start
+                const uint32_t bitOff = carryBits + sB - bpw;    // exclusive bit offset
+                // a chunk of 64 holds at most one macro-block start
+                const uint64_t mb = __ballot(i < cnt && (i % 16384u) == 0);
+                uint32_t myMacroBit = macroBit;
+                if (mb) {
+                    const uint32_t src = (uint32_t)__ffsll((long long)mb) - 1u;
+                    const uint32_t nb = __shfl(bitOff, src, 64);
+                    if (lane >= src) myMacroBit = nb;
+                    macroBit = nb;
+                    if (lane == src) {
+                        macro[2 * (i / 16384u)] = blockIdxIncl - 1;
+                        macro[2 * (i / 16384u) + 1] = bitOff;
+                    }
                 }
-            }
-            sScanA[tid] = start;
-            sScanB[tid] = bpw;
-            __syncthreads();
-            // Hillis-Steele inclusive scan
-            for (uint32_t off = 1; off < kEncBlock; off <<= 1) {
-                const uint32_t a = tid >= off ? sScanA[tid - off] : 0u;
-                const uint32_t b = tid >= off ? sScanB[tid - off] : 0u;
-                __syncthreads();
-                sScanA[tid] += a;
-                sScanB[tid] += b;
-                __syncthreads();
-            }
-            const uint32_t blockIdxIncl = sCarryBlocks + sScanA[tid];  // block index + 1 if start
-            const uint32_t bitOff = sCarryBits + sScanB[tid] - bpw;    // exclusive bit offset
-            if (i < cnt && (i % 16384u) == 0) {
-                sMacroBlock[i / 16384u] = blockIdxIncl - 1;
-                sMacroBit[i / 16384u] = bitOff;
-            }
-            __syncthreads();
-            if (i < cnt) {
-                const uint32_t m = i / 16384u;
-                if (start) {
-                    const uint32_t rel = bitOff - sMacroBit[m];
+                if (i < cnt && start) {
+                    const uint32_t rel = bitOff - myMacroBit;
                     const uint32_t hdr = bpw == 0 ? ((0xFFFFu << 16) | (i % 16384u))
                                                   : ((rel << 16) | ((bpw - 1) << 14) | (i % 16384u));
                     blocks[blockIdxIncl - 1] = ((uint64_t)(uint32_t)c << 32) | hdr;
                 }
+                // weight bits of the chunk: [B0, B1), words w0 .. ; MSB-first stream bit k ->
+                // stored word k / 32, byte (k % 32) / 8, bit 7 - k % 8
+                const uint32_t B0 = carryBits, B1 = carryBits + __shfl(sB, 63, 64);
+                const uint32_t w0 = B0 >> 5;
+                if (lane < 12) buf[lane] = lane == 0 ? carryWord : 0u;
+                wave_sync();
                 if (bpw) {
-                    // MSB-first stream bit k -> stored word k / 32, byte (k % 32) / 8, bit 7 - k % 8
-                    const uint32_t wv = (uint32_t)(c >> 32) & 0xFFu;
+                    const uint32_t wvv = (uint32_t)(c >> 32) & 0xFFu;
                     for (uint32_t b = 0; b < bpw; ++b) {
-                        if (!((wv >> (bpw - 1 - b)) & 1u)) continue;
+                        if (!((wvv >> (bpw - 1 - b)) & 1u)) continue;
                         const uint32_t k = bitOff + b;
-                        atomicOr(&weights[k >> 5], 1u << (((k & 31u) >> 3) * 8u + (7u - (k & 7u))));
+                        atomicOr(&buf[(k >> 5) - w0], 1u << (((k & 31u) >> 3) * 8u + (7u - (k & 7u))));
                     }
                 }
+                wave_sync();
+                const uint32_t full = (B1 >> 5) - w0;  // complete words of the chunk
+                if (lane < full) weights[w0 + lane] = buf[lane];
+                carryWord = (B1 & 31u) ? buf[full] : 0u;
+                wave_sync();
+                carryBlocks = __shfl(blockIdxIncl, 63, 64);
+                carryBits = B1;
             }
-            __syncthreads();
-            if (tid == kEncBlock - 1) {
-                sCarryBlocks = blockIdxIncl;
-                sCarryBits = bitOff + bpw;
+            if (lane == 0 && (carryBits & 31u)) weights[carryBits >> 5] = carryWord;
+            if (lane < 2) weights[nWeights + lane] = 0u;  // padding read by the 16-bit window
+            if (lane == 0) {
+                LeafDesc d;
+                d.blocks = base;
+                d.n_blocks = nBlocks;
+                d.macro = base + nBlocks;
+                d.n_macro = 2ull * nMacro;
+                d.weights32 = 2ull * (base + nBlocks + 2ull * nMacro);
+                d.n_weights = nWeights;
+                d.offset = ~0ull;
+                d.pad = 0;
+                P.leaves[T.newhandle[t] & 0x7FFFFFFFu] = d;
             }
-            __syncthreads();
         }
-        for (uint32_t m = tid; m < nMacro; m += kEncBlock) {
-            macro[2 * m] = sMacroBlock[m];
-            macro[2 * m + 1] = sMacroBit[m];
-        }
-        if (tid == 0) {
-            LeafDesc d;
-            d.blocks = base;
-            d.n_blocks = nBlocks;
-            d.macro = base + nBlocks;
-            d.n_macro = 2ull * nMacro;
-            d.weights32 = 2ull * (base + nBlocks + 2ull * nMacro);
-            d.n_weights = nWeights;
-            d.offset = ~0ull;
-            d.pad = 0;
-            P.leaves[T.newhandle[t] & 0x7FFFFFFFu] = d;
-        }
-        __syncthreads();
     }
 }
 
@@ -951,61 +1091,177 @@ __global__ void k_finish(EditPass P, uint32_t* root_out) {
 }  // namespace
 
 // Host: scratch management and pass orchestration
-hd_status edit_reserve(hd_ctx* ctx, uint32_t item_cap, uint32_t elist_cap, uint64_t color_cap) {
+// Frees the per-pass scratch; the uploaded edit array and the undo log survive.
+static void scratch_free(hd_ctx* ctx) {
     EditScratch& S = ctx->edit;
-    const uint32_t levels = ctx->g.levels;
-    if (S.item_cap >= item_cap && S.elist_cap >= elist_cap && S.color_cap >= color_cap && S.levels == levels) return HD_OK;
-    edit_free(ctx);
-    S.levels = levels;
-    S.item_cap = item_cap;
-    S.elist_cap = elist_cap;
-    S.color_cap = color_cap;
-    uint32_t hcap = 1;
-    while (hcap < 2 * item_cap) hcap <<= 1;
-    S.hcap = hcap;
-    auto alloc = [&](void** p, uint64_t bytes) -> bool {
-        if (hipMalloc(p, std::max<uint64_t>(bytes, 16)) != hipSuccess) return false;
-        S.allocs.push_back(*p);
-        return true;
+    for (void* p : S.allocs) hipFree(p);
+    hipFree(S.hset);
+    hipFree(S.colors);
+    for (void*& p : S.level_mem) {
+        hipFree(p);
+        p = nullptr;
+    }
+    DevEdit* edits = S.edits_dev;
+    const uint32_t ecap = S.edits_cap;
+    uint32_t* uh = S.undo_handles;
+    LeafDesc *ud = S.undo_descs, *un = S.undo_next;
+    const uint32_t ucap = S.undo_cap;
+    S = EditScratch{};
+    S.edits_dev = edits;
+    S.edits_cap = ecap;
+    S.undo_handles = uh;
+    S.undo_descs = ud;
+    S.undo_next = un;
+    S.undo_cap = ucap;
+}
+
+// Carves the item arrays of one level (capacity c) and its edit list (e words) out of one
+// allocation; the leaf level has no child-slot arrays.
+static hipError_t alloc_level(EditScratch& S, uint32_t l, bool leafLevel, uint64_t c, uint64_t e) {
+    const uint64_t c8 = leafLevel ? 0 : 8 * c;
+    struct Field {
+        void** p;
+        uint64_t bytes;
     };
-    const uint64_t c = item_cap, c8 = 8ull * item_cap;
-    for (uint32_t l = 0; l < levels; ++l) {
-        Items& I = S.items[l];
-        bool ok = alloc((void**)&I.px, c * 4) && alloc((void**)&I.py, c * 4) && alloc((void**)&I.pz, c * 4) &&
-                  alloc((void**)&I.old_, c * 4) && alloc((void**)&I.orig, c * 4) && alloc((void**)&I.parent, c * 4) &&
-                  alloc((void**)&I.slot, c) && alloc((void**)&I.estart, c * 4) && alloc((void**)&I.ecount, c * 4) &&
-                  alloc((void**)&I.aux, c * 4) && alloc((void**)&I.handle, c * 4) &&
-                  alloc((void**)&I.newhandle, c * 4) && alloc((void**)&I.cmode, c * 4) &&
-                  alloc((void**)&I.cmodeval, c * 4) && alloc((void**)&I.ckind, c8) && alloc((void**)&I.cval, c8 * 4) &&
-                  alloc((void**)&I.corig, c8 * 4) && alloc((void**)&I.cold, c8 * 4) &&
-                  alloc((void**)&I.cnew, c8 * 4) && alloc((void**)&I.ccolor, c8 * 4) &&
-                  alloc((void**)&I.cstart, c8 * 4) && alloc((void**)&I.costart, c8 * 4) &&
-                  alloc((void**)&I.nidx, c * 4) && alloc((void**)&I.ncount, c * 4) && alloc((void**)&I.cont, c * 36) &&
-                  alloc((void**)&I.clen, c * 4) && alloc((void**)&I.hash, c * 4) && alloc((void**)&I.rep, c * 4) &&
-                  alloc((void**)&I.state, c * 4) && alloc((void**)&I.nstart, c * 4) &&
-                  alloc((void**)&I.ostart, c * 4) && alloc((void**)&I.cbase, c * 8) &&
-                  alloc((void**)&S.elist[l], (uint64_t)elist_cap * 4);
-        if (!ok) {
-            edit_free(ctx);
+    Items& I = S.items[l];
+    I = Items{};
+    const Field f[] = {
+        {(void**)&I.px, c * 4},       {(void**)&I.py, c * 4},      {(void**)&I.pz, c * 4},
+        {(void**)&I.old_, c * 4},     {(void**)&I.orig, c * 4},    {(void**)&I.parent, c * 4},
+        {(void**)&I.slot, c},         {(void**)&I.estart, c * 4},  {(void**)&I.ecount, c * 4},
+        {(void**)&I.aux, c * 4},      {(void**)&I.handle, c * 4},  {(void**)&I.newhandle, c * 4},
+        {(void**)&I.cmode, c * 4},    {(void**)&I.cmodeval, c * 4}, {(void**)&I.ckind, c8},
+        {(void**)&I.cval, c8 * 4},    {(void**)&I.corig, c8 * 4},  {(void**)&I.cold, c8 * 4},
+        {(void**)&I.cnew, c8 * 4},    {(void**)&I.ccolor, c8 * 4}, {(void**)&I.cstart, c8 * 4},
+        {(void**)&I.costart, c8 * 4}, {(void**)&I.nidx, c * 4},    {(void**)&I.ncount, c * 4},
+        {(void**)&I.cont, c * 36},    {(void**)&I.clen, c * 4},    {(void**)&I.hash, c * 4},
+        {(void**)&I.rep, c * 4},      {(void**)&I.state, c * 4},   {(void**)&I.nstart, c * 4},
+        {(void**)&I.ostart, c * 4},   {(void**)&I.cbase, c * 8},   {(void**)&S.elist[l], e * 4},
+    };
+    auto up = [](uint64_t v) { return (v + 255) & ~255ull; };
+    uint64_t total = 0;
+    for (const Field& x : f) total += up(x.bytes);
+    char* mem = nullptr;
+    const hipError_t err = hipMalloc(&mem, std::max<uint64_t>(total, 256));
+    if (err != hipSuccess) return err;
+    S.level_mem[l] = mem;
+    uint64_t off = 0;
+    for (const Field& x : f) {
+        *x.p = x.bytes ? mem + off : nullptr;
+        off += up(x.bytes);
+    }
+    S.item_cap[l] = (uint32_t)c;
+    S.elist_cap[l] = (uint32_t)e;
+    return hipSuccess;
+}
+
+// Grows the scratch to at least items[l] / elist[l] per level (levels 0 .. leaf level) and
+// `colors` staged colours. Levels that already fit keep their memory.
+hd_status edit_reserve(hd_ctx* ctx, const uint32_t* items, const uint32_t* elist, uint64_t colors) {
+    EditScratch& S = ctx->edit;
+    const uint32_t levels = ctx->g.levels, L = ctx->g.leaf_level();
+    if (S.levels != levels) {
+        hipDeviceSynchronize();
+        scratch_free(ctx);
+        S.levels = levels;
+        auto alloc = [&](void** p, uint64_t bytes) -> bool {
+            if (hipMalloc(p, std::max<uint64_t>(bytes, 16)) != hipSuccess) return false;
+            S.allocs.push_back(*p);
+            return true;
+        };
+        if (!(alloc((void**)&S.dcount, 64 * 4) && alloc((void**)&S.decount, 64 * 4) && alloc((void**)&S.dstats, 64 * 8))) {
+            scratch_free(ctx);
             return fail(ctx, HD_ERR_OOM, "edit scratch allocation failed");
         }
     }
-    bool ok = alloc((void**)&S.hset, (uint64_t)hcap * 4) && alloc((void**)&S.colors, color_cap * 8) &&
-              alloc((void**)&S.dcount, 64 * 4) && alloc((void**)&S.decount, 64 * 4) && alloc((void**)&S.dstats, 64 * 8) &&
-              alloc((void**)&S.undo_handles, (uint64_t)kUndoCap * 4) &&
-              alloc((void**)&S.undo_descs, (uint64_t)kUndoCap * sizeof(LeafDesc)) && alloc((void**)&S.edits_dev, 4096 * sizeof(DevEdit));
-    if (!ok) {
-        edit_free(ctx);
-        return fail(ctx, HD_ERR_OOM, "edit scratch allocation failed");
+    bool synced = false;
+    auto sync_once = [&]() {
+        if (!synced) hipDeviceSynchronize();  // no pass may still use memory being freed
+        synced = true;
+    };
+    uint32_t maxItems = 0;
+    for (uint32_t l = 0; l <= L; ++l) {
+        // level l holds at most 8^l items
+        const uint64_t bound = l < 10 ? (1ull << (3 * l)) : ~0ull;
+        const uint64_t want = std::min<uint64_t>(std::max<uint64_t>(items[l], 1), bound);
+        const uint64_t c = std::max<uint64_t>(want, S.item_cap[l]);
+        const uint64_t e = std::max<uint64_t>(std::max<uint64_t>(elist[l], 16), S.elist_cap[l]);
+        maxItems = std::max<uint32_t>(maxItems, (uint32_t)c);
+        if (c == S.item_cap[l] && e == S.elist_cap[l] && S.level_mem[l]) continue;
+        sync_once();
+        hipFree(S.level_mem[l]);
+        S.level_mem[l] = nullptr;
+        S.item_cap[l] = S.elist_cap[l] = 0;
+        const hipError_t err = alloc_level(S, l, l == L, c, e);
+        if (err != hipSuccess) return hip_fail(ctx, err, "edit scratch (items)");
     }
-    S.edits_cap = 4096;
+    uint32_t hcap = 1;
+    while (hcap < 2 * maxItems) hcap <<= 1;
+    if (hcap > S.hcap) {
+        sync_once();
+        hipFree(S.hset);
+        S.hset = nullptr;
+        S.hcap = 0;
+        const hipError_t err = hipMalloc(&S.hset, (uint64_t)hcap * 4);
+        if (err != hipSuccess) return hip_fail(ctx, err, "edit scratch (hash set)");
+        S.hcap = hcap;
+    }
+    if (colors > S.color_cap) {
+        sync_once();
+        hipFree(S.colors);
+        S.colors = nullptr;
+        S.color_cap = 0;
+        const hipError_t err = hipMalloc(&S.colors, colors * 8);
+        if (err != hipSuccess) return hip_fail(ctx, err, "edit scratch (colours)");
+        S.color_cap = colors;
+    }
+    return HD_OK;
+}
+
+// Undo log of a pass (touched leaf slots): kept across scratch re-reserves, grown on demand.
+hd_status edit_reserve_undo(hd_ctx* ctx, uint64_t entries) {
+    EditScratch& S = ctx->edit;
+    if (entries <= S.undo_cap) return HD_OK;
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(entries, kUndoCap), 1u << 30);
+    hipDeviceSynchronize();
+    hipFree(S.undo_handles);
+    hipFree(S.undo_descs);
+    hipFree(S.undo_next);
+    S.undo_handles = nullptr;
+    S.undo_descs = S.undo_next = nullptr;
+    S.undo_cap = 0;
+    hipError_t e = hipMalloc(&S.undo_handles, (uint64_t)cap * 4);
+    if (e == hipSuccess) e = hipMalloc(&S.undo_descs, (uint64_t)cap * sizeof(LeafDesc));
+    if (e == hipSuccess) e = hipMalloc(&S.undo_next, (uint64_t)cap * sizeof(LeafDesc));
+    if (e != hipSuccess) return hip_fail(ctx, e, "undo log");
+    S.undo_cap = cap;
+    return HD_OK;
+}
+
+// Device array for n edits (kept across scratch re-reserves; its contents stay valid).
+hd_status edit_ensure_edits(hd_ctx* ctx, uint32_t n) {
+    EditScratch& S = ctx->edit;
+    if (n <= S.edits_cap) return HD_OK;
+    const uint32_t cap = std::max<uint32_t>(n, 4096);
+    DevEdit* ne = nullptr;
+    const hipError_t e = hipMalloc(&ne, (uint64_t)cap * sizeof(DevEdit));
+    if (e != hipSuccess) return hip_fail(ctx, e, "edit array");
+    if (S.edits_dev) {
+        hipDeviceSynchronize();
+        hipFree(S.edits_dev);
+    }
+    S.edits_dev = ne;
+    S.edits_cap = cap;
     return HD_OK;
 }
 
 void edit_free(hd_ctx* ctx) {
     EditScratch& S = ctx->edit;
-    for (void* p : S.allocs) hipFree(p);
-    S.allocs.clear();
+    scratch_free(ctx);
+    hipFree(S.edits_dev);
+    hipFree(S.undo_handles);
+    hipFree(S.undo_descs);
+    hipFree(S.undo_next);
     S = EditScratch{};
 }
 
@@ -1030,8 +1286,10 @@ static void fill_pass(hd_ctx* ctx, EditPass& P) {
         P.items[l] = S.items[l];
         P.elist[l] = S.elist[l];
     }
-    P.cap = S.item_cap;
-    P.ecap = S.elist_cap;
+    for (uint32_t l = 0; l < kMaxEditLevels; ++l) {
+        P.cap[l] = S.item_cap[l];
+        P.ecap[l] = S.elist_cap[l];
+    }
     P.hset = S.hset;
     P.hcap = S.hcap;
     P.edits = S.edits_dev;
@@ -1051,7 +1309,7 @@ static void fill_pass(hd_ctx* ctx, EditPass& P) {
     P.arena_cap = ctx->cfg.color_arena_words;
     P.undo_handles = S.undo_handles;
     P.undo_descs = S.undo_descs;
-    P.undo_cap = kUndoCap;
+    P.undo_cap = S.undo_cap;
 }
 
 // Enqueues one pass over edits [first, first + n) of the uploaded edit array.
@@ -1077,7 +1335,7 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
     if (colors) {
         for (uint32_t l = P.ctl; l < L; ++l) k_color_starts<<<kGrid, 256, 0, s>>>(P, l);
         k_color_alloc<<<kGrid, 256, 0, s>>>(P);
-        k_color_tree<<<1, 64, 0, s>>>(P);
+        k_color_tree<<<kGrid, 256, 0, s>>>(P);
         for (uint32_t l = P.ctl; l < L; ++l) k_color_copy<<<kGrid, 256, 0, s>>>(P, l);
         k_color_leaf<<<kGrid, 256, 0, s>>>(P);
         k_encode<<<kGrid, kEncBlock, 0, s>>>(P);
@@ -1086,6 +1344,34 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "edit pass launch");
     return HD_OK;
+}
+
+// After a failed pass: give every leaf slot the pass touched its previous leaf back.
+void edit_restore_leaves(hd_ctx* ctx, hipStream_t s) {
+    const EditScratch& S = ctx->edit;
+    k_restore_leaves<<<256, 256, 0, s>>>(ctx->cleaves, S.undo_handles, S.undo_descs, ctx->counters + 4, S.undo_cap);
+}
+
+// Undo / redo: write n host descriptors into their leaf slots (one upload, one scatter).
+hd_status edit_scatter_leaves(hd_ctx* ctx, const uint32_t* handles, const LeafDesc* descs, uint32_t n) {
+    if (n == 0) return HD_OK;
+    const EditScratch& S = ctx->edit;
+    if (n > S.undo_cap) return fail(ctx, HD_ERR_INVALID, "undo frame larger than the undo log");
+    HD_HIP(ctx, hipMemcpy(S.undo_handles, handles, (uint64_t)n * 4, hipMemcpyHostToDevice));
+    HD_HIP(ctx, hipMemcpy(S.undo_next, descs, (uint64_t)n * sizeof(LeafDesc), hipMemcpyHostToDevice));
+    k_scatter_leaves<<<std::min<uint32_t>((n + 255) / 256, 1024), 256>>>(ctx->cleaves, S.undo_handles, S.undo_next, n);
+    HD_HIP(ctx, hipGetLastError());
+    HD_HIP(ctx, hipDeviceSynchronize());
+    return HD_OK;
+}
+
+// After a committed pass: copy the n leaf descriptors the pass wrote (undo-log order) to host.
+hipError_t edit_gather_next(hd_ctx* ctx, uint32_t n, LeafDesc* host, hipStream_t s) {
+    const EditScratch& S = ctx->edit;
+    if (n == 0) return hipSuccess;
+    n = std::min(n, S.undo_cap);
+    k_gather_leaves<<<std::min<uint32_t>((n + 255) / 256, 1024), 256, 0, s>>>(ctx->cleaves, S.undo_handles, S.undo_next, n);
+    return hipMemcpyAsync(host, S.undo_next, (uint64_t)n * sizeof(LeafDesc), hipMemcpyDeviceToHost, s);
 }
 
 }  // namespace hd

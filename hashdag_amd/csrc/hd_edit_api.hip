@@ -12,9 +12,14 @@
 
 namespace hd {
 
-hd_status edit_reserve(hd_ctx* ctx, uint32_t item_cap, uint32_t elist_cap, uint64_t color_cap);
+hd_status edit_reserve(hd_ctx* ctx, const uint32_t* items, const uint32_t* elist, uint64_t colors);
 void edit_free(hd_ctx* ctx);
 hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s);
+hd_status edit_ensure_edits(hd_ctx* ctx, uint32_t n);
+hd_status edit_reserve_undo(hd_ctx* ctx, uint64_t entries);
+hd_status edit_scatter_leaves(hd_ctx* ctx, const uint32_t* handles, const LeafDesc* descs, uint32_t n);
+void edit_restore_leaves(hd_ctx* ctx, hipStream_t s);
+hipError_t edit_gather_next(hd_ctx* ctx, uint32_t n, LeafDesc* host, hipStream_t s);
 
 namespace {
 
@@ -134,9 +139,37 @@ static hd_status grow_arena(hd_ctx* ctx, uint64_t need) {
     return HD_OK;
 }
 
+// Colour tree entries / unique leaf slots: reallocate (copying what is in use) so that at
+// least `nodes` entries and `leaves` slots fit.
+static hd_status grow_color_tables(hd_ctx* ctx, uint64_t nodes, uint64_t leaves) {
+    HD_HIP(ctx, hipDeviceSynchronize());
+    if (nodes > ctx->cfg.color_nodes_cap) {
+        const uint64_t cap = std::min<uint64_t>(std::max<uint64_t>(nodes + nodes / 2, 2ull * ctx->cfg.color_nodes_cap), 0xFFFFFFF0u);
+        if (cap < nodes) return fail(ctx, HD_ERR_COLOR_FULL, "colour tree exceeds 2^32 entries");
+        uint32_t* nn = nullptr;
+        HD_HIP(ctx, hipMalloc(&nn, cap * 4));
+        HD_HIP(ctx, hipMemset(nn, 0, cap * 4));
+        HD_HIP(ctx, hipMemcpy(nn, ctx->cnodes, (uint64_t)ctx->cfg.color_nodes_cap * 4, hipMemcpyDeviceToDevice));
+        hipFree(ctx->cnodes);
+        ctx->cnodes = nn;
+        ctx->cfg.color_nodes_cap = (uint32_t)cap;
+    }
+    if (leaves > ctx->cfg.color_leaves_cap) {
+        const uint64_t cap = std::min<uint64_t>(std::max<uint64_t>(leaves + leaves / 2, 2ull * ctx->cfg.color_leaves_cap), 0x7FFFFFFFu);
+        if (cap < leaves) return fail(ctx, HD_ERR_COLOR_FULL, "more than 2^31 colour leaves");
+        LeafDesc* nl = nullptr;
+        HD_HIP(ctx, hipMalloc(&nl, cap * sizeof(LeafDesc)));
+        HD_HIP(ctx, hipMemcpy(nl, ctx->cleaves, (uint64_t)ctx->cfg.color_leaves_cap * sizeof(LeafDesc), hipMemcpyDeviceToDevice));
+        hipFree(ctx->cleaves);
+        ctx->cleaves = nl;
+        ctx->cfg.color_leaves_cap = (uint32_t)cap;
+    }
+    return HD_OK;
+}
+
 static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s, Readback& rb, UndoFrame* frame) {
     const bool colors = ctx->colors_loaded && ctx->g.leaf_level() > ctx->g.color_tree_levels;
-    for (int attempt = 0; attempt < 6; ++attempt) {
+    for (int attempt = 0; attempt < 40; ++attempt) {
         hd_status st = grow_arena(ctx, 1u << 22);
         if (st != HD_OK) return st;
         EditScratch& S = ctx->edit;
@@ -154,17 +187,62 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
         HD_HIP(ctx, hipMemcpyAsync(&rb.count_ctl, S.dcount + ctx->g.color_tree_levels, 4, hipMemcpyDeviceToHost, s));
         HD_HIP(ctx, hipStreamSynchronize(s));
         const uint32_t err = rb.counters[1];
+        if (err) {
+            // The root was not switched. Leaf slots the pass already re-pointed get their
+            // previous leaves back; nodes it appended stay as unreachable garbage.
+            if (rb.counters[4]) {
+                edit_restore_leaves(ctx, s);
+                HD_HIP(ctx, hipStreamSynchronize(s));
+            }
+        }
         if (err & (ERR_SCRATCH | ERR_COLOR_SCRATCH)) {
-            // nothing was written to the table or the colours: grow the scratch and retry
-            st = edit_reserve(ctx, S.item_cap * 4, S.elist_cap * 4, (err & ERR_COLOR_SCRATCH) ? S.color_cap * 4 : S.color_cap);
+            // The device counters hold what the pass asked for. Levels below the first one that
+            // overflowed only saw that level's capped items: scale their need by its shortfall.
+            uint32_t cnt[64], ecnt[64];
+            HD_HIP(ctx, hipMemcpy(cnt, S.dcount, sizeof(cnt), hipMemcpyDeviceToHost));
+            HD_HIP(ctx, hipMemcpy(ecnt, S.decount, sizeof(ecnt), hipMemcpyDeviceToHost));
+            const uint32_t L = ctx->g.leaf_level();
+            uint32_t items[kMaxEditLevels] = {}, elist[kMaxEditLevels] = {};
+            double scale = 1.0;
+            for (uint32_t l = 0; l <= L; ++l) {
+                const double ci = (double)cnt[l] * scale, ce = (double)ecnt[l] * scale;
+                items[l] = S.item_cap[l];
+                elist[l] = S.elist_cap[l];
+                if (ci > S.item_cap[l]) items[l] = (uint32_t)std::min<double>(ci * 1.25 + 64, 1u << 30);
+                if (ce > S.elist_cap[l]) elist[l] = (uint32_t)std::min<double>(ce * 1.25 + 64, 1u << 30);
+                const double over = std::max((double)cnt[l] / std::max<uint32_t>(S.item_cap[l], 1),
+                                             (double)ecnt[l] / std::max<uint32_t>(S.elist_cap[l], 1));
+                if (over > 1.0) scale = std::max(scale, over);
+            }
+            const uint64_t cc = (err & ERR_COLOR_SCRATCH) ? std::max<uint64_t>(S.color_cap * 2, rb.stats[2] + rb.stats[2] / 4)
+                                                          : S.color_cap;
+            bool grows = cc > S.color_cap;
+            for (uint32_t l = 0; l <= L; ++l) grows |= items[l] > S.item_cap[l] || elist[l] > S.elist_cap[l];
+            if (!grows) return fail(ctx, HD_ERR_OOM, "edit scratch overflow that growth cannot fix");
+            st = edit_reserve(ctx, items, elist, cc);
+            if (st != HD_OK) return st;
+            continue;
+        }
+        if (err & ERR_COLOR_ARENA) {
+            st = grow_arena(ctx, std::max<uint64_t>(rb.stats[3] - ctx->arena_top, 1u << 22) * 2);
             if (st != HD_OK) return st;
             continue;
         }
         if (err & ERR_BUCKET) return fail(ctx, HD_ERR_BUCKET_FULL, "hash bucket full during edit (old root kept)");
         if (err & ERR_POOL) return fail(ctx, HD_ERR_POOL_FULL, "page pool exhausted during edit (old root kept)");
         if (err & ERR_HEADER) return fail(ctx, HD_ERR_HEADER_OVERFLOW, "voxel count >= 2^24 in a node header");
-        if (err & (ERR_COLOR_ARENA | ERR_COLOR_TREE | ERR_UNDO))
-            return fail(ctx, HD_ERR_COLOR_FULL, "colour capacity exhausted during edit");
+        if (err & ERR_LEAF_SIZE) return fail(ctx, HD_ERR_COLOR_FULL, "a colour subtree holds more than 160 macro blocks");
+        if (err & ERR_COLOR_TREE) {
+            // the allocation counters ran past the capacities by what the pass asked for
+            st = grow_color_tables(ctx, (uint64_t)rb.counters[2] + 64, (uint64_t)rb.counters[3] + 64);
+            if (st != HD_OK) return st;
+            continue;
+        }
+        if (err & ERR_UNDO) {
+            st = edit_reserve_undo(ctx, (uint64_t)rb.counters[4] + rb.counters[4] / 2);
+            if (st != HD_OK) return st;
+            continue;
+        }
         if (err & ERR_EMPTY) return fail(ctx, HD_ERR_INVALID, "edit would leave an empty scene (old root kept)");
         ctx->arena_top = rb.stats[3];
         ctx->n_cnodes = rb.counters[2];
@@ -177,12 +255,8 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
             if (nu) {
                 HD_HIP(ctx, hipMemcpyAsync(frame->handles.data(), S.undo_handles, nu * 4, hipMemcpyDeviceToHost, s));
                 HD_HIP(ctx, hipMemcpyAsync(frame->prev.data(), S.undo_descs, nu * sizeof(LeafDesc), hipMemcpyDeviceToHost, s));
+                HD_HIP(ctx, edit_gather_next(ctx, nu, frame->next.data(), s));
                 HD_HIP(ctx, hipStreamSynchronize(s));
-                for (uint32_t k = 0; k < nu; ++k) {
-                    LeafDesc d;
-                    HD_HIP(ctx, hipMemcpy(&d, ctx->cleaves + (frame->handles[k] & 0x7FFFFFFFu), sizeof(d), hipMemcpyDeviceToHost));
-                    frame->next[k] = d;
-                }
             }
         }
         return HD_OK;
@@ -209,28 +283,31 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");
     hd_status st = ensure_color_tree(ctx);
     if (st != HD_OK) return st;
-    if (!ctx->edit.item_cap) {
-        st = edit_reserve(ctx, 1u << 16, 1u << 18, 1u << 22);
+    if (!ctx->edit.level_mem[0]) {
+        uint32_t items[kMaxEditLevels], elist[kMaxEditLevels];
+        for (uint32_t l = 0; l < kMaxEditLevels; ++l) {
+            items[l] = 1u << 16;
+            elist[l] = 1u << 18;
+        }
+        st = edit_reserve(ctx, items, elist, 1u << 22);
         if (st != HD_OK) return st;
     }
+    st = edit_reserve_undo(ctx, kUndoCap);
+    if (st != HD_OK) return st;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     // editors to the device
     std::vector<DevEdit> dev(n);
     for (uint32_t i = 0; i < n; ++i) dev[i] = make_edit(edits[i]);
-    if (n > ctx->edit.edits_cap) {
-        EditScratch& S = ctx->edit;
-        DevEdit* ne = nullptr;
-        HD_HIP(ctx, hipMalloc(&ne, (uint64_t)n * sizeof(DevEdit)));
-        S.allocs.push_back(ne);
-        S.edits_dev = ne;
-        S.edits_cap = n;
-    }
+    st = edit_ensure_edits(ctx, n);
+    if (st != HD_OK) return st;
     if (n) HD_HIP(ctx, hipMemcpyAsync(ctx->edit.edits_dev, dev.data(), (uint64_t)n * sizeof(DevEdit), hipMemcpyHostToDevice, s));
     HD_HIP(ctx, hipMemcpyAsync(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice, s));
     hd_edit_stats out{};
     if (fused) {
-        if (n > ctx->edit.elist_cap) {
-            st = edit_reserve(ctx, ctx->edit.item_cap, std::max<uint32_t>(n * 2, ctx->edit.elist_cap), ctx->edit.color_cap);
+        if (n > ctx->edit.elist_cap[0]) {
+            uint32_t items[kMaxEditLevels] = {}, elist[kMaxEditLevels] = {};
+            elist[0] = n;
+            st = edit_reserve(ctx, items, elist, 0);
             if (st != HD_OK) return st;
         }
         UndoFrame frame;
@@ -269,11 +346,9 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
 }
 
 static hd_status apply_frame(hd_ctx* ctx, const UndoFrame& f, bool undo) {
-    for (size_t k = 0; k < f.handles.size(); ++k) {
-        const size_t idx = undo ? f.handles.size() - 1 - k : k;
-        const LeafDesc& d = undo ? f.prev[idx] : f.next[idx];
-        HD_HIP(ctx, hipMemcpy(ctx->cleaves + (f.handles[idx] & 0x7FFFFFFFu), &d, sizeof(d), hipMemcpyHostToDevice));
-    }
+    const hd_status st = edit_scatter_leaves(ctx, f.handles.data(), undo ? f.prev.data() : f.next.data(),
+                                             (uint32_t)f.handles.size());
+    if (st != HD_OK) return st;
     ctx->root = undo ? f.prev_root : f.new_root;
     HD_HIP(ctx, hipMemcpy(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice));
     return HD_OK;
@@ -317,7 +392,12 @@ hd_status hd_undo_depth(hd_ctx* ctx, uint32_t* undo, uint32_t* redo) {
 
 hd_status hd_reserve_edit_scratch(hd_ctx* ctx, uint32_t items_per_level, uint32_t edit_list_words, uint64_t colors) {
     if (!ctx || !items_per_level) return fail(ctx, HD_ERR_INVALID, "bad arguments");
-    return edit_reserve(ctx, items_per_level, edit_list_words, colors);
+    uint32_t items[kMaxEditLevels], elist[kMaxEditLevels];
+    for (uint32_t l = 0; l < kMaxEditLevels; ++l) {
+        items[l] = items_per_level;
+        elist[l] = edit_list_words;
+    }
+    return edit_reserve(ctx, items, elist, colors);
 }
 
 }  // extern "C"

@@ -117,32 +117,74 @@ __device__ __forceinline__ uint32_t intersection_mask(uint32_t levels, uint32_t 
     return mask;
 }
 
+// A node's header and the 8 words after it, fetched with independent loads (one round trip;
+// the pool has padding words after the virtual space, so reading past the last node is safe).
+__device__ __forceinline__ void load_node(const uint32_t* __restrict__ vpool, uint32_t index, uint32_t nd[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) nd[k] = vpool[index + k];
+}
+
+// nd[k] for a lane-varying k in 1..8, as a select tree over constant indices (a dynamic
+// index would send the array to scratch memory)
+__device__ __forceinline__ uint32_t pick(const uint32_t nd[9], uint32_t k) {
+    const uint32_t a = (k & 1u) ? nd[1] : nd[2];
+    const uint32_t b = (k & 1u) ? nd[3] : nd[4];
+    const uint32_t c = (k & 1u) ? nd[5] : nd[6];
+    const uint32_t d = (k & 1u) ? nd[7] : nd[8];
+    const uint32_t ab = (k <= 2u) ? a : b;
+    const uint32_t cd = (k <= 6u) ? c : d;
+    return k <= 4u ? ab : cd;
+}
+
+// LDS stack of one block: node index per level 1 .. leafLevel-1 (level 0 is the root; at the
+// two leaf levels the 64-bit leaf stays in registers), then (child mask | visit << 8) per level.
+struct Stack {
+    uint32_t* idx;
+    uint16_t* mask;
+    __device__ __forceinline__ uint32_t& at_idx(uint32_t level) { return idx[(level - 1) * kBlock + threadIdx.x]; }
+    __device__ __forceinline__ uint16_t& at_mask(uint32_t level) { return mask[level * kBlock + threadIdx.x]; }
+};
+
+__host__ __device__ __forceinline__ uint32_t stack_bytes(uint32_t levels) {
+    return ((levels - 3) * 4u + levels * 2u) * kBlock;
+}
+
+__device__ __forceinline__ Stack stack_of(uint32_t levels) {
+    extern __shared__ uint32_t sStack[];
+    Stack st;
+    st.idx = sStack;
+    st.mask = reinterpret_cast<uint16_t*>(sStack + (levels - 3) * kBlock);
+    return st;
+}
+
 // DFS over the DAG (tracer.cu:166-249 ordered first hit; tracer.cu:458-542 any hit).
 // Returns true on a hit; (px, py, pz) is the voxel path on a hit, (0,0,0) on a miss.
+// The current node's words are held in registers, so a descent costs one dependent fetch
+// (the child's words) instead of two (the child pointer, then the child's header).
 template <bool kAnyHit, bool kCount>
 __device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, uint32_t order, uint32_t& px,
-                                        uint32_t& py, uint32_t& pz, uint32_t* __restrict__ sIdx,
-                                        uint16_t* __restrict__ sMask, uint32_t tid, uint64_t& bytes) {
+                                        uint32_t& py, uint32_t& pz, Stack st, uint64_t& bytes) {
     const uint32_t* __restrict__ vpool = dag.vpool;
     const uint32_t levels = dag.levels, leafLevel = dag.leaf_level;
     uint32_t level = 0;
     px = py = pz = 0;
     uint32_t index = dag.root;
-    uint32_t childMask = vpool[index] & 0xFFu;
+    uint32_t nd[9];
+    load_node(vpool, index, nd);
+    uint32_t childMask = nd[0] & 0xFFu;
     if (kCount) bytes += 4;
     uint32_t visit = childMask & intersection_mask<true>(levels, 0, 0, 0, 0, o, d, inv);
     uint64_t leaf = 0;
     for (;;) {
-        {
+        if (!visit) {
             uint32_t newLevel = level;
             while (newLevel > 0 && !visit) {
                 newLevel--;
-                index = sIdx[newLevel * kBlock + tid];
-                const uint32_t m = sMask[newLevel * kBlock + tid];
+                const uint32_t m = st.at_mask(newLevel);
                 childMask = m & 0xFFu;
                 visit = m >> 8;
             }
-            if (newLevel == 0 && !visit) {
+            if (!visit) {
                 px = py = pz = 0;
                 return false;
             }
@@ -151,29 +193,32 @@ __device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, u
             py >>= up;
             pz >>= up;
             level = newLevel;
+            if (level < leafLevel) {
+                index = level == 0 ? dag.root : st.at_idx(level);
+                load_node(vpool, index, nd);
+            }
         }
         const uint32_t next = kAnyHit ? (31u - (uint32_t)__builtin_clz(visit)) : next_child(order, visit);
         visit &= ~(1u << next);
         px = (px << 1) | ((next >> 2) & 1u);
         py = (py << 1) | ((next >> 1) & 1u);
         pz = (pz << 1) | (next & 1u);
-        sIdx[level * kBlock + tid] = index;
-        sMask[level * kBlock + tid] = (uint16_t)(childMask | (visit << 8));
+        if (level > 0 && level < leafLevel) st.at_idx(level) = index;
+        st.at_mask(level) = (uint16_t)(childMask | (visit << 8));
         level++;
         if (level == levels) return true;
         if (level < leafLevel) {
-            index = vpool[index + child_off(childMask, next)];
-            childMask = vpool[index] & 0xFFu;
+            index = pick(nd, child_off(childMask, next));
+            load_node(vpool, index, nd);
+            childMask = nd[0] & 0xFFu;
             if (kCount) bytes += 8;
+        } else if (level == leafLevel) {
+            const uint32_t addr = pick(nd, child_off(childMask, next));
+            leaf = *reinterpret_cast<const uint64_t*>(vpool + addr);
+            childMask = leaf_first_mask(leaf);
+            if (kCount) bytes += 12;
         } else {
-            if (level == leafLevel) {
-                const uint32_t addr = vpool[index + child_off(childMask, next)];
-                leaf = *reinterpret_cast<const uint64_t*>(vpool + addr);
-                childMask = leaf_first_mask(leaf);
-                if (kCount) bytes += 12;
-            } else {
-                childMask = leaf_second_mask(leaf, next);
-            }
+            childMask = leaf_second_mask(leaf, next);
         }
         visit = childMask & intersection_mask<false>(levels, level, px, py, pz, o, d, inv);
     }
@@ -206,8 +251,7 @@ template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, uint32_t W, uint32_t H,
                                                         uint4* __restrict__ out, uint32_t tileW, uint32_t tileH,
                                                         unsigned long long* counter) {
-    __shared__ uint32_t sIdx[kMaxLevels * kBlock];
-    __shared__ uint16_t sMask[kMaxLevels * kBlock];
+    const Stack st = stack_of(dag.levels);
     uint32_t x, y;
     pixel_of(tileW, tileH, x, y);
     uint64_t bytes = 0;
@@ -221,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, uint
         const F3 inv{(float)(1.0 / (double)d.x), (float)(1.0 / (double)d.y), (float)(1.0 / (double)d.z)};
         const uint32_t order = (d.x < 0.f ? 4u : 0u) + (d.y < 0.f ? 2u : 0u) + (d.z < 0.f ? 1u : 0u);
         uint32_t vx, vy, vz;
-        descend<false, kCount>(dag, o, d, inv, order, vx, vy, vz, sIdx, sMask, threadIdx.x, bytes);
+        descend<false, kCount>(dag, o, d, inv, order, vx, vy, vz, st, bytes);
         out[(uint64_t)(H - 1 - y) * W + x] = make_uint4(vx, vy, vz, 0u);
         if (kCount) bytes += 16;
     }
@@ -443,8 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, ui
                                                           const uint4* __restrict__ paths, uint32_t* __restrict__ colors,
                                                           float bias, float fog, uint32_t tileW, uint32_t tileH,
                                                           unsigned long long* counter) {
-    __shared__ uint32_t sIdx[kMaxLevels * kBlock];
-    __shared__ uint16_t sMask[kMaxLevels * kBlock];
+    const Stack st = stack_of(dag.levels);
     uint32_t x, y;
     pixel_of(tileW, tileH, x, y);
     uint64_t bytes = 0;
@@ -482,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, ui
             const F3 d{sx, sy, sz};
             const F3 inv{(float)(1.0 / (double)sx), (float)(1.0 / (double)sy), (float)(1.0 / (double)sz)};
             uint32_t vx, vy, vz;
-            const bool shadowed = descend<true, kCount>(dag, o, d, inv, 0, vx, vy, vz, sIdx, sMask, threadIdx.x, bytes);
+            const bool shadowed = descend<true, kCount>(dag, o, d, inv, 0, vx, vy, vz, st, bytes);
             light = shadowed ? 0.f : 1.f;
             const double vxd = bx - p.cam[0], vyd = by - p.cam[1], vzd = bz - p.cam[2];
             distance = __dsqrt_rn(vxd * vxd + vyd * vyd + vzd * vzd);
@@ -554,8 +597,8 @@ hd_status hd_trace_paths(hd_ctx* ctx, const hd_ray_params* params, uint32_t W, u
                          void* stream) {
     if (!ctx || !params || !d_paths || W == 0 || H == 0) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
-    if (ctx->g.levels > kMaxLevels) return fail(ctx, HD_ERR_INVALID, "too many levels");
-    k_trace_paths<false><<<grid_of(ctx, W, H), kBlock, 0, stream_of(ctx, stream)>>>(
+    if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
+    k_trace_paths<false><<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), stream_of(ctx, stream)>>>(
         dev_dag(ctx), to_rayp(params), W, H, reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h, nullptr);
     HD_HIP(ctx, hipGetLastError());
     return HD_OK;
@@ -576,7 +619,8 @@ hd_status hd_trace_shadows(hd_ctx* ctx, const hd_ray_params* params, uint32_t W,
                            uint32_t* d_colors, float bias, float fog, void* stream) {
     if (!ctx || !params || !d_paths || !d_colors || W == 0 || H == 0) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
-    k_trace_shadows<false><<<grid_of(ctx, W, H), kBlock, 0, stream_of(ctx, stream)>>>(
+    if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
+    k_trace_shadows<false><<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), stream_of(ctx, stream)>>>(
         dev_dag(ctx), to_rayp(params), W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
         ctx->tile_w, ctx->tile_h, nullptr);
     HD_HIP(ctx, hipGetLastError());
@@ -587,9 +631,10 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
                                uint32_t* d_colors, float bias, float fog, uint64_t bytes[3]) {
     if (!ctx || !params || !d_paths || !d_colors || !bytes) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
+    if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
     const dim3 grid = grid_of(ctx, W, H);
     HD_HIP(ctx, hipMemsetAsync(ctx->byte_counters, 0, 3 * sizeof(unsigned long long), ctx->stream));
-    k_trace_paths<true><<<grid, kBlock, 0, ctx->stream>>>(dev_dag(ctx), to_rayp(params), W, H,
+    k_trace_paths<true><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), W, H,
                                                           reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h,
                                                           ctx->byte_counters + 0);
     HD_HIP(ctx, hipGetLastError());
@@ -599,7 +644,7 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
                                                                ctx->tile_w, ctx->tile_h, ctx->byte_counters + 1);
         HD_HIP(ctx, hipGetLastError());
     }
-    k_trace_shadows<true><<<grid, kBlock, 0, ctx->stream>>>(dev_dag(ctx), to_rayp(params), W, H,
+    k_trace_shadows<true><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), W, H,
                                                             reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
                                                             ctx->tile_w, ctx->tile_h, ctx->byte_counters + 2);
     HD_HIP(ctx, hipGetLastError());

@@ -38,3 +38,48 @@ def surface_points(params: TerrainParams, n: int, seed: int = 2) -> np.ndarray:
         z = int(rng.integers(0, F))
         pts[i] = (x, terrain_height(params, x, z), z)
     return pts
+
+
+# editor kinds of the C ABI (hd_edit.kind)
+SPHERE_ADD, SPHERE_CARVE, BOX_ADD, BOX_CARVE, SPHERE_PAINT = range(5)
+
+
+def _centre(rng, params: TerrainParams):
+    F = params.footprint or (1 << params.levels)
+    x = int(rng.integers(0, F))
+    z = int(rng.integers(0, F))
+    return (float(x), float(terrain_height(params, x, z)), float(z))
+
+
+def sphere_edits(params: TerrainParams, n: int, seed: int = 3, rmin: float = 8.0, rmax: float = 64.0):
+    """n sphere edits, alternating add / carve, radius uniform in [rmin, rmax], centred at
+    seeded integer surface points: [(kind, (x, y, z), radius), ...]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        c = _centre(rng, params)
+        r = float(np.float32(rng.uniform(rmin, rmax)))
+        out.append((SPHERE_ADD if i % 2 == 0 else SPHERE_CARVE, c, r))
+    return out
+
+
+def mixed_edits(params: TerrainParams, n: int, seed: int = 4, rmin: float = 2.0, rmax: float = 256.0,
+                mix=(0.7, 0.2, 0.1)):
+    """n mixed edits: fractions mix = (sphere add/carve, box add/carve, sphere paint), add and
+    carve equally likely, radius log-uniform in [rmin, rmax], seeded integer surface centres."""
+    rng = np.random.default_rng(seed)
+    p = np.array(mix, np.float64) / sum(mix)
+    out = []
+    for _ in range(n):
+        c = _centre(rng, params)
+        r = float(np.float32(np.exp(rng.uniform(np.log(rmin), np.log(rmax)))))
+        cls = int(rng.choice(3, p=p))
+        add = bool(rng.integers(0, 2))
+        if cls == 0:
+            kind = SPHERE_ADD if add else SPHERE_CARVE
+        elif cls == 1:
+            kind = BOX_ADD if add else BOX_CARVE
+        else:
+            kind = SPHERE_PAINT
+        out.append((kind, c, r))
+    return out

@@ -139,3 +139,21 @@ def test_trace_after_edits_bit_exact():
         cols = Oracle.trace_colors(od, oc, paths, W, H)
         ref = Oracle.trace_shadows(od, op, paths, cols, W, H, 1.0, 0.0)
         assert np.array_equal(gpu, ref), f"{int((gpu != ref).sum())} pixels differ"
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_scratch_growth_keeps_results(fused):
+    """Passes that overflow a deliberately tiny scratch (and colour leaf table) are re-run on
+    a larger one; the uploaded edits must survive the re-reserve and the result must match."""
+    from hashdag_amd import HashDAG
+    p, sc, g, _, od, oc = make("d13")
+    dag = HashDAG(g, pool_pages=POOL_PAGES, color_leaves_cap=16)  # colour slots grow too
+    dag.load_scene(sc)
+    dag.reserve_edit_scratch(64, 256, 1024)
+    edits = random_edits(p, 24, seed=17)
+    st = dag.edit_batch(edits, fused=fused)
+    total = 0
+    for kind, center, r in edits:
+        total += od.edit(kind, center, r, oc)[1]
+    assert st.num_voxels == total
+    assert_same(dag, sc, od, oc, f"after scratch growth (fused={fused})")
