@@ -45,6 +45,8 @@ def main() -> int:
     ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
     ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
     ap.add_argument("--no-edits", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400, help="requests of the served stream (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate, requests/s")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -182,6 +184,8 @@ def main() -> int:
 
     if not args.no_edits:
         result["edits"] = edit_bench(scene, geom, params, dag, dinfo, rparams[:2], W, H, args)
+    if args.serve_requests > 0:
+        result["serving"] = serve_bench(scene, geom, params, args)
     dag.close()
     if not args.no_cpu_baseline:
         od, oc, result["cpu_baseline"] = cpu_baseline(scene, geom, rparams, W, H, args.cpu_seconds)
@@ -251,6 +255,26 @@ def edit_bench(scene, geom, params, dag, dinfo, rparams, W, H, args):
         "plain_ms_per_edit": round(t_p * 1e3 / len(edits), 3),
         "results_identical": same,
     }
+
+
+def serve_bench(scene, geom, params, args):
+    """A served request stream on a fresh copy of the scene: Poisson arrivals, 60% trace
+    requests (640x360 .. 3840x2160, seeded cameras), 40% edit requests (1-64 sphere / box
+    editors, radius log-uniform 2-128); p50 / p99 latency beside throughput."""
+    from hashdag_amd import HashDAG
+    from hashdag_amd.serve import latency_summary, request_stream, serve
+
+    reqs = request_stream(params, args.serve_requests, args.serve_rate, seed=args.seed + 2)
+    d = HashDAG(geom)
+    d.load_scene(scene)
+    serve(d, request_stream(params, 8, 1000.0, seed=args.seed + 3), realtime=False)  # warm-up
+    res = serve(d, reqs, trace_slots=4, realtime=True)
+    d.close()
+    summ = latency_summary(reqs, res)
+    log(f"serving: {summ}")
+    summ["workload"] = (f"{len(reqs)} requests, Poisson {args.serve_rate}/s on the 2^{params.levels} coloured terrain: "
+                        f"60% trace (640x360..3840x2160), 40% edit (1-64 sphere/box, radius log-uniform 2-128)")
+    return summ
 
 
 def cpu_edit_baseline(od, oc, params, args):

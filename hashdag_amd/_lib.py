@@ -99,7 +99,37 @@ class hd_edit_stats(C.Structure):
                 ("root", C.c_uint32), ("passes", C.c_uint32)]
 
 
+HD_REQ_TRACE, HD_REQ_EDIT = 0, 1
+
+
+class hd_request(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("kind", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("first_edit", C.c_uint32), ("n_edits", C.c_uint32), ("pad", C.c_uint32),
+                ("params", hd_ray_params)]
+
+
+class hd_request_result(C.Structure):
+    _fields_ = [("start_s", C.c_double), ("done_s", C.c_double), ("checksum", C.c_uint64), ("voxels", C.c_uint64),
+                ("batch", C.c_uint32), ("status", C.c_int32), ("gc", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class hd_mem_info_t(C.Structure):
+    _fields_ = [("pool_pages", C.c_uint64), ("pool_top", C.c_uint64), ("reusable_pages", C.c_uint64),
+                ("free_pages", C.c_uint64), ("used_words", C.c_uint64), ("arena_words", C.c_uint64),
+                ("arena_top", C.c_uint64)]
+
+
+class hd_serve_config(C.Structure):
+    _fields_ = [("trace_slots", C.c_uint32), ("max_batch_edits", C.c_uint32), ("realtime", C.c_uint32),
+                ("keep_undo", C.c_uint32), ("reserve_pages", C.c_uint64), ("shadow_bias", C.c_float),
+                ("fog_density", C.c_float)]
+
+
 _SIGS = {
+    "hd_serve_run": (C.c_int32, [C.c_void_p, C.POINTER(hd_request), C.c_uint32, C.POINTER(hd_edit), C.c_uint32,
+                                 C.POINTER(hd_serve_config), C.POINTER(hd_request_result)]),
+    "hd_mem_info": (C.c_int32, [C.c_void_p, C.POINTER(hd_mem_info_t)]),
+    "hd_frame_checksum": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, u64p, C.c_void_p]),
     "hd_edit_batch": (C.c_int32, [C.c_void_p, C.POINTER(hd_edit), C.c_uint32, C.c_uint32, C.POINTER(hd_edit_stats),
                                   C.c_void_p]),
     "hd_gc": (C.c_int32, [C.c_void_p, C.c_uint32, u64p, C.c_void_p]),

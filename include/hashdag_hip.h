@@ -182,6 +182,60 @@ hd_status hd_reserve_edit_scratch(hd_ctx* ctx, uint32_t items_per_level, uint32_
  * Clears the undo history. stats (may be NULL): live nodes, words before, words after, 0. */
 hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], void* stream);
 
+/* Request serving: trace and edit requests executed in arrival order with continuous
+ * batching on one device-resident DAG (runs of edit requests fuse into one pass; runs of trace
+ * requests run concurrently on trace slots; admission by free page budget, GC when short).
+ * No counterpart in the reference, whose engine serves one frame loop from the UI thread
+ * (engine.cpp:575-718); the work per request is hd_trace_* (DAGTracer::resolve_*) or
+ * hd_edit_batch (HashDAG::edit_threads). */
+#define HD_REQ_TRACE 0u
+#define HD_REQ_EDIT 1u
+typedef struct hd_request {
+    double arrival_s;        /* seconds after the start of the run, non-decreasing */
+    uint32_t kind;           /* HD_REQ_TRACE / HD_REQ_EDIT */
+    uint32_t width, height;  /* trace: frame size */
+    uint32_t first_edit;     /* edit: range in the edits array */
+    uint32_t n_edits;
+    uint32_t pad;
+    hd_ray_params params;    /* trace: camera rays */
+} hd_request;
+typedef struct hd_request_result {
+    double start_s, done_s;  /* when the request's batch started / the request completed */
+    uint64_t checksum;       /* trace: frame checksum (hd_frame_checksum); edit: root after its batch */
+    uint64_t voxels;         /* trace: pixels; edit: voxels edited by its whole batch */
+    uint32_t batch;          /* batch the request ran in */
+    int32_t status;
+    uint32_t gc;             /* edit: 1 if stale nodes were collected before its batch */
+    uint32_t pad;
+} hd_request_result;
+typedef struct hd_serve_config {
+    uint32_t trace_slots;      /* concurrent traces (streams with frame buffers), 1..16 */
+    uint32_t max_batch_edits;  /* cap on edits fused into one pass */
+    uint32_t realtime;         /* 1: requests arrive at arrival_s; 0: all at once */
+    uint32_t keep_undo;        /* 0: clear the undo history after every edit batch */
+    uint64_t reserve_pages;    /* GC watermark: collect when free pages < estimate + reserve
+                                  (a batch is rejected only if free pages < estimate after GC) */
+    float shadow_bias, fog_density;
+} hd_serve_config;
+hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd_edit* edits, uint32_t n_edits,
+                       const hd_serve_config* cfg, hd_request_result* out);
+/* Memory state for admission control (the proposed hd_mem_info of a serving layer; the
+ * reference has none: HashTable::allocate_page aborts when the pool is exhausted,
+ * hash_table.h:794-809). free_pages = pages never handed out + mapped pages beyond the used
+ * part of their bucket (reusable after GC). */
+typedef struct hd_mem_info_t {
+    uint64_t pool_pages, pool_top;
+    uint64_t reusable_pages;  /* mapped pages past their bucket's used words */
+    uint64_t free_pages;      /* pool_pages - pool_top + reusable_pages */
+    uint64_t used_words;      /* sum of bucket sizes */
+    uint64_t arena_words, arena_top;  /* colour arena capacity / used */
+} hd_mem_info_t;
+hd_status hd_mem_info(hd_ctx* ctx, hd_mem_info_t* out);
+/* Order-independent 64-bit checksum of a width*height RGBA8 frame (sum over pixels of a
+ * 64-bit mix of pixel and index). No counterpart in the reference (a test/serving aid). */
+hd_status hd_frame_checksum(hd_ctx* ctx, const uint32_t* d_colors, uint32_t width, uint32_t height, uint64_t* out,
+                            void* stream);
+
 /* Streaming device copy of `bytes` (multiple of 16) from d_src to d_dst: measures the
  * achievable HBM bandwidth next to the tracer (not in the reference). */
 hd_status hd_stream_copy(const void* d_src, void* d_dst, uint64_t bytes, void* stream);
