@@ -41,7 +41,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1337)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tile", type=str, default="8x8")
+    ap.add_argument("--tile", type=str, default="auto", help="wavefront tile WxH, or auto (autotuned)")
     ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
     ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
     ap.add_argument("--no-edits", action="store_true")
@@ -79,9 +79,15 @@ def main() -> int:
     dinfo = DAGInfo.unit(D)
     cams = cameras(params, args.cameras, seed=args.seed)
     tracer = DAGTracer(W, H)
-    tw, th = (int(v) for v in args.tile.split("x"))
-    tracer.set_tile(dag, tw, th)
     rparams = [ray_params(c, dinfo, D, W, H) for c in cams]
+    if args.tile == "auto":
+        from hashdag_amd.tracer import autotune
+        tuned = autotune(tracer, dag, rparams[:4])
+        args.tile = tuned["tile"]
+        log(f"autotuned tracer tile: {tuned}")
+    else:
+        tw, th = (int(v) for v in args.tile.split("x"))
+        tracer.set_tile(dag, tw, th)
 
     stream = torch.cuda.current_stream()
 

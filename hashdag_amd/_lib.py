@@ -128,6 +128,11 @@ class hd_serve_config(C.Structure):
 _SIGS = {
     "hd_serve_run": (C.c_int32, [C.c_void_p, C.POINTER(hd_request), C.c_uint32, C.POINTER(hd_edit), C.c_uint32,
                                  C.POINTER(hd_serve_config), C.POINTER(hd_request_result)]),
+    "hd_frame_graph_create": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                          C.c_float, C.c_float, C.POINTER(C.c_void_p)]),
+    "hd_frame_graph_launch": (C.c_int32, [C.c_void_p, C.POINTER(hd_edit), C.c_uint32, C.POINTER(hd_ray_params),
+                                          C.POINTER(hd_edit_stats), u32p]),
+    "hd_frame_graph_destroy": (C.c_int32, [C.c_void_p]),
     "hd_mem_info": (C.c_int32, [C.c_void_p, C.POINTER(hd_mem_info_t)]),
     "hd_frame_checksum": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, u64p, C.c_void_p]),
     "hd_edit_batch": (C.c_int32, [C.c_void_p, C.POINTER(hd_edit), C.c_uint32, C.c_uint32, C.POINTER(hd_edit_stats),
@@ -161,6 +166,8 @@ _SIGS = {
     "hd_trace_colors": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hd_trace_shadows": (C.c_int32, [C.c_void_p, C.POINTER(hd_ray_params), C.c_uint32, C.c_uint32, C.c_void_p,
                                      C.c_void_p, C.c_float, C.c_float, C.c_void_p]),
+    "hd_autotune_trace": (C.c_int32, [C.c_void_p, C.POINTER(hd_ray_params), C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_void_p, C.c_void_p, C.c_uint32, u32p, u32p, C.POINTER(C.c_float)]),
     "hd_set_trace_tile": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "hd_trace_count_bytes": (C.c_int32, [C.c_void_p, C.POINTER(hd_ray_params), C.c_uint32, C.c_uint32, C.c_void_p,
                                          C.c_void_p, C.c_float, C.c_float, u64p]),

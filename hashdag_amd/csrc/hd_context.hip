@@ -30,6 +30,7 @@ DevDag dev_dag(const hd_ctx* ctx) {
     d.levels = ctx->g.levels;
     d.leaf_level = ctx->g.leaf_level();
     d.color_tree_levels = ctx->g.color_tree_levels;
+    d.root_ptr = nullptr;
     return d;
 }
 
@@ -317,6 +318,8 @@ hd_status hd_load_colors(hd_ctx* ctx, const uint32_t* nodes, uint64_t n_nodes, c
     // colour-tree allocation tops used by edits: entries in use, unique leaves in use
     const uint32_t tops[2] = {(uint32_t)n_nodes, 0u};
     HD_HIP(ctx, hipMemcpyAsync(ctx->counters + 2, tops, sizeof(tops), hipMemcpyHostToDevice, ctx->stream));
+    const uint64_t top64 = mainWords;  // committed colour arena top (device copy)
+    HD_HIP(ctx, hipMemcpyAsync(ctx->counters + kArenaTopWord, &top64, 8, hipMemcpyHostToDevice, ctx->stream));
     HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->main = m;
     ctx->arena_top = mainWords;
@@ -325,6 +328,7 @@ hd_status hd_load_colors(hd_ctx* ctx, const uint32_t* nodes, uint64_t n_nodes, c
     ctx->n_cleaves = 0;
     ctx->h_leaves.clear();
     ctx->colors_loaded = true;
+    ctx->ptr_gen++;
     return HD_OK;
 }
 

@@ -58,10 +58,13 @@ struct hd_ctx {
     uint32_t* d_root = nullptr;  // device copy of the root (edit passes chain through it)
     hd::EditScratch edit;
     std::vector<hd::UndoFrame> undo, redo;
+    // bumped whenever a device buffer or launch shape that a captured graph bakes in changes
+    uint64_t ptr_gen = 0;
 };
 
 namespace hd {
 constexpr uint64_t kVpoolPad = 64;  // words after the virtual space (vector reads past a node end)
+constexpr uint32_t kArenaTopWord = 8;  // counters[8..9]: committed colour arena top (64-bit)
 
 void edit_free(hd_ctx* ctx);
 hd_status fail(hd_ctx* ctx, hd_status code, const std::string& msg);

@@ -1054,8 +1054,9 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
     }
 }
 
-__global__ void k_init_pass(EditPass P, uint32_t root, uint32_t first, uint32_t nEdits) {
+__global__ void k_init_pass(EditPass P, uint32_t root, uint32_t first, uint32_t nEdits, const uint32_t* dN) {
     // level-0 item = the root, carrying the edits [first, first + nEdits)
+    if (dN) nEdits = min(nEdits, *dN);
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nEdits) P.elist[0][t] = first + t;
     if (t == 0) {
@@ -1083,8 +1084,13 @@ __global__ void k_init_pass(EditPass P, uint32_t root, uint32_t first, uint32_t 
 __global__ void k_finish(EditPass P, uint32_t* root_out) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && *P.errors == 0) {
         const uint32_t r = P.items[0].nidx[0];
-        if (r == kInvalidNode) set_error(P.errors, ERR_EMPTY);
-        else *root_out = r;
+        if (r == kInvalidNode) {
+            set_error(P.errors, ERR_EMPTY);
+        } else {
+            *root_out = r;
+            // commit the colour arena top (a failed pass leaves the committed value)
+            *reinterpret_cast<unsigned long long*>(P.pool_top + kArenaTopWord) = *P.arena_top;
+        }
     }
 }
 
@@ -1162,6 +1168,7 @@ hd_status edit_reserve(hd_ctx* ctx, const uint32_t* items, const uint32_t* elist
     const uint32_t levels = ctx->g.levels, L = ctx->g.leaf_level();
     if (S.levels != levels) {
         hipDeviceSynchronize();
+        ctx->ptr_gen++;
         scratch_free(ctx);
         S.levels = levels;
         auto alloc = [&](void** p, uint64_t bytes) -> bool {
@@ -1178,6 +1185,7 @@ hd_status edit_reserve(hd_ctx* ctx, const uint32_t* items, const uint32_t* elist
     auto sync_once = [&]() {
         if (!synced) hipDeviceSynchronize();  // no pass may still use memory being freed
         synced = true;
+        ctx->ptr_gen++;
     };
     uint32_t maxItems = 0;
     for (uint32_t l = 0; l <= L; ++l) {
@@ -1224,6 +1232,7 @@ hd_status edit_reserve_undo(hd_ctx* ctx, uint64_t entries) {
     if (entries <= S.undo_cap) return HD_OK;
     const uint32_t cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(entries, kUndoCap), 1u << 30);
     hipDeviceSynchronize();
+    ctx->ptr_gen++;
     hipFree(S.undo_handles);
     hipFree(S.undo_descs);
     hipFree(S.undo_next);
@@ -1250,6 +1259,7 @@ hd_status edit_ensure_edits(hd_ctx* ctx, uint32_t n) {
         hipDeviceSynchronize();
         hipFree(S.edits_dev);
     }
+    ctx->ptr_gen++;
     S.edits_dev = ne;
     S.edits_cap = cap;
     return HD_OK;
@@ -1313,12 +1323,13 @@ static void fill_pass(hd_ctx* ctx, EditPass& P) {
 }
 
 // Enqueues one pass over edits [first, first + n) of the uploaded edit array.
-hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s) {
+// dN (optional): the edit count is read from device memory (at most n), for graph replays.
+hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s, const uint32_t* dN) {
     EditPass P{};
     fill_pass(ctx, P);
-    P.single_edit = n == 1 ? 1u : 0u;
+    P.single_edit = (n == 1 && !dN) ? 1u : 0u;
     const uint32_t D = P.levels, L = D - 2;
-    k_init_pass<<<(n + 255) / 256, 256, 0, s>>>(P, 0xFFFFFFFFu, first, n);
+    k_init_pass<<<(std::max<uint32_t>(n, 1) + 255) / 256, 256, 0, s>>>(P, 0xFFFFFFFFu, first, n, dN);
     // the root index lives in ctx->d_root (device) so passes chain without a host round trip
     hipMemcpyAsync(&P.items[0].old_[0], ctx->d_root, 4, hipMemcpyDeviceToDevice, s);
     hipMemcpyAsync(&P.items[0].orig[0], ctx->d_root, 4, hipMemcpyDeviceToDevice, s);

@@ -14,8 +14,11 @@ namespace hd {
 
 hd_status edit_reserve(hd_ctx* ctx, const uint32_t* items, const uint32_t* elist, uint64_t colors);
 void edit_free(hd_ctx* ctx);
-hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s);
+hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s,
+                       const uint32_t* dN = nullptr);
 hd_status edit_ensure_edits(hd_ctx* ctx, uint32_t n);
+hd_status enqueue_frame_from_device(hd_ctx* ctx, const hd_ray_params* d_params, uint32_t W, uint32_t H,
+                                    uint32_t* d_paths, uint32_t* d_colors, float bias, float fog, hipStream_t s);
 hd_status edit_reserve_undo(hd_ctx* ctx, uint64_t entries);
 hd_status edit_scatter_leaves(hd_ctx* ctx, const uint32_t* handles, const LeafDesc* descs, uint32_t n);
 void edit_restore_leaves(hd_ctx* ctx, hipStream_t s);
@@ -44,8 +47,9 @@ __device__ __forceinline__ bool touch_voxel(const DevEdit& e, float x, float y, 
 // One block per editor; threads stride over the (x, z) columns of its bounding square; the
 // touched y values of a column form an interval (the predicate is monotone in |y - c.y|),
 // found by two binary searches around the centre row.
-__global__ void k_count_voxels(const DevEdit* __restrict__ edits, uint32_t n, uint32_t E,
-                               unsigned long long* __restrict__ out) {
+__global__ void k_count_voxels(const DevEdit* __restrict__ edits, uint32_t n, const uint32_t* __restrict__ dN,
+                               uint32_t E, unsigned long long* __restrict__ out) {
+    if (dN) n = min(n, *dN);  // graph replays read the edit count from device memory
     for (uint32_t ei = blockIdx.x; ei < n; ei += gridDim.x) {
         const DevEdit e = edits[ei];
         const float r = e.radius;
@@ -88,6 +92,7 @@ __global__ void k_reset(uint32_t* counters, uint64_t* stats) {
         counters[1] = 0;  // errors
         counters[4] = 0;  // undo log
         for (int i = 0; i < 8; ++i) stats[i] = 0;
+        stats[3] = *reinterpret_cast<const uint64_t*>(counters + kArenaTopWord);  // committed arena top
     }
 }
 
@@ -135,6 +140,7 @@ static hd_status grow_arena(hd_ctx* ctx, uint64_t need) {
     HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     hipFree(ctx->arena);
     ctx->arena = na;
+    ctx->ptr_gen++;
     ctx->cfg.color_arena_words = ncap;
     return HD_OK;
 }
@@ -143,6 +149,7 @@ static hd_status grow_arena(hd_ctx* ctx, uint64_t need) {
 // least `nodes` entries and `leaves` slots fit.
 static hd_status grow_color_tables(hd_ctx* ctx, uint64_t nodes, uint64_t leaves) {
     HD_HIP(ctx, hipDeviceSynchronize());
+    ctx->ptr_gen++;
     if (nodes > ctx->cfg.color_nodes_cap) {
         const uint64_t cap = std::min<uint64_t>(std::max<uint64_t>(nodes + nodes / 2, 2ull * ctx->cfg.color_nodes_cap), 0xFFFFFFF0u);
         if (cap < nodes) return fail(ctx, HD_ERR_COLOR_FULL, "colour tree exceeds 2^32 entries");
@@ -167,6 +174,37 @@ static hd_status grow_color_tables(hd_ctx* ctx, uint64_t nodes, uint64_t leaves)
     return HD_OK;
 }
 
+// The undo frame of a committed pass: the touched slots with their previous and new leaves.
+static hd_status capture_frame(hd_ctx* ctx, uint32_t nu, UndoFrame& frame, hipStream_t s) {
+    const EditScratch& S = ctx->edit;
+    frame.handles.resize(nu);
+    frame.prev.resize(nu);
+    frame.next.resize(nu);
+    if (nu) {
+        HD_HIP(ctx, hipMemcpyAsync(frame.handles.data(), S.undo_handles, nu * 4, hipMemcpyDeviceToHost, s));
+        HD_HIP(ctx, hipMemcpyAsync(frame.prev.data(), S.undo_descs, nu * sizeof(LeafDesc), hipMemcpyDeviceToHost, s));
+        HD_HIP(ctx, edit_gather_next(ctx, nu, frame.next.data(), s));
+        HD_HIP(ctx, hipStreamSynchronize(s));
+    }
+    return HD_OK;
+}
+
+// Colour tree, default scratch and undo log of a context's first edit.
+static hd_status edit_prepare(hd_ctx* ctx) {
+    hd_status st = ensure_color_tree(ctx);
+    if (st != HD_OK) return st;
+    if (!ctx->edit.level_mem[0]) {
+        uint32_t items[kMaxEditLevels], elist[kMaxEditLevels];
+        for (uint32_t l = 0; l < kMaxEditLevels; ++l) {
+            items[l] = 1u << 16;
+            elist[l] = 1u << 18;
+        }
+        st = edit_reserve(ctx, items, elist, 1u << 22);
+        if (st != HD_OK) return st;
+    }
+    return edit_reserve_undo(ctx, kUndoCap);
+}
+
 static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s, Readback& rb, UndoFrame* frame) {
     const bool colors = ctx->colors_loaded && ctx->g.leaf_level() > ctx->g.color_tree_levels;
     for (int attempt = 0; attempt < 40; ++attempt) {
@@ -174,11 +212,11 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
         if (st != HD_OK) return st;
         EditScratch& S = ctx->edit;
         k_reset<<<1, 1, 0, s>>>(ctx->counters, S.dstats);
-        HD_HIP(ctx, hipMemcpyAsync(S.dstats + 3, &ctx->arena_top, 8, hipMemcpyHostToDevice, s));
         st = enqueue_pass(ctx, first, n, colors, s);
         if (st != HD_OK) return st;
         if (n > 1) {
-            k_count_voxels<<<std::min<uint32_t>(n, 4096), 256, 0, s>>>(S.edits_dev + first, n, 1u << ctx->g.levels,
+            k_count_voxels<<<std::min<uint32_t>(n, 4096), 256, 0, s>>>(S.edits_dev + first, n, nullptr,
+                                                                      1u << ctx->g.levels,
                                                                       reinterpret_cast<unsigned long long*>(S.dstats + 0));
         }
         HD_HIP(ctx, hipMemcpyAsync(rb.counters, ctx->counters, sizeof(rb.counters), hipMemcpyDeviceToHost, s));
@@ -247,18 +285,7 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
         ctx->arena_top = rb.stats[3];
         ctx->n_cnodes = rb.counters[2];
         ctx->n_cleaves = rb.counters[3];
-        if (frame) {
-            const uint32_t nu = rb.counters[4];
-            frame->handles.resize(nu);
-            frame->prev.resize(nu);
-            frame->next.resize(nu);
-            if (nu) {
-                HD_HIP(ctx, hipMemcpyAsync(frame->handles.data(), S.undo_handles, nu * 4, hipMemcpyDeviceToHost, s));
-                HD_HIP(ctx, hipMemcpyAsync(frame->prev.data(), S.undo_descs, nu * sizeof(LeafDesc), hipMemcpyDeviceToHost, s));
-                HD_HIP(ctx, edit_gather_next(ctx, nu, frame->next.data(), s));
-                HD_HIP(ctx, hipStreamSynchronize(s));
-            }
-        }
+        if (frame) return capture_frame(ctx, rb.counters[4], *frame, s);
         return HD_OK;
     }
     return fail(ctx, HD_ERR_OOM, "edit scratch could not be grown enough");
@@ -281,18 +308,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
     for (uint32_t i = 0; i < n; ++i)
         if (edits[i].kind > EDIT_SPHERE_PAINT || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
             return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");
-    hd_status st = ensure_color_tree(ctx);
-    if (st != HD_OK) return st;
-    if (!ctx->edit.level_mem[0]) {
-        uint32_t items[kMaxEditLevels], elist[kMaxEditLevels];
-        for (uint32_t l = 0; l < kMaxEditLevels; ++l) {
-            items[l] = 1u << 16;
-            elist[l] = 1u << 18;
-        }
-        st = edit_reserve(ctx, items, elist, 1u << 22);
-        if (st != HD_OK) return st;
-    }
-    st = edit_reserve_undo(ctx, kUndoCap);
+    hd_status st = edit_prepare(ctx);
     if (st != HD_OK) return st;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     // editors to the device
@@ -398,6 +414,175 @@ hd_status hd_reserve_edit_scratch(hd_ctx* ctx, uint32_t items_per_level, uint32_
         elist[l] = edit_list_words;
     }
     return edit_reserve(ctx, items, elist, colors);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Frame graph: one fused edit pass and the trace of the resulting DAG, captured once into a HIP
+// graph and replayed with new edits and a new camera (both staged in pinned memory and copied
+// by the graph's own memcpy nodes). A replay whose pass overflows a capacity is rolled back
+// and re-run eagerly (which grows the capacity); the graph is re-captured when any buffer it
+// bakes in has been reallocated.
+// ---------------------------------------------------------------------------------------------
+struct hd_frame_graph {
+    hd_ctx* ctx = nullptr;
+    uint32_t max_edits = 0, width = 0, height = 0;
+    uint32_t* paths = nullptr;
+    uint32_t* colors = nullptr;
+    float bias = 1.f, fog = 0.f;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t gen = ~0ull;
+    DevEdit* h_edits = nullptr;      // pinned staging
+    uint32_t* h_n = nullptr;
+    hd_ray_params* h_params = nullptr;
+    Readback* h_rb = nullptr;
+    uint32_t* d_n = nullptr;         // device copies read by the replay
+    hd_ray_params* d_params = nullptr;
+};
+
+static void graph_drop(hd_frame_graph* g) {
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    g->exec = nullptr;
+    g->graph = nullptr;
+}
+
+static hd_status graph_capture(hd_frame_graph* g) {
+    hd_ctx* ctx = g->ctx;
+    graph_drop(g);
+    EditScratch& S = ctx->edit;
+    const bool colors = ctx->colors_loaded && ctx->g.leaf_level() > ctx->g.color_tree_levels;
+    hipStream_t s = g->stream;
+    HD_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipMemcpyAsync(g->d_n, g->h_n, 4, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(S.edits_dev, g->h_edits, (uint64_t)g->max_edits * sizeof(DevEdit), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(g->d_params, g->h_params, sizeof(hd_ray_params), hipMemcpyHostToDevice, s);
+    k_reset<<<1, 1, 0, s>>>(ctx->counters, S.dstats);
+    hd_status st = enqueue_pass(ctx, 0, g->max_edits, colors, s, g->d_n);
+    k_count_voxels<<<std::min<uint32_t>(g->max_edits, 4096), 256, 0, s>>>(
+        S.edits_dev, g->max_edits, g->d_n, 1u << ctx->g.levels, reinterpret_cast<unsigned long long*>(S.dstats + 0));
+    if (st == HD_OK) st = enqueue_frame_from_device(ctx, g->d_params, g->width, g->height, g->paths, g->colors, g->bias,
+                                                    g->fog, s);
+    hipMemcpyAsync(g->h_rb->counters, ctx->counters, sizeof(g->h_rb->counters), hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(g->h_rb->stats, S.dstats, sizeof(g->h_rb->stats), hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(&g->h_rb->root, ctx->d_root, 4, hipMemcpyDeviceToHost, s);
+    const hipError_t e = hipStreamEndCapture(s, &g->graph);
+    if (st != HD_OK) return st;
+    if (e != hipSuccess) return hip_fail(ctx, e, "frame graph capture");
+    HD_HIP(ctx, hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    g->gen = ctx->ptr_gen;
+    return HD_OK;
+}
+
+extern "C" {
+
+hd_status hd_frame_graph_create(hd_ctx* ctx, uint32_t max_edits, uint32_t width, uint32_t height, uint32_t* d_paths,
+                                uint32_t* d_colors, float shadow_bias, float fog_density, hd_frame_graph** out) {
+    if (!ctx || !out || !d_paths || !d_colors || !width || !height || !max_edits)
+        return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    if (!ctx->loaded || !ctx->colors_loaded) return fail(ctx, HD_ERR_INVALID, "no DAG or colours loaded");
+    auto* g = new hd_frame_graph();
+    g->ctx = ctx;
+    g->max_edits = max_edits;
+    g->width = width;
+    g->height = height;
+    g->paths = d_paths;
+    g->colors = d_colors;
+    g->bias = shadow_bias;
+    g->fog = fog_density;
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc(&g->h_edits, (uint64_t)max_edits * sizeof(DevEdit));
+    if (e == hipSuccess) e = hipHostMalloc(&g->h_n, 4);
+    if (e == hipSuccess) e = hipHostMalloc(&g->h_params, sizeof(hd_ray_params));
+    if (e == hipSuccess) e = hipHostMalloc(&g->h_rb, sizeof(Readback));
+    if (e == hipSuccess) e = hipMalloc(&g->d_n, 4);
+    if (e == hipSuccess) e = hipMalloc(&g->d_params, sizeof(hd_ray_params));
+    if (e != hipSuccess) {
+        hd_frame_graph_destroy(g);
+        return hip_fail(ctx, e, "frame graph buffers");
+    }
+    std::memset(g->h_edits, 0, (uint64_t)max_edits * sizeof(DevEdit));
+    *g->h_n = 0;
+    hd_status st = edit_prepare(ctx);
+    if (st == HD_OK) st = edit_ensure_edits(ctx, max_edits);
+    if (st != HD_OK) {
+        hd_frame_graph_destroy(g);
+        return st;
+    }
+    *out = g;
+    return HD_OK;
+}
+
+hd_status hd_frame_graph_destroy(hd_frame_graph* g) {
+    if (!g) return HD_OK;
+    if (g->stream) hipStreamSynchronize(g->stream);
+    graph_drop(g);
+    hipHostFree(g->h_edits);
+    hipHostFree(g->h_n);
+    hipHostFree(g->h_params);
+    hipHostFree(g->h_rb);
+    hipFree(g->d_n);
+    hipFree(g->d_params);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+    return HD_OK;
+}
+
+hd_status hd_frame_graph_launch(hd_frame_graph* g, const hd_edit* edits, uint32_t n, const hd_ray_params* params,
+                                hd_edit_stats* stats, uint32_t* replayed) {
+    if (!g || !params || (n && !edits)) return fail(g ? g->ctx : nullptr, HD_ERR_INVALID, "bad arguments");
+    hd_ctx* ctx = g->ctx;
+    if (n > g->max_edits) return fail(ctx, HD_ERR_INVALID, "more edits than the graph was built for");
+    for (uint32_t i = 0; i < n; ++i)
+        if (edits[i].kind > EDIT_SPHERE_PAINT || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
+            return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");
+    // the same free-space guarantees as an eager pass, then (re)capture if buffers moved
+    hd_status st = grow_arena(ctx, 1u << 22);
+    if (st == HD_OK) st = edit_ensure_edits(ctx, g->max_edits);
+    if (st != HD_OK) return st;
+    if (!g->exec || g->gen != ctx->ptr_gen) {
+        st = graph_capture(g);
+        if (st != HD_OK) return st;
+    }
+    for (uint32_t i = 0; i < n; ++i) g->h_edits[i] = make_edit(edits[i]);
+    *g->h_n = n;
+    *g->h_params = *params;
+    HD_HIP(ctx, hipGraphLaunch(g->exec, g->stream));
+    HD_HIP(ctx, hipStreamSynchronize(g->stream));
+    const Readback& rb = *g->h_rb;
+    if (rb.counters[1]) {
+        // a capacity ran out: roll the leaf slots back and run the frame eagerly (which grows it)
+        if (rb.counters[4]) {
+            edit_restore_leaves(ctx, g->stream);
+            HD_HIP(ctx, hipStreamSynchronize(g->stream));
+        }
+        if (replayed) *replayed = 0;
+        st = hd_edit_batch(ctx, edits, n, 1, stats, g->stream);
+        if (st == HD_OK) st = hd_trace_paths(ctx, params, g->width, g->height, g->paths, g->stream);
+        if (st == HD_OK) st = hd_trace_colors(ctx, g->width, g->height, g->paths, g->colors, g->stream);
+        if (st == HD_OK)
+            st = hd_trace_shadows(ctx, params, g->width, g->height, g->paths, g->colors, g->bias, g->fog, g->stream);
+        if (st == HD_OK) HD_HIP(ctx, hipStreamSynchronize(g->stream));
+        return st;
+    }
+    // commit, as a successful eager pass does
+    UndoFrame frame;
+    frame.prev_root = ctx->root;
+    ctx->root = rb.root;
+    frame.new_root = rb.root;
+    ctx->arena_top = rb.stats[3];
+    ctx->n_cnodes = rb.counters[2];
+    ctx->n_cleaves = rb.counters[3];
+    st = capture_frame(ctx, rb.counters[4], frame, g->stream);
+    if (st != HD_OK) return st;
+    ctx->undo.push_back(std::move(frame));
+    ctx->redo.clear();
+    if (stats) *stats = hd_edit_stats{rb.stats[0], rb.stats[1], rb.counters[4], rb.root, 1u};
+    if (replayed) *replayed = 1;
+    return HD_OK;
 }
 
 }  // extern "C"

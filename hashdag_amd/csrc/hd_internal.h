@@ -108,6 +108,7 @@ struct DevDag {
     uint32_t levels;
     uint32_t leaf_level;
     uint32_t color_tree_levels;
+    const uint32_t* root_ptr;  // when set, the root is read from device memory (graph replays)
 };
 
 struct DevColors {

@@ -248,10 +248,12 @@ __device__ __forceinline__ void flush_count(uint64_t v, unsigned long long* coun
 
 // tracer.cu:145-252
 template <bool kCount>
-__global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, uint32_t W, uint32_t H,
+__global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W, uint32_t H,
                                                         uint4* __restrict__ out, uint32_t tileW, uint32_t tileH,
                                                         unsigned long long* counter) {
     const Stack st = stack_of(dag.levels);
+    if (pmem) p = *pmem;
+    if (dag.root_ptr) dag.root = *dag.root_ptr;
     uint32_t x, y;
     pixel_of(tileW, tileH, x, y);
     uint64_t bytes = 0;
@@ -373,6 +375,7 @@ template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_trace_colors(DevDag dag, DevColors colors, uint32_t W, uint32_t H,
                                                          const uint4* __restrict__ paths, uint32_t* __restrict__ out,
                                                          uint32_t tileW, uint32_t tileH, unsigned long long* counter) {
+    if (dag.root_ptr) dag.root = *dag.root_ptr;
     uint32_t x, y;
     pixel_of(tileW, tileH, x, y);
     uint64_t bytes = 0;
@@ -483,11 +486,13 @@ __device__ __forceinline__ void sun_direction(float& x, float& y, float& z) {
 }
 
 template <bool kCount>
-__global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, uint32_t W, uint32_t H,
+__global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W, uint32_t H,
                                                           const uint4* __restrict__ paths, uint32_t* __restrict__ colors,
                                                           float bias, float fog, uint32_t tileW, uint32_t tileH,
                                                           unsigned long long* counter) {
     const Stack st = stack_of(dag.levels);
+    if (pmem) p = *pmem;
+    if (dag.root_ptr) dag.root = *dag.root_ptr;
     uint32_t x, y;
     pixel_of(tileW, tileH, x, y);
     uint64_t bytes = 0;
@@ -577,6 +582,30 @@ inline hipStream_t stream_of(hd_ctx*, void* s) { return reinterpret_cast<hipStre
 }  // namespace
 }  // namespace hd
 
+namespace hd {
+// The three tracer kernels with the ray parameters and the root read from device memory, so a
+// captured graph traces whatever camera and DAG state the replay finds there.
+hd_status enqueue_frame_from_device(hd_ctx* ctx, const hd_ray_params* d_params, uint32_t W, uint32_t H,
+                                    uint32_t* d_paths, uint32_t* d_colors, float bias, float fog, hipStream_t s) {
+    static_assert(sizeof(RayP) == sizeof(hd_ray_params), "ray parameter layouts differ");
+    DevDag dag = dev_dag(ctx);
+    dag.root_ptr = ctx->d_root;
+    const RayP* pm = reinterpret_cast<const RayP*>(d_params);
+    const RayP none{};
+    const dim3 grid = grid_of(ctx, W, H);
+    k_trace_paths<false><<<grid, kBlock, stack_bytes(ctx->g.levels), s>>>(dag, none, pm, W, H,
+                                                                         reinterpret_cast<uint4*>(d_paths), ctx->tile_w,
+                                                                         ctx->tile_h, nullptr);
+    k_trace_colors<false><<<grid, kBlock, 0, s>>>(dag, dev_colors(ctx), W, H, reinterpret_cast<const uint4*>(d_paths),
+                                                  d_colors, ctx->tile_w, ctx->tile_h, nullptr);
+    k_trace_shadows<false><<<grid, kBlock, stack_bytes(ctx->g.levels), s>>>(
+        dag, none, pm, W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog, ctx->tile_w, ctx->tile_h,
+        nullptr);
+    HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+}  // namespace hd
+
 using namespace hd;
 
 extern "C" {
@@ -590,6 +619,7 @@ hd_status hd_set_trace_tile(hd_ctx* ctx, uint32_t tw, uint32_t th) {
     if (tw * th != 64) return fail(ctx, HD_ERR_INVALID, "tile must cover 64 pixels (one wavefront)");
     ctx->tile_w = tw;
     ctx->tile_h = th;
+    ctx->ptr_gen++;
     return HD_OK;
 }
 
@@ -599,7 +629,7 @@ hd_status hd_trace_paths(hd_ctx* ctx, const hd_ray_params* params, uint32_t W, u
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
     if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
     k_trace_paths<false><<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), stream_of(ctx, stream)>>>(
-        dev_dag(ctx), to_rayp(params), W, H, reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h, nullptr);
+        dev_dag(ctx), to_rayp(params), nullptr, W, H, reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h, nullptr);
     HD_HIP(ctx, hipGetLastError());
     return HD_OK;
 }
@@ -621,9 +651,62 @@ hd_status hd_trace_shadows(hd_ctx* ctx, const hd_ray_params* params, uint32_t W,
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
     if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
     k_trace_shadows<false><<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), stream_of(ctx, stream)>>>(
-        dev_dag(ctx), to_rayp(params), W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
+        dev_dag(ctx), to_rayp(params), nullptr, W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
         ctx->tile_w, ctx->tile_h, nullptr);
     HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+
+hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n_params, uint32_t W, uint32_t H,
+                            uint32_t* d_paths, uint32_t* d_colors, uint32_t reps, uint32_t* best_w, uint32_t* best_h,
+                            float* best_ms) {
+    if (!ctx || !params || !n_params || !d_paths || !d_colors || W == 0 || H == 0)
+        return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    if (!ctx->loaded || !ctx->colors_loaded) return fail(ctx, HD_ERR_INVALID, "no DAG or colours loaded");
+    static const uint32_t shapes[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}, {2, 32}, {64, 1}};
+    hipEvent_t e0, e1;
+    HD_HIP(ctx, hipEventCreate(&e0));
+    HD_HIP(ctx, hipEventCreate(&e1));
+    float bestMs = 1e30f;
+    uint32_t bw = ctx->tile_w, bh = ctx->tile_h;
+    reps = std::max<uint32_t>(reps, 1);
+    for (const auto& sh : shapes) {
+        ctx->tile_w = sh[0];
+        ctx->tile_h = sh[1];
+        float total = 0;
+        for (uint32_t rep = 0; rep <= reps; ++rep) {  // rep 0 warms up
+            HD_HIP(ctx, hipEventRecord(e0, ctx->stream));
+            for (uint32_t k = 0; k < n_params; ++k) {
+                hd_status st = hd_trace_paths(ctx, &params[k], W, H, d_paths, ctx->stream);
+                if (st == HD_OK) st = hd_trace_colors(ctx, W, H, d_paths, d_colors, ctx->stream);
+                if (st == HD_OK) st = hd_trace_shadows(ctx, &params[k], W, H, d_paths, d_colors, 1.0f, 0.0f, ctx->stream);
+                if (st != HD_OK) {
+                    hipEventDestroy(e0);
+                    hipEventDestroy(e1);
+                    return st;
+                }
+            }
+            HD_HIP(ctx, hipEventRecord(e1, ctx->stream));
+            HD_HIP(ctx, hipEventSynchronize(e1));
+            float ms = 0;
+            HD_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+            if (rep) total += ms;
+        }
+        total /= (float)reps;
+        if (total < bestMs) {
+            bestMs = total;
+            bw = sh[0];
+            bh = sh[1];
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    ctx->tile_w = bw;
+    ctx->tile_h = bh;
+    ctx->ptr_gen++;
+    if (best_w) *best_w = bw;
+    if (best_h) *best_h = bh;
+    if (best_ms) *best_ms = bestMs / (float)n_params;
     return HD_OK;
 }
 
@@ -634,7 +717,7 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
     if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
     const dim3 grid = grid_of(ctx, W, H);
     HD_HIP(ctx, hipMemsetAsync(ctx->byte_counters, 0, 3 * sizeof(unsigned long long), ctx->stream));
-    k_trace_paths<true><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), W, H,
+    k_trace_paths<true><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), nullptr, W, H,
                                                           reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h,
                                                           ctx->byte_counters + 0);
     HD_HIP(ctx, hipGetLastError());
@@ -644,7 +727,7 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
                                                                ctx->tile_w, ctx->tile_h, ctx->byte_counters + 1);
         HD_HIP(ctx, hipGetLastError());
     }
-    k_trace_shadows<true><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), W, H,
+    k_trace_shadows<true><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), nullptr, W, H,
                                                             reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
                                                             ctx->tile_w, ctx->tile_h, ctx->byte_counters + 2);
     HD_HIP(ctx, hipGetLastError());

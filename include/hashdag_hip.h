@@ -132,6 +132,14 @@ hd_status hd_trace_shadows(hd_ctx* ctx, const hd_ray_params* params, uint32_t wi
  * at (4,64) blocks, dag_tracer.cu:121-122). */
 hd_status hd_set_trace_tile(hd_ctx* ctx, uint32_t tile_w, uint32_t tile_h);
 
+/* Autotuner for the tracer launch shape: times full frames (paths + colours + shadows) over
+ * n_params cameras for each wavefront tile shape (8x8, 16x4, 4x16, 32x2, 2x32, 64x1), keeps
+ * the fastest in ctx and returns it with its ms per frame. No counterpart in the reference
+ * (fixed (4,64) blocks, dag_tracer.cu:121-122). */
+hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n_params, uint32_t width,
+                            uint32_t height, uint32_t* d_paths, uint32_t* d_colors, uint32_t reps, uint32_t* best_w,
+                            uint32_t* best_h, float* best_ms);
+
 /* Algorithmic HBM bytes of the last traced frame per kernel (paths, colors, shadows), filled
  * by an instrumented launch: hd_trace_count_bytes runs the three kernels once with counters.
  * No counterpart in the reference (a measurement hook for the roofline of the three tracer
@@ -181,6 +189,20 @@ hd_status hd_reserve_edit_scratch(hd_ctx* ctx, uint32_t items_per_level, uint32_
  * re-appended level by level bottom-up with their children remapped, full nodes first.
  * Clears the undo history. stats (may be NULL): live nodes, words before, words after, 0. */
 hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], void* stream);
+
+/* Frame graph: one fused edit pass (up to max_edits editors) followed by the trace of the
+ * resulting DAG (paths, colours, shadows into d_paths / d_colors), captured once as a HIP graph
+ * and replayed per call with new edits and camera. The replay is committed like
+ * hd_edit_batch(fused = 1) + hd_trace_*; a replay that runs out of a capacity is rolled back
+ * and run eagerly instead (*replayed = 0), and the graph is re-captured when a buffer it uses
+ * has been reallocated. No counterpart in the reference (one edit per UI event, then a frame:
+ * engine.cpp:575-718). */
+typedef struct hd_frame_graph hd_frame_graph;
+hd_status hd_frame_graph_create(hd_ctx* ctx, uint32_t max_edits, uint32_t width, uint32_t height, uint32_t* d_paths,
+                                uint32_t* d_colors, float shadow_bias, float fog_density, hd_frame_graph** out);
+hd_status hd_frame_graph_launch(hd_frame_graph* graph, const hd_edit* edits, uint32_t n, const hd_ray_params* params,
+                                hd_edit_stats* stats, uint32_t* replayed);
+hd_status hd_frame_graph_destroy(hd_frame_graph* graph);
 
 /* Request serving: trace and edit requests executed in arrival order with continuous
  * batching on one device-resident DAG (runs of edit requests fuse into one pass; runs of trace

@@ -118,3 +118,35 @@ def test_fog_close_to_oracle():
     ocols = Oracle.trace_colors(od, oc, opaths, W, H)
     o8 = Oracle.trace_shadows(od, oparams, opaths, ocols, W, H, 1.0, 2000.0).view(np.uint8).reshape(H, W, 4).astype(int)
     assert np.abs(g8 - o8).max() <= 1
+
+
+@pytest.mark.parametrize("tile", [(16, 4), (4, 16), (64, 1), "auto"])
+def test_tile_shapes_bit_exact(tile):
+    """Every wavefront tile shape (and the autotuner's pick) traces the same frame."""
+    import torch
+    from hashdag_amd import DAGInfo, DAGTracer, ray_params
+    from hashdag_amd.tracer import autotune
+    from hashdag_amd.workload import cameras
+    from oracle import Oracle
+
+    p, sc, g, dag, od, oc = _setup("d13")
+    W, H = 100, 60  # not a multiple of the tile sizes
+    info = DAGInfo.unit(g.levels)
+    tracer = DAGTracer(W, H)
+    cams = cameras(p, 2, seed=21)
+    prms = [ray_params(c, info, g.levels, W, H) for c in cams]
+    if tile == "auto":
+        got = autotune(tracer, dag, prms, reps=1)
+        assert got["tile"] in {"8x8", "16x4", "4x16", "32x2", "2x32", "64x1"}
+    else:
+        tracer.set_tile(dag, *tile)
+    for cam, prm in zip(cams, prms):
+        tracer.resolve_paths(cam, info, dag, params=prm)
+        tracer.resolve_colors(dag)
+        tracer.resolve_shadows(cam, info, dag, 1.0, 0.0, params=prm)
+        torch.cuda.synchronize()
+        gpu = tracer.colors.cpu().numpy().view(np.uint32)
+        op = Oracle.ray_params(cam, info, g.levels, W, H)
+        paths = Oracle.trace_paths(od, op, W, H)
+        ref = Oracle.trace_shadows(od, op, paths, Oracle.trace_colors(od, oc, paths, W, H), W, H, 1.0, 0.0)
+        assert np.array_equal(gpu, ref)
