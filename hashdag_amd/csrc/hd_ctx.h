@@ -52,6 +52,7 @@ struct hd_ctx {
 
     // tracer
     uint32_t tile_w = 8, tile_h = 8;
+    uint32_t fetch_mode = 0;  // tracer node fetch: 0 = pointer then header, 1 = whole node
     unsigned long long* byte_counters = nullptr;  // 3 counters for instrumented launches
 
     // edits

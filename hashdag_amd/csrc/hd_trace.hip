@@ -117,11 +117,29 @@ __device__ __forceinline__ uint32_t intersection_mask(uint32_t levels, uint32_t 
     return mask;
 }
 
-// A node's header and the 8 words after it, fetched with independent loads (one round trip;
-// the pool has padding words after the virtual space, so reading past the last node is safe).
+// Node fetch strategies (picked per context by the autotuner):
+//   kFetchPointer  a descent loads the child pointer, then the child's header (two dependent
+//                  loads of one word each)
+//   kFetchNode     a descent loads the child's whole node (header + up to 8 children) with
+//                  4-byte-aligned vector loads, so the next descent needs no pointer load; the
+//                  pool has padding words after the virtual space, so reading past the last
+//                  node is safe
+constexpr int kFetchPointer = 0, kFetchNode = 1;
+
+typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+
 __device__ __forceinline__ void load_node(const uint32_t* __restrict__ vpool, uint32_t index, uint32_t nd[9]) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) nd[k] = vpool[index + k];
+    const u32x4_a4 a = *reinterpret_cast<const u32x4_a4*>(vpool + index);
+    const u32x4_a4 b = *reinterpret_cast<const u32x4_a4*>(vpool + index + 4);
+    nd[0] = a.x;
+    nd[1] = a.y;
+    nd[2] = a.z;
+    nd[3] = a.w;
+    nd[4] = b.x;
+    nd[5] = b.y;
+    nd[6] = b.z;
+    nd[7] = b.w;
+    nd[8] = vpool[index + 8];
 }
 
 // nd[k] for a lane-varying k in 1..8, as a select tree over constant indices (a dynamic
@@ -159,9 +177,7 @@ __device__ __forceinline__ Stack stack_of(uint32_t levels) {
 
 // DFS over the DAG (tracer.cu:166-249 ordered first hit; tracer.cu:458-542 any hit).
 // Returns true on a hit; (px, py, pz) is the voxel path on a hit, (0,0,0) on a miss.
-// The current node's words are held in registers, so a descent costs one dependent fetch
-// (the child's words) instead of two (the child pointer, then the child's header).
-template <bool kAnyHit, bool kCount>
+template <bool kAnyHit, bool kCount, int kFetch>
 __device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, uint32_t order, uint32_t& px,
                                         uint32_t& py, uint32_t& pz, Stack st, uint64_t& bytes) {
     const uint32_t* __restrict__ vpool = dag.vpool;
@@ -170,8 +186,13 @@ __device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, u
     px = py = pz = 0;
     uint32_t index = dag.root;
     uint32_t nd[9];
-    load_node(vpool, index, nd);
-    uint32_t childMask = nd[0] & 0xFFu;
+    uint32_t childMask;
+    if (kFetch == kFetchNode) {
+        load_node(vpool, index, nd);
+        childMask = nd[0] & 0xFFu;
+    } else {
+        childMask = vpool[index] & 0xFFu;
+    }
     if (kCount) bytes += 4;
     uint32_t visit = childMask & intersection_mask<true>(levels, 0, 0, 0, 0, o, d, inv);
     uint64_t leaf = 0;
@@ -195,7 +216,7 @@ __device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, u
             level = newLevel;
             if (level < leafLevel) {
                 index = level == 0 ? dag.root : st.at_idx(level);
-                load_node(vpool, index, nd);
+                if (kFetch == kFetchNode) load_node(vpool, index, nd);
             }
         }
         const uint32_t next = kAnyHit ? (31u - (uint32_t)__builtin_clz(visit)) : next_child(order, visit);
@@ -208,12 +229,18 @@ __device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, u
         level++;
         if (level == levels) return true;
         if (level < leafLevel) {
-            index = pick(nd, child_off(childMask, next));
-            load_node(vpool, index, nd);
-            childMask = nd[0] & 0xFFu;
+            if (kFetch == kFetchNode) {
+                index = pick(nd, child_off(childMask, next));
+                load_node(vpool, index, nd);
+                childMask = nd[0] & 0xFFu;
+            } else {
+                index = vpool[index + child_off(childMask, next)];
+                childMask = vpool[index] & 0xFFu;
+            }
             if (kCount) bytes += 8;
         } else if (level == leafLevel) {
-            const uint32_t addr = pick(nd, child_off(childMask, next));
+            const uint32_t addr =
+                kFetch == kFetchNode ? pick(nd, child_off(childMask, next)) : vpool[index + child_off(childMask, next)];
             leaf = *reinterpret_cast<const uint64_t*>(vpool + addr);
             childMask = leaf_first_mask(leaf);
             if (kCount) bytes += 12;
@@ -247,7 +274,7 @@ __device__ __forceinline__ void flush_count(uint64_t v, unsigned long long* coun
 }
 
 // tracer.cu:145-252
-template <bool kCount>
+template <bool kCount, int kFetch>
 __global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W, uint32_t H,
                                                         uint4* __restrict__ out, uint32_t tileW, uint32_t tileH,
                                                         unsigned long long* counter) {
@@ -267,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, cons
         const F3 inv{(float)(1.0 / (double)d.x), (float)(1.0 / (double)d.y), (float)(1.0 / (double)d.z)};
         const uint32_t order = (d.x < 0.f ? 4u : 0u) + (d.y < 0.f ? 2u : 0u) + (d.z < 0.f ? 1u : 0u);
         uint32_t vx, vy, vz;
-        descend<false, kCount>(dag, o, d, inv, order, vx, vy, vz, st, bytes);
+        descend<false, kCount, kFetch>(dag, o, d, inv, order, vx, vy, vz, st, bytes);
         out[(uint64_t)(H - 1 - y) * W + x] = make_uint4(vx, vy, vz, 0u);
         if (kCount) bytes += 16;
     }
@@ -485,7 +512,7 @@ __device__ __forceinline__ void sun_direction(float& x, float& y, float& z) {
     z = s * 0.5f;
 }
 
-template <bool kCount>
+template <bool kCount, int kFetch>
 __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W, uint32_t H,
                                                           const uint4* __restrict__ paths, uint32_t* __restrict__ colors,
                                                           float bias, float fog, uint32_t tileW, uint32_t tileH,
@@ -530,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, co
             const F3 d{sx, sy, sz};
             const F3 inv{(float)(1.0 / (double)sx), (float)(1.0 / (double)sy), (float)(1.0 / (double)sz)};
             uint32_t vx, vy, vz;
-            const bool shadowed = descend<true, kCount>(dag, o, d, inv, 0, vx, vy, vz, st, bytes);
+            const bool shadowed = descend<true, kCount, kFetch>(dag, o, d, inv, 0, vx, vy, vz, st, bytes);
             light = shadowed ? 0.f : 1.f;
             const double vxd = bx - p.cam[0], vyd = by - p.cam[1], vzd = bz - p.cam[2];
             distance = __dsqrt_rn(vxd * vxd + vyd * vyd + vzd * vzd);
@@ -593,12 +620,14 @@ hd_status enqueue_frame_from_device(hd_ctx* ctx, const hd_ray_params* d_params, 
     const RayP* pm = reinterpret_cast<const RayP*>(d_params);
     const RayP none{};
     const dim3 grid = grid_of(ctx, W, H);
-    k_trace_paths<false><<<grid, kBlock, stack_bytes(ctx->g.levels), s>>>(dag, none, pm, W, H,
+    const auto kp = ctx->fetch_mode == kFetchNode ? k_trace_paths<false, kFetchNode> : k_trace_paths<false, kFetchPointer>;
+    const auto ks = ctx->fetch_mode == kFetchNode ? k_trace_shadows<false, kFetchNode> : k_trace_shadows<false, kFetchPointer>;
+    kp<<<grid, kBlock, stack_bytes(ctx->g.levels), s>>>(dag, none, pm, W, H,
                                                                          reinterpret_cast<uint4*>(d_paths), ctx->tile_w,
                                                                          ctx->tile_h, nullptr);
     k_trace_colors<false><<<grid, kBlock, 0, s>>>(dag, dev_colors(ctx), W, H, reinterpret_cast<const uint4*>(d_paths),
                                                   d_colors, ctx->tile_w, ctx->tile_h, nullptr);
-    k_trace_shadows<false><<<grid, kBlock, stack_bytes(ctx->g.levels), s>>>(
+    ks<<<grid, kBlock, stack_bytes(ctx->g.levels), s>>>(
         dag, none, pm, W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog, ctx->tile_w, ctx->tile_h,
         nullptr);
     HD_HIP(ctx, hipGetLastError());
@@ -628,7 +657,8 @@ hd_status hd_trace_paths(hd_ctx* ctx, const hd_ray_params* params, uint32_t W, u
     if (!ctx || !params || !d_paths || W == 0 || H == 0) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
     if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
-    k_trace_paths<false><<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), stream_of(ctx, stream)>>>(
+    const auto kp = ctx->fetch_mode == kFetchNode ? k_trace_paths<false, kFetchNode> : k_trace_paths<false, kFetchPointer>;
+    kp<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), stream_of(ctx, stream)>>>(
         dev_dag(ctx), to_rayp(params), nullptr, W, H, reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h, nullptr);
     HD_HIP(ctx, hipGetLastError());
     return HD_OK;
@@ -650,7 +680,8 @@ hd_status hd_trace_shadows(hd_ctx* ctx, const hd_ray_params* params, uint32_t W,
     if (!ctx || !params || !d_paths || !d_colors || W == 0 || H == 0) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
     if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
-    k_trace_shadows<false><<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), stream_of(ctx, stream)>>>(
+    const auto ks = ctx->fetch_mode == kFetchNode ? k_trace_shadows<false, kFetchNode> : k_trace_shadows<false, kFetchPointer>;
+    ks<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), stream_of(ctx, stream)>>>(
         dev_dag(ctx), to_rayp(params), nullptr, W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
         ctx->tile_w, ctx->tile_h, nullptr);
     HD_HIP(ctx, hipGetLastError());
@@ -664,15 +695,18 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
         return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded || !ctx->colors_loaded) return fail(ctx, HD_ERR_INVALID, "no DAG or colours loaded");
     static const uint32_t shapes[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}, {2, 32}, {64, 1}};
+    static const uint32_t fetches[] = {kFetchPointer, kFetchNode};
     hipEvent_t e0, e1;
     HD_HIP(ctx, hipEventCreate(&e0));
     HD_HIP(ctx, hipEventCreate(&e1));
     float bestMs = 1e30f;
-    uint32_t bw = ctx->tile_w, bh = ctx->tile_h;
+    uint32_t bw = ctx->tile_w, bh = ctx->tile_h, bf = ctx->fetch_mode;
     reps = std::max<uint32_t>(reps, 1);
-    for (const auto& sh : shapes) {
+    for (uint32_t trial = 0; trial < 12; ++trial) {
+        const auto& sh = shapes[trial % 6];
         ctx->tile_w = sh[0];
         ctx->tile_h = sh[1];
+        ctx->fetch_mode = fetches[trial / 6];
         float total = 0;
         for (uint32_t rep = 0; rep <= reps; ++rep) {  // rep 0 warms up
             HD_HIP(ctx, hipEventRecord(e0, ctx->stream));
@@ -697,12 +731,14 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
             bestMs = total;
             bw = sh[0];
             bh = sh[1];
+            bf = ctx->fetch_mode;
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     ctx->tile_w = bw;
     ctx->tile_h = bh;
+    ctx->fetch_mode = bf;
     ctx->ptr_gen++;
     if (best_w) *best_w = bw;
     if (best_h) *best_h = bh;
@@ -717,7 +753,7 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
     if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
     const dim3 grid = grid_of(ctx, W, H);
     HD_HIP(ctx, hipMemsetAsync(ctx->byte_counters, 0, 3 * sizeof(unsigned long long), ctx->stream));
-    k_trace_paths<true><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), nullptr, W, H,
+    k_trace_paths<true, kFetchPointer><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), nullptr, W, H,
                                                           reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h,
                                                           ctx->byte_counters + 0);
     HD_HIP(ctx, hipGetLastError());
@@ -727,7 +763,7 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
                                                                ctx->tile_w, ctx->tile_h, ctx->byte_counters + 1);
         HD_HIP(ctx, hipGetLastError());
     }
-    k_trace_shadows<true><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), nullptr, W, H,
+    k_trace_shadows<true, kFetchPointer><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), nullptr, W, H,
                                                             reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
                                                             ctx->tile_w, ctx->tile_h, ctx->byte_counters + 2);
     HD_HIP(ctx, hipGetLastError());
