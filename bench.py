@@ -1,4 +1,6 @@
-"""Benchmark: primary Mrays/s @1080p on the 2^15 coloured terrain (one GPU).
+"""Benchmark (one GPU): primary Mrays/s @1080p on the 2^15 coloured terrain; edited voxels/s of
+4096 mixed edits batched (one fused pass) vs plain (one pass per edit) on the 2^16 coloured
+terrain; p50 / p99 latency of a served request stream.
 
 Workload (one step = one frame for one camera): trace primary paths, resolve colours and
 one shadow ray per pixel at 1920x1080, cameras cycling over 64 seeded views placed 5-20 %
@@ -45,6 +47,8 @@ def main() -> int:
     ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
     ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
     ap.add_argument("--no-edits", action="store_true")
+    ap.add_argument("--edit-levels", type=int, default=16,
+                    help="scene depth of the batched-vs-plain edit comparison (0 = the traced scene)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the served stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate, requests/s")
     args = ap.parse_args()
@@ -68,7 +72,7 @@ def main() -> int:
     log(f"scene 2^{D}: {scene.dag.size} BasicDAG words, {scene.num_voxels} voxels, "
         f"{scene.blocks.size} colour blocks, generated in {time.time() - t0:.1f}s")
 
-    geom = Geometry.fit(D, scene.level_words, headroom=1.5)
+    geom = fit_geometry(D, scene.level_words)
     log(f"bucket geometry: {geom.num_top_levels} top-geometry levels, {geom.total_virtual()} virtual words; "
         f"level words {[int(w) for w in scene.level_words]}")
     t0 = time.time()
@@ -169,8 +173,10 @@ def main() -> int:
                         f"{len(cams)} seeded cameras (one camera per step)",
             "levels": D, "width": W, "height": H, "cameras": len(cams), "seed": args.seed,
             "voxels": int(scene.num_voxels), "tile": args.tile,
-            "bucket_geometry": {"num_top_levels": geom.num_top_levels, "top": "1024 x 1024 words",
-                                "low": "65536 x 4096 words", "page_words": geom.page_size},
+            "bucket_geometry": {"num_top_levels": geom.num_top_levels,
+                                "top": f"{1 << geom.bucket_bits_top} x {geom.bucket_size_top} words",
+                                "low": f"{1 << geom.bucket_bits_low} x {geom.bucket_size_low} words",
+                                "page_words": geom.page_size},
         },
         "kernels_ms": {n: round(float(v), 4) for n, v in zip(names, k_ms)},
         "primary_only_mrays_s": round(W * H / (k_ms[0] * 1e-3) / 1e6, 2),
@@ -188,20 +194,44 @@ def main() -> int:
         },
     }
 
-    if not args.no_edits:
-        result["edits"] = edit_bench(scene, geom, params, dag, dinfo, rparams[:2], W, H, args)
     if args.serve_requests > 0:
         result["serving"] = serve_bench(scene, geom, params, args)
     dag.close()
     if not args.no_cpu_baseline:
         od, oc, result["cpu_baseline"] = cpu_baseline(scene, geom, rparams, W, H, args.cpu_seconds)
-        if "edits" in result:
-            result["edits"]["cpu_baseline"] = cpu_edit_baseline(od, oc, params, args)
+        if not args.no_edits:
+            edit_cpu = cpu_edit_baseline(od, oc, params, args)
+        del od, oc
+    if not args.no_edits:
+        ED = args.edit_levels or D
+        if ED != D:
+            # the batched-vs-plain comparison on its own (deeper) scene
+            del scene
+            eparams = TerrainParams(levels=ED, seed=args.seed, colors=True)
+            t0 = time.time()
+            escene = generate_terrain(eparams)
+            egeom = fit_geometry(ED, escene.level_words)
+            log(f"edit scene 2^{ED}: {escene.num_voxels} voxels, generated in {time.time() - t0:.1f}s; "
+                f"buckets top {egeom.bucket_size_top} low {egeom.bucket_size_low} words")
+        else:
+            eparams, escene, egeom = params, scene, geom
+        result["edits"] = edit_bench(escene, egeom, eparams, W, H, args)
+        if not args.no_cpu_baseline:
+            result["edits"]["cpu_baseline"] = edit_cpu
     print(json.dumps(result), flush=True)
     return 0
 
 
-def edit_bench(scene, geom, params, dag, dinfo, rparams, W, H, args):
+def fit_geometry(levels, level_words):
+    """Bucket geometry with 50% headroom for edits when it fits, else the tightest that does."""
+    from hashdag_amd import Geometry
+    try:
+        return Geometry.fit(levels, level_words, headroom=1.5)
+    except ValueError:
+        return Geometry.fit(levels, level_words, headroom=1.0)
+
+
+def edit_bench(scene, geom, params, W, H, args):
     """Edited voxels/s of the same seeded mixed edit list applied one pass per edit (plain)
     and as one fused pass (batched), each on a freshly loaded copy of the scene with the edit
     scratch already sized by an untimed run (undone and garbage-collected before timing).
@@ -209,10 +239,12 @@ def edit_bench(scene, geom, params, dag, dinfo, rparams, W, H, args):
     import numpy as np
     import torch
 
-    from hashdag_amd import DAGTracer, HashDAG
-    from hashdag_amd.workload import mixed_edits
+    from hashdag_amd import DAGInfo, DAGTracer, HashDAG, ray_params
+    from hashdag_amd.workload import cameras, mixed_edits
 
     edits = mixed_edits(params, args.edits, seed=args.seed + 1)
+    dinfo = DAGInfo.unit(params.levels)
+    rparams = [ray_params(c, dinfo, params.levels, W, H) for c in cameras(params, 2, seed=args.seed)]
     log(f"edits: {len(edits)} mixed edits (70% sphere, 20% box, 10% paint; radius log-uniform 2-256)")
 
     def frames(d):
@@ -237,9 +269,12 @@ def edit_bench(scene, geom, params, dag, dinfo, rparams, W, H, args):
         torch.cuda.synchronize()
         return st, time.perf_counter() - t0
 
+    dag = HashDAG(geom)
+    dag.load_scene(scene)
     st_f, t_f = timed(dag, True, edits)
     log(f"fused: {st_f.num_voxels} voxels in {t_f * 1e3:.1f} ms ({st_f.passes} pass)")
     img_f = frames(dag)
+    dag.close()
     plain = HashDAG(geom)
     plain.load_scene(scene)
     st_p, t_p = timed(plain, False, edits[:16])
@@ -298,7 +333,7 @@ def cpu_edit_baseline(od, oc, params, args):
         spent += time.perf_counter() - t
         n += 1
     return {"value": round(voxels / spent, 1), "unit": "edited voxels/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} edits of the list ({voxels} voxels) in {spent:.1f}s"}
+            "sample": f"first {n} edits of the same list on the 2^{params.levels} scene ({voxels} voxels) in {spent:.1f}s"}
 
 
 def measured_traffic(kernel: str, levels: int, W: int, H: int):

@@ -50,22 +50,33 @@ class Geometry:
     @staticmethod
     def fit(levels: int, level_words, color_tree_levels: int = 10, top_fill: float = 0.4,
             low_fill: float = 0.6, headroom: float = 1.0) -> "Geometry":
-        """Reference bucket geometry with as many top-geometry levels as the measured per-level
-        node words allow (a level is 'top' while its words fit `top_fill` of the top capacity;
-        the reference's 9 when they all do). `headroom` scales the words for later growth."""
-        g = Geometry(levels=levels, color_tree_levels=color_tree_levels)
-        top_cap = (1 << g.bucket_bits_top) * g.bucket_size_top
-        n_top = 0
-        while n_top < min(9, levels) and n_top < len(level_words) and \
-                level_words[n_top] * headroom <= top_fill * top_cap:
-            n_top += 1
-        g.num_top_levels = n_top
-        for lvl, w in enumerate(level_words):
-            if w * headroom > (low_fill if lvl >= n_top else top_fill) * g.capacity(lvl):
-                raise ValueError(f"level {lvl}: {int(w)} words do not fit the bucket geometry")
-        if g.total_virtual() >= 1 << 32:
-            raise ValueError("virtual address space exceeds 32 bits")
-        return g
+        """Bucket geometry for measured per-level node words: the reference's bucket counts,
+        with the reference's bucket sizes when they hold the scene and larger buckets (in that
+        order of preference) when they do not. A level is 'top' while its words fit `top_fill`
+        of the top capacity (the reference's 9 when all do); every level must fit its fill
+        limit, and the virtual space must stay below 2^32 words. `headroom` scales the words
+        for later growth."""
+        last = None
+        for size_top, size_low in ((1024, 4096), (2048, 4096), (4096, 4096), (1024, 8192), (2048, 8192),
+                                   (4096, 8192)):
+            g = Geometry(levels=levels, color_tree_levels=color_tree_levels, bucket_size_top=size_top,
+                         bucket_size_low=size_low)
+            top_cap = (1 << g.bucket_bits_top) * g.bucket_size_top
+            n_top = 0
+            while n_top < min(9, levels) and n_top < len(level_words) and \
+                    level_words[n_top] * headroom <= top_fill * top_cap:
+                n_top += 1
+            g.num_top_levels = n_top
+            bad = [lvl for lvl, w in enumerate(level_words)
+                   if w * headroom > (low_fill if lvl >= n_top else top_fill) * g.capacity(lvl)]
+            if bad:
+                last = f"level {bad[0]}: {int(level_words[bad[0]])} words do not fit the bucket geometry"
+                continue
+            if g.total_virtual() >= 1 << 32:
+                last = "virtual address space exceeds 32 bits"
+                continue
+            return g
+        raise ValueError(last)
 
 
 @dataclass
@@ -95,8 +106,9 @@ class HashDAG:
         cfg.bucket_size_top = geometry.bucket_size_top
         cfg.bucket_size_low = geometry.bucket_size_low
         cfg.color_tree_levels = geometry.color_tree_levels
-        if pool_pages is not None:
-            cfg.pool_pages = pool_pages
+        # the device pool maps the whole virtual space, so by default every page may be handed
+        # out (the reference's budget, main.cpp:46, sized its CPU pool)
+        cfg.pool_pages = pool_pages if pool_pages is not None else min(geometry.total_pages(), 0xFFFFFFFF)
         cfg.device = device
         cfg.color_leaves_cap = color_leaves_cap
         self.pool_pages = cfg.pool_pages
