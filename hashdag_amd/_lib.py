@@ -91,7 +91,8 @@ class hd_terrain_params(C.Structure):
 
 
 class hd_edit(C.Structure):
-    _fields_ = [("kind", C.c_uint32), ("center", C.c_float * 3), ("radius", C.c_float)]
+    _fields_ = [("kind", C.c_uint32), ("center", C.c_float * 3), ("radius", C.c_float), ("source", C.c_float * 3),
+                ("dest", C.c_float * 3)]
 
 
 class hd_edit_stats(C.Structure):

@@ -13,7 +13,7 @@ struct hd_ctx;
 namespace hd {
 
 constexpr uint32_t EDIT_SPHERE_ADD = 0, EDIT_SPHERE_CARVE = 1, EDIT_BOX_ADD = 2, EDIT_BOX_CARVE = 3,
-                   EDIT_SPHERE_PAINT = 4;
+                   EDIT_SPHERE_PAINT = 4, EDIT_COPY = 5;
 
 // error bits reported by a pass
 constexpr uint32_t ERR_SCRATCH = 1, ERR_BUCKET = 2, ERR_POOL = 4, ERR_HEADER = 8, ERR_COLOR_SCRATCH = 16,
@@ -25,6 +25,7 @@ constexpr uint32_t kUndoCap = 1u << 20;
 
 // An editor with its derived bounds (hash_dag_editors.h:57-127): box min/max = c -/+ r,
 // r2 = r * r, `single` = rgb101210 colour of add editors (sphere red, box magenta).
+// A copy is a box around dest + (center - source) whose voxels read source + (p - dest).
 struct DevEdit {
     uint32_t kind;
     float c[3];
@@ -32,7 +33,8 @@ struct DevEdit {
     float r2;
     float bmin[3], bmax[3];
     uint32_t single;
-    uint32_t pad[3];
+    float src[3], dst[3];
+    uint32_t pad;
 };
 
 // Work items of one level (structure of arrays, capacity `cap`; child arrays are 8 * cap).

@@ -96,7 +96,7 @@ __device__ __forceinline__ float tmaxf2(float a, float b) { return a > b ? a : b
 
 // hash_dag_editors.h:76-91 / 112-126
 __device__ __forceinline__ bool e_touch(const DevEdit& e, float x0, float y0, float z0, float x1, float y1, float z1) {
-    if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE) {
+    if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE || e.kind == EDIT_COPY) {
         return !(e.bmin[0] >= x1 || e.bmin[1] >= y1 || e.bmin[2] >= z1 || e.bmax[0] <= x0 || e.bmax[1] <= y0 ||
                  e.bmax[2] <= z0 || x0 >= e.bmax[0] || y0 >= e.bmax[1] || z0 >= e.bmax[2] || x1 <= e.bmin[0] ||
                  y1 <= e.bmin[1] || z1 <= e.bmin[2]);
@@ -207,6 +207,41 @@ __device__ __forceinline__ bool leaf_valid_index(const LeafDesc& d, uint64_t i) 
     if (d.n_blocks == 0) return false;
     if (d.offset != ~0ull) i += d.offset;
     return 2 * i / 16384u < d.n_macro;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Copy editor reads (hash_dag_editors.h:364-434): the DAG as it was when the pass started
+// ---------------------------------------------------------------------------------------------
+// float -> uint32 as x86-64 code converts it (truncate to int64, keep the low 32 bits)
+__device__ __forceinline__ uint32_t f2u_x86(float f) { return (uint32_t)(int64_t)f; }
+
+// source voxel of a destination voxel corner: source + (p - dest), per component in FP32
+__device__ __forceinline__ void copy_source(const DevEdit& e, float px, float py, float pz, uint32_t& sx, uint32_t& sy,
+                                            uint32_t& sz) {
+    sx = f2u_x86(e.src[0] + (px - e.dst[0]));
+    sy = f2u_x86(e.src[1] + (py - e.dst[1]));
+    sz = f2u_x86(e.src[2] + (pz - e.dst[2]));
+}
+
+// 64-bit leaf mask of the leaf at leaf coordinates (lx, ly, lz) (voxel >> 2), 0 when absent
+// (DAGUtils::get_value, dag_utils.h:138-172, stopped at the leaf)
+__device__ uint64_t source_leaf(const uint32_t* __restrict__ vpool, uint32_t root, uint32_t levels, uint32_t lx,
+                                uint32_t ly, uint32_t lz) {
+    const uint32_t L = levels - 2;
+    uint32_t va = root;
+    for (uint32_t level = 0; level < L; ++level) {
+        const uint32_t s = L - level - 1;  // bit of the leaf coordinates below this level
+        const uint32_t c = (((lx >> s) & 1u) << 2) | (((ly >> s) & 1u) << 1) | ((lz >> s) & 1u);
+        const uint32_t m = vpool[va] & 0xFFu;
+        if (!(m & (1u << c))) return 0;
+        va = vpool[va + __popc(m & ((1u << c) - 1u)) + 1u];
+    }
+    return *reinterpret_cast<const uint64_t*>(vpool + va);
+}
+
+__device__ __forceinline__ uint32_t leaf_bit(uint32_t x, uint32_t y, uint32_t z) {
+    return ((x & 1u) ? 4u : 0u) | ((y & 1u) ? 2u : 0u) | ((z & 1u) ? 1u : 0u) | ((x & 2u) ? 32u : 0u) |
+           ((y & 2u) ? 16u : 0u) | ((z & 2u) ? 8u : 0u);
 }
 
 __device__ __forceinline__ uint32_t child_off(uint32_t mask, uint32_t c) { return __popc(mask & ((1u << c) - 1u)) + 1u; }
@@ -404,9 +439,35 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
             uint64_t v = old != kInvalidNode ? *reinterpret_cast<const uint64_t*>(P.vpool + old) : 0ull;
             const uint64_t origMask = orig != kInvalidNode ? *reinterpret_cast<const uint64_t*>(P.vpool + orig) : 0ull;
             const uint32_t es = I.estart[i], ec = I.ecount[i];
+            uint32_t cx = ~0u, cy = ~0u, cz = ~0u;  // copies: last source leaf looked up
+            uint64_t cachedLeaf = 0;
             for (uint32_t k = 0; k < ec; ++k) {
                 const DevEdit& e = P.edits[P.elist[L][es + k]];
                 const uint64_t tm = leaf_touch_mask(e, bx, by, bz);
+                if (e.kind == EDIT_COPY) {
+                    // get_new_value: an empty touched voxel takes its source voxel's value
+                    const uint32_t srcRoot = P.items[0].orig[0];
+                    uint64_t cand = tm & ~v, add = 0;
+                    while (cand) {
+                        const uint32_t b = (uint32_t)__builtin_ctzll(cand);
+                        cand &= cand - 1;
+                        const uint32_t c1 = b >> 3, c2 = b & 7u;
+                        const float fx = (float)(bx | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u));
+                        const float fy = (float)(by | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u));
+                        const float fz = (float)(bz | ((c1 & 1u) << 1) | (c2 & 1u));
+                        uint32_t sx, sy, sz;
+                        copy_source(e, fx, fy, fz, sx, sy, sz);
+                        if ((sx >> 2) != cx || (sy >> 2) != cy || (sz >> 2) != cz) {
+                            cx = sx >> 2;
+                            cy = sy >> 2;
+                            cz = sz >> 2;
+                            cachedLeaf = source_leaf(P.vpool, srcRoot, P.levels, cx, cy, cz);
+                        }
+                        if ((cachedLeaf >> leaf_bit(sx, sy, sz)) & 1ull) add |= 1ull << b;
+                    }
+                    v |= add;
+                    continue;  // a copy's "num voxels" is its box volume, counted on the host
+                }
                 if (e.kind == EDIT_SPHERE_ADD || e.kind == EDIT_BOX_ADD) v |= tm;
                 else if (e.kind != EDIT_SPHERE_PAINT) v &= ~tm;
                 touched |= tm;
@@ -711,6 +772,48 @@ __device__ __forceinline__ LeafDesc task_leaf(const EditPass& P, uint32_t handle
     return d;
 }
 
+// DAGUtils::get_color (dag_utils.h:423-523) of voxel (x, y, z) in the DAG at `root`: the colour
+// at the voxel's rank in its colour subtree, CompressedColor{} (0) when absent. The colour
+// slots hold the pass's starting colours until k_encode writes the new leaves.
+__device__ uint64_t source_color(const EditPass& P, uint32_t root, uint32_t x, uint32_t y, uint32_t z) {
+    const uint32_t levels = P.levels, L = levels - 2, ctl = P.ctl;
+    const uint32_t* __restrict__ vpool = P.vpool;
+    uint64_t nof = 0;
+    uint32_t colorNode = 0, handle = 0;
+    bool haveLeaf = false;
+    uint32_t va = root;
+    for (uint32_t level = 1; level <= L; ++level) {
+        const uint32_t m = vpool[va] & 0xFFu;
+        const uint32_t s = levels - level;
+        const uint32_t child = (((x >> s) & 1u) << 2) | (((y >> s) & 1u) << 1) | ((z >> s) & 1u);
+        if (!(m & (1u << child))) return 0;
+        if (level - 1 < ctl) {
+            colorNode = P.cnodes[colorNode + child];
+            if (level == ctl) {
+                handle = colorNode;
+                haveLeaf = true;
+            } else if (!colorNode) {
+                return 0;
+            }
+        }
+        if (level == L) {
+            for (uint32_t c = 0; c < child; ++c)
+                if (m & (1u << c)) nof += __popcll(*reinterpret_cast<const uint64_t*>(vpool + vpool[va + child_off(m, c)]));
+            const uint64_t lf = *reinterpret_cast<const uint64_t*>(vpool + vpool[va + child_off(m, child)]);
+            nof += __popcll(lf & ((1ull << leaf_bit(x, y, z)) - 1ull));
+            break;
+        }
+        if (level > ctl)
+            for (uint32_t c = 0; c < child; ++c)
+                if (m & (1u << c)) nof += vpool[vpool[va + child_off(m, c)]] >> 8;
+        va = vpool[va + child_off(m, child)];
+    }
+    if (!haveLeaf) return 0;
+    const LeafDesc d = task_leaf(P, handle);
+    if (!leaf_valid_index(d, nof)) return 0;
+    return leaf_decode(P.arena, d, nof);
+}
+
 __device__ __forceinline__ uint64_t old_color(const EditPass& P, uint32_t mode, uint32_t modeval, const LeafDesc& d,
                                               uint64_t index) {
     if (mode == 1) return pack_color(modeval, 0, 0);
@@ -771,6 +874,21 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
         for (uint32_t k = 0; k < I.ecount[i]; ++k) {
             const DevEdit& e = P.edits[P.elist[L][I.estart[i] + k]];
             if (!e_touch(e, fx, fy, fz, fx + 1.0f, fy + 1.0f, fz + 1.0f)) continue;
+            if (e.kind == EDIT_COPY) {
+                // get_new_value / get_new_color: an empty voxel takes its source's value and colour
+                if (!v) {
+                    uint32_t sx, sy, sz;
+                    copy_source(e, fx, fy, fz, sx, sy, sz);
+                    const uint32_t srcRoot = P.items[0].orig[0];
+                    const uint64_t sl = source_leaf(P.vpool, srcRoot, P.levels, sx >> 2, sy >> 2, sz >> 2);
+                    if ((sl >> leaf_bit(sx, sy, sz)) & 1ull) {
+                        col = source_color(P, srcRoot, sx, sy, sz);
+                        haveCol = true;
+                        v = true;
+                    }
+                }
+                continue;
+            }
             const bool nv = e_value(e, v);
             if (nv) {
                 uint64_t pc = 0;

@@ -52,6 +52,7 @@ __global__ void k_count_voxels(const DevEdit* __restrict__ edits, uint32_t n, co
     if (dN) n = min(n, *dN);  // graph replays read the edit count from device memory
     for (uint32_t ei = blockIdx.x; ei < n; ei += gridDim.x) {
         const DevEdit e = edits[ei];
+        if (e.kind == EDIT_COPY) continue;  // counted on the host (box volume)
         const float r = e.radius;
         const int64_t x0 = std::max<int64_t>(0, (int64_t)floorf(e.c[0] - r) - 1);
         const int64_t x1 = std::min<int64_t>((int64_t)E - 1, (int64_t)ceilf(e.c[0] + r) + 1);
@@ -103,14 +104,31 @@ DevEdit make_edit(const hd_edit& e) {
     d.radius = e.radius;
     d.r2 = e.radius * e.radius;
     for (int i = 0; i < 3; ++i) {
-        d.bmin[i] = e.center[i] - e.radius;
-        d.bmax[i] = e.center[i] + e.radius;
+        // a copy's box is centred at dest + (center - source) (hash_dag_editors.h:350-351)
+        const float c = e.kind == EDIT_COPY ? e.dest[i] + (e.center[i] - e.source[i]) : e.center[i];
+        d.bmin[i] = c - e.radius;
+        d.bmax[i] = c + e.radius;
+        d.src[i] = e.source[i];
+        d.dst[i] = e.dest[i];
     }
     auto c101210 = [](float r, float g, float b) {
         return (uint32_t)(r * 1023.0f) | ((uint32_t)(g * 4095.0f) << 10) | ((uint32_t)(b * 1023.0f) << 22);
     };
     d.single = (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE) ? c101210(1, 0, 1) : c101210(1, 0, 0);
     return d;
+}
+
+// "num voxels" of copies: the box volume as a 32-bit product (BoxEditorBase::edit_count_impl,
+// hash_dag_editors.h:92-96; hash_dag.h:374-378 reports it instead of the voxel count)
+uint64_t copy_voxels(const hd_edit* e, uint32_t n) {
+    uint64_t sum = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (e[i].kind != EDIT_COPY) continue;
+        const DevEdit d = make_edit(e[i]);
+        sum += (uint32_t)(uint32_t)(d.bmax[0] - d.bmin[0]) * (uint32_t)(d.bmax[1] - d.bmin[1]) *
+               (uint32_t)(d.bmax[2] - d.bmin[2]);
+    }
+    return sum;
 }
 
 struct Readback {
@@ -306,7 +324,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
         return HD_OK;
     }
     for (uint32_t i = 0; i < n; ++i)
-        if (edits[i].kind > EDIT_SPHERE_PAINT || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
+        if (edits[i].kind > EDIT_COPY || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
             return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");
     hd_status st = edit_prepare(ctx);
     if (st != HD_OK) return st;
@@ -326,19 +344,26 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             st = edit_reserve(ctx, items, elist, 0);
             if (st != HD_OK) return st;
         }
-        UndoFrame frame;
-        frame.prev_root = ctx->root;
-        Readback rb{};
-        st = run_pass(ctx, 0, n, s, rb, &frame);
-        if (st != HD_OK) return st;
-        ctx->root = rb.root;
-        frame.new_root = rb.root;
-        ctx->undo.push_back(std::move(frame));
-        ctx->redo.clear();
-        out.num_voxels += rb.stats[0];
-        out.new_words += rb.stats[1];
-        out.leaves_rebuilt += rb.counters[4];
-        out.passes = 1;
+        // A copy reads the DAG as the edits before it left it, so each copy starts a new pass;
+        // every other run of editors fuses into one pass.
+        for (uint32_t first = 0; first < n;) {
+            uint32_t last = first + 1;
+            while (last < n && edits[last].kind != EDIT_COPY) ++last;
+            UndoFrame frame;
+            frame.prev_root = ctx->root;
+            Readback rb{};
+            st = run_pass(ctx, first, last - first, s, rb, &frame);
+            if (st != HD_OK) return st;
+            ctx->root = rb.root;
+            frame.new_root = rb.root;
+            ctx->undo.push_back(std::move(frame));
+            ctx->redo.clear();
+            out.num_voxels += rb.stats[0] + copy_voxels(edits + first, last - first);
+            out.new_words += rb.stats[1];
+            out.leaves_rebuilt += rb.counters[4];
+            out.passes++;
+            first = last;
+        }
     } else {
         for (uint32_t i = 0; i < n; ++i) {
             UndoFrame frame;
@@ -350,7 +375,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             frame.new_root = rb.root;
             ctx->undo.push_back(std::move(frame));
             ctx->redo.clear();
-            out.num_voxels += rb.stats[0];
+            out.num_voxels += rb.stats[0] + copy_voxels(edits + i, 1);
             out.new_words += rb.stats[1];
             out.leaves_rebuilt += rb.counters[4];
             out.passes++;
@@ -538,7 +563,7 @@ hd_status hd_frame_graph_launch(hd_frame_graph* g, const hd_edit* edits, uint32_
     if (n > g->max_edits) return fail(ctx, HD_ERR_INVALID, "more edits than the graph was built for");
     for (uint32_t i = 0; i < n; ++i)
         if (edits[i].kind > EDIT_SPHERE_PAINT || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
-            return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");
+            return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius (copies run through hd_edit_batch)");
     // the same free-space guarantees as an eager pass, then (re)capture if buffers moved
     hd_status st = grow_arena(ctx, 1u << 22);
     if (st == HD_OK) st = edit_ensure_edits(ctx, g->max_edits);

@@ -86,33 +86,26 @@ __device__ __forceinline__ uint32_t intersection_mask(uint32_t levels, uint32_t 
         const uint32_t first = (qx >= rx ? 4u : 0u) + (qy >= ry ? 2u : 0u) + (qz >= rz ? 1u : 0u);
         mask = 1u << first;
     }
+    // the three face terms are evaluated unconditionally and kept by predicate (same IEEE
+    // operations as the branches of tracer.cu:70-130, without divergent control flow)
     const float eps = 1e-4f;
-    if (tmin <= tx && tx <= tmax) {
+    {
         const float qy = tx * d.y, qz = tx * d.z;
-        uint32_t A = 0, B = 0;
-        if (qy >= ry - eps) A |= 0xCC;
-        if (qy <= ry + eps) A |= 0x33;
-        if (qz >= rz - eps) B |= 0xAA;
-        if (qz <= rz + eps) B |= 0x55;
-        mask |= A & B;
+        const uint32_t A = (qy >= ry - eps ? 0xCCu : 0u) | (qy <= ry + eps ? 0x33u : 0u);
+        const uint32_t B = (qz >= rz - eps ? 0xAAu : 0u) | (qz <= rz + eps ? 0x55u : 0u);
+        mask |= (tmin <= tx && tx <= tmax) ? (A & B) : 0u;
     }
-    if (tmin <= ty && ty <= tmax) {
+    {
         const float qx = ty * d.x, qz = ty * d.z;
-        uint32_t C = 0, D = 0;
-        if (qx >= rx - eps) C |= 0xF0;
-        if (qx <= rx + eps) C |= 0x0F;
-        if (qz >= rz - eps) D |= 0xAA;
-        if (qz <= rz + eps) D |= 0x55;
-        mask |= C & D;
+        const uint32_t C = (qx >= rx - eps ? 0xF0u : 0u) | (qx <= rx + eps ? 0x0Fu : 0u);
+        const uint32_t D = (qz >= rz - eps ? 0xAAu : 0u) | (qz <= rz + eps ? 0x55u : 0u);
+        mask |= (tmin <= ty && ty <= tmax) ? (C & D) : 0u;
     }
-    if (tmin <= tz && tz <= tmax) {
+    {
         const float qx = tz * d.x, qy = tz * d.y;
-        uint32_t E = 0, F = 0;
-        if (qx >= rx - eps) E |= 0xF0;
-        if (qx <= rx + eps) E |= 0x0F;
-        if (qy >= ry - eps) F |= 0xCC;
-        if (qy <= ry + eps) F |= 0x33;
-        mask |= E & F;
+        const uint32_t E = (qx >= rx - eps ? 0xF0u : 0u) | (qx <= rx + eps ? 0x0Fu : 0u);
+        const uint32_t F = (qy >= ry - eps ? 0xCCu : 0u) | (qy <= ry + eps ? 0x33u : 0u);
+        mask |= (tmin <= tz && tz <= tmax) ? (E & F) : 0u;
     }
     return mask;
 }
@@ -196,19 +189,19 @@ __device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, u
     if (kCount) bytes += 4;
     uint32_t visit = childMask & intersection_mask<true>(levels, 0, 0, 0, 0, o, d, inv);
     uint64_t leaf = 0;
+    uint32_t alive = 0;  // bit l: the stack entry of level l still has children to visit
     for (;;) {
         if (!visit) {
-            uint32_t newLevel = level;
-            while (newLevel > 0 && !visit) {
-                newLevel--;
-                const uint32_t m = st.at_mask(newLevel);
-                childMask = m & 0xFFu;
-                visit = m >> 8;
-            }
-            if (!visit) {
+            // back up to the deepest level with children left (the DFS backtrack)
+            const uint32_t below = alive & ((1u << level) - 1u);
+            if (!below) {
                 px = py = pz = 0;
                 return false;
             }
+            const uint32_t newLevel = 31u - (uint32_t)__builtin_clz(below);
+            const uint32_t m = st.at_mask(newLevel);
+            childMask = m & 0xFFu;
+            visit = m >> 8;
             const uint32_t up = level - newLevel;
             px >>= up;
             py >>= up;
@@ -224,8 +217,14 @@ __device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, u
         px = (px << 1) | ((next >> 2) & 1u);
         py = (py << 1) | ((next >> 1) & 1u);
         pz = (pz << 1) | (next & 1u);
-        if (level > 0 && level < leafLevel) st.at_idx(level) = index;
-        st.at_mask(level) = (uint16_t)(childMask | (visit << 8));
+        // only entries with children left are ever read back
+        if (visit) {
+            if (level > 0 && level < leafLevel) st.at_idx(level) = index;
+            st.at_mask(level) = (uint16_t)(childMask | (visit << 8));
+            alive |= 1u << level;
+        } else {
+            alive &= ~(1u << level);
+        }
         level++;
         if (level == levels) return true;
         if (level < leafLevel) {

@@ -192,18 +192,29 @@ class HashDAG:
 
     # -- edits: HashDAG::edit_threads<Editor> (hash_dag.h:255-405) --
     def edit_batch(self, edits, fused: bool = False, stream=None) -> hd_edit_stats:
-        """Applies [(kind, (x, y, z), radius), ...] in order; fused=True runs them in one pass."""
-        arr = (hd_edit * len(edits))()
-        for i, (kind, center, radius) in enumerate(edits):
+        """Applies [(kind, (x, y, z), radius), ...] in order (a copy adds source and dest:
+        (5, center, radius, source, dest)); fused=True runs each run of editors between copies
+        in one pass."""
+        arr = (hd_edit * max(len(edits), 1))()
+        for i, ed in enumerate(edits):
+            kind, center, radius = ed[:3]
             arr[i].kind = int(kind)
             arr[i].center[:] = [float(c) for c in center]
             arr[i].radius = float(radius)
+            if len(ed) > 3:
+                arr[i].source[:] = [float(c) for c in ed[3]]
+                arr[i].dest[:] = [float(c) for c in ed[4]]
         st = hd_edit_stats()
         check(lib.hd_edit_batch(self._h, arr, len(edits), 1 if fused else 0, C.byref(st), stream), self._h)
         return st
 
     def edit(self, kind: int, center, radius: float) -> hd_edit_stats:
         return self.edit_batch([(kind, center, radius)])
+
+    def copy(self, center, radius: float, source, dest) -> hd_edit_stats:
+        """CopyEditor (hash_dag_editors.h:332-435, identity transform): the box of `radius`
+        around dest + (center - source) gets the voxels (and colours) at source + (p - dest)."""
+        return self.edit_batch([(5, center, radius, source, dest)])
 
     def remove_stale_nodes(self, max_level: int | None = None, stream=None) -> dict:
         """GC (hash_dag.h:407-547); the engine calls it with levels - 2 (engine.cpp:182)."""

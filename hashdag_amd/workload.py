@@ -41,7 +41,7 @@ def surface_points(params: TerrainParams, n: int, seed: int = 2) -> np.ndarray:
 
 
 # editor kinds of the C ABI (hd_edit.kind)
-SPHERE_ADD, SPHERE_CARVE, BOX_ADD, BOX_CARVE, SPHERE_PAINT = range(5)
+SPHERE_ADD, SPHERE_CARVE, BOX_ADD, BOX_CARVE, SPHERE_PAINT, COPY = range(6)
 
 
 def _centre(rng, params: TerrainParams):

@@ -155,10 +155,13 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
 #define HD_EDIT_BOX_ADD 2u
 #define HD_EDIT_BOX_CARVE 3u
 #define HD_EDIT_SPHERE_PAINT 4u
+#define HD_EDIT_COPY 5u  /* CopyEditorWithoutDecompression, identity transform (hash_dag_editors.h:332-435) */
 typedef struct hd_edit {
     uint32_t kind;
     float center[3];
     float radius;
+    float source[3];  /* copy: the box around center is copied from source to dest */
+    float dest[3];
 } hd_edit;
 
 typedef struct hd_edit_stats {

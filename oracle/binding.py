@@ -62,6 +62,8 @@ _SIGS = {
                                     C.c_float, C.c_float, u64p, C.c_int]),
     "orc_edit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, fltp, C.c_float, u64p]),
     "orc_gc": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "orc_edit_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "orc_get_color": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
     "orc_dag_reachable": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
 }
 
@@ -169,6 +171,20 @@ class OracleDAG:
         if r != 0:
             raise RuntimeError(_err())
         return [int(v) for v in stats]
+
+    def edit_copy(self, center, radius: float, source, dest, colors: "OracleColors"):
+        c, s, d = (C.c_float * 3)(*center), (C.c_float * 3)(*source), (C.c_float * 3)(*dest)
+        stats = np.zeros(3, np.uint64)
+        if orc.orc_edit_copy(self.h, colors.h, c, radius, s, d, stats.ctypes.data) != 0:
+            raise RuntimeError(_err())
+        return [int(v) for v in stats]
+
+    def color(self, colors: "OracleColors", x: int, y: int, z: int):
+        out = np.zeros(2, np.uint32)
+        r = orc.orc_get_color(self.h, colors.h, x, y, z, out.ctypes.data)
+        if r < 0:
+            raise RuntimeError(_err())
+        return bool(r), int(out[0]), int(out[1])
 
     def gc(self, max_level: int) -> None:
         if orc.orc_gc(self.h, max_level) != 0:

@@ -380,6 +380,35 @@ int orc_edit(void* d, void* c, uint32_t kind, const float center[3], float radiu
     GUARD_END(-1)
 }
 
+int orc_edit_copy(void* d, void* c, const float center[3], float radius, const float source[3], const float dest[3],
+                  uint64_t stats[3]) {
+    GUARD_BEGIN
+    Edit e{};
+    e.kind = COPY;
+    e.center = {center[0], center[1], center[2]};
+    e.radius = radius;
+    e.source = {source[0], source[1], source[2]};
+    e.dest = {dest[0], dest[1], dest[2]};
+    const EditStats s = apply_edit(*static_cast<DAG*>(d), static_cast<HashDAGColors*>(c), e, nullptr);
+    if (stats) {
+        stats[0] = s.numNodes;
+        stats[1] = s.numVoxels;
+        stats[2] = s.leavesRebuilt;
+    }
+    return 0;
+    GUARD_END(-1)
+}
+
+int orc_get_color(void* d, void* c, uint32_t x, uint32_t y, uint32_t z, uint32_t out[2]) {
+    GUARD_BEGIN
+    bool ok = false;
+    const CompressedColor col = get_color_at(*static_cast<DAG*>(d), *static_cast<HashDAGColors*>(c), Path(x, y, z), &ok);
+    out[0] = col.colorBits;
+    out[1] = col.weight | (col.bitsPerWeight << 8);
+    return ok ? 1 : 0;
+    GUARD_END(-1)
+}
+
 int orc_gc(void* d, uint32_t max_level) {
     GUARD_BEGIN
     remove_stale_nodes(*static_cast<DAG*>(d), max_level);

@@ -14,17 +14,109 @@ namespace {
 static float sq(float f) { return f * f; }
 static float tmaxf(float a, float b) { return a > b ? a : b; }  // host ::max (cuda_math.h:7-8)
 
+// float -> uint32 as x86-64 code converts it (truncate to int64, keep the low 32 bits): what
+// the reference's uint32(p.x) yields for the source positions of a copy
+static u32 to_u32(float f) { return (u32)(int64_t)f; }
+
+// DAGUtils::get_value (dag_utils.h:138-172) from an explicit root
+static bool value_at(const HashTable& t, u32 root, Path path) {
+    const Geometry& g = t.g;
+    u32 va = root;
+    for (u32 level = 0; level < g.levels; ++level) {
+        if (level < g.leaf_level()) {
+            const u32 hdr = *t.ptr(va);
+            const u8 mask = child_mask(hdr);
+            const u8 c = path.child_index(level + 1, g.levels);
+            if (!(mask & (1u << c))) return false;
+            va = *t.ptr(va + child_offset(mask, c));
+        } else {
+            const u32* p = t.ptr(va);
+            const u64 leaf = (u64)p[0] | ((u64)p[1] << 32);
+            const u32 bit = ((path.p.x & 1) ? 4 : 0) | ((path.p.y & 1) ? 2 : 0) | ((path.p.z & 1) ? 1 : 0) |
+                            ((path.p.x & 2) ? 32 : 0) | ((path.p.y & 2) ? 16 : 0) | ((path.p.z & 2) ? 8 : 0);
+            return (leaf >> bit) & 1;
+        }
+    }
+    return true;
+}
+
+// DAGUtils::get_color (dag_utils.h:423-523) from an explicit root: colour index = voxels
+// before the path's voxel in its level-color_tree_levels subtree, looked up in that subtree's
+// leaf; an absent voxel or colour gives CompressedColor{}
+static CompressedColor color_at(const HashTable& t, u32 root, const HashDAGColors& colors, Path path) {
+    const Geometry& g = t.g;
+    const u32 ctl = g.color_tree_levels, L = g.leaf_level();
+    u64 nof = 0;
+    u32 colorNode = 0;
+    LeafView leaf{};
+    bool haveLeaf = false;
+    u32 level = 0;
+    u32 va = root;
+    while (level < L) {
+        level++;
+        const u32 hdr = *t.ptr(va);
+        const u8 mask = child_mask(hdr);
+        const u8 child = path.child_index(level, g.levels);
+        if (!(mask & (1u << child))) return CompressedColor{};
+        if (level - 1 < ctl) {
+            colorNode = colors.nodes[colorNode + child];
+            if (level == ctl) {
+                leaf = colors.get_leaf(colorNode);
+                haveLeaf = true;
+            } else if (!colorNode) {
+                return CompressedColor{};
+            }
+        }
+        if (level == L) {
+            for (u8 c = 0; c < child; ++c)
+                if (mask & (1u << c)) {
+                    const u32* p = t.ptr(*t.ptr(va + child_offset(mask, c)));
+                    nof += popcll((u64)p[0] | ((u64)p[1] << 32));
+                }
+            const u32* p = t.ptr(*t.ptr(va + child_offset(mask, child)));
+            const u64 lf = (u64)p[0] | ((u64)p[1] << 32);
+            const u32 bit = ((path.p.x & 1) ? 4 : 0) | ((path.p.y & 1) ? 2 : 0) | ((path.p.z & 1) ? 1 : 0) |
+                            ((path.p.x & 2) ? 32 : 0) | ((path.p.y & 2) ? 16 : 0) | ((path.p.z & 2) ? 8 : 0);
+            nof += popcll(lf & ((1ull << bit) - 1));
+            break;
+        }
+        if (level > ctl)
+            for (u8 c = 0; c < child; ++c)
+                if (mask & (1u << c)) nof += *t.ptr(*t.ptr(va + child_offset(mask, c))) >> 8;
+        va = *t.ptr(va + child_offset(mask, child));
+    }
+    if (!haveLeaf || !leaf.is_valid() || !leaf.is_valid_index(nof)) return CompressedColor{};
+    return leaf.get_color(nof);
+}
+
 struct Editor {
     Edit e;
     F3 bmin{}, bmax{};  // BoxEditorBase (hash_dag_editors.h:57-74)
     float r2 = 0;       // SphereEditorBase (hash_dag_editors.h:99-110)
+    // copy (CopyEditorWithoutDecompression<true>, hash_dag_editors.h:332-435, identity
+    // transform, no swirl): the DAG before the edit and the colours it reads from
+    const HashTable* srcTable = nullptr;
+    u32 srcRoot = 0;
+    const HashDAGColors* srcColors = nullptr;
 
     explicit Editor(const Edit& ed) : e(ed) {
-        bmin = e.center - f3(e.radius);
-        bmax = e.center + f3(e.radius);
+        // a copy is a box around dest + (center - source) (hash_dag_editors.h:350-351)
+        const F3 c = e.kind == COPY ? e.dest + (e.center - e.source) : e.center;
+        bmin = c - f3(e.radius);
+        bmax = c + f3(e.radius);
         r2 = e.radius * e.radius;
     }
-    bool is_box() const { return e.kind == BOX_ADD || e.kind == BOX_CARVE; }
+    bool is_box() const { return e.kind == BOX_ADD || e.kind == BOX_CARVE || e.kind == COPY; }
+    // hash_dag_editors.h:364-379: source + (position - dest)
+    Path source_of(F3 pos) const {
+        const F3 p = e.source + (pos - e.dest);
+        return Path(to_u32(p.x), to_u32(p.y), to_u32(p.z));
+    }
+    // BoxEditorBase::edit_count_impl (hash_dag_editors.h:92-96), 32-bit product
+    u32 edit_count() const {
+        const F3 ext = bmax - bmin;
+        return (u32)ext.x * (u32)ext.y * (u32)ext.z;
+    }
     bool adding() const { return e.kind == SPHERE_ADD || e.kind == BOX_ADD; }
 
     // hash_dag_editors.h:112-126 / 76-91
@@ -77,11 +169,19 @@ struct Editor {
         return is_empty_impl(s, s + f3((float)(1u << depth)));
     }
     F3 single_color() const { return is_box() ? F3{1, 0, 1} : F3{1, 0, 0}; }
-    bool new_value(bool old) const { return e.kind == SPHERE_PAINT ? old : adding(); }
-    // hash_dag_editors.h:264-269, 307-312, 323-329
+    // get_new_value (hash_dag_editors.h:263, 306, 322, 416-424)
+    bool new_value(bool old, F3 pos) const {
+        if (e.kind == COPY) return old ? old : value_at(*srcTable, srcRoot, source_of(pos));
+        return e.kind == SPHERE_PAINT ? old : adding();
+    }
+    // hash_dag_editors.h:264-269, 307-312, 323-329, 425-434
     template <typename Fn>
-    CompressedColor new_color(F3 pos, Fn oldColor) const {
+    CompressedColor new_color(F3 pos, Fn oldColor, bool oldValue, bool newValue) const {
         CompressedColor c;
+        if (e.kind == COPY) {
+            if (oldValue == newValue || !newValue) return oldColor();
+            return color_at(*srcTable, srcRoot, *srcColors, source_of(pos));
+        }
         if (e.kind == SPHERE_PAINT) {
             const CompressedColor prev = oldColor();
             const float f = tclamp(1 - lengthf(pos - e.center) / e.radius, 0.f, .1f);
@@ -160,7 +260,7 @@ LeafResult edit_leaf(u32 level, u32 nodeIndex, Path path, const Editor& ed, Leaf
                 const u64 bit = 1ull << (c1 * 8u + c2);
                 const bool shouldEdit = ed.should_edit_impl(pos, pos + 1.0f);
                 const bool prev = (mask & bit) != 0;
-                const bool nv = shouldEdit ? ed.new_value(prev) : prev;
+                const bool nv = shouldEdit ? ed.new_value(prev, pos) : prev;
                 ctx.stats.numVoxels += shouldEdit;
                 if (ctx.builder) {
                     if (nv) {
@@ -169,7 +269,7 @@ LeafResult edit_leaf(u32 level, u32 nodeIndex, Path path, const Editor& ed, Leaf
                                 return ctx.leaf.get_color(ctx.oldLeavesCount);
                             return CompressedColor{};
                         };
-                        const CompressedColor col = shouldEdit ? ed.new_color(pos, prevColor) : prevColor();
+                        const CompressedColor col = shouldEdit ? ed.new_color(pos, prevColor, prev, nv) : prevColor();
                         ctx.builder->add(col);
                     }
                     ctx.oldLeavesCount += prev;
@@ -335,9 +435,19 @@ EditIndex edit_threaded(bool queue, u32 level, u32 nodeIndex, u32 colorNodeIndex
 
 }  // namespace
 
+CompressedColor get_color_at(const DAG& dag, const HashDAGColors& colors, Path path, bool* ok) {
+    const bool present = value_at(dag.t, dag.root, path);
+    if (ok) *ok = present;
+    return color_at(dag.t, dag.root, colors, path);
+}
+
 // hash_dag.h:255-405 (tasks run in creation order; undo frame kept when requested)
 EditStats apply_edit(DAG& dag, HashDAGColors* colors, const Edit& e, UndoFrame* undo) {
     Editor ed(e);
+    ed.srcTable = &dag.t;  // the copy reads the DAG as it was before this edit
+    ed.srcRoot = dag.root;
+    ed.srcColors = colors;
+    if (e.kind == COPY && !colors) throw std::runtime_error("oracle: copy needs colours");
     EditCtx ctx{dag.t, colors, ed, {}, {}, {}, {}};
     if (undo) {
         undo->root = dag.root;
@@ -352,6 +462,7 @@ EditStats apply_edit(DAG& dag, HashDAGColors* colors, const Edit& e, UndoFrame* 
     }
     dag.root = edit_threaded(false, 0, dag.root, 0, Path(0, 0, 0), ctx).nodeIndex;
     ctx.stats.leavesRebuilt = ctx.toBuild.size();
+    if (e.kind == COPY) ctx.stats.numVoxels = ed.edit_count();  // "num voxels" (hash_dag.h:374-378)
     for (auto& tb : ctx.toBuild) {
         ColorLeaf& ref = colors->leaves[tb.newLeafIndex & 0x7FFFFFFFu];
         // The undo frame keeps the leaf the subtree used before the edit: the shared main

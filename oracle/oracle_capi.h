@@ -105,6 +105,12 @@ int orc_trace_shadows(void* dag, const double params[12], uint32_t W, uint32_t H
 /* edits: kind 0 sphere add, 1 sphere carve, 2 box add, 3 box carve, 4 sphere paint.
  * stats out: num nodes, num voxels, colour leaves rebuilt. colors may be NULL. */
 int orc_edit(void* dag, void* colors, uint32_t kind, const float center[3], float radius, uint64_t stats[3]);
+/* copy editor (CopyEditorWithoutDecompression, identity transform): box of `radius` around
+ * dest + (center - source), filled from source + (p - dest) of the DAG before the edit */
+int orc_edit_copy(void* dag, void* colors, const float center[3], float radius, const float source[3],
+                  const float dest[3], uint64_t stats[3]);
+/* DAGUtils::get_color at a voxel (out: colorBits, weight | bpw << 8); returns 1 if the voxel is set */
+int orc_get_color(void* dag, void* colors, uint32_t x, uint32_t y, uint32_t z, uint32_t out[2]);
 int orc_gc(void* dag, uint32_t max_level);
 /* distinct node addresses reachable from the root, per level 0..max_level */
 int orc_dag_reachable(void* dag, uint32_t max_level, uint64_t* out);

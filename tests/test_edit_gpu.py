@@ -157,3 +157,46 @@ def test_scratch_growth_keeps_results(fused):
         total += od.edit(kind, center, r, oc)[1]
     assert st.num_voxels == total
     assert_same(dag, sc, od, oc, f"after scratch growth (fused={fused})")
+
+
+COPY = 5
+
+
+def copy_edits(p, n, seed):
+    """Copies of terrain boxes (radius 4-24) to seeded destinations nearby, integer valued."""
+    from hashdag_amd import terrain_height
+    rng = np.random.default_rng(seed)
+    F = p.footprint or (1 << p.levels)
+    out = []
+    for _ in range(n):
+        r = float(rng.integers(4, 25))
+        x, z = (int(v) for v in rng.integers(32, F - 32, size=2))
+        y = terrain_height(p, x, z)
+        src = (float(x), float(y), float(z))
+        dx, dz = (int(v) for v in rng.integers(-24, 25, size=2))
+        dst = (float(x + dx), float(y + int(rng.integers(8, 40))), float(z + dz))
+        out.append((COPY, src, r, src, dst))
+    return out
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_copy_matches_oracle(fused):
+    """CopyEditor (identity transform) alone and between other editors: same DAG, colours and
+    "num voxels" (box volume for copies) as the oracle applying the edits in order."""
+    p, sc, g, dag, od, oc = make("d13")
+    edits = []
+    others = random_edits(p, 9, seed=23)
+    copies = copy_edits(p, 3, seed=29)
+    for k in range(3):
+        edits += others[3 * k:3 * k + 3] + [copies[k]]
+    st = dag.edit_batch(edits, fused=fused)
+    total = 0
+    for ed in edits:
+        if ed[0] == COPY:
+            total += od.edit_copy(ed[1], ed[2], ed[3], ed[4], oc)[1]
+        else:
+            total += od.edit(ed[0], ed[1], ed[2], oc)[1]
+    assert st.num_voxels == total
+    if fused:
+        assert st.passes == 4  # each copy starts a new pass: [o o o] [C o o o] [C o o o] [C]
+    assert_same(dag, sc, od, oc, f"copies (fused={fused})")

@@ -172,3 +172,59 @@ def test_edit_sequence_then_gc_keeps_canonical_form():
     # after GC every node left in the table is reachable: bucket words == reachable words
     pool, pt, bs, full, top = od.export()
     assert int(bs.sum()) <= int(sum(canon.size for _ in [0])) * 4
+
+
+COPIES = [
+    # center, radius, source, dest (integer valued; the source box lies inside the cube)
+    ((40.0, 60.0, 40.0), 10.0, (40.0, 60.0, 40.0), (90.0, 80.0, 90.0)),
+    ((30.0, 55.0, 90.0), 7.0, (20.0, 10.0, 30.0), (75.0, 20.0, 20.0)),
+    ((64.0, 64.0, 64.0), 16.0, (60.0, 40.0, 20.0), (20.0, 50.0, 100.0)),
+]
+
+
+@pytest.mark.parametrize("center,radius,source,dest", COPIES)
+def test_copy_brute_force(center, radius, source, dest):
+    """CopyEditorWithoutDecompression (hash_dag_editors.h:332-435), identity transform: the
+    box of `radius` around dest + (center - source) gets old | before[v - dest + source];
+    newly set voxels take the colour of their source voxel, all others keep theirs; the
+    edited-voxel count is the box volume."""
+    p, sc, g = scene("d7")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    before, cols_before = per_voxel_colours(od, oc, g)
+    stats = od.edit_copy(center, radius, source, dest, oc)
+    after, cols_after = per_voxel_colours(od, oc, g)
+    E = 1 << g.levels
+    box_c = tuple(d + (c - s) for c, s, d in zip(center, source, dest))
+    t = touched(BOX_ADD, box_c, radius, E)
+    off = np.array(source, np.int64) - np.array(dest, np.int64)
+    expect = before.copy()
+    for v in zip(*np.nonzero(t)):
+        src = tuple(int(a) for a in np.array(v) + off)
+        assert all(0 <= s < E for s in src)
+        if before[src]:
+            expect[v] = 1
+    assert np.array_equal(after, expect)
+    assert stats[1] == int((2 * radius) ** 3)
+    for v, col in cols_after.items():
+        if t[v] and not before[v]:
+            src = tuple(int(a) for a in np.array(v) + off)
+            assert col == cols_before[src], v
+        else:
+            assert col == cols_before[v], v
+
+
+def test_color_lookup_matches_canonical_order():
+    """DAGUtils::get_color (dag_utils.h:423-523) == the colour at the voxel's Morton rank."""
+    p, sc, g = scene("d7")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    grid, cols = per_voxel_colours(od, oc, g)
+    rng = np.random.default_rng(5)
+    keys = list(cols)
+    for k in rng.choice(len(keys), 300):
+        v = keys[int(k)]
+        ok, bits, wb = od.color(oc, *v)
+        assert ok and (bits, wb) == cols[v]
+    ok, bits, wb = od.color(oc, 0, 127, 0)  # empty (above the terrain)
+    assert not ok and (bits, wb) == (0, 0)
