@@ -168,86 +168,119 @@ __device__ __forceinline__ Stack stack_of(uint32_t levels) {
     return st;
 }
 
-// DFS over the DAG (tracer.cu:166-249 ordered first hit; tracer.cu:458-542 any hit).
-// Returns true on a hit; (px, py, pz) is the voxel path on a hit, (0,0,0) on a miss.
-template <bool kAnyHit, bool kCount, int kFetch>
-__device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, uint32_t order, uint32_t& px,
-                                        uint32_t& py, uint32_t& pz, Stack st, uint64_t& bytes) {
-    const uint32_t* __restrict__ vpool = dag.vpool;
-    const uint32_t levels = dag.levels, leafLevel = dag.leaf_level;
-    uint32_t level = 0;
-    px = py = pz = 0;
-    uint32_t index = dag.root;
-    uint32_t nd[9];
-    uint32_t childMask;
+// DFS state of one ray: the current node, its child mask and the children left to visit,
+// the voxel path, and (bit l of alive) which stack levels still hold children to visit.
+struct Walk {
+    uint32_t level, px, py, pz, index, childMask, visit, alive;
+    uint64_t leaf;
+    uint32_t nd[9];  // the current node's words (kFetchNode)
+};
+
+constexpr uint32_t kWalkOn = 0, kWalkHit = 1, kWalkMiss = 2;
+
+template <bool kCount, int kFetch>
+__device__ __forceinline__ void walk_begin(const DevDag& dag, Walk& w, F3 o, F3 d, F3 inv, uint64_t& bytes) {
+    w.level = 0;
+    w.px = w.py = w.pz = 0;
+    w.index = dag.root;
+    w.leaf = 0;
+    w.alive = 0;
     if (kFetch == kFetchNode) {
-        load_node(vpool, index, nd);
-        childMask = nd[0] & 0xFFu;
+        load_node(dag.vpool, w.index, w.nd);
+        w.childMask = w.nd[0] & 0xFFu;
     } else {
-        childMask = vpool[index] & 0xFFu;
+        w.childMask = dag.vpool[w.index] & 0xFFu;
     }
     if (kCount) bytes += 4;
-    uint32_t visit = childMask & intersection_mask<true>(levels, 0, 0, 0, 0, o, d, inv);
-    uint64_t leaf = 0;
-    uint32_t alive = 0;  // bit l: the stack entry of level l still has children to visit
-    for (;;) {
-        if (!visit) {
-            // back up to the deepest level with children left (the DFS backtrack)
-            const uint32_t below = alive & ((1u << level) - 1u);
-            if (!below) {
-                px = py = pz = 0;
-                return false;
-            }
-            const uint32_t newLevel = 31u - (uint32_t)__builtin_clz(below);
-            const uint32_t m = st.at_mask(newLevel);
-            childMask = m & 0xFFu;
-            visit = m >> 8;
-            const uint32_t up = level - newLevel;
-            px >>= up;
-            py >>= up;
-            pz >>= up;
-            level = newLevel;
-            if (level < leafLevel) {
-                index = level == 0 ? dag.root : st.at_idx(level);
-                if (kFetch == kFetchNode) load_node(vpool, index, nd);
-            }
+    w.visit = w.childMask & intersection_mask<true>(dag.levels, 0, 0, 0, 0, o, d, inv);
+}
+
+// One step of the DFS over the DAG (tracer.cu:166-249 ordered first hit; tracer.cu:458-542
+// any hit): back up if the current node has nothing left to visit, then descend into the
+// next child. Returns kWalkHit with the voxel path in (px, py, pz), kWalkMiss with (0, 0, 0),
+// or kWalkOn.
+template <bool kCount, int kFetch>
+__device__ __forceinline__ uint32_t walk_step(const DevDag& dag, Walk& w, F3 o, F3 d, F3 inv, uint32_t order, Stack st,
+                                              uint64_t& bytes) {
+    const uint32_t* __restrict__ vpool = dag.vpool;
+    const uint32_t levels = dag.levels, leafLevel = dag.leaf_level;
+    if (!w.visit) {
+        // back up to the deepest level with children left (the DFS backtrack)
+        const uint32_t below = w.alive & ((1u << w.level) - 1u);
+        if (!below) {
+            w.px = w.py = w.pz = 0;
+            return kWalkMiss;
         }
-        const uint32_t next = kAnyHit ? (31u - (uint32_t)__builtin_clz(visit)) : next_child(order, visit);
-        visit &= ~(1u << next);
-        px = (px << 1) | ((next >> 2) & 1u);
-        py = (py << 1) | ((next >> 1) & 1u);
-        pz = (pz << 1) | (next & 1u);
-        // only entries with children left are ever read back
-        if (visit) {
-            if (level > 0 && level < leafLevel) st.at_idx(level) = index;
-            st.at_mask(level) = (uint16_t)(childMask | (visit << 8));
-            alive |= 1u << level;
-        } else {
-            alive &= ~(1u << level);
+        const uint32_t newLevel = 31u - (uint32_t)__builtin_clz(below);
+        const uint32_t m = st.at_mask(newLevel);
+        w.childMask = m & 0xFFu;
+        w.visit = m >> 8;
+        const uint32_t up = w.level - newLevel;
+        w.px >>= up;
+        w.py >>= up;
+        w.pz >>= up;
+        w.level = newLevel;
+        if (w.level < leafLevel) {
+            w.index = w.level == 0 ? dag.root : st.at_idx(w.level);
+            if (kFetch == kFetchNode) load_node(vpool, w.index, w.nd);
         }
-        level++;
-        if (level == levels) return true;
-        if (level < leafLevel) {
-            if (kFetch == kFetchNode) {
-                index = pick(nd, child_off(childMask, next));
-                load_node(vpool, index, nd);
-                childMask = nd[0] & 0xFFu;
-            } else {
-                index = vpool[index + child_off(childMask, next)];
-                childMask = vpool[index] & 0xFFu;
-            }
-            if (kCount) bytes += 8;
-        } else if (level == leafLevel) {
-            const uint32_t addr =
-                kFetch == kFetchNode ? pick(nd, child_off(childMask, next)) : vpool[index + child_off(childMask, next)];
-            leaf = *reinterpret_cast<const uint64_t*>(vpool + addr);
-            childMask = leaf_first_mask(leaf);
-            if (kCount) bytes += 12;
-        } else {
-            childMask = leaf_second_mask(leaf, next);
-        }
-        visit = childMask & intersection_mask<false>(levels, level, px, py, pz, o, d, inv);
     }
+    // children front to back along the ray, also for the any-hit walk: its result does not
+    // depend on the order (tracer.cu:493 takes the highest bit first), and front to back
+    // reaches the nearest occluder of a shadowed ray first
+    const uint32_t next = next_child(order, w.visit);
+    w.visit &= ~(1u << next);
+    w.px = (w.px << 1) | ((next >> 2) & 1u);
+    w.py = (w.py << 1) | ((next >> 1) & 1u);
+    w.pz = (w.pz << 1) | (next & 1u);
+    // only entries with children left are ever read back
+    if (w.visit) {
+        if (w.level > 0 && w.level < leafLevel) st.at_idx(w.level) = w.index;
+        st.at_mask(w.level) = (uint16_t)(w.childMask | (w.visit << 8));
+        w.alive |= 1u << w.level;
+    } else {
+        w.alive &= ~(1u << w.level);
+    }
+    w.level++;
+    if (w.level == levels) return kWalkHit;
+    if (w.level < leafLevel) {
+        if (kFetch == kFetchNode) {
+            w.index = pick(w.nd, child_off(w.childMask, next));
+            load_node(vpool, w.index, w.nd);
+            w.childMask = w.nd[0] & 0xFFu;
+        } else {
+            w.index = vpool[w.index + child_off(w.childMask, next)];
+            w.childMask = vpool[w.index] & 0xFFu;
+        }
+        if (kCount) bytes += 8;
+    } else if (w.level == leafLevel) {
+        const uint32_t addr = kFetch == kFetchNode ? pick(w.nd, child_off(w.childMask, next))
+                                                   : vpool[w.index + child_off(w.childMask, next)];
+        w.leaf = *reinterpret_cast<const uint64_t*>(vpool + addr);
+        w.childMask = leaf_first_mask(w.leaf);
+        if (kCount) bytes += 12;
+    } else {
+        w.childMask = leaf_second_mask(w.leaf, next);
+    }
+    w.visit = w.childMask & intersection_mask<false>(levels, w.level, w.px, w.py, w.pz, o, d, inv);
+    return kWalkOn;
+}
+
+// The whole walk of one ray. Returns true on a hit; (px, py, pz) is the voxel path on a hit,
+// (0,0,0) on a miss.
+template <bool kCount, int kFetch>
+__device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, uint32_t order, uint32_t& px,
+                                        uint32_t& py, uint32_t& pz, Stack st, uint64_t& bytes) {
+    Walk w;
+    walk_begin<kCount, kFetch>(dag, w, o, d, inv, bytes);
+    uint32_t r;
+    do {
+        r = walk_step<kCount, kFetch>(dag, w, o, d, inv, order, st, bytes);
+    } while (r == kWalkOn);
+    px = w.px;
+    py = w.py;
+    pz = w.pz;
+    return r == kWalkHit;
 }
 
 __device__ __forceinline__ void dnormalize(double& x, double& y, double& z) {
@@ -272,6 +305,20 @@ __device__ __forceinline__ void flush_count(uint64_t v, unsigned long long* coun
     if ((threadIdx.x & 63u) == 0) atomicAdd(counter, (unsigned long long)v);
 }
 
+// Primary ray of pixel (x, y) (tracer.cu:160-180): FP64 direction, FP32 origin, direction and
+// inverse direction, child order from the direction's signs.
+__device__ __forceinline__ void primary_ray(const RayP& p, uint32_t x, uint32_t y, F3& o, F3& d, F3& inv,
+                                            uint32_t& order) {
+    double dx = p.rmin[0] + (double)x * p.ddx[0] + (double)y * p.ddy[0] - p.cam[0];
+    double dy = p.rmin[1] + (double)x * p.ddx[1] + (double)y * p.ddy[1] - p.cam[1];
+    double dz = p.rmin[2] + (double)x * p.ddx[2] + (double)y * p.ddy[2] - p.cam[2];
+    dnormalize(dx, dy, dz);
+    o = F3{(float)p.cam[0], (float)p.cam[1], (float)p.cam[2]};
+    d = F3{(float)dx, (float)dy, (float)dz};
+    inv = F3{(float)(1.0 / (double)d.x), (float)(1.0 / (double)d.y), (float)(1.0 / (double)d.z)};
+    order = (d.x < 0.f ? 4u : 0u) + (d.y < 0.f ? 2u : 0u) + (d.z < 0.f ? 1u : 0u);
+}
+
 // tracer.cu:145-252
 template <bool kCount, int kFetch>
 __global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W, uint32_t H,
@@ -284,16 +331,11 @@ __global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, cons
     pixel_of(tileW, tileH, x, y);
     uint64_t bytes = 0;
     if (x < W && y < H) {
-        double dx = p.rmin[0] + (double)x * p.ddx[0] + (double)y * p.ddy[0] - p.cam[0];
-        double dy = p.rmin[1] + (double)x * p.ddx[1] + (double)y * p.ddy[1] - p.cam[1];
-        double dz = p.rmin[2] + (double)x * p.ddx[2] + (double)y * p.ddy[2] - p.cam[2];
-        dnormalize(dx, dy, dz);
-        const F3 o{(float)p.cam[0], (float)p.cam[1], (float)p.cam[2]};
-        const F3 d{(float)dx, (float)dy, (float)dz};
-        const F3 inv{(float)(1.0 / (double)d.x), (float)(1.0 / (double)d.y), (float)(1.0 / (double)d.z)};
-        const uint32_t order = (d.x < 0.f ? 4u : 0u) + (d.y < 0.f ? 2u : 0u) + (d.z < 0.f ? 1u : 0u);
+        F3 o, d, inv;
+        uint32_t order;
+        primary_ray(p, x, y, o, d, inv, order);
         uint32_t vx, vy, vz;
-        descend<false, kCount, kFetch>(dag, o, d, inv, order, vx, vy, vz, st, bytes);
+        descend<kCount, kFetch>(dag, o, d, inv, order, vx, vy, vz, st, bytes);
         out[(uint64_t)(H - 1 - y) * W + x] = make_uint4(vx, vy, vz, 0u);
         if (kCount) bytes += 16;
     }
@@ -511,6 +553,75 @@ __device__ __forceinline__ void sun_direction(float& x, float& y, float& z) {
     z = s * 0.5f;
 }
 
+// Camera ray direction of pixel (x, yy) in FP64 (tracer.cu:630-636).
+__device__ __forceinline__ void camera_dir(const RayP& p, uint32_t x, uint32_t yy, double& cx, double& cy, double& cz) {
+    cx = p.rmin[0] + (double)x * p.ddx[0] + (double)yy * p.ddy[0] - p.cam[0];
+    cy = p.rmin[1] + (double)x * p.ddx[1] + (double)yy * p.ddy[1] - p.cam[1];
+    cz = p.rmin[2] + (double)x * p.ddx[2] + (double)yy * p.ddy[2] - p.cam[2];
+    dnormalize(cx, cy, cz);
+}
+
+// Shadow ray origin: the exact FP64 hit point on the voxel's box (ray_box_intersection,
+// tracer.cu:574-587) moved by bias along the sun direction (tracer.cu:640-655).
+__device__ __forceinline__ F3 shadow_origin(const RayP& p, uint4 pp, double cx, double cy, double cz, float bias,
+                                            F3 sun) {
+    const double bx = (double)(float)pp.x, by = (double)(float)pp.y, bz = (double)(float)pp.z;
+    const double t0x = (bx - p.cam[0]) / cx, t0y = (by - p.cam[1]) / cy, t0z = (bz - p.cam[2]) / cz;
+    const double t1x = (bx + 1 - p.cam[0]) / cx, t1y = (by + 1 - p.cam[1]) / cy, t1z = (bz + 1 - p.cam[2]) / cz;
+    const double mx = t0x < t1x ? t0x : t1x, my = t0y < t1y ? t0y : t1y, mz = t0z < t1z ? t0z : t1z;
+    const double maxmin = fmax(fmax(mx, my), mz);
+    const float hx = (float)(p.cam[0] + cx * maxmin);
+    const float hy = (float)(p.cam[1] + cy * maxmin);
+    const float hz = (float)(p.cam[2] + cz * maxmin);
+    return F3{hx + bias * sun.x, hy + bias * sun.y, hz + bias * sun.z};
+}
+
+// Distance from the camera to the hit voxel's corner and the direction to it (tracer.cu:656-660).
+__device__ __forceinline__ void hit_distance(const RayP& p, uint4 pp, double& distance, double& nx, double& ny,
+                                             double& nz) {
+    const double bx = (double)(float)pp.x, by = (double)(float)pp.y, bz = (double)(float)pp.z;
+    const double vxd = bx - p.cam[0], vyd = by - p.cam[1], vzd = bz - p.cam[2];
+    distance = __dsqrt_rn(vxd * vxd + vyd * vyd + vzd * vzd);
+    nx = vxd / distance;
+    ny = vyd / distance;
+    nz = vzd / distance;
+}
+
+// Shading (tracer.cu:604-619) and applyFog (tracer.cu:551-572).
+__device__ __forceinline__ uint32_t shade(uint32_t colorIn, float light, double distance, double nx, double ny,
+                                          double nz, float fog, F3 sun) {
+    const float k = clamp01(0.5f + light);
+    float r = (float)((colorIn >> 0) & 0xFFu) / 255.0f * k;
+    float g = (float)((colorIn >> 8) & 0xFFu) / 255.0f * k;
+    float b = (float)((colorIn >> 16) & 0xFFu) / 255.0f * k;
+    const float fd = fog * 0.00001f;
+    const double fogAmount = 1.0 - exp(-distance * (double)fd);
+    const double sunAmount = (double)1.01f * fmax(nx * (double)sun.x + ny * (double)sun.y + nz * (double)sun.z, 0.0);
+    const float sa = (float)pow(sunAmount, 30.0);
+    const float oms = 1 - sa;
+    const float fr = 187.0f / 255.f * oms + 1.0f * sa;
+    const float fgc = 242.0f / 255.f * oms + 1.0f * sa;
+    const float fb = 250.0f / 255.f * oms + 1.0f * sa;
+    const float fa = clamp01((float)fogAmount);
+    const float omf = 1 - fa;
+    r = r * omf + fr * fa;
+    g = g * omf + fgc * fa;
+    b = b * omf + fb * fa;
+    return to_rgb888(r, g, b);
+}
+
+// shade() without fog: the colour scaled by clamp(0.5 + light, 0, 1).
+__device__ __forceinline__ uint32_t shade_clear(uint32_t colorIn, float light) {
+    const float k = clamp01(0.5f + light);
+    const float r = (float)((colorIn >> 0) & 0xFFu) / 255.0f * k;
+    const float g = (float)((colorIn >> 8) & 0xFFu) / 255.0f * k;
+    const float b = (float)((colorIn >> 16) & 0xFFu) / 255.0f * k;
+    return to_rgb888(r, g, b);
+}
+
+__device__ __forceinline__ bool null_path(uint4 pp) { return pp.x == 0 && pp.y == 0 && pp.z == 0; }
+
+// tracer.cu:589-697
 template <bool kCount, int kFetch>
 __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W, uint32_t H,
                                                           const uint4* __restrict__ paths, uint32_t* __restrict__ colors,
@@ -527,62 +638,35 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, co
         const uint4 pp = paths[pix];
         const uint32_t colorIn = colors[pix];
         if (kCount) bytes += 24;
-        const float ox = (float)pp.x, oy = (float)pp.y, oz = (float)pp.z;
-        double cx = p.rmin[0] + (double)x * p.ddx[0] + (double)(H - 1 - y) * p.ddy[0] - p.cam[0];
-        double cy = p.rmin[1] + (double)x * p.ddx[1] + (double)(H - 1 - y) * p.ddy[1] - p.cam[1];
-        double cz = p.rmin[2] + (double)x * p.ddx[2] + (double)(H - 1 - y) * p.ddy[2] - p.cam[2];
-        dnormalize(cx, cy, cz);
-        float sx, sy, sz;
-        sun_direction(sx, sy, sz);
-        float light;
-        double distance, nx, ny, nz;
-        if (pp.x == 0 && pp.y == 0 && pp.z == 0) {  // length(rayOrigin) == 0 (tracer.cu:644)
-            light = 1.f;
-            distance = 1e9;
-            nx = cx;
-            ny = cy;
-            nz = cz;
+        F3 sun;
+        sun_direction(sun.x, sun.y, sun.z);
+        // With zero fog density applyFog leaves the colour as it is (fogAmount = 1 - exp(-0) = 0,
+        // so r * 1 + fogColor * 0 == r), and the FP64 fog terms are skipped.
+        const bool noFog = fog == 0.f;
+        if (null_path(pp)) {  // length(rayOrigin) == 0 (tracer.cu:644)
+            if (noFog) {
+                colors[pix] = shade_clear(colorIn, 1.f);
+            } else {
+                double cx, cy, cz;
+                camera_dir(p, x, H - 1 - y, cx, cy, cz);
+                colors[pix] = shade(colorIn, 1.f, 1e9, cx, cy, cz, fog, sun);
+            }
         } else {
-            // ray_box_intersection (tracer.cu:574-587)
-            const double bx = (double)ox, by = (double)oy, bz = (double)oz;
-            const double t0x = (bx - p.cam[0]) / cx, t0y = (by - p.cam[1]) / cy, t0z = (bz - p.cam[2]) / cz;
-            const double t1x = (bx + 1 - p.cam[0]) / cx, t1y = (by + 1 - p.cam[1]) / cy, t1z = (bz + 1 - p.cam[2]) / cz;
-            const double mx = t0x < t1x ? t0x : t1x, my = t0y < t1y ? t0y : t1y, mz = t0z < t1z ? t0z : t1z;
-            const double maxmin = fmax(fmax(mx, my), mz);
-            const float hx = (float)(p.cam[0] + cx * maxmin);
-            const float hy = (float)(p.cam[1] + cy * maxmin);
-            const float hz = (float)(p.cam[2] + cz * maxmin);
-            const F3 o{hx + bias * sx, hy + bias * sy, hz + bias * sz};
-            const F3 d{sx, sy, sz};
-            const F3 inv{(float)(1.0 / (double)sx), (float)(1.0 / (double)sy), (float)(1.0 / (double)sz)};
+            double cx, cy, cz;
+            camera_dir(p, x, H - 1 - y, cx, cy, cz);
+            const F3 o = shadow_origin(p, pp, cx, cy, cz, bias, sun);
+            const F3 inv{(float)(1.0 / (double)sun.x), (float)(1.0 / (double)sun.y), (float)(1.0 / (double)sun.z)};
+            const uint32_t order = (sun.x < 0.f ? 4u : 0u) + (sun.y < 0.f ? 2u : 0u) + (sun.z < 0.f ? 1u : 0u);
             uint32_t vx, vy, vz;
-            const bool shadowed = descend<true, kCount, kFetch>(dag, o, d, inv, 0, vx, vy, vz, st, bytes);
-            light = shadowed ? 0.f : 1.f;
-            const double vxd = bx - p.cam[0], vyd = by - p.cam[1], vzd = bz - p.cam[2];
-            distance = __dsqrt_rn(vxd * vxd + vyd * vyd + vzd * vzd);
-            nx = vxd / distance;
-            ny = vyd / distance;
-            nz = vzd / distance;
+            const bool shadowed = descend<kCount, kFetch>(dag, o, sun, inv, order, vx, vy, vz, st, bytes);
+            if (noFog) {
+                colors[pix] = shade_clear(colorIn, shadowed ? 0.f : 1.f);
+            } else {
+                double distance, nx, ny, nz;
+                hit_distance(p, pp, distance, nx, ny, nz);
+                colors[pix] = shade(colorIn, shadowed ? 0.f : 1.f, distance, nx, ny, nz, fog, sun);
+            }
         }
-        // shading (tracer.cu:604-619) and applyFog (tracer.cu:551-572)
-        const float k = clamp01(0.5f + light);
-        float r = (float)((colorIn >> 0) & 0xFFu) / 255.0f * k;
-        float g = (float)((colorIn >> 8) & 0xFFu) / 255.0f * k;
-        float b = (float)((colorIn >> 16) & 0xFFu) / 255.0f * k;
-        const float fd = fog * 0.00001f;
-        const double fogAmount = 1.0 - exp(-distance * (double)fd);
-        const double sunAmount = (double)1.01f * fmax(nx * (double)sx + ny * (double)sy + nz * (double)sz, 0.0);
-        const float sa = (float)pow(sunAmount, 30.0);
-        const float oms = 1 - sa;
-        const float fr = 187.0f / 255.f * oms + 1.0f * sa;
-        const float fgc = 242.0f / 255.f * oms + 1.0f * sa;
-        const float fb = 250.0f / 255.f * oms + 1.0f * sa;
-        const float fa = clamp01((float)fogAmount);
-        const float omf = 1 - fa;
-        r = r * omf + fr * fa;
-        g = g * omf + fgc * fa;
-        b = b * omf + fb * fa;
-        colors[pix] = to_rgb888(r, g, b);
     }
     flush_count<kCount>(bytes, counter);
 }
@@ -605,6 +689,25 @@ inline dim3 grid_of(const hd_ctx* ctx, uint32_t W, uint32_t H) {
 // The caller's stream as given; NULL is the HIP default stream (torch's default stream).
 inline hipStream_t stream_of(hd_ctx*, void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Primary paths with the fetch strategy of the context; pm (device) overrides p when given.
+hd_status launch_paths(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP* pm, uint32_t W, uint32_t H,
+                       uint4* out, hipStream_t s) {
+    const auto k = ctx->fetch_mode == kFetchNode ? k_trace_paths<false, kFetchNode> : k_trace_paths<false, kFetchPointer>;
+    k<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), s>>>(dag, p, pm, W, H, out, ctx->tile_w, ctx->tile_h,
+                                                                     nullptr);
+    HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+
+hd_status launch_shadows(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP* pm, uint32_t W, uint32_t H,
+                         const uint4* paths, uint32_t* colors, float bias, float fog, hipStream_t s) {
+    const auto k = ctx->fetch_mode == kFetchNode ? k_trace_shadows<false, kFetchNode> : k_trace_shadows<false, kFetchPointer>;
+    k<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), s>>>(dag, p, pm, W, H, paths, colors, bias, fog,
+                                                                     ctx->tile_w, ctx->tile_h, nullptr);
+    HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+
 }  // namespace
 }  // namespace hd
 
@@ -618,19 +721,13 @@ hd_status enqueue_frame_from_device(hd_ctx* ctx, const hd_ray_params* d_params, 
     dag.root_ptr = ctx->d_root;
     const RayP* pm = reinterpret_cast<const RayP*>(d_params);
     const RayP none{};
-    const dim3 grid = grid_of(ctx, W, H);
-    const auto kp = ctx->fetch_mode == kFetchNode ? k_trace_paths<false, kFetchNode> : k_trace_paths<false, kFetchPointer>;
-    const auto ks = ctx->fetch_mode == kFetchNode ? k_trace_shadows<false, kFetchNode> : k_trace_shadows<false, kFetchPointer>;
-    kp<<<grid, kBlock, stack_bytes(ctx->g.levels), s>>>(dag, none, pm, W, H,
-                                                                         reinterpret_cast<uint4*>(d_paths), ctx->tile_w,
-                                                                         ctx->tile_h, nullptr);
-    k_trace_colors<false><<<grid, kBlock, 0, s>>>(dag, dev_colors(ctx), W, H, reinterpret_cast<const uint4*>(d_paths),
-                                                  d_colors, ctx->tile_w, ctx->tile_h, nullptr);
-    ks<<<grid, kBlock, stack_bytes(ctx->g.levels), s>>>(
-        dag, none, pm, W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog, ctx->tile_w, ctx->tile_h,
-        nullptr);
+    hd_status st = launch_paths(ctx, dag, none, pm, W, H, reinterpret_cast<uint4*>(d_paths), s);
+    if (st != HD_OK) return st;
+    k_trace_colors<false><<<grid_of(ctx, W, H), kBlock, 0, s>>>(dag, dev_colors(ctx), W, H,
+                                                                reinterpret_cast<const uint4*>(d_paths), d_colors,
+                                                                ctx->tile_w, ctx->tile_h, nullptr);
     HD_HIP(ctx, hipGetLastError());
-    return HD_OK;
+    return launch_shadows(ctx, dag, none, pm, W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog, s);
 }
 }  // namespace hd
 
@@ -656,11 +753,8 @@ hd_status hd_trace_paths(hd_ctx* ctx, const hd_ray_params* params, uint32_t W, u
     if (!ctx || !params || !d_paths || W == 0 || H == 0) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
     if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
-    const auto kp = ctx->fetch_mode == kFetchNode ? k_trace_paths<false, kFetchNode> : k_trace_paths<false, kFetchPointer>;
-    kp<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), stream_of(ctx, stream)>>>(
-        dev_dag(ctx), to_rayp(params), nullptr, W, H, reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h, nullptr);
-    HD_HIP(ctx, hipGetLastError());
-    return HD_OK;
+    return launch_paths(ctx, dev_dag(ctx), to_rayp(params), nullptr, W, H, reinterpret_cast<uint4*>(d_paths),
+                        stream_of(ctx, stream));
 }
 
 hd_status hd_trace_colors(hd_ctx* ctx, uint32_t W, uint32_t H, const uint32_t* d_paths, uint32_t* d_colors,
@@ -679,11 +773,23 @@ hd_status hd_trace_shadows(hd_ctx* ctx, const hd_ray_params* params, uint32_t W,
     if (!ctx || !params || !d_paths || !d_colors || W == 0 || H == 0) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
     if (ctx->g.levels > kMaxLevels || ctx->g.levels < 3) return fail(ctx, HD_ERR_INVALID, "levels outside 3..17");
-    const auto ks = ctx->fetch_mode == kFetchNode ? k_trace_shadows<false, kFetchNode> : k_trace_shadows<false, kFetchPointer>;
-    ks<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), stream_of(ctx, stream)>>>(
-        dev_dag(ctx), to_rayp(params), nullptr, W, H, reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
-        ctx->tile_w, ctx->tile_h, nullptr);
-    HD_HIP(ctx, hipGetLastError());
+    return launch_shadows(ctx, dev_dag(ctx), to_rayp(params), nullptr, W, H, reinterpret_cast<const uint4*>(d_paths),
+                          d_colors, bias, fog, stream_of(ctx, stream));
+}
+
+hd_status hd_set_trace_fetch(hd_ctx* ctx, uint32_t fetch_mode) {
+    if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
+    if (fetch_mode > 1) return fail(ctx, HD_ERR_INVALID, "fetch mode is 0 or 1");
+    ctx->fetch_mode = fetch_mode;
+    ctx->ptr_gen++;
+    return HD_OK;
+}
+
+hd_status hd_get_trace_launch(hd_ctx* ctx, uint32_t out[3]) {
+    if (!ctx || !out) return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    out[0] = ctx->tile_w;
+    out[1] = ctx->tile_h;
+    out[2] = ctx->fetch_mode;
     return HD_OK;
 }
 
@@ -695,13 +801,14 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
     if (!ctx->loaded || !ctx->colors_loaded) return fail(ctx, HD_ERR_INVALID, "no DAG or colours loaded");
     static const uint32_t shapes[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}, {2, 32}, {64, 1}};
     static const uint32_t fetches[] = {kFetchPointer, kFetchNode};
+    constexpr uint32_t kTrials = 6 * 2;  // tile shape x fetch strategy
     hipEvent_t e0, e1;
     HD_HIP(ctx, hipEventCreate(&e0));
     HD_HIP(ctx, hipEventCreate(&e1));
     float bestMs = 1e30f;
     uint32_t bw = ctx->tile_w, bh = ctx->tile_h, bf = ctx->fetch_mode;
     reps = std::max<uint32_t>(reps, 1);
-    for (uint32_t trial = 0; trial < 12; ++trial) {
+    for (uint32_t trial = 0; trial < kTrials; ++trial) {
         const auto& sh = shapes[trial % 6];
         ctx->tile_w = sh[0];
         ctx->tile_h = sh[1];

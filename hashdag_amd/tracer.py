@@ -97,13 +97,24 @@ class DAGTracer:
     def set_tile(self, dag: HashDAG, tile_w: int, tile_h: int) -> None:
         check(lib.hd_set_trace_tile(dag.handle, tile_w, tile_h), dag.handle)
 
+    def set_fetch(self, dag: HashDAG, fetch: int = 0) -> None:
+        """Node fetch strategy: 0 = child pointer then header, 1 = whole node."""
+        check(lib.hd_set_trace_fetch(dag.handle, fetch), dag.handle)
+
+
+def launch_config(dag: HashDAG) -> dict:
+    out = (C.c_uint32 * 3)()
+    check(lib.hd_get_trace_launch(dag.handle, out), dag.handle)
+    return {"tile": f"{out[0]}x{out[1]}", "fetch": ["pointer", "node"][out[2]]}
+
 
 def autotune(tracer: "DAGTracer", dag: HashDAG, params_list, reps: int = 2) -> dict:
-    """Picks the tracer's wavefront tile shape by timing full frames over the given cameras
-    (hd_autotune_trace); the choice stays in the DAG context for later traces."""
+    """Picks the tracer's wavefront tile shape and node fetch strategy by timing full frames
+    over the given cameras (hd_autotune_trace); the choice stays in the DAG context for later
+    traces."""
     arr = (hd_ray_params * len(params_list))(*params_list)
     bw, bh, ms = C.c_uint32(), C.c_uint32(), C.c_float()
     check(lib.hd_autotune_trace(dag.handle, arr, len(params_list), tracer.width, tracer.height,
                                 C.c_void_p(tracer.paths.data_ptr()), C.c_void_p(tracer.colors.data_ptr()), reps,
                                 C.byref(bw), C.byref(bh), C.byref(ms)), dag.handle)
-    return {"tile": f"{bw.value}x{bh.value}", "ms_per_frame": round(ms.value, 4)}
+    return {**launch_config(dag), "ms_per_frame": round(ms.value, 4)}

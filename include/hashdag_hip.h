@@ -131,6 +131,11 @@ hd_status hd_trace_shadows(hd_ctx* ctx, const hd_ray_params* params, uint32_t wi
 /* Tracer tuning (tile shape); 0 = default. Not in the reference (its launch shape is fixed
  * at (4,64) blocks, dag_tracer.cu:121-122). */
 hd_status hd_set_trace_tile(hd_ctx* ctx, uint32_t tile_w, uint32_t tile_h);
+/* Tracer tuning, not in the reference: node fetch strategy (0 = child pointer then header,
+ * 1 = whole node). Both give the same images. */
+hd_status hd_set_trace_fetch(hd_ctx* ctx, uint32_t fetch_mode);
+/* out: tile_w, tile_h, fetch mode */
+hd_status hd_get_trace_launch(hd_ctx* ctx, uint32_t out[3]);
 
 /* Autotuner for the tracer launch shape: times full frames (paths + colours + shadows) over
  * n_params cameras for each wavefront tile shape (8x8, 16x4, 4x16, 32x2, 2x32, 64x1), keeps

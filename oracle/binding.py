@@ -58,6 +58,7 @@ _SIGS = {
                                   C.c_int]),
     "orc_trace_colors": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p,
                                    u32p, C.c_int]),
+    "orc_set_any_hit_order": (None, [C.c_int]),
     "orc_trace_shadows": (C.c_int, [C.c_void_p, dblp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p,
                                     C.c_float, C.c_float, u64p, C.c_int]),
     "orc_edit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, fltp, C.c_float, u64p]),
@@ -270,6 +271,12 @@ class Oracle:
         paths = np.ascontiguousarray(paths, np.uint32)
         orc.orc_trace_colors(dag.h, colors.h, W, H, y0, y1, _p(paths), _p(out), threads)
         return out
+
+    @staticmethod
+    def set_any_hit_front_to_back(on: bool) -> None:
+        """Child order of the shadow walk: the reference's highest-bit-first order (default) or
+        front to back along the ray (the device's). The result is the same; the bytes differ."""
+        orc.orc_set_any_hit_order(1 if on else 0)
 
     @staticmethod
     def trace_shadows(dag: OracleDAG, params, paths, colors_in, W, H, bias=1.0, fog=0.0, rows=None, threads=8,

@@ -345,6 +345,8 @@ int orc_trace_colors(void* d, void* c, uint32_t W, uint32_t H, uint32_t y0, uint
     return 0;
 }
 
+void orc_set_any_hit_order(int front_to_back) { set_any_hit_front_to_back(front_to_back != 0); }
+
 int orc_trace_shadows(void* d, const double params[12], uint32_t W, uint32_t H, uint32_t y0, uint32_t y1,
                       const uint32_t* paths, uint32_t* colors, float bias, float fog, uint64_t* bytes, int threads) {
     const DAG& dag = *static_cast<DAG*>(d);

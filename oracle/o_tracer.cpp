@@ -109,6 +109,13 @@ struct StackEntry { u32 index; u8 childMask; u8 visitMask; };
 
 // Algorithmic HBM bytes (page-table words excluded: the build's device pool is addressed
 // by virtual word index): node header 4 B, child pointer 4 B, leaf 8 B.
+// Child order of the any-hit walk. The reference visits the highest set bit first
+// (tracer.cu:493); the device visits children front to back along the ray. The result of an
+// any-hit walk does not depend on the order (it is whether any voxel is reached through the
+// intersection masks, which depend only on the node and the ray), but the bytes it reads do:
+// the byte-counter comparison switches this to the device's order.
+static bool g_any_hit_front_to_back = false;
+
 template <bool anyHit>
 static bool descend(const DAG& dag, F3 o, F3 d, F3 inv, u8 order, Path& path, u64* bytes) {
     const Geometry& g = dag.g();
@@ -136,7 +143,8 @@ static bool descend(const DAG& dag, F3 o, F3 d, F3 inv, u8 order, Path& path, u6
             level = newLevel;
         }
         // tracer.cu:201 (ordered) / :493 (highest set bit first, 31 - clz)
-        const u8 next = anyHit ? (u8)(31 - __builtin_clz((u32)cache.visitMask)) : next_child(order, cache.visitMask);
+        const u8 next = anyHit && !g_any_hit_front_to_back ? (u8)(31 - __builtin_clz((u32)cache.visitMask))
+                                                            : next_child(order, cache.visitMask);
         cache.visitMask &= (u8)~(1u << next);
         path.descend(next);
         stack[level] = cache;
@@ -180,8 +188,11 @@ U3 trace_path(const DAG& dag, const RayParams& p, u32 px, u32 py, u64* bytes) {
 bool intersect_any(const DAG& dag, F3 o, F3 d, u64* bytes) {
     const F3 inv = to_f3(D3{1. / (double)d.x, 1. / (double)d.y, 1. / (double)d.z});
     Path path(0, 0, 0);
-    return descend<true>(dag, o, d, inv, 0, path, bytes);
+    const u8 order = (u8)((d.x < 0.f ? 4 : 0) + (d.y < 0.f ? 2 : 0) + (d.z < 0.f ? 1 : 0));
+    return descend<true>(dag, o, d, inv, order, path, bytes);
 }
+
+void set_any_hit_front_to_back(bool on) { g_any_hit_front_to_back = on; }
 
 // tracer.cu:254-451 for one pixel (debugColors = None, TOOL_OVERLAY = 0)
 u32 trace_color(const DAG& dag, const HashDAGColors& colors, U3 pp) {

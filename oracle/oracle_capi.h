@@ -19,7 +19,8 @@
  *                                                                tests/test_oracle_tracer.py
  *   trace_colors      colour index == Morton rank of the voxel in its level-10 subtree
  *                     (dense-grid brute force)                   tests/test_oracle_tracer.py
- *   trace_shadows     any-hit result == brute-force segment/voxel test on a dense grid
+ *   trace_shadows     any-hit result == brute-force segment/voxel test on a dense grid;
+ *                     the same result with either any-hit child order  tests/test_oracle_tracer.py
  *   edits             dense-grid brute force of sphere/box/paint voxel sets and colours
  *                                                                tests/test_oracle_edits.py
  *   GC                canonical form unchanged, idempotent, used words == reachable words +
@@ -98,6 +99,10 @@ int orc_trace_paths(void* dag, const double params[12], uint32_t W, uint32_t H, 
                     uint32_t* paths, uint64_t* bytes, int threads);
 int orc_trace_colors(void* dag, void* colors, uint32_t W, uint32_t H, uint32_t y0, uint32_t y1, const uint32_t* paths,
                      uint32_t* colors_out, int threads);
+/* child order of the any-hit (shadow) walk: 0 = the reference's highest-bit-first order
+ * (default), 1 = front to back along the ray. The shadow result is the same either way; the
+ * bytes read differ. Process-wide. */
+void orc_set_any_hit_order(int front_to_back);
 int orc_trace_shadows(void* dag, const double params[12], uint32_t W, uint32_t H, uint32_t y0, uint32_t y1,
                       const uint32_t* paths, uint32_t* colors_inout, float bias, float fog, uint64_t* bytes,
                       int threads);

@@ -284,6 +284,7 @@ u32 trace_color(const DAG& dag, const HashDAGColors& colors, U3 path);
 u32 shade_shadow(const DAG& dag, const RayParams& p, u32 W, u32 H, u32 px, u32 py, U3 path, u32 colorIn,
                  float bias, float fog, u64* bytes);
 bool intersect_any(const DAG& dag, F3 origin, F3 dir, u64* bytes);
+void set_any_hit_front_to_back(bool on);
 F3 sun_direction();
 
 // ---------------------------------------------------------------------------------------------

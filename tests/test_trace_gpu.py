@@ -88,7 +88,11 @@ def test_byte_counters_match_oracle():
     oparams = Oracle.ray_params(cam, info, g.levels, W, H)
     opaths, ob = Oracle.trace_paths(od, oparams, W, H, count=True)
     ocols = Oracle.trace_colors(od, oc, opaths, W, H)
-    _, osb = Oracle.trace_shadows(od, oparams, opaths, ocols, W, H, 1.0, 0.0, count=True)
+    Oracle.set_any_hit_front_to_back(True)  # the device walks shadow rays front to back
+    try:
+        _, osb = Oracle.trace_shadows(od, oparams, opaths, ocols, W, H, 1.0, 0.0, count=True)
+    finally:
+        Oracle.set_any_hit_front_to_back(False)
     assert b[0] == ob
     assert b[2] == osb
     assert b[1] > 0
@@ -120,9 +124,10 @@ def test_fog_close_to_oracle():
     assert np.abs(g8 - o8).max() <= 1
 
 
-@pytest.mark.parametrize("tile", [(16, 4), (4, 16), (64, 1), "auto"])
+@pytest.mark.parametrize("tile", [(16, 4, 0), (4, 16, 1), (64, 1, 0), (8, 8, 1), (2, 32, 1), "auto"])
 def test_tile_shapes_bit_exact(tile):
-    """Every wavefront tile shape (and the autotuner's pick) traces the same frame."""
+    """Every wavefront tile shape and node fetch strategy (and the autotuner's pick) traces
+    the same frame."""
     import torch
     from hashdag_amd import DAGInfo, DAGTracer, ray_params
     from hashdag_amd.tracer import autotune
@@ -139,7 +144,8 @@ def test_tile_shapes_bit_exact(tile):
         got = autotune(tracer, dag, prms, reps=1)
         assert got["tile"] in {"8x8", "16x4", "4x16", "32x2", "2x32", "64x1"}
     else:
-        tracer.set_tile(dag, *tile)
+        tracer.set_tile(dag, tile[0], tile[1])
+        tracer.set_fetch(dag, tile[2])
     for cam, prm in zip(cams, prms):
         tracer.resolve_paths(cam, info, dag, params=prm)
         tracer.resolve_colors(dag)
@@ -148,5 +154,6 @@ def test_tile_shapes_bit_exact(tile):
         gpu = tracer.colors.cpu().numpy().view(np.uint32)
         op = Oracle.ray_params(cam, info, g.levels, W, H)
         paths = Oracle.trace_paths(od, op, W, H)
+        assert np.array_equal(tracer.paths.cpu().numpy().view(np.uint32).reshape(paths.shape), paths)
         ref = Oracle.trace_shadows(od, op, paths, Oracle.trace_colors(od, oc, paths, W, H), W, H, 1.0, 0.0)
         assert np.array_equal(gpu, ref)
