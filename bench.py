@@ -1,6 +1,6 @@
 """Benchmark (one GPU): primary Mrays/s @1080p on the 2^15 coloured terrain; edited voxels/s of
-4096 mixed edits batched (one fused pass) vs plain (one pass per edit) on the 2^16 coloured
-terrain; p50 / p99 latency of a served request stream.
+4096 mixed edits batched (one fused pass) vs plain (one pass per edit), each followed by GC, on
+the 2^16 coloured terrain; p50 / p99 latency of a served request stream on the 2^16 terrain.
 
 Workload (one step = one frame for one camera): trace primary paths, resolve colours and
 one shadow ray per pixel at 1920x1080, cameras cycling over 64 seeded views placed 5-20 %
@@ -194,15 +194,13 @@ def main() -> int:
         },
     }
 
-    if args.serve_requests > 0:
-        result["serving"] = serve_bench(scene, geom, params, args)
     dag.close()
     if not args.no_cpu_baseline:
         od, oc, result["cpu_baseline"] = cpu_baseline(scene, geom, rparams, W, H, args.cpu_seconds)
         if not args.no_edits:
             edit_cpu = cpu_edit_baseline(od, oc, params, args)
         del od, oc
-    if not args.no_edits:
+    if not args.no_edits or args.serve_requests > 0:
         ED = args.edit_levels or D
         if ED != D:
             # the batched-vs-plain comparison on its own (deeper) scene
@@ -215,9 +213,12 @@ def main() -> int:
                 f"buckets top {egeom.bucket_size_top} low {egeom.bucket_size_low} words")
         else:
             eparams, escene, egeom = params, scene, geom
-        result["edits"] = edit_bench(escene, egeom, eparams, W, H, args)
-        if not args.no_cpu_baseline:
-            result["edits"]["cpu_baseline"] = edit_cpu
+        if not args.no_edits:
+            result["edits"] = edit_bench(escene, egeom, eparams, W, H, args)
+            if not args.no_cpu_baseline:
+                result["edits"]["cpu_baseline"] = edit_cpu
+        if args.serve_requests > 0:
+            result["serving"] = serve_bench(escene, egeom, eparams, args)
     print(json.dumps(result), flush=True)
     return 0
 
@@ -259,6 +260,7 @@ def edit_bench(scene, geom, params, W, H, args):
         return out
 
     def timed(d, fused, warm_edits):
+        """The edit list, then the GC pass that follows it (mark / compact of the stale nodes)."""
         d.edit_batch(warm_edits, fused=fused)
         for _ in range(d.undo_depth()[0]):
             d.undo()
@@ -267,18 +269,22 @@ def edit_bench(scene, geom, params, W, H, args):
         t0 = time.perf_counter()
         st = d.edit_batch(edits, fused=fused)
         torch.cuda.synchronize()
-        return st, time.perf_counter() - t0
+        t1 = time.perf_counter()
+        gc = d.remove_stale_nodes()
+        torch.cuda.synchronize()
+        return st, t1 - t0, time.perf_counter() - t1, gc
 
     dag = HashDAG(geom)
     dag.load_scene(scene)
-    st_f, t_f = timed(dag, True, edits)
-    log(f"fused: {st_f.num_voxels} voxels in {t_f * 1e3:.1f} ms ({st_f.passes} pass)")
+    st_f, t_f, gc_f, gcs_f = timed(dag, True, edits)
+    log(f"fused: {st_f.num_voxels} voxels in {t_f * 1e3:.1f} ms ({st_f.passes} pass), then GC {gc_f * 1e3:.1f} ms "
+        f"({gcs_f['words_before']} -> {gcs_f['words_after']} words)")
     img_f = frames(dag)
     dag.close()
     plain = HashDAG(geom)
     plain.load_scene(scene)
-    st_p, t_p = timed(plain, False, edits[:16])
-    log(f"plain: {st_p.num_voxels} voxels in {t_p * 1e3:.1f} ms ({st_p.passes} passes)")
+    st_p, t_p, gc_p, _ = timed(plain, False, edits[:16])
+    log(f"plain: {st_p.num_voxels} voxels in {t_p * 1e3:.1f} ms ({st_p.passes} passes), then GC {gc_p * 1e3:.1f} ms")
     img_p = frames(plain)
     plain.close()
     same = st_p.num_voxels == st_f.num_voxels and all(np.array_equal(a, b) for a, b in zip(img_f, img_p))
@@ -294,6 +300,11 @@ def edit_bench(scene, geom, params, W, H, args):
         "plain_ms": round(t_p * 1e3, 2),
         "batched_ms": round(t_f * 1e3, 2),
         "plain_ms_per_edit": round(t_p * 1e3 / len(edits), 3),
+        "gc_after_batched_ms": round(gc_f * 1e3, 2),
+        "gc_after_plain_ms": round(gc_p * 1e3, 2),
+        "gc_words_before_after": [gcs_f["words_before"], gcs_f["words_after"]],
+        "batched_with_gc_voxels_per_s": round(st_f.num_voxels / (t_f + gc_f), 1),
+        "batched_over_plain_with_gc": round((t_p + gc_p) / (t_f + gc_f), 2),
         "results_identical": same,
     }
 

@@ -139,6 +139,8 @@ _SIGS = {
     "hd_edit_batch": (C.c_int32, [C.c_void_p, C.POINTER(hd_edit), C.c_uint32, C.c_uint32, C.POINTER(hd_edit_stats),
                                   C.c_void_p]),
     "hd_gc": (C.c_int32, [C.c_void_p, C.c_uint32, u64p, C.c_void_p]),
+    "hd_gc_roots": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, u64p,
+                                C.c_void_p]),
     "hd_undo": (C.c_int32, [C.c_void_p]),
     "hd_redo": (C.c_int32, [C.c_void_p]),
     "hd_undo_clear": (C.c_int32, [C.c_void_p]),

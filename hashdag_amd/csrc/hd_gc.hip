@@ -29,12 +29,15 @@ __global__ void k_gc_fill(uint32_t* p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-__global__ void k_gc_seed(uint32_t* frontier0, uint32_t* counts, const uint32_t* d_root, uint32_t levels) {
+// level 0 frontier = the (distinct) roots kept alive
+__global__ void k_gc_seed(uint32_t* frontier0, uint32_t* counts, const uint32_t* roots, uint32_t n_roots,
+                          uint32_t levels) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (uint32_t l = 0; l <= levels; ++l) counts[l] = 0;
-        frontier0[0] = *d_root;
-        counts[0] = 1;
+        counts[0] = n_roots;
     }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_roots; i += gridDim.x * blockDim.x)
+        frontier0[i] = roots[i];
 }
 
 // mark children of the live nodes of `level` (an interior level)
@@ -170,12 +173,14 @@ __global__ __launch_bounds__(256) void k_gc_add(TableRef T, uint32_t level, cons
     }
 }
 
-__global__ void k_gc_root(const uint32_t* mkeys, const uint32_t* mvals, uint32_t mmask, uint32_t* d_root,
-                          uint32_t* errors) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && *errors == 0) {
-        const uint32_t r = map_find(mkeys, mvals, mmask, *d_root);
+// every kept root -> its address after GC
+__global__ void k_gc_roots(const uint32_t* mkeys, const uint32_t* mvals, uint32_t mmask, uint32_t* roots,
+                           uint32_t n_roots, uint32_t* errors) {
+    if (*errors) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_roots; i += gridDim.x * blockDim.x) {
+        const uint32_t r = map_find(mkeys, mvals, mmask, roots[i]);
         if (r == kInvalidNode) atomicOr(errors, GC_ERR_MAP);
-        else *d_root = r;
+        else roots[i] = r;
     }
 }
 
@@ -184,12 +189,37 @@ __global__ void k_gc_root(const uint32_t* mkeys, const uint32_t* mvals, uint32_t
 
 using namespace hd;
 
-extern "C" hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], void* stream) {
+// Roots kept alive by a GC: the current root, the caller's extra roots and (keep_undo) the
+// roots of every undo / redo frame, each once.
+static std::vector<uint32_t> gc_root_set(const hd_ctx* ctx, const uint32_t* extra, uint32_t n_extra, bool keep_undo) {
+    std::vector<uint32_t> r{ctx->root};
+    for (uint32_t i = 0; i < n_extra; ++i) r.push_back(extra[i]);
+    if (keep_undo)
+        for (const auto* frames : {&ctx->undo, &ctx->redo})
+            for (const UndoFrame& f : *frames) {
+                r.push_back(f.prev_root);
+                r.push_back(f.new_root);
+            }
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+    return r;
+}
+
+static uint32_t remap(const std::vector<uint32_t>& from, const std::vector<uint32_t>& to, uint32_t v) {
+    return to[std::lower_bound(from.begin(), from.end(), v) - from.begin()];
+}
+
+static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32_t n_extra, bool keep_undo,
+                        uint64_t stats[4], void* stream) {
     if (!ctx || !ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
     const Geom& g = ctx->g;
     const uint32_t L = g.leaf_level();
     if (max_level > L) return fail(ctx, HD_ERR_INVALID, "max_level above the leaf level");
-    if (ctx->root >= g.total_virtual()) return fail(ctx, HD_ERR_INVALID, "root is not an address of the table");
+    if (n_extra && !extra) return fail(ctx, HD_ERR_INVALID, "extra roots missing");
+    const std::vector<uint32_t> roots = gc_root_set(ctx, extra, n_extra, keep_undo);
+    for (uint32_t r : roots)
+        if (r >= g.total_virtual()) return fail(ctx, HD_ERR_INVALID, "a root is not an address of the table");
+    const uint32_t nRoots = (uint32_t)roots.size();
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     // capacities: used words per level bound the number of nodes
     std::vector<uint32_t> bs(g.total_buckets());
@@ -198,9 +228,12 @@ extern "C" hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], v
     std::vector<uint64_t> used(g.levels, 0);
     for (uint32_t l = 0; l < g.levels; ++l)
         for (uint32_t b = 0; b < g.buckets_per_level(l); ++b) used[l] += bs[g.bucket_global(l, b)];
-    uint64_t capMax = 16;
     // every node takes at least 2 words, so a level holds at most used / 2 nodes
-    for (uint32_t l = 0; l <= max_level; ++l) capMax = std::max<uint64_t>(capMax, used[l] / 2 + 16);
+    auto fcap = [&](uint32_t l) {
+        return (uint32_t)std::max<uint64_t>(std::max<uint64_t>(used[l] / 2 + 16, 16), l == 0 ? nRoots : 0);
+    };
+    uint64_t capMax = 16;
+    for (uint32_t l = 0; l <= max_level; ++l) capMax = std::max<uint64_t>(capMax, fcap(l));
     if (capMax >= (1u << 30)) return fail(ctx, HD_ERR_INVALID, "level too large for GC scratch");
     const uint32_t cap = (uint32_t)capMax;
     uint32_t mcap = 1;
@@ -217,13 +250,13 @@ extern "C" hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], v
         for (void* p : allocs) hipFree(p);
     };
     const uint64_t bitmapWords = (g.total_virtual() + 31) / 32;
-    uint32_t *bitmap, *counts, *errors, *tmp, *tlen, *mk[2], *mv[2];
+    uint32_t *bitmap, *counts, *errors, *tmp, *tlen, *mk[2], *mv[2], *droots;
     std::vector<uint32_t*> frontier(max_level + 1, nullptr);
     hipError_t e = alloc((void**)&bitmap, bitmapWords * 4);
     if (e == hipSuccess) e = alloc((void**)&counts, 64 * 4);
     if (e == hipSuccess) e = alloc((void**)&errors, 4);
-    for (uint32_t l = 0; l <= max_level && e == hipSuccess; ++l)
-        e = alloc((void**)&frontier[l], (uint64_t)std::max<uint64_t>(used[l] / 2 + 16, 16) * 4);
+    if (e == hipSuccess) e = alloc((void**)&droots, (uint64_t)nRoots * 4);
+    for (uint32_t l = 0; l <= max_level && e == hipSuccess; ++l) e = alloc((void**)&frontier[l], (uint64_t)fcap(l) * 4);
     if (e == hipSuccess) e = alloc((void**)&tmp, (uint64_t)cap * 36);
     if (e == hipSuccess) e = alloc((void**)&tlen, (uint64_t)cap * 4);
     for (int k = 0; k < 2 && e == hipSuccess; ++k) {
@@ -236,9 +269,12 @@ extern "C" hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], v
     }
     k_gc_fill<<<1024, 256, 0, s>>>(bitmap, bitmapWords, 0u);
     k_gc_fill<<<1, 64, 0, s>>>(errors, 1, 0u);
-    HD_HIP(ctx, hipMemcpyAsync(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice, s));
-    k_gc_seed<<<1, 1, 0, s>>>(frontier[0], counts, ctx->d_root, g.levels);
-    auto fcap = [&](uint32_t l) { return (uint32_t)std::max<uint64_t>(used[l] / 2 + 16, 16); };
+    e = hipMemcpyAsync(droots, roots.data(), (uint64_t)nRoots * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(ctx, e, "GC roots");
+    }
+    k_gc_seed<<<(nRoots + 255) / 256, 256, 0, s>>>(frontier[0], counts, droots, nRoots, g.levels);
     for (uint32_t l = 0; l < max_level; ++l)
         k_gc_mark<<<kGcGrid, 256, 0, s>>>(ctx->vpool, bitmap, frontier[l], frontier[l + 1], counts, l, fcap(l),
                                           fcap(l + 1), g.total_virtual(), errors);
@@ -266,20 +302,31 @@ extern "C" hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], v
         k_gc_add<<<kGcGrid, 256, 0, s>>>(T, l, counts, cap, frontier[l], tmp, tlen, mk[cur], mv[cur], mcap - 1, errors);
         cur ^= 1;
     }
-    k_gc_root<<<1, 1, 0, s>>>(mk[cur ^ 1], mv[cur ^ 1], mcap - 1, ctx->d_root, errors);
-    uint32_t err = 0, root = 0;
+    k_gc_roots<<<(nRoots + 255) / 256, 256, 0, s>>>(mk[cur ^ 1], mv[cur ^ 1], mcap - 1, droots, nRoots, errors);
+    uint32_t err = 0;
+    std::vector<uint32_t> newRoots(nRoots);
     std::vector<uint32_t> cnt(max_level + 1);
     HD_HIP(ctx, hipMemcpyAsync(&err, errors, 4, hipMemcpyDeviceToHost, s));
-    HD_HIP(ctx, hipMemcpyAsync(&root, ctx->d_root, 4, hipMemcpyDeviceToHost, s));
+    HD_HIP(ctx, hipMemcpyAsync(newRoots.data(), droots, (uint64_t)nRoots * 4, hipMemcpyDeviceToHost, s));
     HD_HIP(ctx, hipMemcpyAsync(cnt.data(), counts, cnt.size() * 4, hipMemcpyDeviceToHost, s));
     e = hipStreamSynchronize(s);
     release();
     if (e != hipSuccess) return hip_fail(ctx, e, "GC");
     if (err) return fail(ctx, err & GC_ERR_POOL ? HD_ERR_POOL_FULL : HD_ERR_BUCKET_FULL,
                          "GC failed (capacity); the table is no longer consistent");
-    ctx->root = root;
-    ctx->undo.clear();  // old roots point at collected nodes (engine.cpp:178-185 frees undo)
-    ctx->redo.clear();
+    ctx->root = remap(roots, newRoots, ctx->root);
+    HD_HIP(ctx, hipMemcpyAsync(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice, s));
+    for (uint32_t i = 0; i < n_extra; ++i) extra[i] = remap(roots, newRoots, extra[i]);
+    if (keep_undo) {
+        for (auto* frames : {&ctx->undo, &ctx->redo})
+            for (UndoFrame& f : *frames) {
+                f.prev_root = remap(roots, newRoots, f.prev_root);
+                f.new_root = remap(roots, newRoots, f.new_root);
+            }
+    } else {
+        ctx->undo.clear();  // old roots point at collected nodes (engine.cpp:178-185 frees undo)
+        ctx->redo.clear();
+    }
     if (stats) {
         uint64_t live = 0;
         for (uint32_t v : cnt) live += v;
@@ -296,4 +343,13 @@ extern "C" hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], v
         stats[3] = 0;
     }
     return HD_OK;
+}
+
+extern "C" hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], void* stream) {
+    return gc_run(ctx, max_level, nullptr, 0, false, stats, stream);
+}
+
+extern "C" hd_status hd_gc_roots(hd_ctx* ctx, uint32_t max_level, uint32_t keep_undo, uint32_t* roots,
+                                 uint32_t n_roots, uint64_t stats[4], void* stream) {
+    return gc_run(ctx, max_level, roots, n_roots, keep_undo != 0, stats, stream);
 }

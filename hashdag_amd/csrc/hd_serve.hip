@@ -270,7 +270,7 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
             const uint64_t need = estimate_pages(fused.data(), (uint32_t)fused.size(), ctx->g.page_size);
             uint32_t gcRan = 0;
             if (st == HD_OK && mem.free_pages < need + cfg->reserve_pages) {
-                st = hd_gc(ctx, ctx->g.leaf_level(), nullptr, editStream);
+                st = hd_gc_roots(ctx, ctx->g.leaf_level(), cfg->keep_undo, nullptr, 0, nullptr, editStream);
                 gcRan = 1;
                 if (st == HD_OK) st = hd_mem_info(ctx, &mem);
                 if (st == HD_OK && mem.free_pages < need) st = HD_ERR_POOL_FULL;
@@ -280,7 +280,7 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
             if ((st == HD_ERR_BUCKET_FULL || st == HD_ERR_POOL_FULL) && !gcRan) {
                 // a full bucket or page budget: collect stale nodes once and retry
                 gcRan = 1;
-                if (hd_gc(ctx, ctx->g.leaf_level(), nullptr, editStream) == HD_OK)
+                if (hd_gc_roots(ctx, ctx->g.leaf_level(), cfg->keep_undo, nullptr, 0, nullptr, editStream) == HD_OK)
                     st = hd_edit_batch(ctx, fused.data(), (uint32_t)fused.size(), 1, &es, editStream);
             }
             if (st == HD_OK && !cfg->keep_undo) hd_undo_clear(ctx);

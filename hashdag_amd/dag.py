@@ -216,12 +216,24 @@ class HashDAG:
         around dest + (center - source) gets the voxels (and colours) at source + (p - dest)."""
         return self.edit_batch([(5, center, radius, source, dest)])
 
-    def remove_stale_nodes(self, max_level: int | None = None, stream=None) -> dict:
-        """GC (hash_dag.h:407-547); the engine calls it with levels - 2 (engine.cpp:182)."""
+    def remove_stale_nodes(self, max_level: int | None = None, stream=None, keep_undo: bool = False,
+                           extra_roots=None) -> dict:
+        """GC (hash_dag.h:407-547); the engine calls it with levels - 2 (engine.cpp:182).
+
+        Without options the undo history is cleared, as the engine does. keep_undo keeps the
+        nodes of every undo / redo frame alive and the history usable; extra_roots (level-0
+        node addresses) are kept alive too and returned as "roots", rewritten to their
+        addresses after GC."""
         st = np.zeros(4, np.uint64)
         lvl = self.geometry.leaf_level if max_level is None else max_level
-        check(lib.hd_gc(self._h, lvl, ptr(st, C.c_uint64), stream), self._h)
-        return {"live_nodes": int(st[0]), "words_before": int(st[1]), "words_after": int(st[2])}
+        roots = np.ascontiguousarray(extra_roots if extra_roots is not None else [], np.uint32)
+        if keep_undo or roots.size:
+            check(lib.hd_gc_roots(self._h, lvl, 1 if keep_undo else 0, ptr(roots, C.c_uint32), roots.size,
+                                  ptr(st, C.c_uint64), stream), self._h)
+        else:
+            check(lib.hd_gc(self._h, lvl, ptr(st, C.c_uint64), stream), self._h)
+        return {"live_nodes": int(st[0]), "words_before": int(st[1]), "words_after": int(st[2]),
+                "roots": [int(r) for r in roots]}
 
     def undo(self) -> None:
         check(lib.hd_undo(self._h), self._h)

@@ -197,6 +197,13 @@ hd_status hd_reserve_edit_scratch(hd_ctx* ctx, uint32_t items_per_level, uint32_
  * re-appended level by level bottom-up with their children remapped, full nodes first.
  * Clears the undo history. stats (may be NULL): live nodes, words before, words after, 0. */
 hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], void* stream);
+/* GC that also keeps alive the nodes reachable from extra level-0 roots (rewritten in place to
+ * their addresses after GC) and, with keep_undo = 1, from the roots of every undo / redo
+ * frame, which are rewritten so the history survives the GC. No counterpart in the reference:
+ * its GC marks from firstNodeIndex only and the engine frees the undo history with it
+ * (hash_dag.h:426, engine.cpp:178-185). hd_gc is this call with no extra roots. */
+hd_status hd_gc_roots(hd_ctx* ctx, uint32_t max_level, uint32_t keep_undo, uint32_t* roots, uint32_t n_roots,
+                      uint64_t stats[4], void* stream);
 
 /* Frame graph: one fused edit pass (up to max_edits editors) followed by the trace of the
  * resulting DAG (paths, colours, shadows into d_paths / d_colors), captured once as a HIP graph
@@ -242,7 +249,7 @@ typedef struct hd_serve_config {
     uint32_t trace_slots;      /* concurrent traces (streams with frame buffers), 1..16 */
     uint32_t max_batch_edits;  /* cap on edits fused into one pass */
     uint32_t realtime;         /* 1: requests arrive at arrival_s; 0: all at once */
-    uint32_t keep_undo;        /* 0: clear the undo history after every edit batch */
+    uint32_t keep_undo;        /* 0: clear the undo history after every edit batch; 1: keep it, also across GC */
     uint64_t reserve_pages;    /* GC watermark: collect when free pages < estimate + reserve
                                   (a batch is rejected only if free pages < estimate after GC) */
     float shadow_bias, fog_density;

@@ -103,3 +103,69 @@ def test_gc_keeps_trace_bit_exact():
         frames.append(out)
     for a, b in zip(*frames):
         assert np.array_equal(a, b)
+
+
+def test_gc_keeps_undo_history():
+    """GC with keep_undo: the nodes of every undo / redo frame stay alive and are rewritten, so
+    undo and redo after the GC reach the same DAGs and colours as before it; the nodes of
+    redo frames discarded by a new edit are collected."""
+    from tests.test_edit_gpu import gpu_state_in_oracle
+
+    p, sc, g, dag, od, oc = make("d13")
+    edits = random_edits(p, 7, seed=41)
+    states = [(od.canonical(), oc.canonical(od))]
+    for kind, center, r in edits[:5]:
+        dag.edit(kind, center, r)
+    dag.undo()
+    dag.undo()  # the last two edits become redo frames ...
+    for kind, center, r in edits[:3]:
+        od.edit(kind, center, r, oc)
+        states.append((od.canonical(), oc.canonical(od)))
+    for kind, center, r in edits[5:]:  # ... and are discarded by these
+        dag.edit(kind, center, r)
+        od.edit(kind, center, r, oc)
+        states.append((od.canonical(), oc.canonical(od)))
+    assert dag.undo_depth() == (5, 0)
+    live = reachable_on_device(dag, g.leaf_level)
+    st = dag.remove_stale_nodes(keep_undo=True)
+    assert st["words_after"] < st["words_before"]
+    assert st["live_nodes"] >= live
+    assert dag.undo_depth() == (5, 0)
+    assert_same(dag, sc, od, oc, "after GC")
+    for k in range(5):
+        dag.undo()
+        god, goc = gpu_state_in_oracle(dag, sc)
+        want_dag, want_col = states[-2 - k]
+        assert np.array_equal(god.canonical(), want_dag), f"undo {k + 1}: DAG differs"
+        assert np.array_equal(goc.canonical(god), want_col), f"undo {k + 1}: colours differ"
+    for k in range(5):
+        dag.redo()
+    assert_same(dag, sc, od, oc, "redo after GC")
+    # a second GC with the history kept changes nothing reachable
+    st2 = dag.remove_stale_nodes(keep_undo=True)
+    assert st2["live_nodes"] == st["live_nodes"]
+    assert_same(dag, sc, od, oc, "second GC")
+
+
+def test_gc_extra_roots_survive():
+    """Extra roots keep their DAGs alive through a GC and come back rewritten."""
+    from tests.test_edit_gpu import gpu_state_in_oracle
+
+    p, sc, g, dag, od, oc = make("d13")
+    base = od.canonical()
+    r0 = dag.root
+    for kind, center, r in random_edits(p, 6, seed=43):
+        dag.edit(kind, center, r)
+        od.edit(kind, center, r, oc)
+    st = dag.remove_stale_nodes(extra_roots=[r0])
+    assert dag.undo_depth() == (0, 0)
+    assert_same(dag, sc, od, oc, "current DAG after GC")
+    edited = dag.root
+    dag.root = st["roots"][0]
+    god, _ = gpu_state_in_oracle(dag, sc)
+    assert np.array_equal(god.canonical(), base)
+    dag.root = edited
+    # without the extra root the base DAG's own nodes are collected
+    st2 = dag.remove_stale_nodes()
+    assert st2["live_nodes"] < st["live_nodes"]
+    assert_same(dag, sc, od, oc, "after second GC")
