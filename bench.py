@@ -51,6 +51,9 @@ def main() -> int:
                     help="scene depth of the batched-vs-plain edit comparison (0 = the traced scene)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the served stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--c5", action="store_true",
+                    help="also run the depth-17 workload: instancing, 512^3 copies, large carves in a frame graph, "
+                         "undo kept across a full GC, a 3840x2160 frame")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -219,6 +222,8 @@ def main() -> int:
                 result["edits"]["cpu_baseline"] = edit_cpu
         if args.serve_requests > 0:
             result["serving"] = serve_bench(escene, egeom, eparams, args)
+    if args.c5:
+        result["c5"] = c5_bench(args)
     print(json.dumps(result), flush=True)
     return 0
 
@@ -327,6 +332,108 @@ def serve_bench(scene, geom, params, args):
     summ["workload"] = (f"{len(reqs)} requests, Poisson {args.serve_rate}/s on the 2^{params.levels} coloured terrain: "
                         f"60% trace (640x360..3840x2160), 40% edit (1-64 sphere/box, radius log-uniform 2-128)")
     return summ
+
+
+def c5_bench(args):
+    """Depth-17 coloured terrain (a 65536^2 footprint: the full 131072^2 footprint needs about
+    1.3 G words at one level, beyond what the u32 virtual address space holds with the uniform
+    bucket geometry of the low levels), then, each phase timed on its own:
+      2000 instanced boxes and spheres standing on the terrain (heights 16-512), one fused pass;
+      32 copies of 512^3 regions to seeded destinations (one pass each);
+      64 sphere carves of radius 512-1024 (eagerly once, undone), then the same carves and a
+      3840x2160 frame as one replay of a captured graph;
+      a full GC that keeps the undo history;
+      3840x2160 primary + colour + shadow frames.
+    Checked on the way: GC leaves the traced frame unchanged and the undo history usable (undo
+    then redo of the carve pass reproduces the frame)."""
+    import numpy as np
+    import torch
+
+    from hashdag_amd import DAGInfo, DAGTracer, HashDAG, TerrainParams, generate_terrain, ray_params
+    from hashdag_amd.graph import FrameGraph
+    from hashdag_amd.serve import frame_checksum
+    from hashdag_amd.workload import cameras, carves, copy_edits, instances
+
+    D, W, H = 17, 3840, 2160
+    params = TerrainParams(levels=D, footprint=1 << 16, seed=args.seed, colors=True)
+    t0 = time.time()
+    scene = generate_terrain(params)
+    geom = fit_geometry(D, scene.level_words)
+    log(f"c5 scene 2^{D} (footprint 2^16): {scene.num_voxels} voxels, {scene.dag.size} BasicDAG words, generated in "
+        f"{time.time() - t0:.1f}s; buckets top {geom.bucket_size_top} low {geom.bucket_size_low} words, "
+        f"{geom.total_virtual()} virtual words")
+    t0 = time.time()
+    dag = HashDAG(geom)
+    dag.load_scene(scene)
+    del scene
+    log(f"c5 loaded in {time.time() - t0:.1f}s")
+    dinfo = DAGInfo.unit(D)
+    cam = cameras(params, 1, seed=args.seed)[0]
+    prm = ray_params(cam, dinfo, D, W, H)
+    tr = DAGTracer(W, H)
+    out = {"workload": "depth 17, 65536^2 coloured terrain footprint; 2000 instances (heights 16-512), 32 copies "
+                       "of 512^3 regions, 64 carves (radius 512-1024) + 3840x2160 frame in one graph replay, "
+                       "GC keeping undo, 3840x2160 frames", "phases_ms": {}, "voxels": {}}
+
+    def timed(name, fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        out["phases_ms"][name] = round((time.perf_counter() - t) * 1e3, 2)
+        return r
+
+    st = timed("instances_2000", lambda: dag.edit_batch(instances(params, 2000, seed=args.seed + 5), fused=True))
+    out["voxels"]["instances_2000"] = int(st.num_voxels)
+    log(f"c5 instances: {st.num_voxels} voxels in {out['phases_ms']['instances_2000']} ms")
+    st = timed("copies_32", lambda: dag.edit_batch(copy_edits(params, 32, seed=args.seed + 6), fused=True))
+    out["voxels"]["copies_32"] = int(st.num_voxels)
+    out["copy_passes"] = int(st.passes)
+    log(f"c5 copies: {st.num_voxels} voxels in {out['phases_ms']['copies_32']} ms ({st.passes} passes)")
+    carve_list = carves(params, 64, seed=args.seed + 7)
+    # the carves once eagerly (sizes the edit scratch the graph bakes in), then undone
+    st = timed("carves_64_eager", lambda: dag.edit_batch(carve_list, fused=True))
+    dag.undo()
+    fg = FrameGraph(dag, tr, 64)
+    timed("graph_capture_empty_batch", lambda: fg.launch([], prm))
+    st, replayed = timed("carves_64_and_frame_graph", lambda: fg.launch(carve_list, prm))
+    out["voxels"]["carves_64"] = int(st.num_voxels)
+    out["graph_replayed"] = bool(replayed)
+    log(f"c5 carves + frame (graph replay {replayed}): {st.num_voxels} voxels in "
+        f"{out['phases_ms']['carves_64_and_frame_graph']} ms")
+    fg.close()
+    before = frame_checksum(dag, tr.colors, W, H)
+    depth = dag.undo_depth()
+    gc = timed("gc_keep_undo", lambda: dag.remove_stale_nodes(keep_undo=True))
+    out["gc_words_before_after"] = [gc["words_before"], gc["words_after"]]
+    out["undo_depth"] = list(depth)
+    log(f"c5 GC: {gc['words_before']} -> {gc['words_after']} words in {out['phases_ms']['gc_keep_undo']} ms, "
+        f"undo depth {depth} -> {dag.undo_depth()}")
+
+    def frame():
+        tr.resolve_paths(None, dinfo, dag, params=prm)
+        tr.resolve_colors(dag)
+        tr.resolve_shadows(None, dinfo, dag, 1.0, 0.0, params=prm)
+
+    frame()
+    after = frame_checksum(dag, tr.colors, W, H)
+    reps = 8
+    timed("frames_4k_x8", lambda: [frame() for _ in range(reps)])
+    out["frame_mrays_per_s"] = round(W * H * reps / (out["phases_ms"]["frames_4k_x8"] * 1e-3) / 1e6, 1)
+    dag.undo()
+    frame()
+    undone = frame_checksum(dag, tr.colors, W, H)
+    dag.redo()
+    frame()
+    redone = frame_checksum(dag, tr.colors, W, H)
+    out["gc_keeps_frame"] = bool(before == after)
+    out["undo_redo_after_gc_keeps_frame"] = bool(redone == after and undone != after)
+    out["undo_depth_after_gc"] = list(dag.undo_depth())
+    dag.close()
+    log(f"c5: {out}")
+    if not (out["gc_keeps_frame"] and out["undo_redo_after_gc_keeps_frame"] and out["undo_depth_after_gc"] == list(depth)):
+        raise RuntimeError("depth-17 workload: GC or undo changed the traced frame")
+    return out
 
 
 def cpu_edit_baseline(od, oc, params, args):

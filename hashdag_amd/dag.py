@@ -54,16 +54,17 @@ class Geometry:
         with the reference's bucket sizes when they hold the scene and larger buckets (in that
         order of preference) when they do not. A level is 'top' while its words fit `top_fill`
         of the top capacity (the reference's 9 when all do); every level must fit its fill
-        limit, and the virtual space must stay below 2^32 words. `headroom` scales the words
-        for later growth."""
+        limit, and the virtual space must stay below 2^32 words. More than the reference's 9
+        top levels are used only when no geometry with 9 fits (deep scenes whose uniform low
+        levels would not fit 32-bit addresses). `headroom` scales the words for later growth."""
         last = None
-        for size_top, size_low in ((1024, 4096), (2048, 4096), (4096, 4096), (1024, 8192), (2048, 8192),
-                                   (4096, 8192)):
+        sizes = ((1024, 4096), (2048, 4096), (4096, 4096), (1024, 8192), (2048, 8192), (4096, 8192))
+        for max_top, (size_top, size_low) in [(9, sz) for sz in sizes] + [(levels - 1, sz) for sz in sizes]:
             g = Geometry(levels=levels, color_tree_levels=color_tree_levels, bucket_size_top=size_top,
                          bucket_size_low=size_low)
             top_cap = (1 << g.bucket_bits_top) * g.bucket_size_top
             n_top = 0
-            while n_top < min(9, levels) and n_top < len(level_words) and \
+            while n_top < min(max_top, levels) and n_top < len(level_words) and \
                     level_words[n_top] * headroom <= top_fill * top_cap:
                 n_top += 1
             g.num_top_levels = n_top

@@ -83,3 +83,35 @@ def mixed_edits(params: TerrainParams, n: int, seed: int = 4, rmin: float = 2.0,
             kind = SPHERE_PAINT
         out.append((kind, c, r))
     return out
+
+
+def instances(params: TerrainParams, n: int, seed: int = 6, hmin: float = 16.0, hmax: float = 512.0):
+    """n instanced objects standing on the terrain: boxes and spheres (equally likely) of
+    height log-uniform in [hmin, hmax], centred one radius above a seeded surface point."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        x, y, z = _centre(rng, params)
+        h = float(np.exp(rng.uniform(np.log(hmin), np.log(hmax))))
+        r = float(np.float32(h / 2))
+        kind = BOX_ADD if rng.integers(0, 2) else SPHERE_ADD
+        out.append((kind, (x, float(np.float32(y + r)), z), r))
+    return out
+
+
+def copy_edits(params: TerrainParams, n: int, seed: int = 7, radius: float = 256.0):
+    """n copies of the (2 radius)^3 region around a seeded surface point to another seeded
+    surface point: [(COPY, source, radius, source, dest), ...] (translation only)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        src = _centre(rng, params)
+        dst = _centre(rng, params)
+        out.append((COPY, src, float(radius), src, dst))
+    return out
+
+
+def carves(params: TerrainParams, n: int, seed: int = 8, rmin: float = 512.0, rmax: float = 1024.0):
+    """n sphere carves of radius uniform in [rmin, rmax] at seeded surface points."""
+    rng = np.random.default_rng(seed)
+    return [(SPHERE_CARVE, _centre(rng, params), float(np.float32(rng.uniform(rmin, rmax)))) for _ in range(n)]
