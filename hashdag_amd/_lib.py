@@ -97,7 +97,7 @@ class hd_edit(C.Structure):
 
 class hd_edit_stats(C.Structure):
     _fields_ = [("num_voxels", C.c_uint64), ("new_words", C.c_uint64), ("leaves_rebuilt", C.c_uint64),
-                ("root", C.c_uint32), ("passes", C.c_uint32)]
+                ("root", C.c_uint32), ("passes", C.c_uint32), ("colors_encoded", C.c_uint64)]
 
 
 HD_REQ_TRACE, HD_REQ_EDIT = 0, 1

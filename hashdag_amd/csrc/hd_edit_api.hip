@@ -320,7 +320,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
     if (!ctx || (!edits && n) ) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
     if (n == 0) {
-        if (stats) *stats = hd_edit_stats{0, 0, 0, ctx->root, 0};
+        if (stats) *stats = hd_edit_stats{0, 0, 0, ctx->root, 0, 0};
         return HD_OK;
     }
     for (uint32_t i = 0; i < n; ++i)
@@ -361,6 +361,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             out.num_voxels += rb.stats[0] + copy_voxels(edits + first, last - first);
             out.new_words += rb.stats[1];
             out.leaves_rebuilt += rb.counters[4];
+            out.colors_encoded += rb.stats[2];
             out.passes++;
             first = last;
         }
@@ -378,6 +379,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             out.num_voxels += rb.stats[0] + copy_voxels(edits + i, 1);
             out.new_words += rb.stats[1];
             out.leaves_rebuilt += rb.counters[4];
+            out.colors_encoded += rb.stats[2];
             out.passes++;
         }
     }
@@ -605,7 +607,7 @@ hd_status hd_frame_graph_launch(hd_frame_graph* g, const hd_edit* edits, uint32_
     if (st != HD_OK) return st;
     ctx->undo.push_back(std::move(frame));
     ctx->redo.clear();
-    if (stats) *stats = hd_edit_stats{rb.stats[0], rb.stats[1], rb.counters[4], rb.root, 1u};
+    if (stats) *stats = hd_edit_stats{rb.stats[0], rb.stats[1], rb.counters[4], rb.root, 1u, rb.stats[2]};
     if (replayed) *replayed = 1;
     return HD_OK;
 }

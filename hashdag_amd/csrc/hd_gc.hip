@@ -40,29 +40,46 @@ __global__ void k_gc_seed(uint32_t* frontier0, uint32_t* counts, const uint32_t*
         frontier0[i] = roots[i];
 }
 
-// mark children of the live nodes of `level` (an interior level)
+// mark children of the live nodes of `level` (an interior level): 8 lanes per node, one per
+// child slot; a wavefront appends the children it marked first with one atomic
 __global__ __launch_bounds__(256) void k_gc_mark(const uint32_t* __restrict__ vpool, uint32_t* __restrict__ bitmap,
                                                  const uint32_t* __restrict__ frontier, uint32_t* __restrict__ next,
                                                  uint32_t* counts, uint32_t level, uint32_t capCur, uint32_t cap,
                                                  uint64_t numVirtual, uint32_t* errors) {
     const uint32_t n = min(counts[level], capCur);
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (uint64_t)n * 8; t += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t va = frontier[t >> 3], c = (uint32_t)(t & 7u);
-        const uint32_t mask = vpool[va] & 0xFFu;
-        if (!(mask & (1u << c))) continue;
-        const uint32_t child = vpool[va + child_off(mask, c)];
-        if (child >= numVirtual) {  // not an address of this table
-            atomicOr(errors, GC_ERR_MAP);
-            continue;
+    const uint64_t total = (uint64_t)n * 8;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < total; t0 += stride) {
+        const uint64_t t = t0 + lane;
+        uint32_t child = 0;
+        bool first = false;
+        if (t < total) {
+            const uint32_t va = frontier[t >> 3], c = (uint32_t)(t & 7u);
+            const uint32_t mask = vpool[va] & 0xFFu;
+            if (mask & (1u << c)) {
+                child = vpool[va + child_off(mask, c)];
+                if (child >= numVirtual) {  // not an address of this table
+                    atomicOr(errors, GC_ERR_MAP);
+                } else {
+                    const uint32_t bit = 1u << (child & 31u);
+                    first = !(atomicOr(&bitmap[child >> 5], bit) & bit);
+                }
+            }
         }
-        const uint32_t bit = 1u << (child & 31u);
-        if (atomicOr(&bitmap[child >> 5], bit) & bit) continue;
-        const uint32_t slot = atomicAdd(&counts[level + 1], 1u);
-        if (slot >= cap) {
-            atomicOr(errors, GC_ERR_CAP);
-            continue;
+        const uint64_t m = __ballot(first);
+        if (m) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&counts[level + 1], (uint32_t)__popcll(m));
+            base = __shfl(base, (int)leader, 64);
+            if (first) {
+                const uint32_t slot =
+                    base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (slot >= cap) atomicOr(errors, GC_ERR_CAP);
+                else next[slot] = child;
+            }
         }
-        next[slot] = child;
     }
 }
 

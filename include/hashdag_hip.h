@@ -175,6 +175,7 @@ typedef struct hd_edit_stats {
     uint64_t leaves_rebuilt; /* "num color leaves rebuilt" (hash_dag.h:379) */
     uint32_t root;           /* new firstNodeIndex */
     uint32_t passes;         /* device passes run */
+    uint64_t colors_encoded; /* colours written into the rebuilt colour leaves */
 } hd_edit_stats;
 
 /* Engine::edit<T> -> HashDAG::edit_threads<T> (engine.h:77-103, hash_dag.h:255-405) for each
