@@ -92,7 +92,8 @@ class hd_terrain_params(C.Structure):
 
 class hd_edit(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("center", C.c_float * 3), ("radius", C.c_float), ("source", C.c_float * 3),
-                ("dest", C.c_float * 3)]
+                ("dest", C.c_float * 3), ("swirl", C.c_uint32), ("swirl_period", C.c_float),
+                ("transform", C.c_double * 9)]
 
 
 class hd_edit_stats(C.Structure):

@@ -34,7 +34,10 @@ struct DevEdit {
     float bmin[3], bmax[3];
     uint32_t single;
     float src[3], dst[3];
+    uint32_t swirl;      // copy: turn the relative position about y by y / swirl_period
+    float swirl_period;
     uint32_t pad;
+    double inv[9];       // copy: inverse of the copy's transform (row-major, FP64)
 };
 
 // Work items of one level (structure of arrays, capacity `cap`; child arrays are 8 * cap).

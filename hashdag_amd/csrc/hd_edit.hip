@@ -215,12 +215,26 @@ __device__ __forceinline__ bool leaf_valid_index(const LeafDesc& d, uint64_t i) 
 // float -> uint32 as x86-64 code converts it (truncate to int64, keep the low 32 bits)
 __device__ __forceinline__ uint32_t f2u_x86(float f) { return (uint32_t)(int64_t)f; }
 
-// source voxel of a destination voxel corner: source + (p - dest), per component in FP32
+// source voxel of a destination voxel corner (destination_to_source, hash_dag_editors.h:364-379):
+// p - dest in FP32, swirled about y, through the inverse transform in FP64, plus source. The
+// swirl's cos / sin are FP64 rounded to FP32 (the reference calls the host's cosf / sinf).
 __device__ __forceinline__ void copy_source(const DevEdit& e, float px, float py, float pz, uint32_t& sx, uint32_t& sy,
                                             uint32_t& sz) {
-    sx = f2u_x86(e.src[0] + (px - e.dst[0]));
-    sy = f2u_x86(e.src[1] + (py - e.dst[1]));
-    sz = f2u_x86(e.src[2] + (pz - e.dst[2]));
+    float rx = px - e.dst[0], ry = py - e.dst[1], rz = pz - e.dst[2];
+    if (e.swirl) {
+        const float angle = __fdiv_rn(ry, e.swirl_period);
+        const float c = (float)cos((double)angle), s = (float)sin((double)angle);
+        const float x = rx, z = rz;
+        rx = x * c - z * s;
+        rz = x * s + z * c;
+    }
+    const double vx = (double)rx, vy = (double)ry, vz = (double)rz;
+    const float tx = (float)(e.inv[0] * vx + e.inv[1] * vy + e.inv[2] * vz);
+    const float ty = (float)(e.inv[3] * vx + e.inv[4] * vy + e.inv[5] * vz);
+    const float tz = (float)(e.inv[6] * vx + e.inv[7] * vy + e.inv[8] * vz);
+    sx = f2u_x86(e.src[0] + tx);
+    sy = f2u_x86(e.src[1] + ty);
+    sz = f2u_x86(e.src[2] + tz);
 }
 
 // 64-bit leaf mask of the leaf at leaf coordinates (lx, ly, lz) (voxel >> 2), 0 when absent

@@ -97,17 +97,48 @@ __global__ void k_reset(uint32_t* counters, uint64_t* stats) {
     }
 }
 
+// The copy's transform; all zeros stands for the identity (callers that zero the struct).
+static void copy_transform(const hd_edit& e, double T[9]) {
+    bool zero = true;
+    for (int i = 0; i < 9; ++i) zero &= e.transform[i] == 0.0;
+    for (int i = 0; i < 9; ++i) T[i] = zero ? (i % 4 == 0 ? 1.0 : 0.0) : e.transform[i];
+}
+
 DevEdit make_edit(const hd_edit& e) {
     DevEdit d{};
     d.kind = e.kind;
     for (int i = 0; i < 3; ++i) d.c[i] = e.center[i];
     d.radius = e.radius;
     d.r2 = e.radius * e.radius;
+    float box[3] = {e.center[0], e.center[1], e.center[2]};
+    float boxRadius = e.radius;
+    if (e.kind == EDIT_COPY) {
+        // BoxEditorBase(dest + float3(T * (center - source)), radius * max|T * (1,1,1)|)
+        // (hash_dag_editors.h:350-351; Matrix3x3 in FP64, Matrix3x3.h:190-243, 399-406)
+        double T[9];
+        copy_transform(e, T);
+        const double r[3] = {(double)(e.center[0] - e.source[0]), (double)(e.center[1] - e.source[1]),
+                             (double)(e.center[2] - e.source[2])};
+        double rowMax = 0;
+        for (int i = 0; i < 3; ++i) {
+            box[i] = e.dest[i] + (float)(T[3 * i] * r[0] + T[3 * i + 1] * r[1] + T[3 * i + 2] * r[2]);
+            rowMax = std::max(rowMax, std::fabs(T[3 * i] * 1.0 + T[3 * i + 1] * 1.0 + T[3 * i + 2] * 1.0));
+        }
+        boxRadius = (float)((double)e.radius * rowMax);
+        // Matrix3x3::Inverse: adjugate times 1 / Determinate
+        const double a[9] = {T[8] * T[4] - T[7] * T[5], T[7] * T[2] - T[8] * T[1], T[5] * T[1] - T[4] * T[2],
+                             T[6] * T[5] - T[8] * T[3], T[8] * T[0] - T[6] * T[2], T[3] * T[2] - T[5] * T[0],
+                             T[7] * T[3] - T[6] * T[4], T[6] * T[1] - T[7] * T[0], T[4] * T[0] - T[3] * T[1]};
+        const double det = T[0] * (T[8] * T[4] - T[7] * T[5]) - T[3] * (T[8] * T[1] - T[7] * T[2]) +
+                           T[6] * (T[5] * T[1] - T[4] * T[2]);
+        const double sc = 1 / det;
+        for (int i = 0; i < 9; ++i) d.inv[i] = a[i] * sc;
+        d.swirl = e.swirl ? 1u : 0u;
+        d.swirl_period = e.swirl_period;
+    }
     for (int i = 0; i < 3; ++i) {
-        // a copy's box is centred at dest + (center - source) (hash_dag_editors.h:350-351)
-        const float c = e.kind == EDIT_COPY ? e.dest[i] + (e.center[i] - e.source[i]) : e.center[i];
-        d.bmin[i] = c - e.radius;
-        d.bmax[i] = c + e.radius;
+        d.bmin[i] = box[i] - boxRadius;
+        d.bmax[i] = box[i] + boxRadius;
         d.src[i] = e.source[i];
         d.dst[i] = e.dest[i];
     }
@@ -326,6 +357,17 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
     for (uint32_t i = 0; i < n; ++i)
         if (edits[i].kind > EDIT_COPY || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
             return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (edits[i].kind != EDIT_COPY) continue;
+        // Matrix3x3::IsInvertible (Matrix3x3.h:245-248)
+        double T[9];
+        copy_transform(edits[i], T);
+        const double det = T[0] * (T[8] * T[4] - T[7] * T[5]) - T[3] * (T[8] * T[1] - T[7] * T[2]) +
+                           T[6] * (T[5] * T[1] - T[4] * T[2]);
+        if (!(std::fabs(det) > 0.00001)) return fail(ctx, HD_ERR_INVALID, "copy transform is not invertible");
+        if (edits[i].swirl && !(std::isfinite(edits[i].swirl_period) && edits[i].swirl_period != 0.f))
+            return fail(ctx, HD_ERR_INVALID, "copy swirl period must be finite and non-zero");
+    }
     hd_status st = edit_prepare(ctx);
     if (st != HD_OK) return st;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);

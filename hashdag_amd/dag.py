@@ -193,9 +193,10 @@ class HashDAG:
 
     # -- edits: HashDAG::edit_threads<Editor> (hash_dag.h:255-405) --
     def edit_batch(self, edits, fused: bool = False, stream=None) -> hd_edit_stats:
-        """Applies [(kind, (x, y, z), radius), ...] in order (a copy adds source and dest:
-        (5, center, radius, source, dest)); fused=True runs each run of editors between copies
-        in one pass."""
+        """Applies [(kind, (x, y, z), radius), ...] in order. A copy adds source and dest, and
+        optionally a 3x3 transform and a swirl period: (5, center, radius, source, dest[,
+        transform[, swirl_period]]). fused=True runs each run of editors between copies in one
+        pass."""
         arr = (hd_edit * max(len(edits), 1))()
         for i, ed in enumerate(edits):
             kind, center, radius = ed[:3]
@@ -205,6 +206,11 @@ class HashDAG:
             if len(ed) > 3:
                 arr[i].source[:] = [float(c) for c in ed[3]]
                 arr[i].dest[:] = [float(c) for c in ed[4]]
+            if len(ed) > 5 and ed[5] is not None:
+                arr[i].transform[:] = [float(v) for v in np.asarray(ed[5], np.float64).reshape(9)]
+            if len(ed) > 6 and ed[6] is not None:
+                arr[i].swirl = 1
+                arr[i].swirl_period = float(ed[6])
         st = hd_edit_stats()
         check(lib.hd_edit_batch(self._h, arr, len(edits), 1 if fused else 0, C.byref(st), stream), self._h)
         return st
@@ -212,10 +218,11 @@ class HashDAG:
     def edit(self, kind: int, center, radius: float) -> hd_edit_stats:
         return self.edit_batch([(kind, center, radius)])
 
-    def copy(self, center, radius: float, source, dest) -> hd_edit_stats:
-        """CopyEditor (hash_dag_editors.h:332-435, identity transform): the box of `radius`
-        around dest + (center - source) gets the voxels (and colours) at source + (p - dest)."""
-        return self.edit_batch([(5, center, radius, source, dest)])
+    def copy(self, center, radius: float, source, dest, transform=None, swirl_period=None) -> hd_edit_stats:
+        """CopyEditor (hash_dag_editors.h:332-435): the box of radius * max|T (1,1,1)| around
+        dest + T (center - source) gets the voxels (and colours) at source + T^-1 (p - dest),
+        the relative position first swirled about y when swirl_period is given."""
+        return self.edit_batch([(5, center, radius, source, dest, transform, swirl_period)])
 
     def remove_stale_nodes(self, max_level: int | None = None, stream=None, keep_undo: bool = False,
                            extra_roots=None) -> dict:

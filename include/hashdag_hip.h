@@ -167,6 +167,9 @@ typedef struct hd_edit {
     float radius;
     float source[3];  /* copy: the box around center is copied from source to dest */
     float dest[3];
+    uint32_t swirl;       /* copy: 1 = swirl about y with period swirl_period (hash_dag_editors.h:366-372) */
+    float swirl_period;
+    double transform[9];  /* copy: Matrix3x3 transform, row-major; all zeros = identity */
 } hd_edit;
 
 typedef struct hd_edit_stats {

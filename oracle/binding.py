@@ -64,6 +64,8 @@ _SIGS = {
     "orc_edit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, fltp, C.c_float, u64p]),
     "orc_gc": (C.c_int, [C.c_void_p, C.c_uint32]),
     "orc_edit_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "orc_edit_copy_tf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_int, C.c_float, C.c_void_p]),
     "orc_get_color": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
     "orc_dag_reachable": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
 }
@@ -173,10 +175,18 @@ class OracleDAG:
             raise RuntimeError(_err())
         return [int(v) for v in stats]
 
-    def edit_copy(self, center, radius: float, source, dest, colors: "OracleColors"):
+    def edit_copy(self, center, radius: float, source, dest, colors: "OracleColors", transform=None,
+                  swirl_period=None):
+        """CopyEditor; transform: 3x3 row-major matrix (FP64), swirl_period: None = no swirl."""
         c, s, d = (C.c_float * 3)(*center), (C.c_float * 3)(*source), (C.c_float * 3)(*dest)
         stats = np.zeros(3, np.uint64)
-        if orc.orc_edit_copy(self.h, colors.h, c, radius, s, d, stats.ctypes.data) != 0:
+        if transform is None and swirl_period is None:
+            rc = orc.orc_edit_copy(self.h, colors.h, c, radius, s, d, stats.ctypes.data)
+        else:
+            t = np.ascontiguousarray(np.eye(3) if transform is None else transform, np.float64).reshape(9)
+            rc = orc.orc_edit_copy_tf(self.h, colors.h, c, radius, s, d, t.ctypes.data, 0 if swirl_period is None else 1,
+                                      1.0 if swirl_period is None else swirl_period, stats.ctypes.data)
+        if rc != 0:
             raise RuntimeError(_err())
         return [int(v) for v in stats]
 

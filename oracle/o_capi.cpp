@@ -401,6 +401,28 @@ int orc_edit_copy(void* d, void* c, const float center[3], float radius, const f
     GUARD_END(-1)
 }
 
+int orc_edit_copy_tf(void* d, void* c, const float center[3], float radius, const float source[3],
+                     const float dest[3], const double transform[9], int swirl, float swirl_period, uint64_t stats[3]) {
+    GUARD_BEGIN
+    Edit e{};
+    e.kind = COPY;
+    e.center = {center[0], center[1], center[2]};
+    e.radius = radius;
+    e.source = {source[0], source[1], source[2]};
+    e.dest = {dest[0], dest[1], dest[2]};
+    for (int i = 0; i < 9; ++i) e.transform[i] = transform[i];
+    e.swirl = swirl != 0;
+    e.swirlPeriod = swirl_period;
+    const EditStats s = apply_edit(*static_cast<DAG*>(d), static_cast<HashDAGColors*>(c), e, nullptr);
+    if (stats) {
+        stats[0] = s.numNodes;
+        stats[1] = s.numVoxels;
+        stats[2] = s.leavesRebuilt;
+    }
+    return 0;
+    GUARD_END(-1)
+}
+
 int orc_get_color(void* d, void* c, uint32_t x, uint32_t y, uint32_t z, uint32_t out[2]) {
     GUARD_BEGIN
     bool ok = false;

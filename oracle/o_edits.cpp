@@ -3,6 +3,8 @@
 // TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
 #include "oracle_core.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <map>
 #include <stdexcept>
 #include <tuple>
@@ -93,23 +95,59 @@ struct Editor {
     Edit e;
     F3 bmin{}, bmax{};  // BoxEditorBase (hash_dag_editors.h:57-74)
     float r2 = 0;       // SphereEditorBase (hash_dag_editors.h:99-110)
-    // copy (CopyEditorWithoutDecompression<true>, hash_dag_editors.h:332-435, identity
-    // transform, no swirl): the DAG before the edit and the colours it reads from
+    // copy (CopyEditorWithoutDecompression<true>, hash_dag_editors.h:332-435): the DAG before
+    // the edit and the colours it reads from, and the inverse of the copy's transform
     const HashTable* srcTable = nullptr;
     u32 srcRoot = 0;
     const HashDAGColors* srcColors = nullptr;
+    double inv[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 
     explicit Editor(const Edit& ed) : e(ed) {
-        // a copy is a box around dest + (center - source) (hash_dag_editors.h:350-351)
-        const F3 c = e.kind == COPY ? e.dest + (e.center - e.source) : e.center;
-        bmin = c - f3(e.radius);
-        bmax = c + f3(e.radius);
+        F3 c = e.center;
+        float radius = e.radius;
+        if (e.kind == COPY) {
+            // BoxEditorBase(dest + float3(T * (center - source)), radius * max|T * (1,1,1)|)
+            // (hash_dag_editors.h:350-351), Matrix3x3 in FP64 (Matrix3x3.h:190-243, 399-406)
+            const double* T = e.transform;
+            const F3 rel = e.center - e.source;
+            const double r[3] = {(double)rel.x, (double)rel.y, (double)rel.z};
+            c = e.dest + F3{(float)(T[0] * r[0] + T[1] * r[1] + T[2] * r[2]),
+                            (float)(T[3] * r[0] + T[4] * r[1] + T[5] * r[2]),
+                            (float)(T[6] * r[0] + T[7] * r[1] + T[8] * r[2])};
+            const double rx = std::fabs(T[0] * 1.0 + T[1] * 1.0 + T[2] * 1.0);
+            const double ry = std::fabs(T[3] * 1.0 + T[4] * 1.0 + T[5] * 1.0);
+            const double rz = std::fabs(T[6] * 1.0 + T[7] * 1.0 + T[8] * 1.0);
+            radius = (float)((double)e.radius * std::max(rx, std::max(ry, rz)));
+            // Matrix3x3::Inverse: adjugate times 1 / Determinate
+            const double a[9] = {T[8] * T[4] - T[7] * T[5], T[7] * T[2] - T[8] * T[1], T[5] * T[1] - T[4] * T[2],
+                                 T[6] * T[5] - T[8] * T[3], T[8] * T[0] - T[6] * T[2], T[3] * T[2] - T[5] * T[0],
+                                 T[7] * T[3] - T[6] * T[4], T[6] * T[1] - T[7] * T[0], T[4] * T[0] - T[3] * T[1]};
+            const double v1 = T[0] * (T[8] * T[4] - T[7] * T[5]);
+            const double v2 = T[3] * (T[8] * T[1] - T[7] * T[2]);
+            const double v3 = T[6] * (T[5] * T[1] - T[4] * T[2]);
+            const double s = 1 / (v1 - v2 + v3);
+            for (int i = 0; i < 9; ++i) inv[i] = a[i] * s;
+        }
+        bmin = c - f3(radius);
+        bmax = c + f3(radius);
         r2 = e.radius * e.radius;
     }
     bool is_box() const { return e.kind == BOX_ADD || e.kind == BOX_CARVE || e.kind == COPY; }
-    // hash_dag_editors.h:364-379: source + (position - dest)
+    // destination_to_source (hash_dag_editors.h:364-379): position - dest, swirled about the y
+    // axis (host cosf / sinf), through the inverse transform in FP64, plus source
     Path source_of(F3 pos) const {
-        const F3 p = e.source + (pos - e.dest);
+        F3 rel = pos - e.dest;
+        if (e.swirl) {
+            const float angle = rel.y / e.swirlPeriod;
+            const float x = rel.x, z = rel.z;
+            rel.x = x * std::cos(angle) - z * std::sin(angle);
+            rel.z = x * std::sin(angle) + z * std::cos(angle);
+        }
+        const double v[3] = {(double)rel.x, (double)rel.y, (double)rel.z};
+        rel = F3{(float)(inv[0] * v[0] + inv[1] * v[1] + inv[2] * v[2]),
+                 (float)(inv[3] * v[0] + inv[4] * v[1] + inv[5] * v[2]),
+                 (float)(inv[6] * v[0] + inv[7] * v[1] + inv[8] * v[2])};
+        const F3 p = e.source + rel;
         return Path(to_u32(p.x), to_u32(p.y), to_u32(p.z));
     }
     // BoxEditorBase::edit_count_impl (hash_dag_editors.h:92-96), 32-bit product

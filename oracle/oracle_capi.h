@@ -21,8 +21,11 @@
  *                     (dense-grid brute force)                   tests/test_oracle_tracer.py
  *   trace_shadows     any-hit result == brute-force segment/voxel test on a dense grid;
  *                     the same result with either any-hit child order  tests/test_oracle_tracer.py
- *   edits             dense-grid brute force of sphere/box/paint voxel sets and colours
- *                                                                tests/test_oracle_edits.py
+ *   edits             dense-grid brute force of sphere/box/paint voxel sets and colours, and of
+ *                     copies (translations, and transforms that permute / mirror / scale
+ *                     axes exactly)                              tests/test_oracle_edits.py
+ *   copy swirl        host cosf / sinf as the reference's host edit path calls them: parity
+ *                     unpinned (read from hash_dag_editors.h:364-379 only)
  *   GC                canonical form unchanged, idempotent, used words == reachable words +
  *                     full node + padding        tests/test_oracle_table.py, test_oracle_edits.py
  *   interior find     the reference's walk rule (start + n < min(bucket size, page end)) is
@@ -114,6 +117,10 @@ int orc_edit(void* dag, void* colors, uint32_t kind, const float center[3], floa
  * dest + (center - source), filled from source + (p - dest) of the DAG before the edit */
 int orc_edit_copy(void* dag, void* colors, const float center[3], float radius, const float source[3],
                   const float dest[3], uint64_t stats[3]);
+/* the same with the copy's Matrix3x3 transform (row-major, FP64) and swirl (swirl != 0: the
+ * relative position turns about y by angle y / swirl_period, hash_dag_editors.h:364-379) */
+int orc_edit_copy_tf(void* dag, void* colors, const float center[3], float radius, const float source[3],
+                     const float dest[3], const double transform[9], int swirl, float swirl_period, uint64_t stats[3]);
 /* DAGUtils::get_color at a voxel (out: colorBits, weight | bpw << 8); returns 1 if the voxel is set */
 int orc_get_color(void* dag, void* colors, uint32_t x, uint32_t y, uint32_t z, uint32_t out[2]);
 int orc_gc(void* dag, uint32_t max_level);

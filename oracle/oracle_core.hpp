@@ -297,6 +297,11 @@ struct Edit {
     float radius;
     F3 source;   // copy only
     F3 dest;     // copy only
+    // copy only: Matrix3x3 transform (row-major D00 .. D22, FP64) and the swirl
+    // (hash_dag_editors.h:332-379); identity / off by default
+    double transform[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    bool swirl = false;
+    float swirlPeriod = 1.f;
 };
 struct EditStats { u64 numNodes = 0, numVoxels = 0, leavesRebuilt = 0; };
 

@@ -200,3 +200,54 @@ def test_copy_matches_oracle(fused):
     if fused:
         assert st.passes == 4  # each copy starts a new pass: [o o o] [C o o o] [C o o o] [C]
     assert_same(dag, sc, od, oc, f"copies (fused={fused})")
+
+
+def _rotation(deg, axis):
+    a = np.radians(deg)
+    c, s = np.cos(a), np.sin(a)
+    m = {"y": [[c, 0, s], [0, 1, 0], [-s, 0, c]], "z": [[c, -s, 0], [s, c, 0], [0, 0, 1]]}[axis]
+    return np.array(m, np.float64)
+
+
+TRANSFORMS = [
+    np.array([[0, 0, 1], [0, 1, 0], [1, 0, 0]], np.float64),        # axis swap
+    np.array([[-1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64),       # mirror
+    np.diag([2.0, 2.0, 2.0]),                                        # scale up
+    np.diag([0.5, 0.5, 0.5]) @ _rotation(30, "y"),                   # rotate and scale down
+    _rotation(45, "z") * 1.1,                                        # the replays' kind of transform
+]
+
+
+def test_transformed_copy_matches_oracle():
+    """CopyEditor with a Matrix3x3 transform (hash_dag_editors.h:332-379): the FP64 inverse and
+    matrix-vector products run the same operations on both sides, so the result is identical."""
+    p, sc, g, dag, od, oc = make("d13")
+    copies = copy_edits(p, len(TRANSFORMS), seed=31)
+    edits = [c + (T,) for c, T in zip(copies, TRANSFORMS)]
+    st = dag.edit_batch(edits, fused=True)
+    total = 0
+    for ed in edits:
+        total += od.edit_copy(ed[1], ed[2], ed[3], ed[4], oc, transform=ed[5])[1]
+    assert st.passes == len(edits)
+    assert st.num_voxels == total
+    assert_same(dag, sc, od, oc, "transformed copies")
+
+
+def test_swirl_copy_close_to_oracle():
+    """Swirled copies: the reference turns by host cosf / sinf, the device by FP64 cos / sin
+    rounded to FP32; the two may differ in the last bit of a source coordinate, which moves a
+    voxel only when that coordinate sits within an ulp of an integer. The edited DAGs must
+    agree on all but a tiny fraction of the voxels of the copied boxes (observed: none)."""
+    from tests.test_edit_gpu import gpu_state_in_oracle
+
+    p, sc, g, dag, od, oc = make("d8")
+    copies = copy_edits(p, 3, seed=37)
+    edits = [c + (None, period) for c, period in zip(copies, (12.0, 40.0, 7.5))]
+    dag.edit_batch(edits, fused=True)
+    for ed in edits:
+        od.edit_copy(ed[1], ed[2], ed[3], ed[4], oc, swirl_period=ed[6])
+    god, _ = gpu_state_in_oracle(dag, sc)
+    a, b = god.dense(), od.dense()
+    diff = int((a != b).sum())
+    boxes = sum(int((2 * ed[2]) ** 3) for ed in edits)
+    assert diff <= max(1, boxes // 100000), f"{diff} voxels differ"

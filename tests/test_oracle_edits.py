@@ -214,6 +214,68 @@ def test_copy_brute_force(center, radius, source, dest):
             assert col == cols_before[v], v
 
 
+TRANSFORMED_COPIES = [
+    # center, radius, source, dest, transform: axis swaps, mirrors and scales by powers of two
+    # keep every step of destination_to_source exact, so the brute force below is exact too
+    ((40.0, 60.0, 40.0), 10.0, (40.0, 60.0, 40.0), (90.0, 80.0, 90.0), [[0, 0, 1], [0, 1, 0], [1, 0, 0]]),
+    ((30.0, 55.0, 90.0), 7.0, (30.0, 50.0, 60.0), (75.0, 30.0, 20.0), [[-1, 0, 0], [0, 1, 0], [0, 0, 1]]),
+    ((64.0, 60.0, 64.0), 6.0, (64.0, 60.0, 64.0), (30.0, 70.0, 100.0), [[2, 0, 0], [0, 2, 0], [0, 0, 2]]),
+    ((50.0, 58.0, 70.0), 12.0, (50.0, 58.0, 70.0), (90.0, 40.0, 30.0), [[0.5, 0, 0], [0, 0, 0.5], [0, 0.5, 0]]),
+]
+
+
+@pytest.mark.parametrize("center,radius,source,dest,transform", TRANSFORMED_COPIES)
+def test_transformed_copy_brute_force(center, radius, source, dest, transform):
+    """CopyEditorWithoutDecompression with a Matrix3x3 transform (hash_dag_editors.h:332-379):
+    the box of radius * max|T (1,1,1)| around dest + T (center - source) gets
+    old | before[floor(source + T^-1 (v - dest))], newly set voxels take their source's colour."""
+    p, sc, g = scene("d7")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    before, cols_before = per_voxel_colours(od, oc, g)
+    T = np.array(transform, np.float64)
+    stats = od.edit_copy(center, radius, source, dest, oc, transform=T)
+    after, cols_after = per_voxel_colours(od, oc, g)
+    E = 1 << g.levels
+    box_c = np.array(dest) + T @ (np.array(center) - np.array(source))
+    box_r = radius * np.abs(T @ np.ones(3)).max()
+    t = touched(BOX_ADD, tuple(box_c), box_r, E)
+    Tinv = np.linalg.inv(T)
+
+    def src_of(v):
+        s_ = np.array(source) + Tinv @ (np.array(v, np.float64) - np.array(dest))
+        assert (s_ >= 0).all() and (s_ < E).all()
+        return tuple(int(a) for a in np.floor(s_))
+
+    expect = before.copy()
+    for v in zip(*np.nonzero(t)):
+        if before[src_of(v)]:
+            expect[v] = 1
+    assert np.array_equal(after, expect)
+    assert stats[1] == int((2 * box_r) ** 3)
+    for v, col in cols_after.items():
+        if t[v] and not before[v]:
+            assert col == cols_before[src_of(v)], v
+        else:
+            assert col == cols_before[v], v
+
+
+def test_swirl_copy_moves_voxels_by_rotation():
+    """Swirl (hash_dag_editors.h:366-372): with a period so long that every angle in the box is
+    below one voxel's worth of turn, the swirled copy equals the plain copy; with a shorter period
+    the copied set differs but keeps its size within the box's rotation error."""
+    p, sc, g = scene("d7")
+    center, radius, source, dest = (40.0, 60.0, 40.0), 10.0, (40.0, 60.0, 40.0), (90.0, 80.0, 90.0)
+    grids = []
+    for period in (None, 1e9, 30.0):
+        od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+        oc = OracleColors.from_scene(sc)
+        od.edit_copy(center, radius, source, dest, oc, swirl_period=period)
+        grids.append(od.dense())
+    assert np.array_equal(grids[0], grids[1])
+    assert not np.array_equal(grids[0], grids[2])
+
+
 def test_color_lookup_matches_canonical_order():
     """DAGUtils::get_color (dag_utils.h:423-523) == the colour at the voxel's Morton rank."""
     p, sc, g = scene("d7")
