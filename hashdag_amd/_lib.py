@@ -96,12 +96,20 @@ class hd_edit(C.Structure):
                 ("transform", C.c_double * 9)]
 
 
+class hd_replay_action(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("frame", C.c_uint32), ("location", C.c_double * 3),
+                ("rotation", C.c_double * 9), ("edit", hd_edit)]
+
+
 class hd_edit_stats(C.Structure):
     _fields_ = [("num_voxels", C.c_uint64), ("new_words", C.c_uint64), ("leaves_rebuilt", C.c_uint64),
                 ("root", C.c_uint32), ("passes", C.c_uint32), ("colors_encoded", C.c_uint64)]
 
 
-HD_REQ_TRACE, HD_REQ_EDIT = 0, 1
+HD_REQ_TRACE, HD_REQ_EDIT, HD_REQ_UNDO, HD_REQ_REDO = 0, 1, 2, 3
+
+(HD_REPLAY_SET_LOCATION, HD_REPLAY_SET_ROTATION, HD_REPLAY_SET_TOOL, HD_REPLAY_EDIT, HD_REPLAY_UNDO, HD_REPLAY_REDO,
+ HD_REPLAY_END_FRAME, HD_REPLAY_FILL) = range(8)
 
 
 class hd_request(C.Structure):
@@ -140,6 +148,10 @@ _SIGS = {
     "hd_edit_batch": (C.c_int32, [C.c_void_p, C.POINTER(hd_edit), C.c_uint32, C.c_uint32, C.POINTER(hd_edit_stats),
                                   C.c_void_p]),
     "hd_gc": (C.c_int32, [C.c_void_p, C.c_uint32, u64p, C.c_void_p]),
+    "hd_replay_parse": (C.c_int32, [C.c_char_p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "hd_replay_actions": (C.c_int32, [C.c_void_p, C.POINTER(C.POINTER(hd_replay_action)), u64p,
+                                      C.POINTER(C.c_uint32)]),
+    "hd_replay_free": (None, [C.c_void_p]),
     "hd_gc_roots": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, u64p,
                                 C.c_void_p]),
     "hd_undo": (C.c_int32, [C.c_void_p]),

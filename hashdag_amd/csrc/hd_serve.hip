@@ -162,7 +162,7 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
             maxPixels = std::max<uint64_t>(maxPixels, (uint64_t)r.width * r.height);
         } else if (r.kind == HD_REQ_EDIT) {
             if ((uint64_t)r.first_edit + r.n_edits > n_edits) return fail(ctx, HD_ERR_INVALID, "edit range out of bounds");
-        } else {
+        } else if (r.kind != HD_REQ_UNDO && r.kind != HD_REQ_REDO) {
             return fail(ctx, HD_ERR_INVALID, "unknown request kind");
         }
         if (i && r.arrival_s < reqs[i - 1].arrival_s) return fail(ctx, HD_ERR_INVALID, "requests not in arrival order");
@@ -243,6 +243,25 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
             continue;
         }
         const uint32_t head = queue.front();
+        if (reqs[head].kind == HD_REQ_UNDO || reqs[head].kind == HD_REQ_REDO) {
+            // undo / redo swap the root and colour leaves in place: everything queued before
+            // finishes first, everything after starts after
+            queue.pop_front();
+            const double start = now_s();
+            HD_HIP(ctx, hipStreamSynchronize(editStream));
+            for (Slot& sl : slots)
+                if (sl.req >= 0) HD_HIP(ctx, hipStreamSynchronize(sl.stream));
+            poll_slots(false);
+            const hd_status st = reqs[head].kind == HD_REQ_UNDO ? hd_undo(ctx) : hd_redo(ctx);
+            out[head].start_s = start;
+            out[head].done_s = now_s();
+            out[head].status = st;
+            out[head].batch = batch++;
+            out[head].checksum = ctx->root;
+            ++finished;
+            if (st != HD_OK) result = st;
+            continue;
+        }
         if (reqs[head].kind == HD_REQ_EDIT) {
             // fuse the run of edit requests at the head of the queue
             std::vector<hd_edit> fused;

@@ -92,6 +92,22 @@ class HostImage:
     root: int
 
 
+def fill_edit(e: hd_edit, ed) -> None:
+    """hd_edit from (kind, center, radius[, source, dest[, transform[, swirl_period]]])."""
+    kind, center, radius = ed[:3]
+    e.kind = int(kind)
+    e.center[:] = [float(c) for c in center]
+    e.radius = float(radius)
+    if len(ed) > 3:
+        e.source[:] = [float(c) for c in ed[3]]
+        e.dest[:] = [float(c) for c in ed[4]]
+    if len(ed) > 5 and ed[5] is not None:
+        e.transform[:] = [float(v) for v in np.asarray(ed[5], np.float64).reshape(9)]
+    if len(ed) > 6 and ed[6] is not None:
+        e.swirl = 1
+        e.swirl_period = float(ed[6])
+
+
 class HashDAG:
     """One device-resident HashDAG (+ colours) on one GPU."""
 
@@ -199,18 +215,7 @@ class HashDAG:
         pass."""
         arr = (hd_edit * max(len(edits), 1))()
         for i, ed in enumerate(edits):
-            kind, center, radius = ed[:3]
-            arr[i].kind = int(kind)
-            arr[i].center[:] = [float(c) for c in center]
-            arr[i].radius = float(radius)
-            if len(ed) > 3:
-                arr[i].source[:] = [float(c) for c in ed[3]]
-                arr[i].dest[:] = [float(c) for c in ed[4]]
-            if len(ed) > 5 and ed[5] is not None:
-                arr[i].transform[:] = [float(v) for v in np.asarray(ed[5], np.float64).reshape(9)]
-            if len(ed) > 6 and ed[6] is not None:
-                arr[i].swirl = 1
-                arr[i].swirl_period = float(ed[6])
+            fill_edit(arr[i], ed)
         st = hd_edit_stats()
         check(lib.hd_edit_batch(self._h, arr, len(edits), 1 if fused else 0, C.byref(st), stream), self._h)
         return st

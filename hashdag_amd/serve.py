@@ -13,9 +13,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import (HD_REQ_EDIT, HD_REQ_TRACE, check, hd_edit, hd_ray_params, hd_request, hd_request_result,
-                   hd_serve_config, lib)
-from .dag import HashDAG
+from ._lib import (HD_REQ_EDIT, HD_REQ_REDO, HD_REQ_TRACE, HD_REQ_UNDO, check, hd_edit, hd_ray_params, hd_request,
+                   hd_request_result, hd_serve_config, lib)
+from .dag import HashDAG, fill_edit
 from .scene import TerrainParams, terrain_height
 from .tracer import DAGInfo, ray_params
 from .workload import BOX_ADD, BOX_CARVE, SPHERE_ADD, SPHERE_CARVE, cameras
@@ -26,7 +26,7 @@ RESOLUTIONS = ((640, 360), (1280, 720), (1920, 1080), (3840, 2160))
 @dataclass
 class Request:
     arrival_s: float
-    kind: int                     # HD_REQ_TRACE / HD_REQ_EDIT
+    kind: int                     # HD_REQ_TRACE / HD_REQ_EDIT / HD_REQ_UNDO / HD_REQ_REDO
     width: int = 0
     height: int = 0
     params: hd_ray_params | None = None
@@ -78,15 +78,13 @@ def serve(dag: HashDAG, requests: list[Request], trace_slots: int = 4, max_batch
         if r.kind == HD_REQ_TRACE:
             q.width, q.height = r.width, r.height
             q.params = r.params
-        else:
+        elif r.kind == HD_REQ_EDIT:
             q.first_edit = len(flat)
             q.n_edits = len(r.edits)
             flat.extend(r.edits)
     edits = (hd_edit * max(len(flat), 1))()
-    for i, (kind, c, rad) in enumerate(flat):
-        edits[i].kind = int(kind)
-        edits[i].center[:] = [float(v) for v in c]
-        edits[i].radius = float(rad)
+    for i, ed in enumerate(flat):
+        fill_edit(edits[i], ed)
     cfg = hd_serve_config(trace_slots, max_batch_edits, 1 if realtime else 0, 1 if keep_undo else 0, reserve_pages,
                           shadow_bias, fog_density)
     res = (hd_request_result * max(n, 1))()
@@ -111,7 +109,7 @@ def latency_summary(requests: list[Request], results: np.ndarray) -> dict:
     span = float(results["done_s"].max() - min(arrival.min(), results["start_s"].min()))
     out = {"requests": len(requests), "span_s": round(span, 4), "requests_per_s": round(len(requests) / span, 2),
            "batches": int(results["batch"].max()) + 1 if len(results) else 0}
-    for name, k in (("trace", HD_REQ_TRACE), ("edit", HD_REQ_EDIT)):
+    for name, k in (("trace", HD_REQ_TRACE), ("edit", HD_REQ_EDIT), ("undo", HD_REQ_UNDO), ("redo", HD_REQ_REDO)):
         sel = lat[kinds == k]
         if sel.size:
             out[f"{name}_p50_ms"] = round(float(np.percentile(sel, 50)) * 1e3, 3)

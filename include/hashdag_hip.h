@@ -231,9 +231,11 @@ hd_status hd_frame_graph_destroy(hd_frame_graph* graph);
  * hd_edit_batch (HashDAG::edit_threads). */
 #define HD_REQ_TRACE 0u
 #define HD_REQ_EDIT 1u
+#define HD_REQ_UNDO 2u  /* HashDAGUndoRedo::undo after every earlier request (replay.cpp:7-12) */
+#define HD_REQ_REDO 3u
 typedef struct hd_request {
     double arrival_s;        /* seconds after the start of the run, non-decreasing */
-    uint32_t kind;           /* HD_REQ_TRACE / HD_REQ_EDIT */
+    uint32_t kind;           /* HD_REQ_TRACE / HD_REQ_EDIT / HD_REQ_UNDO / HD_REQ_REDO */
     uint32_t width, height;  /* trace: frame size */
     uint32_t first_edit;     /* edit: range in the edits array */
     uint32_t n_edits;
@@ -305,6 +307,38 @@ hd_status hd_scene_copy(const hd_scene* s, uint32_t* dag, uint32_t* color_nodes,
 /* unique node counts per level (levels-1 entries), then BasicDAG words per level (levels-1) */
 hd_status hd_scene_level_nodes(const hd_scene* s, uint64_t* out);
 int32_t hd_terrain_height(const hd_terrain_params* p, uint32_t x, uint32_t z);
+
+/* ---- replays (ReplayManager, replay.h:12-635 / replay.cpp:1-101) ----
+ * A replay is the reference's CSV action stream: camera moves, edits, undo / redo and frame
+ * ends. hd_replay_parse reads it as ReplayManager::load_csv does (cells split at ',' and
+ * lines at '\n', trailing empty cells dropped, numbers read as std::stof / std::stoi would)
+ * and scales every edit by float(2^scene_depth) / float(2^replay_depth) (replay.cpp:5). */
+#define HD_REPLAY_SET_LOCATION 0u
+#define HD_REPLAY_SET_ROTATION 1u
+#define HD_REPLAY_SET_TOOL 2u    /* SetToolParameters: interactive tool state, no effect on the DAG */
+#define HD_REPLAY_EDIT 3u        /* EditSphere / EditCube / EditCopy / EditPaint: `edit` */
+#define HD_REPLAY_UNDO 4u
+#define HD_REPLAY_REDO 5u
+#define HD_REPLAY_END_FRAME 6u
+#define HD_REPLAY_FILL 7u        /* EditFill (FillEditorColors): not built; `edit` holds centre and radius */
+
+typedef struct hd_replay hd_replay;
+typedef struct hd_replay_action {
+    uint32_t type;          /* HD_REPLAY_* */
+    uint32_t frame;         /* frame the action belongs to (frames end at HD_REPLAY_END_FRAME) */
+    double location[3];     /* SetLocation: camera position */
+    double rotation[9];     /* SetRotation: camera rotation, rows */
+    hd_edit edit;           /* edits: scaled to the scene depth */
+} hd_replay_action;
+
+/* ReplayManager::load_csv (replay.h:565-607); unknown actions or malformed cells are errors
+ * (message: hd_last_error(NULL)). */
+hd_status hd_replay_parse(const char* text, uint64_t length, uint32_t scene_depth, uint32_t replay_depth,
+                          hd_replay** out);
+/* the parsed actions (owned by the replay) and the frame count */
+hd_status hd_replay_actions(const hd_replay* replay, const hd_replay_action** actions, uint64_t* n_actions,
+                            uint32_t* n_frames);
+void hd_replay_free(hd_replay* replay);
 
 #ifdef __cplusplus
 }
