@@ -51,6 +51,8 @@ def main() -> int:
                     help="scene depth of the batched-vs-plain edit comparison (0 = the traced scene)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the served stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--no-replays", action="store_true",
+                    help="skip serving the reference's benchmark replays (edits, small / large edits, copies)")
     ap.add_argument("--c5", action="store_true",
                     help="also run the depth-17 workload: instancing, 512^3 copies, large carves in a frame graph, "
                          "undo kept across a full GC, a 3840x2160 frame")
@@ -203,7 +205,7 @@ def main() -> int:
         if not args.no_edits:
             edit_cpu = cpu_edit_baseline(od, oc, params, args)
         del od, oc
-    if not args.no_edits or args.serve_requests > 0:
+    if not args.no_edits or args.serve_requests > 0 or not args.no_replays:
         ED = args.edit_levels or D
         if ED != D:
             # the batched-vs-plain comparison on its own (deeper) scene
@@ -222,6 +224,8 @@ def main() -> int:
                 result["edits"]["cpu_baseline"] = edit_cpu
         if args.serve_requests > 0:
             result["serving"] = serve_bench(escene, egeom, eparams, args)
+        if not args.no_replays:
+            result["replays"] = replay_bench(escene, egeom, eparams, W, H, args)
     if args.c5:
         result["c5"] = c5_bench(args)
     print(json.dumps(result), flush=True)
@@ -332,6 +336,45 @@ def serve_bench(scene, geom, params, args):
     summ["workload"] = (f"{len(reqs)} requests, Poisson {args.serve_rate}/s on the 2^{params.levels} coloured terrain: "
                         f"60% trace (640x360..3840x2160), 40% edit (1-64 sphere/box, radius log-uniform 2-128)")
     return summ
+
+
+REPLAYS = ("epiccitadel_edits", "epiccitadel_small_edits", "epiccitadel_large_edits", "epiccitadel_copy")
+
+
+def replay_bench(scene, geom, params, W, H, args):
+    """The reference's benchmark replays (tests/golden/replays, recorded at depth 17) served on
+    the terrain at 60 frames/s in real time: each frame's edits (scaled to the scene depth) and
+    a W x H frame traced from a seeded camera over this terrain (the replays' own cameras
+    belong to their scenes). Per replay: p50 / p99 latency of frames and edits, edited voxels."""
+    from hashdag_amd import DAGInfo, HashDAG, ray_params
+    from hashdag_amd.replay import Replay, replay_requests
+    from hashdag_amd.serve import latency_summary, serve
+    from hashdag_amd.workload import cameras
+
+    dinfo = DAGInfo.unit(params.levels)
+    prms = [ray_params(c, dinfo, params.levels, W, H) for c in cameras(params, 16, seed=args.seed + 4)]
+    out = {}
+    d = HashDAG(geom)
+    d.load_scene(scene)
+    for name in REPLAYS:
+        rep = Replay.load(os.path.join(ROOT, "tests", "golden", "replays", name + ".csv"), params.levels)
+        reqs, skipped = replay_requests(rep, W, H, params.levels, fps=60.0, params=prms)
+        res = serve(d, reqs, trace_slots=4, realtime=True, keep_undo=True)
+        # back to the loaded scene for the next replay
+        for _ in range(d.undo_depth()[0]):
+            d.undo()
+        d.remove_stale_nodes()
+        summ = latency_summary(reqs, res)
+        summ["frames"] = rep.frames
+        summ["replay_edits"] = rep.counts()["edits"]
+        summ["skipped_fill"] = skipped
+        summ["rejected"] = int((res["status"] != 0).sum())
+        log(f"replay {name}: {summ}")
+        out[name] = summ
+    d.close()
+    out["workload"] = (f"the reference's replays {', '.join(REPLAYS)} at 60 frames/s on the 2^{params.levels} "
+                       f"terrain, edits scaled from depth 17, {W}x{H} frames from seeded cameras")
+    return out
 
 
 def c5_bench(args):

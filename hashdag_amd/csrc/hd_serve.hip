@@ -234,8 +234,12 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
         poll_slots(false);
         if (queue.empty()) {
             if (next < n) {
+                // sleep until the next arrival, but keep noticing finished traces while any runs
+                bool busy = false;
+                for (const Slot& sl : slots) busy |= sl.req >= 0;
                 const double wait = reqs[next].arrival_s - now_s();
-                if (wait > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 100e-6));
+                const double nap = busy ? std::min(wait - 100e-6, 20e-6) : wait - 100e-6;
+                if (nap > 0) std::this_thread::sleep_for(std::chrono::duration<double>(nap));
                 else std::this_thread::yield();
             } else {
                 poll_slots(true);
