@@ -148,6 +148,13 @@ _SIGS = {
     "hd_edit_batch": (C.c_int32, [C.c_void_p, C.POINTER(hd_edit), C.c_uint32, C.c_uint32, C.POINTER(hd_edit_stats),
                                   C.c_void_p]),
     "hd_gc": (C.c_int32, [C.c_void_p, C.c_uint32, u64p, C.c_void_p]),
+    "hd_write_basic_dag_file": (C.c_int32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint32, u32p,
+                                            C.c_uint64]),
+    "hd_read_basic_dag_file": (C.c_int32, [C.c_char_p, C.POINTER(C.c_double), u32p, u32p, C.c_uint64, u64p]),
+    "hd_save_hash_dag_file": (C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "hd_load_hash_dag_file": (C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
+    "hd_checkpoint_save": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "hd_checkpoint_load": (C.c_int32, [C.c_void_p, C.c_char_p]),
     "hd_replay_parse": (C.c_int32, [C.c_char_p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "hd_replay_actions": (C.c_int32, [C.c_void_p, C.POINTER(C.POINTER(hd_replay_action)), u64p,
                                       C.POINTER(C.c_uint32)]),

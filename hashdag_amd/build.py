@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libhashdag_hip.so")
 
 HIP_SOURCES = ["hd_context.hip", "hd_trace.hip", "hd_stream.hip", "hd_edit.hip", "hd_edit_api.hip", "hd_gc.hip",
-               "hd_serve.hip", "hd_replay.hip"]
+               "hd_serve.hip", "hd_replay.hip", "hd_files.hip"]
 CPP_SOURCES = ["hd_scene.cpp", "hd_factory.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

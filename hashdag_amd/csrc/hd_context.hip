@@ -224,6 +224,24 @@ hd_status hd_download_image(hd_ctx* ctx, uint32_t* pool, uint32_t* page_table, u
     if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
     const Geom& g = ctx->g;
     if (pool) {
+        const hd_status st = download_pool(ctx, pool, (uint64_t)ctx->cfg.pool_pages * g.page_size);
+        if (st != HD_OK) return st;
+    }
+    if (page_table) HD_HIP(ctx, hipMemcpyAsync(page_table, ctx->page_table, g.total_pages() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (bucket_sizes)
+        HD_HIP(ctx, hipMemcpyAsync(bucket_sizes, ctx->bucket_sizes, g.total_buckets() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (full_nodes) HD_HIP(ctx, hipMemcpyAsync(full_nodes, ctx->full_nodes, g.levels * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+}  // extern "C"
+
+namespace hd {
+// The physical pool pages in use (page 0 .. pool_top - 1), then zeros up to cap_words.
+hd_status download_pool(hd_ctx* ctx, uint32_t* pool, uint64_t cap_words) {
+    const Geom& g = ctx->g;
+    {
         uint32_t top = 0;
         HD_HIP(ctx, hipMemcpyAsync(&top, ctx->counters, 4, hipMemcpyDeviceToHost, ctx->stream));
         HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -236,21 +254,18 @@ hd_status hd_download_image(hd_ctx* ctx, uint32_t* pool, uint32_t* page_table, u
                                                                               g.total_pages(), g.page_size);
             e = hipGetLastError();
         }
-        if (e == hipSuccess) e = hipMemcpyAsync(pool, dphys, words * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(pool, dphys, std::min(words, cap_words) * 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         hipFree(dphys);
         if (e != hipSuccess) return hip_fail(ctx, e, "download pool");
-        // the caller's buffer holds pool_pages pages; pages at and above pool_top are zero
-        const uint64_t cap = (uint64_t)ctx->cfg.pool_pages * g.page_size;
-        if (cap > words) std::memset(pool + words, 0, (cap - words) * 4);
+        // pages at and above pool_top are zero
+        if (cap_words > words) std::memset(pool + words, 0, (cap_words - words) * 4);
     }
-    if (page_table) HD_HIP(ctx, hipMemcpyAsync(page_table, ctx->page_table, g.total_pages() * 4, hipMemcpyDeviceToHost, ctx->stream));
-    if (bucket_sizes)
-        HD_HIP(ctx, hipMemcpyAsync(bucket_sizes, ctx->bucket_sizes, g.total_buckets() * 4, hipMemcpyDeviceToHost, ctx->stream));
-    if (full_nodes) HD_HIP(ctx, hipMemcpyAsync(full_nodes, ctx->full_nodes, g.levels * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return HD_OK;
 }
+}  // namespace hd
+
+extern "C" {
 
 hd_status hd_upload_image(hd_ctx* ctx, const uint32_t* pool, const uint32_t* page_table, const uint32_t* bucket_sizes,
                           const uint32_t* full_nodes, uint32_t pool_top, uint32_t root) {

@@ -72,6 +72,8 @@ hd_status fail(hd_ctx* ctx, hd_status code, const std::string& msg);
 hd_status hip_fail(hd_ctx* ctx, hipError_t e, const char* what);
 DevDag dev_dag(const hd_ctx* ctx);
 DevColors dev_colors(const hd_ctx* ctx);
+// physical pool pages 0 .. pool_top - 1 into `pool` (at most cap_words words), zeros after them
+hd_status download_pool(hd_ctx* ctx, uint32_t* pool, uint64_t cap_words);
 }  // namespace hd
 
 #define HD_HIP(ctx, call)                                          \

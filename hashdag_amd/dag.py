@@ -248,6 +248,26 @@ class HashDAG:
         return {"live_nodes": int(st[0]), "words_before": int(st[1]), "words_after": int(st[2]),
                 "roots": [int(r) for r in roots]}
 
+    # -- files --
+    def save_checkpoint(self, path: str) -> None:
+        """The table image and colour state, restorable exactly by load_checkpoint."""
+        check(lib.hd_checkpoint_save(self._h, path.encode()), self._h)
+
+    def load_checkpoint(self, path: str) -> None:
+        check(lib.hd_checkpoint_load(self._h, path.encode()), self._h)
+
+    def save_hash_dag_file(self, path: str, bounds_min, bounds_max) -> None:
+        """HashDAGFactory::save_dag_to_file (hash_dag_factory.cpp:200-223)."""
+        bmin, bmax = (C.c_double * 3)(*bounds_min), (C.c_double * 3)(*bounds_max)
+        check(lib.hd_save_hash_dag_file(self._h, path.encode(), bmin, bmax), self._h)
+
+    def load_hash_dag_file(self, path: str):
+        """HashDAGFactory::load_dag_from_file (hash_dag_factory.cpp:225-246): the dumped DAG,
+        re-interned into this table; returns the bounds (min, max)."""
+        b = (C.c_double * 6)()
+        check(lib.hd_load_hash_dag_file(self._h, path.encode(), b), self._h)
+        return tuple(b[:3]), tuple(b[3:])
+
     def undo(self) -> None:
         check(lib.hd_undo(self._h), self._h)
 

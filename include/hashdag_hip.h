@@ -308,6 +308,26 @@ hd_status hd_scene_copy(const hd_scene* s, uint32_t* dag, uint32_t* color_nodes,
 hd_status hd_scene_level_nodes(const hd_scene* s, uint64_t* out);
 int32_t hd_terrain_height(const hd_terrain_params* p, uint32_t x, uint32_t z);
 
+/* ---- files ---- (errors without a context: hd_last_error(NULL))
+ * BasicDAGFactory::save_dag_to_file / load_dag_from_file (basic_dag.cpp:89-117, serializer.h:11-122):
+ * DAGInfo (6 doubles), levels (uint32), uint64 word count, the BasicDAG words. Reading with
+ * words = NULL returns the sizes only. */
+hd_status hd_write_basic_dag_file(const char* path, const double bounds_min[3], const double bounds_max[3],
+                                  uint32_t levels, const uint32_t* words, uint64_t n_words);
+hd_status hd_read_basic_dag_file(const char* path, double bounds[6], uint32_t* levels, uint32_t* words,
+                                 uint64_t capacity, uint64_t* n_words);
+/* HashDAGFactory::save_dag_to_file / load_dag_from_file (hash_dag_factory.cpp:200-246): DAGInfo,
+ * levels, firstNodeIndex, poolMaxSize, pageTableSize, poolTop, the pool pages, the page table.
+ * The dump holds no bucket sizes or colours; loading walks the DAG from its root and builds the
+ * context's table from it (the same DAG, re-interned). */
+hd_status hd_save_hash_dag_file(hd_ctx* ctx, const char* path, const double bounds_min[3], const double bounds_max[3]);
+hd_status hd_load_hash_dag_file(hd_ctx* ctx, const char* path, double bounds[6]);
+/* Checkpoint of a context, no counterpart in the reference: the table image (with bucket sizes
+ * and full nodes) and the colour state, restored exactly into a context of the same geometry.
+ * The undo history is not saved and is cleared on load. */
+hd_status hd_checkpoint_save(hd_ctx* ctx, const char* path);
+hd_status hd_checkpoint_load(hd_ctx* ctx, const char* path);
+
 /* ---- replays (ReplayManager, replay.h:12-635 / replay.cpp:1-101) ----
  * A replay is the reference's CSV action stream: camera moves, edits, undo / redo and frame
  * ends. hd_replay_parse reads it as ReplayManager::load_csv does (cells split at ',' and
