@@ -62,8 +62,12 @@ __global__ __launch_bounds__(256) void k_gc_mark(const uint32_t* __restrict__ vp
                 if (child >= numVirtual) {  // not an address of this table
                     atomicOr(errors, GC_ERR_MAP);
                 } else {
+                    // shared children are marked by many parents: read first, so only the
+                    // first few pay for the atomic on a hot bitmap word
                     const uint32_t bit = 1u << (child & 31u);
-                    first = !(atomicOr(&bitmap[child >> 5], bit) & bit);
+                    uint32_t* word = &bitmap[child >> 5];
+                    first = !(__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit) &&
+                            !(atomicOr(word, bit) & bit);
                 }
             }
         }
