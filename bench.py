@@ -358,7 +358,7 @@ def replay_bench(scene, geom, params, W, H, args):
     d.load_scene(scene)
     for name in REPLAYS:
         rep = Replay.load(os.path.join(ROOT, "tests", "golden", "replays", name + ".csv"), params.levels)
-        reqs, skipped = replay_requests(rep, W, H, params.levels, fps=60.0, params=prms)
+        reqs = replay_requests(rep, W, H, params.levels, fps=60.0, params=prms)
         res = serve(d, reqs, trace_slots=4, realtime=True, keep_undo=True)
         # back to the loaded scene for the next replay
         for _ in range(d.undo_depth()[0]):
@@ -367,7 +367,6 @@ def replay_bench(scene, geom, params, W, H, args):
         summ = latency_summary(reqs, res)
         summ["frames"] = rep.frames
         summ["replay_edits"] = rep.counts()["edits"]
-        summ["skipped_fill"] = skipped
         summ["rejected"] = int((res["status"] != 0).sum())
         log(f"replay {name}: {summ}")
         out[name] = summ

@@ -109,7 +109,7 @@ class hd_edit_stats(C.Structure):
 HD_REQ_TRACE, HD_REQ_EDIT, HD_REQ_UNDO, HD_REQ_REDO = 0, 1, 2, 3
 
 (HD_REPLAY_SET_LOCATION, HD_REPLAY_SET_ROTATION, HD_REPLAY_SET_TOOL, HD_REPLAY_EDIT, HD_REPLAY_UNDO, HD_REPLAY_REDO,
- HD_REPLAY_END_FRAME, HD_REPLAY_FILL) = range(8)
+ HD_REPLAY_END_FRAME) = range(7)
 
 
 class hd_request(C.Structure):

@@ -13,7 +13,7 @@ struct hd_ctx;
 namespace hd {
 
 constexpr uint32_t EDIT_SPHERE_ADD = 0, EDIT_SPHERE_CARVE = 1, EDIT_BOX_ADD = 2, EDIT_BOX_CARVE = 3,
-                   EDIT_SPHERE_PAINT = 4, EDIT_COPY = 5;
+                   EDIT_SPHERE_PAINT = 4, EDIT_COPY = 5, EDIT_FILL = 6;
 
 // error bits reported by a pass
 constexpr uint32_t ERR_SCRATCH = 1, ERR_BUCKET = 2, ERR_POOL = 4, ERR_HEADER = 8, ERR_COLOR_SCRATCH = 16,
@@ -38,7 +38,31 @@ struct DevEdit {
     float swirl_period;
     uint32_t pad;
     double inv[9];       // copy: inverse of the copy's transform (row-major, FP64)
+    // fill: per buffer voxel (x + size * (y + size * z), buffer origin src[]) bit 0 = the voxel
+    // is set after the fill, bit 1 = the flood fill set it; colours of the fill's bounding box
+    // (lo, dim) for the voxels it set
+    const uint8_t* fill_state;
+    const uint64_t* fill_colors;
+    uint32_t fill_size;
+    uint32_t fill_lo[3], fill_dim[3];
+    uint32_t pad2;
 };
+
+// Device buffers of one prepared fill edit (owned by the host call that runs its pass).
+struct FillBuffers {
+    uint8_t* state = nullptr;
+    uint64_t* colors = nullptr;
+    void release() {
+        hipFree(state);
+        hipFree(colors);
+        state = nullptr;
+        colors = nullptr;
+    }
+    ~FillBuffers() { release(); }
+};
+
+// float -> uint32 as x86-64 code converts it (truncate to int64, keep the low 32 bits)
+inline uint32_t f2u_host(float f) { return (uint32_t)(int64_t)f; }
 
 // Work items of one level (structure of arrays, capacity `cap`; child arrays are 8 * cap).
 struct Items {

@@ -96,7 +96,7 @@ __device__ __forceinline__ float tmaxf2(float a, float b) { return a > b ? a : b
 
 // hash_dag_editors.h:76-91 / 112-126
 __device__ __forceinline__ bool e_touch(const DevEdit& e, float x0, float y0, float z0, float x1, float y1, float z1) {
-    if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE || e.kind == EDIT_COPY) {
+    if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE || e.kind == EDIT_COPY || e.kind == EDIT_FILL) {
         return !(e.bmin[0] >= x1 || e.bmin[1] >= y1 || e.bmin[2] >= z1 || e.bmax[0] <= x0 || e.bmax[1] <= y0 ||
                  e.bmax[2] <= z0 || x0 >= e.bmax[0] || y0 >= e.bmax[1] || z0 >= e.bmax[2] || x1 <= e.bmin[0] ||
                  y1 <= e.bmin[1] || z1 <= e.bmin[2]);
@@ -235,6 +235,22 @@ __device__ __forceinline__ void copy_source(const DevEdit& e, float px, float py
     sx = f2u_x86(e.src[0] + tx);
     sy = f2u_x86(e.src[1] + ty);
     sz = f2u_x86(e.src[2] + tz);
+}
+
+// fill buffer voxel of a voxel corner: truncate(position - min) (hash_dag_editors.h:213-214);
+// returns the state byte (0 outside the buffer)
+__device__ __forceinline__ uint32_t fill_state_at(const DevEdit& e, float px, float py, float pz, uint64_t& idx) {
+    const uint32_t x = f2u_x86(px - e.src[0]), y = f2u_x86(py - e.src[1]), z = f2u_x86(pz - e.src[2]);
+    const uint32_t S = e.fill_size;
+    if (x >= S || y >= S || z >= S) return 0u;
+    idx = (uint64_t)x + (uint64_t)S * y + (uint64_t)S * S * z;
+    return e.fill_state[idx];
+}
+__device__ __forceinline__ uint64_t fill_color_at(const DevEdit& e, uint64_t idx) {
+    const uint64_t S = e.fill_size;
+    const uint32_t x = (uint32_t)(idx % S), y = (uint32_t)(idx / S % S), z = (uint32_t)(idx / (S * S));
+    return e.fill_colors[(uint64_t)(x - e.fill_lo[0]) +
+                         (uint64_t)e.fill_dim[0] * ((y - e.fill_lo[1]) + (uint64_t)e.fill_dim[1] * (z - e.fill_lo[2]))];
 }
 
 // 64-bit leaf mask of the leaf at leaf coordinates (lx, ly, lz) (voxel >> 2), 0 when absent
@@ -481,6 +497,23 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
                     }
                     v |= add;
                     continue;  // a copy's "num voxels" is its box volume, counted on the host
+                }
+                if (e.kind == EDIT_FILL) {
+                    // BufferEditor::get_new_value: set where the buffer voxel is set
+                    uint64_t cand = tm & ~v, add = 0;
+                    while (cand) {
+                        const uint32_t b = (uint32_t)__builtin_ctzll(cand);
+                        cand &= cand - 1;
+                        const uint32_t c1 = b >> 3, c2 = b & 7u;
+                        const float fx = (float)(bx | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u));
+                        const float fy = (float)(by | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u));
+                        const float fz = (float)(bz | ((c1 & 1u) << 1) | (c2 & 1u));
+                        uint64_t idx;
+                        if (fill_state_at(e, fx, fy, fz, idx) & 1u) add |= 1ull << b;
+                    }
+                    v |= add;
+                    touched |= tm;
+                    continue;
                 }
                 if (e.kind == EDIT_SPHERE_ADD || e.kind == EDIT_BOX_ADD) v |= tm;
                 else if (e.kind != EDIT_SPHERE_PAINT) v &= ~tm;
@@ -888,6 +921,21 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
         for (uint32_t k = 0; k < I.ecount[i]; ++k) {
             const DevEdit& e = P.edits[P.elist[L][I.estart[i] + k]];
             if (!e_touch(e, fx, fy, fz, fx + 1.0f, fy + 1.0f, fz + 1.0f)) continue;
+            if (e.kind == EDIT_FILL) {
+                // FillEditorColors::get_new_color: a voxel the flood fill set takes its colour
+                if (!v) {
+                    uint64_t idx = 0;
+                    const uint32_t st = fill_state_at(e, fx, fy, fz, idx);
+                    if (st & 1u) {
+                        if (st & 2u) {
+                            col = fill_color_at(e, idx);
+                            haveCol = true;
+                        }
+                        v = true;
+                    }
+                }
+                continue;
+            }
             if (e.kind == EDIT_COPY) {
                 // get_new_value / get_new_color: an empty voxel takes its source's value and colour
                 if (!v) {
@@ -907,7 +955,7 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
             if (nv) {
                 uint64_t pc = 0;
                 if (e.kind == EDIT_SPHERE_PAINT && v) {
-                    if (!haveCol) col = old_color(P, I.cmode[i], I.cmodeval[i], d, I.ostart[i] + oldRank);
+                    if (!haveCol) col = prev ? old_color(P, I.cmode[i], I.cmodeval[i], d, I.ostart[i] + oldRank) : 0ull;
                     pc = col;
                 }
                 col = e_color(e, fx, fy, fz, pc);
@@ -917,7 +965,8 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
         }
         const uint64_t newMask = __ballot(v);
         if (v) {
-            if (!haveCol) col = old_color(P, I.cmode[i], I.cmodeval[i], d, I.ostart[i] + oldRank);
+            // oldColor() of a voxel that was empty is CompressedColor{} (hash_dag_edits.h:520-540)
+            if (!haveCol) col = prev ? old_color(P, I.cmode[i], I.cmodeval[i], d, I.ostart[i] + oldRank) : 0ull;
             const uint32_t newRank = __popcll(newMask & ((1ull << lane) - 1ull));
             P.colors[T.cbase[task] + I.nstart[i] + newRank] = col;
         }
@@ -1184,6 +1233,29 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
             }
         }
     }
+}
+
+// ---- fill editor preparation (BufferEditor / FillEditorBase, hash_dag_editors.h:144-209, 629-671) ----
+// value of every buffer voxel: DAGUtils::get_value at get_world_path (min + float(x), ...)
+__global__ __launch_bounds__(256) void k_fill_load(const uint32_t* __restrict__ vpool, const uint32_t* d_root,
+                                                   uint32_t levels, float mx, float my, float mz, uint32_t S,
+                                                   uint8_t* __restrict__ out) {
+    const uint64_t n = (uint64_t)S * S * S;
+    const uint32_t root = *d_root;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(i % S), y = (uint32_t)(i / S % S), z = (uint32_t)(i / ((uint64_t)S * S));
+        const uint32_t wx = f2u_x86(mx + (float)x), wy = f2u_x86(my + (float)y), wz = f2u_x86(mz + (float)z);
+        const uint64_t lf = source_leaf(vpool, root, levels, wx >> 2, wy >> 2, wz >> 2);
+        out[i] = (uint8_t)((lf >> leaf_bit(wx, wy, wz)) & 1ull);
+    }
+}
+
+// DAGUtils::get_color of the given voxels (the solid voxels the flood fill loaded)
+__global__ __launch_bounds__(256) void k_fill_colors(EditPass P, const uint32_t* d_root, const uint32_t* __restrict__ xyz,
+                                                     uint32_t n, uint64_t* __restrict__ out) {
+    const uint32_t root = *d_root;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = source_color(P, root, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
 }
 
 __global__ void k_init_pass(EditPass P, uint32_t root, uint32_t first, uint32_t nEdits, const uint32_t* dN) {
@@ -1496,6 +1568,131 @@ void edit_restore_leaves(hd_ctx* ctx, hipStream_t s) {
 }
 
 // Undo / redo: write n host descriptors into their leaf slots (one upload, one scatter).
+// FillEditorBase + FillEditorColors construction (hash_dag_editors.h:629-728) for one fill edit
+// on the DAG at the context's current root: the buffer voxels' values come from the device,
+// the flood fill (a stack walk, as the reference's) and the colour scan run on the host, and
+// the result is uploaded for the pass. The buffers stay in `fb` until the pass has run.
+hd_status fill_prepare(hd_ctx* ctx, const hd_edit& e, DevEdit& d, FillBuffers& fb, hipStream_t s) {
+    const float fmin[3] = {e.center[0] - e.radius, e.center[1] - e.radius, e.center[2] - e.radius};
+    const double ce = std::ceil((double)e.radius);
+    if (!(ce >= 1.0) || ce > 512.0) return fail(ctx, HD_ERR_INVALID, "fill radius must be in (0, 512]");
+    const uint32_t S = 2u * (uint32_t)ce;
+    const uint64_t n = (uint64_t)S * S * S;
+    enum : uint8_t { Unknown = 0, LoadedTrue = 1, LoadedFalse = 2, SetTrue = 3 };
+    std::vector<uint8_t> solid(n);
+    {
+        uint8_t* dsolid = nullptr;
+        HD_HIP(ctx, hipMalloc(&dsolid, n));
+        k_fill_load<<<(uint32_t)std::min<uint64_t>((n + 255) / 256, 65536), 256, 0, s>>>(
+            ctx->vpool, ctx->d_root, ctx->g.levels, fmin[0], fmin[1], fmin[2], S, dsolid);
+        hipError_t err = hipGetLastError();
+        if (err == hipSuccess) err = hipMemcpyAsync(solid.data(), dsolid, n, hipMemcpyDeviceToHost, s);
+        if (err == hipSuccess) err = hipStreamSynchronize(s);
+        hipFree(dsolid);
+        if (err != hipSuccess) return hip_fail(ctx, err, "fill buffer load");
+    }
+    auto idx = [S](uint32_t x, uint32_t y, uint32_t z) { return (uint64_t)x + (uint64_t)S * y + (uint64_t)S * S * z; };
+    std::vector<uint8_t> st(n, Unknown);
+    auto get_value = [&](uint32_t x, uint32_t y, uint32_t z) {
+        uint8_t& v = st[idx(x, y, z)];
+        if (v == Unknown) v = solid[idx(x, y, z)] ? LoadedTrue : LoadedFalse;
+        return v == LoadedTrue || v == SetTrue;
+    };
+    const uint32_t c0 = S / 2;
+    uint32_t lo[3] = {c0, c0, c0}, hi[3] = {c0, c0, c0};
+    std::vector<uint32_t> stack{c0, c0, c0};
+    while (!stack.empty()) {
+        const uint32_t z = stack.back();
+        stack.pop_back();
+        const uint32_t y = stack.back();
+        stack.pop_back();
+        const uint32_t x = stack.back();
+        stack.pop_back();
+        st[idx(x, y, z)] = SetTrue;
+        const uint32_t q[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], q[a]);
+            hi[a] = std::max(hi[a], q[a]);
+        }
+        static const int dd[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+        for (const auto& o : dd) {
+            const uint32_t nx = x + (uint32_t)o[0], ny = y + (uint32_t)o[1], nz = z + (uint32_t)o[2];
+            if (nx < S && ny < S && nz < S && !get_value(nx, ny, nz)) {
+                stack.push_back(nx);
+                stack.push_back(ny);
+                stack.push_back(nz);
+            }
+        }
+    }
+    // colours of the loaded solid voxels in the filled bounding box, in the scan order
+    const uint32_t dim[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+    std::vector<uint32_t> loaded;
+    for (uint32_t x = lo[0]; x <= hi[0]; ++x)
+        for (uint32_t y = lo[1]; y <= hi[1]; ++y)
+            for (uint32_t z = lo[2]; z <= hi[2]; ++z)
+                if (st[idx(x, y, z)] == LoadedTrue) {
+                    loaded.push_back(f2u_host(fmin[0] + (float)x));
+                    loaded.push_back(f2u_host(fmin[1] + (float)y));
+                    loaded.push_back(f2u_host(fmin[2] + (float)z));
+                }
+    const uint32_t nl = (uint32_t)(loaded.size() / 3);
+    std::vector<uint64_t> lcol(nl);
+    if (nl && ctx->colors_loaded) {
+        uint32_t* dxyz = nullptr;
+        uint64_t* dcol = nullptr;
+        hipError_t err = hipMalloc(&dxyz, loaded.size() * 4);
+        if (err == hipSuccess) err = hipMalloc(&dcol, (uint64_t)nl * 8);
+        if (err == hipSuccess) err = hipMemcpyAsync(dxyz, loaded.data(), loaded.size() * 4, hipMemcpyHostToDevice, s);
+        if (err == hipSuccess) {
+            EditPass P{};
+            fill_pass(ctx, P);
+            k_fill_colors<<<std::min<uint32_t>((nl + 255) / 256, 4096), 256, 0, s>>>(P, ctx->d_root, dxyz, nl, dcol);
+            err = hipGetLastError();
+        }
+        if (err == hipSuccess) err = hipMemcpyAsync(lcol.data(), dcol, (uint64_t)nl * 8, hipMemcpyDeviceToHost, s);
+        if (err == hipSuccess) err = hipStreamSynchronize(s);
+        hipFree(dxyz);
+        hipFree(dcol);
+        if (err != hipSuccess) return hip_fail(ctx, err, "fill colours");
+    }
+    const uint64_t nb = (uint64_t)dim[0] * dim[1] * dim[2];
+    std::vector<uint64_t> colors(nb, 0);
+    {
+        uint64_t color = 0;  // CompressedColor{}
+        uint32_t k = 0;
+        for (uint32_t x = lo[0]; x <= hi[0]; ++x)
+            for (uint32_t y = lo[1]; y <= hi[1]; ++y)
+                for (uint32_t z = lo[2]; z <= hi[2]; ++z) {
+                    const uint8_t v = st[idx(x, y, z)];
+                    if (v == LoadedTrue) color = lcol[k++];
+                    else if (v == SetTrue)
+                        colors[(x - lo[0]) + (uint64_t)dim[0] * ((y - lo[1]) + (uint64_t)dim[1] * (z - lo[2]))] = color;
+                }
+    }
+    // state for the pass: bit 0 = set after the edit (get_value loads unknown voxels from the
+    // same DAG), bit 1 = set by the flood fill
+    std::vector<uint8_t> state(n);
+    for (uint64_t i = 0; i < n; ++i) state[i] = (uint8_t)((st[i] == SetTrue ? 3u : 0u) | (solid[i] ? 1u : 0u));
+    fb.release();
+    HD_HIP(ctx, hipMalloc(&fb.state, n));
+    HD_HIP(ctx, hipMalloc(&fb.colors, nb * 8));
+    HD_HIP(ctx, hipMemcpyAsync(fb.state, state.data(), n, hipMemcpyHostToDevice, s));
+    HD_HIP(ctx, hipMemcpyAsync(fb.colors, colors.data(), nb * 8, hipMemcpyHostToDevice, s));
+    HD_HIP(ctx, hipStreamSynchronize(s));
+    d.fill_state = fb.state;
+    d.fill_colors = fb.colors;
+    d.fill_size = S;
+    for (int a = 0; a < 3; ++a) {
+        d.src[a] = fmin[a];
+        d.fill_lo[a] = lo[a];
+        d.fill_dim[a] = dim[a];
+        // FillEditorBase: boundsMin / boundsMax from the min corners of the filled voxels
+        d.bmin[a] = fmin[a] + (float)lo[a];
+        d.bmax[a] = fmin[a] + (float)hi[a];
+    }
+    return HD_OK;
+}
+
 hd_status edit_scatter_leaves(hd_ctx* ctx, const uint32_t* handles, const LeafDesc* descs, uint32_t n) {
     if (n == 0) return HD_OK;
     const EditScratch& S = ctx->edit;

@@ -21,6 +21,7 @@ hd_status enqueue_frame_from_device(hd_ctx* ctx, const hd_ray_params* d_params, 
                                     uint32_t* d_paths, uint32_t* d_colors, float bias, float fog, hipStream_t s);
 hd_status edit_reserve_undo(hd_ctx* ctx, uint64_t entries);
 hd_status edit_scatter_leaves(hd_ctx* ctx, const uint32_t* handles, const LeafDesc* descs, uint32_t n);
+hd_status fill_prepare(hd_ctx* ctx, const hd_edit& e, DevEdit& d, FillBuffers& fb, hipStream_t s);
 void edit_restore_leaves(hd_ctx* ctx, hipStream_t s);
 hipError_t edit_gather_next(hd_ctx* ctx, uint32_t n, LeafDesc* host, hipStream_t s);
 
@@ -52,7 +53,7 @@ __global__ void k_count_voxels(const DevEdit* __restrict__ edits, uint32_t n, co
     if (dN) n = min(n, *dN);  // graph replays read the edit count from device memory
     for (uint32_t ei = blockIdx.x; ei < n; ei += gridDim.x) {
         const DevEdit e = edits[ei];
-        if (e.kind == EDIT_COPY) continue;  // counted on the host (box volume)
+        if (e.kind == EDIT_COPY || e.kind == EDIT_FILL) continue;  // counted on the host
         const float r = e.radius;
         const int64_t x0 = std::max<int64_t>(0, (int64_t)floorf(e.c[0] - r) - 1);
         const int64_t x1 = std::min<int64_t>((int64_t)E - 1, (int64_t)ceilf(e.c[0] + r) + 1);
@@ -160,6 +161,19 @@ uint64_t copy_voxels(const hd_edit* e, uint32_t n) {
                (uint32_t)(d.bmax[2] - d.bmin[2]);
     }
     return sum;
+}
+
+// "num voxels" of a fill in a pass with other editors: the voxels of the scene whose unit box
+// overlaps the fill's bounds, the voxels should_edit lets through (hash_dag_editors.h:76-91)
+uint64_t fill_voxels(const DevEdit& d, uint32_t E) {
+    uint64_t n = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t lo = std::max<int64_t>(0, (int64_t)std::floor((double)d.bmin[a]));
+        const int64_t hi = std::min<int64_t>((int64_t)E - 1, (int64_t)std::ceil((double)d.bmax[a]) - 1);
+        if (hi < lo) return 0;
+        n *= (uint64_t)(hi - lo + 1);
+    }
+    return n;
 }
 
 struct Readback {
@@ -355,7 +369,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
         return HD_OK;
     }
     for (uint32_t i = 0; i < n; ++i)
-        if (edits[i].kind > EDIT_COPY || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
+        if (edits[i].kind > EDIT_FILL || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
             return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");
     for (uint32_t i = 0; i < n; ++i) {
         if (edits[i].kind != EDIT_COPY) continue;
@@ -379,6 +393,15 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
     if (n) HD_HIP(ctx, hipMemcpyAsync(ctx->edit.edits_dev, dev.data(), (uint64_t)n * sizeof(DevEdit), hipMemcpyHostToDevice, s));
     HD_HIP(ctx, hipMemcpyAsync(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice, s));
     hd_edit_stats out{};
+    // a fill's buffers are built from the DAG as it is when its pass starts
+    FillBuffers fb;
+    auto prepare_fill = [&](uint32_t i) -> hd_status {
+        if (edits[i].kind != EDIT_FILL) return HD_OK;
+        hd_status r = fill_prepare(ctx, edits[i], dev[i], fb, s);
+        if (r != HD_OK) return r;
+        HD_HIP(ctx, hipMemcpyAsync(ctx->edit.edits_dev + i, &dev[i], sizeof(DevEdit), hipMemcpyHostToDevice, s));
+        return HD_OK;
+    };
     if (fused) {
         if (n > ctx->edit.elist_cap[0]) {
             uint32_t items[kMaxEditLevels] = {}, elist[kMaxEditLevels] = {};
@@ -386,14 +409,16 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             st = edit_reserve(ctx, items, elist, 0);
             if (st != HD_OK) return st;
         }
-        // A copy reads the DAG as the edits before it left it, so each copy starts a new pass;
-        // every other run of editors fuses into one pass.
+        // A copy or a fill reads the DAG as the edits before it left it, so each starts a new
+        // pass; every other run of editors fuses into one pass.
         for (uint32_t first = 0; first < n;) {
             uint32_t last = first + 1;
-            while (last < n && edits[last].kind != EDIT_COPY) ++last;
+            while (last < n && edits[last].kind != EDIT_COPY && edits[last].kind != EDIT_FILL) ++last;
             UndoFrame frame;
             frame.prev_root = ctx->root;
             Readback rb{};
+            st = prepare_fill(first);
+            if (st != HD_OK) return st;
             st = run_pass(ctx, first, last - first, s, rb, &frame);
             if (st != HD_OK) return st;
             ctx->root = rb.root;
@@ -401,6 +426,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             ctx->undo.push_back(std::move(frame));
             ctx->redo.clear();
             out.num_voxels += rb.stats[0] + copy_voxels(edits + first, last - first);
+            if (edits[first].kind == EDIT_FILL && last - first > 1) out.num_voxels += fill_voxels(dev[first], 1u << ctx->g.levels);
             out.new_words += rb.stats[1];
             out.leaves_rebuilt += rb.counters[4];
             out.colors_encoded += rb.stats[2];
@@ -412,6 +438,8 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             UndoFrame frame;
             frame.prev_root = ctx->root;
             Readback rb{};
+            st = prepare_fill(i);
+            if (st != HD_OK) return st;
             st = run_pass(ctx, i, 1, s, rb, &frame);
             if (st != HD_OK) return st;
             ctx->root = rb.root;
@@ -607,7 +635,7 @@ hd_status hd_frame_graph_launch(hd_frame_graph* g, const hd_edit* edits, uint32_
     if (n > g->max_edits) return fail(ctx, HD_ERR_INVALID, "more edits than the graph was built for");
     for (uint32_t i = 0; i < n; ++i)
         if (edits[i].kind > EDIT_SPHERE_PAINT || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
-            return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius (copies run through hd_edit_batch)");
+            return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius (copies and fills run through hd_edit_batch)");
     // the same free-space guarantees as an eager pass, then (re)capture if buffers moved
     hd_status st = grow_arena(ctx, 1u << 22);
     if (st == HD_OK) st = edit_ensure_edits(ctx, g->max_edits);

@@ -85,8 +85,8 @@ hd_replay_action parse_action(const std::string& name, const std::vector<std::st
         a.edit.radius = to_f(d[3]) * scale;
     } else if (name == "EditPaint" || name == "EditFill") {
         need(d, 4, name.c_str());
-        a.type = name == "EditPaint" ? HD_REPLAY_EDIT : HD_REPLAY_FILL;
-        a.edit.kind = HD_EDIT_SPHERE_PAINT;
+        a.type = HD_REPLAY_EDIT;
+        a.edit.kind = name == "EditPaint" ? HD_EDIT_SPHERE_PAINT : HD_EDIT_FILL;
         loc3(0, a.edit.center);
         a.edit.radius = to_f(d[3]) * scale;
     } else if (name == "EditCopy") {

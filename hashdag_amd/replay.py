@@ -13,9 +13,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (HD_REPLAY_EDIT, HD_REPLAY_END_FRAME, HD_REPLAY_FILL, HD_REPLAY_REDO, HD_REPLAY_SET_LOCATION,
-                   HD_REPLAY_SET_ROTATION, HD_REPLAY_UNDO, HD_REQ_EDIT, HD_REQ_REDO, HD_REQ_TRACE, HD_REQ_UNDO,
-                   check, hd_replay_action, lib)
+from ._lib import (HD_REPLAY_EDIT, HD_REPLAY_END_FRAME, HD_REPLAY_REDO, HD_REPLAY_SET_LOCATION, HD_REPLAY_SET_ROTATION,
+                   HD_REPLAY_UNDO, HD_REQ_EDIT, HD_REQ_REDO, HD_REQ_TRACE, HD_REQ_UNDO, check, hd_replay_action, lib)
 from .tracer import CameraView, DAGInfo, ray_params
 
 COPY = 5
@@ -64,7 +63,7 @@ class Replay:
                     r.location = tuple(a.location)
                 elif a.type == HD_REPLAY_SET_ROTATION:
                     r.rotation = tuple(a.rotation)
-                elif a.type in (HD_REPLAY_EDIT, HD_REPLAY_FILL):
+                elif a.type == HD_REPLAY_EDIT:
                     r.edit = _edit_tuple(a.edit)
                 out.append(r)
             return Replay(out, int(frames.value))
@@ -77,8 +76,7 @@ class Replay:
             return Replay.parse(f.read(), scene_depth, replay_depth)
 
     def counts(self) -> dict:
-        names = {HD_REPLAY_EDIT: "edits", HD_REPLAY_UNDO: "undo", HD_REPLAY_REDO: "redo", HD_REPLAY_FILL: "fill",
-                 HD_REPLAY_END_FRAME: "frames"}
+        names = {HD_REPLAY_EDIT: "edits", HD_REPLAY_UNDO: "undo", HD_REPLAY_REDO: "redo", HD_REPLAY_END_FRAME: "frames"}
         out = {v: 0 for v in names.values()}
         for a in self.actions:
             if a.type in names:
@@ -101,15 +99,15 @@ class Replay:
 
 def replay_requests(replay: Replay, width: int, height: int, levels: int, fps: float = 60.0, params=None,
                     info: DAGInfo | None = None):
-    """The replay as a served request stream (serve.Request list) and the number of skipped fill
-    actions (the fill editor is not built). Trace requests use `params` (hd_ray_params, cycled
-    per frame) when given, else the replay's own camera of the frame with scene bounds `info`."""
+    """The replay as a served request stream (serve.Request list). Trace requests use `params`
+    (hd_ray_params, cycled per frame) when given, else the replay's own camera of the frame with
+    scene bounds `info`."""
     from .serve import Request
 
     if params is None:
         info = info or DAGInfo.unit(levels)
         cams = replay.cameras()
-    reqs, pending, skipped = [], [], 0
+    reqs, pending = [], []
     for a in replay.actions:
         t = a.frame / fps
         if a.type == HD_REPLAY_EDIT:
@@ -119,8 +117,6 @@ def replay_requests(replay: Replay, width: int, height: int, levels: int, fps: f
                 reqs.append(Request(t, HD_REQ_EDIT, edits=pending))
                 pending = []
             reqs.append(Request(t, HD_REQ_UNDO if a.type == HD_REPLAY_UNDO else HD_REQ_REDO))
-        elif a.type == HD_REPLAY_FILL:
-            skipped += 1
         elif a.type == HD_REPLAY_END_FRAME:
             if pending:
                 reqs.append(Request(t, HD_REQ_EDIT, edits=pending))
@@ -130,4 +126,4 @@ def replay_requests(replay: Replay, width: int, height: int, levels: int, fps: f
             reqs.append(Request(t, HD_REQ_TRACE, width, height, prm))
     if pending:
         reqs.append(Request(replay.frames / fps, HD_REQ_EDIT, edits=pending))
-    return reqs, skipped
+    return reqs

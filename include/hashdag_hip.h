@@ -160,7 +160,8 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
 #define HD_EDIT_BOX_ADD 2u
 #define HD_EDIT_BOX_CARVE 3u
 #define HD_EDIT_SPHERE_PAINT 4u
-#define HD_EDIT_COPY 5u  /* CopyEditorWithoutDecompression, identity transform (hash_dag_editors.h:332-435) */
+#define HD_EDIT_COPY 5u  /* CopyEditorWithoutDecompression with its transform and swirl (hash_dag_editors.h:332-435) */
+#define HD_EDIT_FILL 6u  /* FillEditorColors: flood fill of the empty voxels around center (hash_dag_editors.h:624-768) */
 typedef struct hd_edit {
     uint32_t kind;
     float center[3];
@@ -336,11 +337,10 @@ hd_status hd_checkpoint_load(hd_ctx* ctx, const char* path);
 #define HD_REPLAY_SET_LOCATION 0u
 #define HD_REPLAY_SET_ROTATION 1u
 #define HD_REPLAY_SET_TOOL 2u    /* SetToolParameters: interactive tool state, no effect on the DAG */
-#define HD_REPLAY_EDIT 3u        /* EditSphere / EditCube / EditCopy / EditPaint: `edit` */
+#define HD_REPLAY_EDIT 3u        /* EditSphere / EditCube / EditCopy / EditPaint / EditFill: `edit` */
 #define HD_REPLAY_UNDO 4u
 #define HD_REPLAY_REDO 5u
 #define HD_REPLAY_END_FRAME 6u
-#define HD_REPLAY_FILL 7u        /* EditFill (FillEditorColors): not built; `edit` holds centre and radius */
 
 typedef struct hd_replay hd_replay;
 typedef struct hd_replay_action {

@@ -4,6 +4,7 @@
 #include "oracle_core.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <map>
 #include <stdexcept>
@@ -101,6 +102,13 @@ struct Editor {
     u32 srcRoot = 0;
     const HashDAGColors* srcColors = nullptr;
     double inv[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    // fill (FillEditorColors over BufferEditor, hash_dag_editors.h:129-222, 624-768): the buffer
+    // of voxel states around the centre and the colours of the voxels the flood fill set
+    enum : u8 { Unknown = 0, LoadedTrue = 1, LoadedFalse = 2, SetTrue = 3, SetFalse = 4 };
+    mutable std::vector<u8> values;
+    std::vector<CompressedColor> fillColors;
+    F3 fmin{};
+    u32 fsize = 0;
 
     explicit Editor(const Edit& ed) : e(ed) {
         F3 c = e.center;
@@ -132,7 +140,57 @@ struct Editor {
         bmax = c + f3(radius);
         r2 = e.radius * e.radius;
     }
-    bool is_box() const { return e.kind == BOX_ADD || e.kind == BOX_CARVE || e.kind == COPY; }
+    bool is_box() const { return e.kind == BOX_ADD || e.kind == BOX_CARVE || e.kind == COPY || e.kind == FILL; }
+
+    // ---- BufferEditor (hash_dag_editors.h:144-209) ----
+    bool fvalid(u32 x, u32 y, u32 z) const { return x < fsize && y < fsize && z < fsize; }
+    u64 findex(u32 x, u32 y, u32 z) const { return (u64)x + (u64)fsize * y + (u64)fsize * fsize * z; }
+    Path world_path(u32 x, u32 y, u32 z) const {
+        return Path(to_u32(fmin.x + (float)x), to_u32(fmin.y + (float)y), to_u32(fmin.z + (float)z));
+    }
+    // get_value: loads an unknown voxel from the DAG before the edit
+    bool fget(u32 x, u32 y, u32 z) const {
+        u8& v = values[findex(x, y, z)];
+        if (v == Unknown) {
+            const Path p = world_path(x, y, z);
+            v = value_at(*srcTable, srcRoot, p) ? LoadedTrue : LoadedFalse;
+        }
+        return v == LoadedTrue || v == SetTrue;
+    }
+    // FillEditorBase (hash_dag_editors.h:629-671) then FillEditorColors (:688-728)
+    void build_fill() {
+        fmin = e.center - f3(e.radius);
+        fsize = 2 * (u32)std::ceil(e.radius);
+        values.assign((u64)fsize * fsize * fsize, Unknown);
+        fillColors.assign(values.size(), CompressedColor{});
+        const u32 c0 = fsize / 2;
+        u32 mn[3] = {c0, c0, c0}, mx[3] = {c0, c0, c0};
+        std::vector<std::array<u32, 3>> stack{{c0, c0, c0}};
+        while (!stack.empty()) {
+            const auto q = stack.back();
+            stack.pop_back();
+            values[findex(q[0], q[1], q[2])] = SetTrue;
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = std::min(mn[a], q[a]);
+                mx[a] = std::max(mx[a], q[a]);
+            }
+            static const int d[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+            for (const auto& o : d) {
+                const u32 n0 = q[0] + (u32)o[0], n1 = q[1] + (u32)o[1], n2 = q[2] + (u32)o[2];
+                if (fvalid(n0, n1, n2) && !fget(n0, n1, n2)) stack.push_back({n0, n1, n2});
+            }
+        }
+        bmin = fmin + F3{(float)mn[0], (float)mn[1], (float)mn[2]};
+        bmax = fmin + F3{(float)mx[0], (float)mx[1], (float)mx[2]};
+        CompressedColor color{};
+        for (u32 x = mn[0]; x <= mx[0]; ++x)
+            for (u32 y = mn[1]; y <= mx[1]; ++y)
+                for (u32 z = mn[2]; z <= mx[2]; ++z) {
+                    const u64 i = findex(x, y, z);
+                    if (values[i] == LoadedTrue) color = color_at(*srcTable, srcRoot, *srcColors, world_path(x, y, z));
+                    else if (values[i] == SetTrue) fillColors[i] = color;
+                }
+    }
     // destination_to_source (hash_dag_editors.h:364-379): position - dest, swirled about the y
     // axis (host cosf / sinf), through the inverse transform in FP64, plus source
     Path source_of(F3 pos) const {
@@ -210,6 +268,12 @@ struct Editor {
     // get_new_value (hash_dag_editors.h:263, 306, 322, 416-424)
     bool new_value(bool old, F3 pos) const {
         if (e.kind == COPY) return old ? old : value_at(*srcTable, srcRoot, source_of(pos));
+        if (e.kind == FILL) {  // BufferEditor::get_new_value (hash_dag_editors.h:211-220)
+            const F3 fp = pos - fmin;
+            const u32 x = to_u32(fp.x), y = to_u32(fp.y), z = to_u32(fp.z);
+            if (!fvalid(x, y, z) || !fget(x, y, z)) return old;
+            return true;
+        }
         return e.kind == SPHERE_PAINT ? old : adding();
     }
     // hash_dag_editors.h:264-269, 307-312, 323-329, 425-434
@@ -219,6 +283,14 @@ struct Editor {
         if (e.kind == COPY) {
             if (oldValue == newValue || !newValue) return oldColor();
             return color_at(*srcTable, srcRoot, *srcColors, source_of(pos));
+        }
+        if (e.kind == FILL) {  // FillEditorColors::get_new_color (hash_dag_editors.h:737-754)
+            if (newValue && !oldValue) {
+                const F3 fp = pos - fmin;
+                const u32 x = to_u32(fp.x), y = to_u32(fp.y), z = to_u32(fp.z);
+                if (fvalid(x, y, z) && values[findex(x, y, z)] == SetTrue) return fillColors[findex(x, y, z)];
+            }
+            return oldColor();
         }
         if (e.kind == SPHERE_PAINT) {
             const CompressedColor prev = oldColor();
@@ -485,7 +557,8 @@ EditStats apply_edit(DAG& dag, HashDAGColors* colors, const Edit& e, UndoFrame* 
     ed.srcTable = &dag.t;  // the copy reads the DAG as it was before this edit
     ed.srcRoot = dag.root;
     ed.srcColors = colors;
-    if (e.kind == COPY && !colors) throw std::runtime_error("oracle: copy needs colours");
+    if ((e.kind == COPY || e.kind == FILL) && !colors) throw std::runtime_error("oracle: copy and fill need colours");
+    if (e.kind == FILL) ed.build_fill();
     EditCtx ctx{dag.t, colors, ed, {}, {}, {}, {}};
     if (undo) {
         undo->root = dag.root;

@@ -24,6 +24,8 @@
  *   edits             dense-grid brute force of sphere/box/paint voxel sets and colours, and of
  *                     copies (translations, and transforms that permute / mirror / scale
  *                     axes exactly)                              tests/test_oracle_edits.py
+ *   fill              dense-grid flood fill with the buffer's loaded set, bounds and colour
+ *                     scan written out independently                tests/test_oracle_edits.py
  *   copy swirl        host cosf / sinf as the reference's host edit path calls them: parity
  *                     unpinned (read from hash_dag_editors.h:364-379 only)
  *   GC                canonical form unchanged, idempotent, used words == reachable words +
@@ -110,7 +112,7 @@ int orc_trace_shadows(void* dag, const double params[12], uint32_t W, uint32_t H
                       const uint32_t* paths, uint32_t* colors_inout, float bias, float fog, uint64_t* bytes,
                       int threads);
 
-/* edits: kind 0 sphere add, 1 sphere carve, 2 box add, 3 box carve, 4 sphere paint.
+/* edits: kind 0 sphere add, 1 sphere carve, 2 box add, 3 box carve, 4 sphere paint, 6 fill.
  * stats out: num nodes, num voxels, colour leaves rebuilt. colors may be NULL. */
 int orc_edit(void* dag, void* colors, uint32_t kind, const float center[3], float radius, uint64_t stats[3]);
 /* copy editor (CopyEditorWithoutDecompression, identity transform): box of `radius` around

@@ -290,7 +290,7 @@ F3 sun_direction();
 // ---------------------------------------------------------------------------------------------
 // Editors (hash_dag_editors.h) and edits (hash_dag_edits.h, hash_dag.h:255-405)
 // ---------------------------------------------------------------------------------------------
-enum EditKind : u32 { SPHERE_ADD = 0, SPHERE_CARVE = 1, BOX_ADD = 2, BOX_CARVE = 3, SPHERE_PAINT = 4, COPY = 5 };
+enum EditKind : u32 { SPHERE_ADD = 0, SPHERE_CARVE = 1, BOX_ADD = 2, BOX_CARVE = 3, SPHERE_PAINT = 4, COPY = 5, FILL = 6 };
 struct Edit {
     u32 kind;
     F3 center;

@@ -251,3 +251,36 @@ def test_swirl_copy_close_to_oracle():
     diff = int((a != b).sum())
     boxes = sum(int((2 * ed[2]) ** 3) for ed in edits)
     assert diff <= max(1, boxes // 100000), f"{diff} voxels differ"
+
+
+def fill_edits(p, seed):
+    """Fills above the surface, below the terrain shell and on a solid voxel, integer and
+    fractional radii."""
+    from hashdag_amd import terrain_height
+    rng = np.random.default_rng(seed)
+    F = p.footprint or (1 << p.levels)
+    out = []
+    for dy, r in ((5, 9.0), (-5, 7.0), (0, 6.0), (6, 5.5), (-3, 12.0)):
+        x, z = (int(v) for v in rng.integers(40, F - 40, size=2))
+        y = terrain_height(p, x, z) + dy
+        out.append((6, (float(x), float(y), float(z)), r))
+    return out
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_fill_matches_oracle(fused):
+    """FillEditorColors (hash_dag_editors.h:624-768): flood fill of the empty voxels around the
+    centre within its box, colours carried from the last loaded solid voxel; alone and between
+    other editors (each fill starts a new pass, built from the DAG the edits before it left)."""
+    p, sc, g, dag, od, oc = make("d13")
+    fills = fill_edits(p, seed=61)
+    others = random_edits(p, 6, seed=62)
+    edits = [fills[0], others[0], others[1], fills[1], fills[2], others[2], fills[3], others[3], fills[4]]
+    st = dag.edit_batch(edits, fused=fused)
+    total = 0
+    for ed in edits:
+        total += od.edit(ed[0], ed[1], ed[2], oc)[1]
+    assert st.num_voxels == total
+    if fused:
+        assert st.passes == 5  # [F o o] [F] [F o] [F o] [F]
+    assert_same(dag, sc, od, oc, f"fills (fused={fused})")

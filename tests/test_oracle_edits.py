@@ -260,6 +260,98 @@ def test_transformed_copy_brute_force(center, radius, source, dest, transform):
             assert col == cols_before[v], v
 
 
+FILLS = [
+    # center (integer valued), radius: above the surface, below the shell, on a solid voxel,
+    # a fractional radius
+    ("above", 8.0),
+    ("below", 7.0),
+    ("solid", 6.0),
+    ("above", 5.5),
+]
+
+
+def brute_force_fill(before, colours_before, center, radius):
+    """FillEditorColors (hash_dag_editors.h:129-222, 624-768) on a dense grid: 6-connected
+    flood fill of empty voxels from the box centre (set even if solid) over buffer voxels
+    loaded from world voxel floor(min + p); the voxels it checked are 'loaded'. The edit region
+    is every voxel overlapping (min + minEdit, min + maxEdit) (bounds from the min corners);
+    there voxel w becomes set if buffer voxel p = floor(w - min) is filled or solid. A voxel set
+    from a filled buffer voxel takes the colour of the last loaded solid voxel of an x, y, z
+    scan over the filled voxels' bounding box. Returns the grid and {voxel: colour} for the
+    voxels set from filled buffer voxels."""
+    from collections import deque
+    f32 = np.float32
+    fmin = np.array(center, np.float32) - f32(radius)
+    size = 2 * int(np.ceil(f32(radius)))
+    world = lambda q: tuple(int(f32(fmin[a]) + f32(q[a])) for a in range(3))
+    solid = lambda q: bool(before[world(q)])
+    c0 = (size // 2,) * 3
+    filled, loaded, todo = {c0}, set(), deque([c0])
+    while todo:
+        q = todo.popleft()
+        for a in range(3):
+            for d in (-1, 1):
+                n = list(q)
+                n[a] += d
+                n = tuple(n)
+                if not all(0 <= v < size for v in n) or n in filled:
+                    continue
+                loaded.add(n)
+                if not solid(n):
+                    filled.add(n)
+                    todo.append(n)
+    lo = [min(q[a] for q in filled) for a in range(3)]
+    hi = [max(q[a] for q in filled) for a in range(3)]
+    cols, color = {}, (0, 0)
+    for x in range(lo[0], hi[0] + 1):
+        for y in range(lo[1], hi[1] + 1):
+            for z in range(lo[2], hi[2] + 1):
+                q = (x, y, z)
+                if q in filled:
+                    cols[q] = color
+                elif q in loaded and solid(q):
+                    color = colours_before[world(q)]
+    bmin = [f32(fmin[a] + f32(lo[a])) for a in range(3)]
+    bmax = [f32(fmin[a] + f32(hi[a])) for a in range(3)]
+    after, new_cols = before.copy(), {}
+    ranges = [range(max(0, int(np.floor(bmin[a])) - 1), min(before.shape[a], int(np.ceil(bmax[a])) + 1))
+              for a in range(3)]
+    for w in ((x, y, z) for x in ranges[0] for y in ranges[1] for z in ranges[2]):
+        if not all(f32(w[a] + 1) > bmin[a] and f32(w[a]) < bmax[a] for a in range(3)):
+            continue
+        q = tuple(int(f32(f32(w[a]) - fmin[a])) for a in range(3))
+        if not all(0 <= v < size for v in q):
+            continue
+        if q in filled or solid(q):
+            if not before[w] and q in filled:
+                new_cols[w] = cols[q]
+            after[w] = 1
+    return after, new_cols
+
+
+@pytest.mark.parametrize("where,radius", FILLS)
+def test_fill_brute_force(where, radius):
+    from hashdag_amd.scene import terrain_height
+    p, sc, g = scene("d7")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    before, cols_before = per_voxel_colours(od, oc, g)
+    x, z = 60, 70
+    h = terrain_height(p, x, z)
+    y = {"above": h + 4, "below": h - 4, "solid": h}[where]
+    center = (float(x), float(y), float(z))
+    want, want_cols = brute_force_fill(before, cols_before, center, radius)
+    od.edit(6, center, radius, oc)
+    after, cols_after = per_voxel_colours(od, oc, g)
+    assert np.array_equal(after, want)
+    assert int(after.sum()) > int(before.sum()) or where == "solid"
+    for v, col in cols_after.items():
+        if v in want_cols:
+            assert col == want_cols[v], v
+        elif v in cols_before:
+            assert col == cols_before[v], v
+
+
 def test_swirl_copy_moves_voxels_by_rotation():
     """Swirl (hash_dag_editors.h:366-372): with a period so long that every angle in the box is
     below one voxel's worth of turn, the swirled copy equals the plain copy; with a shorter period

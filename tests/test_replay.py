@@ -30,8 +30,8 @@ def expected_edits(path, scale):
         if name in ("EditSphere", "EditCube"):
             add = d[4] in ("true", "TRUE")
             out.append((KINDS[(name, add)], tuple(float(x * scale) for x in v[:3]), float(v[3] * scale)))
-        elif name == "EditPaint":
-            out.append((4, tuple(float(x * scale) for x in v[:3]), float(v[3] * scale)))
+        elif name in ("EditPaint", "EditFill"):
+            out.append((4 if name == "EditPaint" else 6, tuple(float(x * scale) for x in v[:3]), float(v[3] * scale)))
         elif name == "EditCopy":
             T = np.eye(3) if len(d) == 10 else np.array([float(x) for x in v[10:19]]).reshape(3, 3)
             swirl = None if len(d) == 10 or d[19] not in ("true", "TRUE") else float(v[19])
@@ -50,7 +50,7 @@ def test_replay_parse_matches_independent_reading(name, scene_depth):
     assert rep.frames == lines.count("EndFrame")
     c = rep.counts()
     assert c["undo"] == lines.count("Undo") and c["redo"] == lines.count("Redo")
-    assert c["edits"] == sum(lines.count(k) for k in ("EditSphere", "EditCube", "EditPaint", "EditCopy"))
+    assert c["edits"] == sum(lines.count(k) for k in ("EditSphere", "EditCube", "EditPaint", "EditCopy", "EditFill"))
     scale = np.float32(2.0 ** scene_depth) / np.float32(2.0 ** 17)
     got = [a.edit for a in rep.actions if a.edit is not None and a.type == 3]
     want = expected_edits(path, scale)
@@ -89,9 +89,10 @@ def test_replay_requests_follow_frames():
     text = ("SetLocation,1,2,3\nSetRotation,1,0,0,0,1,0,0,0,1\nEditSphere,10,10,10,4,true\nEndFrame\n"
             "EditCube,20,20,20,3,false\nUndo\nRedo\nEndFrame\nEditFill,1,1,1,1\nEndFrame\n")
     rep = Replay.parse(text, 13, 13)
-    reqs, skipped = replay_requests(rep, 64, 32, 13, fps=10.0)
+    reqs = replay_requests(rep, 64, 32, 13, fps=10.0)
     kinds = [r.kind for r in reqs]
-    assert kinds == [HD_REQ_EDIT, HD_REQ_TRACE, HD_REQ_EDIT, HD_REQ_UNDO, HD_REQ_REDO, HD_REQ_TRACE, HD_REQ_TRACE]
-    assert skipped == 1
-    assert [r.arrival_s for r in reqs] == [0.0, 0.0, 0.1, 0.1, 0.1, 0.1, 0.2]
+    assert kinds == [HD_REQ_EDIT, HD_REQ_TRACE, HD_REQ_EDIT, HD_REQ_UNDO, HD_REQ_REDO, HD_REQ_TRACE, HD_REQ_EDIT,
+                     HD_REQ_TRACE]
+    assert [r.arrival_s for r in reqs] == [0.0, 0.0, 0.1, 0.1, 0.1, 0.1, 0.2, 0.2]
     assert reqs[0].edits == [(0, (10.0, 10.0, 10.0), 4.0)]
+    assert reqs[6].edits == [(6, (1.0, 1.0, 1.0), 1.0)]
