@@ -110,7 +110,9 @@ struct EditPass {
     unsigned long long* stat_new_words;
     unsigned long long* color_top;
     unsigned long long* arena_top;
+    unsigned long long* stat_colors;  // colours staged (color_top also counts the chunk padding)
     uint64_t* colors;
+    uint64_t* cmarks;  // per 64-colour chunk of `colors`: kUniformChunk | colour, or 0 (explicit)
     uint64_t color_cap;
     uint32_t* cnodes;
     uint32_t cnodes_cap;
@@ -136,6 +138,7 @@ struct EditScratch {
     void* level_mem[kMaxEditLevels]{};
     uint32_t* hset = nullptr;
     uint64_t* colors = nullptr;
+    uint64_t* cmarks = nullptr;
     uint32_t* dcount = nullptr;
     uint32_t* decount = nullptr;
     uint64_t* dstats = nullptr;

@@ -87,6 +87,9 @@ __device__ __forceinline__ uint64_t wave_alloc64(unsigned long long* ctr, uint64
     return b + incl - v;
 }
 __device__ __forceinline__ void wave_add64(unsigned long long* ctr, uint64_t v) { (void)wave_alloc64(ctr, v); }
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
+    return ((uint64_t)__shfl((uint32_t)(v >> 32), src, 64) << 32) | __shfl((uint32_t)v, src, 64);
+}
 
 // ---------------------------------------------------------------------------------------------
 // Editors (hash_dag_editors.h:9-330). FP32, contraction off.
@@ -793,20 +796,29 @@ __global__ __launch_bounds__(256) void k_color_starts(EditPass P, uint32_t l) {
     }
 }
 
-// colour array of every task with colours: base offset in the colour scratch
+// colour array of every task with colours: base offset in the colour scratch. Arrays start on
+// a 64-colour chunk; the chunk markers of the wave's range are cleared here (the colour
+// writers mark the chunks they fill with one colour).
 __global__ __launch_bounds__(256) void k_color_alloc(EditPass P) {
     const uint32_t ctl = P.ctl;
     const uint32_t n = min(P.count[ctl], P.cap[ctl]);
     if (*P.errors) return;
     const Items T = P.items[ctl];
+    const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += gridDim.x * blockDim.x) {
-        const uint32_t i = t0 + (threadIdx.x & 63u);
+        const uint32_t i = t0 + lane;
         const uint32_t cnt = i < n ? T.ncount[i] : 0u;
-        const uint64_t base = wave_alloc64(P.color_top, cnt);
-        if (i < n) {
-            if (cnt && base + cnt > P.color_cap) set_error(P.errors, ERR_COLOR_SCRATCH);
-            T.cbase[i] = base;
+        const uint64_t acnt = ((uint64_t)cnt + 63u) & ~63ull;
+        const uint64_t base = wave_alloc64(P.color_top, acnt);
+        wave_add64(P.stat_colors, cnt);
+        if (i < n) T.cbase[i] = base;
+        // the wave's arrays are contiguous: [base of lane 0, end of lane 63)
+        const uint64_t lo = shfl64(base, 0), hi = shfl64(base + acnt, 63);
+        if (hi > P.color_cap) {
+            if (lane == 0) set_error(P.errors, ERR_COLOR_SCRATCH);
+            continue;
         }
+        for (uint64_t k = lo / 64 + lane; k < hi / 64; k += 64) P.cmarks[k] = 0;
     }
 }
 
@@ -868,7 +880,14 @@ __device__ __forceinline__ uint64_t old_color(const EditPass& P, uint32_t mode, 
     return 0;  // CompressedColor{}
 }
 
-// keep / full slots of interior items at levels >= ctl: one wavefront per slot
+// A chunk of 64 staged colours that all share one colour with no weights is stored as a marker
+// (kUniformChunk | colour bits) instead of the colours. Block starts compare the colour bits
+// and the bits per weight (kColorKey) of neighbouring colours.
+constexpr uint64_t kUniformChunk = 1ull << 63;
+constexpr uint64_t kColorKey = 0x0000FF00FFFFFFFFull;
+
+// keep / full slots of interior items at levels >= ctl: one wavefront per slot. Whole chunks
+// of one colour become markers; the partial chunks at either end are written as colours.
 __global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
     const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
@@ -882,15 +901,33 @@ __global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
         const uint32_t cnt = I.cnew[slot];
         if (cnt == 0) continue;
         const uint32_t task = I.aux[i];
-        uint64_t* out = P.colors + T.cbase[task] + I.cstart[slot];
+        const uint64_t cb = T.cbase[task];  // a multiple of 64
+        uint64_t* out = P.colors + cb;
+        uint64_t* marks = P.cmarks + cb / 64;
+        const uint32_t cs = I.cstart[slot], ce = cs + cnt;
         if (kind == KIND_FULL) {
             const uint64_t col = pack_color(I.ccolor[slot], 0, 0);
-            for (uint32_t k = lane; k < cnt; k += 64) out[k] = col;
+            const uint32_t a = min((cs + 63u) & ~63u, ce), b = max(ce & ~63u, a);
+            for (uint32_t k = cs + lane; k < a; k += 64) out[k] = col;
+            for (uint32_t k = a / 64 + lane; k < b / 64; k += 64) marks[k] = kUniformChunk | col;
+            for (uint32_t k = b + lane; k < ce; k += 64) out[k] = col;
         } else {
             const LeafDesc d = task_leaf(P, T.handle[task]);
             const uint32_t mode = I.cmode[i], mv = I.cmodeval[i];
-            const uint64_t os = I.costart[slot];
-            for (uint32_t k = lane; k < cnt; k += 64) out[k] = old_color(P, mode, mv, d, os + k);
+            const uint64_t os = (uint64_t)I.costart[slot] - cs;
+            for (uint32_t k0 = cs & ~63u; k0 < ce; k0 += 64) {
+                const uint32_t k = k0 + lane;
+                const bool in = k >= cs && k < ce;
+                const uint64_t col = in ? old_color(P, mode, mv, d, os + k) : 0ull;
+                if (k0 >= cs && k0 + 64 <= ce) {
+                    const uint64_t c0 = shfl64(col, 0);
+                    if (__ballot(((col ^ c0) & kColorKey) != 0 || (col >> 40) != 0) == 0) {
+                        if (lane == 0) marks[k0 / 64] = kUniformChunk | (uint32_t)c0;
+                        continue;
+                    }
+                }
+                if (in) out[k] = col;
+            }
         }
     }
 }
@@ -1089,110 +1126,90 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ void color_at(const uint64_t* cols, uint32_t i, uint32_t cnt, uint64_t& c, uint32_t& bpw,
-                                         uint32_t& start) {
-    c = 0;
-    bpw = 0;
-    start = 0;
-    if (i >= cnt) return;
-    c = cols[i];
-    bpw = (uint32_t)(c >> 40) & 0xFFu;
-    start = (i % 16384u) == 0;
-    if (!start) {
-        const uint64_t p = cols[i - 1];
-        start = (uint32_t)p != (uint32_t)c || ((p >> 40) & 0xFFu) != bpw;
-    }
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v) {
+    return ((uint64_t)__shfl_up((uint32_t)(v >> 32), 1, 64) << 32) | __shfl_up((uint32_t)v, 1, 64);
 }
 
-__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
-    return ((uint64_t)__shfl((uint32_t)(v >> 32), src, 64) << 32) | __shfl((uint32_t)v, src, 64);
-}
-
-// Tasks are taken in rounds of 64 per wavefront (lane j records task j's sizes in pass 1), so
-// the arena is reserved with one atomic per round.
-__global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
-    const uint32_t ctl = P.ctl;
-    const uint32_t n = min(P.count[ctl], P.cap[ctl]);
-    if (*P.errors) return;
-    const Items T = P.items[ctl];
-    __shared__ uint32_t sBits[kEncWaves][12];
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    uint32_t* buf = sBits[wv];
-    const uint32_t nW = gridDim.x * kEncWaves;
-    for (uint32_t r = blockIdx.x * kEncWaves + wv; r < n; r += 64 * nW) {
-        // pass 1: blocks and weight bits of each task of the round
-        uint32_t myBlocks = 0, myBits = 0, myCnt = 0;
-        for (uint32_t j = 0; j < 64; ++j) {
-            const uint32_t t = r + j * nW;
-            if (t >= n) break;
-            const uint32_t cnt = T.ncount[t];
-            if (cnt == 0) continue;
-            if ((cnt + 16383u) / 16384u > 160) {  // more than 160 * 16384 colours in one subtree
-                if (lane == 0) set_error(P.errors, ERR_LEAF_SIZE);
+// One task's staged colours, chunk by chunk, in colour order. A block starts where the colour
+// index is a multiple of 16384 or the colour bits / bits per weight differ from the previous
+// colour's (the same rule as the leaf builder). Uniform chunks (markers) hold no weights and
+// can start a block only at their first colour, so a run of them is handled 64 chunks at a
+// time. kWrite = false counts blocks and weight bits; kWrite = true writes blocks, macro-block
+// records and weight words at the counted sizes. Weight bits of an explicit chunk are gathered
+// in LDS and each weight word is written once (the word a chunk leaves partial is carried).
+template <bool kWrite>
+__device__ __forceinline__ void encode_colors(const uint64_t* __restrict__ cols, const uint64_t* __restrict__ marks,
+                                              uint32_t cnt, uint32_t lane, uint32_t* buf, uint64_t* blocks,
+                                              uint64_t* macro, uint32_t* weights, uint32_t& outBlocks,
+                                              uint32_t& outBits) {
+    uint32_t carryBlocks = 0, carryBits = 0, macroBit = 0, carryWord = 0, laneBits = 0;
+    uint64_t prev = 0;  // the colour before the current chunk (unused at colour 0)
+    const uint32_t nCh = (cnt + 63u) / 64u;
+    for (uint32_t g0 = 0; g0 < nCh; g0 += 64) {
+        const uint32_t gl = min(64u, nCh - g0);
+        const uint64_t mk = lane < gl ? marks[g0 + lane] : 0ull;
+        const uint64_t marked = __ballot((mk & kUniformChunk) != 0);
+        uint32_t j = 0;
+        while (j < gl) {
+            if ((marked >> j) & 1ull) {
+                // uniform chunks j .. e-1 (lane k: chunk g0 + k)
+                const uint64_t open = ~marked >> j;
+                const uint32_t e = min(open ? j + (uint32_t)__ffsll((long long)open) - 1u : 64u, gl);
+                const uint64_t col = mk & 0xFFFFFFFFull;
+                const uint32_t upc = __shfl_up((uint32_t)col, 1, 64);
+                const bool in = lane >= j && lane < e;
+                const uint32_t pos = (g0 + lane) * 64u;
+                const bool mstart = in && (pos & 16383u) == 0;
+                const bool st = mstart || (in && (lane == j ? ((prev ^ col) & kColorKey) != 0 : upc != (uint32_t)col));
+                const uint64_t sb = __ballot(st);
+                if constexpr (kWrite) {
+                    const uint32_t bi = carryBlocks + (uint32_t)__popcll(sb & (~0ull >> (63u - lane)));
+                    if (st) blocks[bi - 1] = (col << 32) | (0xFFFFu << 16) | (pos & 16383u);
+                    if (mstart) {
+                        macro[2 * (pos >> 14)] = bi - 1;
+                        macro[2 * (pos >> 14) + 1] = carryBits;
+                    }
+                    if (__ballot(mstart)) macroBit = carryBits;
+                }
+                carryBlocks += (uint32_t)__popcll(sb);
+                prev = __shfl((uint32_t)col, e - 1, 64);
+                j = e;
                 continue;
             }
-            const uint64_t* cols = P.colors + T.cbase[t];
-            uint32_t nb = 0, bits = 0;
-            for (uint32_t i = lane; i < cnt; i += 64) {
-                uint64_t c;
-                uint32_t bpw, start;
-                color_at(cols, i, cnt, c, bpw, start);
-                nb += start;
-                bits += bpw;
+            // explicit chunk
+            const uint32_t c0 = (g0 + j) * 64u, i = c0 + lane;
+            uint64_t c = 0;
+            uint32_t bpw = 0;
+            if (i < cnt) {
+                c = cols[i];
+                bpw = (uint32_t)(c >> 40) & 0xFFu;
             }
-            nb = wave_sum(nb);
-            bits = wave_sum(bits);
-            if (lane == j) {
-                myBlocks = nb;
-                myBits = bits;
-                myCnt = cnt;
-            }
-        }
-        const uint32_t myMacro = (myCnt + 16383u) / 16384u;
-        const uint64_t myWords = myCnt ? (uint64_t)myBlocks + 2ull * myMacro + ((myBits + 31) / 32 + 1) / 2 + 1 : 0ull;
-        const uint64_t myBase = wave_alloc64(P.arena_top, myWords);
-        if (myCnt && myBase + myWords > P.arena_cap) set_error(P.errors, ERR_COLOR_ARENA);
-        // pass 2
-        for (uint32_t j = 0; j < 64; ++j) {
-            const uint32_t t = r + j * nW;
-            if (t >= n) break;
-            const uint32_t cnt = __shfl(myCnt, j, 64);
-            if (cnt == 0) continue;
-            const uint64_t base = shfl64(myBase, j), words = shfl64(myWords, j);
-            if (base + words > P.arena_cap) continue;
-            const uint32_t nBlocks = __shfl(myBlocks, j, 64), nBits = __shfl(myBits, j, 64);
-            const uint32_t nMacro = (cnt + 16383u) / 16384u;
-            const uint32_t nWeights = (nBits + 31) / 32;
-            const uint64_t* cols = P.colors + T.cbase[t];
-            uint64_t* blocks = P.arena + base;
-            uint64_t* macro = blocks + nBlocks;
-            uint32_t* weights = reinterpret_cast<uint32_t*>(macro + 2 * nMacro);
-            uint32_t carryBlocks = 0, carryBits = 0, macroBit = 0, carryWord = 0;
-            for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
-                const uint32_t i = c0 + lane;
-                uint64_t c;
-                uint32_t bpw, start;
-                color_at(cols, i, cnt, c, bpw, start);
+            const uint64_t up = shfl_up64(c);
+            const uint64_t p = lane == 0 ? prev : up;
+            const uint32_t start = i < cnt && ((i & 16383u) == 0 || ((p ^ c) & kColorKey) != 0);
+            prev = shfl64(c, 63);
+            ++j;
+            if constexpr (!kWrite) {
+                carryBlocks += (uint32_t)__popcll(__ballot(start));
+                laneBits += bpw;
+            } else {
                 const uint32_t sA = wave_incl_scan(start, lane), sB = wave_incl_scan(bpw, lane);
                 const uint32_t blockIdxIncl = carryBlocks + sA;  // block index + 1 if start
                 const uint32_t bitOff = carryBits + sB - bpw;    // exclusive bit offset
-                // a chunk of 64 holds at most one macro-block start
-                const uint64_t mb = __ballot(i < cnt && (i % 16384u) == 0);
+                // only lane 0 can sit on a macro-block start (16384 is a multiple of 64)
                 uint32_t myMacroBit = macroBit;
-                if (mb) {
-                    const uint32_t src = (uint32_t)__ffsll((long long)mb) - 1u;
-                    const uint32_t nb = __shfl(bitOff, src, 64);
-                    if (lane >= src) myMacroBit = nb;
-                    macroBit = nb;
-                    if (lane == src) {
-                        macro[2 * (i / 16384u)] = blockIdxIncl - 1;
-                        macro[2 * (i / 16384u) + 1] = bitOff;
+                if ((c0 & 16383u) == 0) {
+                    myMacroBit = carryBits;
+                    macroBit = carryBits;
+                    if (lane == 0) {
+                        macro[2 * (c0 >> 14)] = blockIdxIncl - 1;
+                        macro[2 * (c0 >> 14) + 1] = bitOff;
                     }
                 }
                 if (i < cnt && start) {
                     const uint32_t rel = bitOff - myMacroBit;
-                    const uint32_t hdr = bpw == 0 ? ((0xFFFFu << 16) | (i % 16384u))
-                                                  : ((rel << 16) | ((bpw - 1) << 14) | (i % 16384u));
+                    const uint32_t hdr = bpw == 0 ? ((0xFFFFu << 16) | (i & 16383u))
+                                                  : ((rel << 16) | ((bpw - 1) << 14) | (i & 16383u));
                     blocks[blockIdxIncl - 1] = ((uint64_t)(uint32_t)c << 32) | hdr;
                 }
                 // weight bits of the chunk: [B0, B1), words w0 .. ; MSB-first stream bit k ->
@@ -1217,7 +1234,71 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
                 carryBlocks = __shfl(blockIdxIncl, 63, 64);
                 carryBits = B1;
             }
-            if (lane == 0 && (carryBits & 31u)) weights[carryBits >> 5] = carryWord;
+        }
+    }
+    outBlocks = carryBlocks;
+    if constexpr (kWrite) {
+        outBits = carryBits;
+        if (lane == 0 && (carryBits & 31u)) weights[carryBits >> 5] = carryWord;
+    } else {
+        outBits = wave_sum(laneBits);
+    }
+}
+
+// One wavefront per task: encode its colour array into a new unique leaf in the arena. Tasks
+// are taken in rounds of 64 per wavefront (lane j records task j's sizes in the counting
+// walk), so the arena is reserved with one atomic per round.
+__global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
+    const uint32_t ctl = P.ctl;
+    const uint32_t n = min(P.count[ctl], P.cap[ctl]);
+    if (*P.errors) return;
+    const Items T = P.items[ctl];
+    __shared__ uint32_t sBits[kEncWaves][12];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint32_t* buf = sBits[wv];
+    const uint32_t nW = gridDim.x * kEncWaves;
+    for (uint32_t r = blockIdx.x * kEncWaves + wv; r < n; r += 64 * nW) {
+        // blocks and weight bits of each task of the round
+        uint32_t myBlocks = 0, myBits = 0, myCnt = 0;
+        for (uint32_t j = 0; j < 64; ++j) {
+            const uint32_t t = r + j * nW;
+            if (t >= n) break;
+            const uint32_t cnt = T.ncount[t];
+            if (cnt == 0) continue;
+            if ((cnt + 16383u) / 16384u > 160) {  // more than 160 * 16384 colours in one subtree
+                if (lane == 0) set_error(P.errors, ERR_LEAF_SIZE);
+                continue;
+            }
+            const uint64_t cb = T.cbase[t];
+            uint32_t nb, bits;
+            encode_colors<false>(P.colors + cb, P.cmarks + cb / 64, cnt, lane, buf, nullptr, nullptr, nullptr, nb,
+                                 bits);
+            if (lane == j) {
+                myBlocks = nb;
+                myBits = bits;
+                myCnt = cnt;
+            }
+        }
+        const uint32_t myMacro = (myCnt + 16383u) / 16384u;
+        const uint64_t myWords = myCnt ? (uint64_t)myBlocks + 2ull * myMacro + ((myBits + 31) / 32 + 1) / 2 + 1 : 0ull;
+        const uint64_t myBase = wave_alloc64(P.arena_top, myWords);
+        if (myCnt && myBase + myWords > P.arena_cap) set_error(P.errors, ERR_COLOR_ARENA);
+        for (uint32_t j = 0; j < 64; ++j) {
+            const uint32_t t = r + j * nW;
+            if (t >= n) break;
+            const uint32_t cnt = __shfl(myCnt, j, 64);
+            if (cnt == 0) continue;
+            const uint64_t base = shfl64(myBase, j), words = shfl64(myWords, j);
+            if (base + words > P.arena_cap) continue;
+            const uint32_t nBlocks = __shfl(myBlocks, j, 64), nBits = __shfl(myBits, j, 64);
+            const uint32_t nMacro = (cnt + 16383u) / 16384u;
+            const uint32_t nWeights = (nBits + 31) / 32;
+            uint64_t* blocks = P.arena + base;
+            uint64_t* macro = blocks + nBlocks;
+            uint32_t* weights = reinterpret_cast<uint32_t*>(macro + 2 * nMacro);
+            const uint64_t cb = T.cbase[t];
+            uint32_t wb, wbits;
+            encode_colors<true>(P.colors + cb, P.cmarks + cb / 64, cnt, lane, buf, blocks, macro, weights, wb, wbits);
             if (lane < 2) weights[nWeights + lane] = 0u;  // padding read by the 16-bit window
             if (lane == 0) {
                 LeafDesc d;
@@ -1307,6 +1388,7 @@ static void scratch_free(hd_ctx* ctx) {
     for (void* p : S.allocs) hipFree(p);
     hipFree(S.hset);
     hipFree(S.colors);
+    hipFree(S.cmarks);
     for (void*& p : S.level_mem) {
         hipFree(p);
         p = nullptr;
@@ -1421,9 +1503,13 @@ hd_status edit_reserve(hd_ctx* ctx, const uint32_t* items, const uint32_t* elist
     if (colors > S.color_cap) {
         sync_once();
         hipFree(S.colors);
+        hipFree(S.cmarks);
         S.colors = nullptr;
+        S.cmarks = nullptr;
         S.color_cap = 0;
-        const hipError_t err = hipMalloc(&S.colors, colors * 8);
+        colors = (colors + 63) & ~63ull;  // whole chunks, one marker each
+        hipError_t err = hipMalloc(&S.colors, colors * 8);
+        if (err == hipSuccess) err = hipMalloc(&S.cmarks, colors / 64 * 8);
         if (err != hipSuccess) return hip_fail(ctx, err, "edit scratch (colours)");
         S.color_cap = colors;
     }
@@ -1511,7 +1597,9 @@ static void fill_pass(hd_ctx* ctx, EditPass& P) {
     P.stat_new_words = reinterpret_cast<unsigned long long*>(S.dstats + 1);
     P.color_top = reinterpret_cast<unsigned long long*>(S.dstats + 2);
     P.arena_top = reinterpret_cast<unsigned long long*>(S.dstats + 3);
+    P.stat_colors = reinterpret_cast<unsigned long long*>(S.dstats + 4);
     P.colors = S.colors;
+    P.cmarks = S.cmarks;
     P.color_cap = S.color_cap;
     P.cnodes = ctx->cnodes;
     P.cnodes_cap = ctx->cfg.color_nodes_cap;

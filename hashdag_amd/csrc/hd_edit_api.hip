@@ -429,7 +429,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             if (edits[first].kind == EDIT_FILL && last - first > 1) out.num_voxels += fill_voxels(dev[first], 1u << ctx->g.levels);
             out.new_words += rb.stats[1];
             out.leaves_rebuilt += rb.counters[4];
-            out.colors_encoded += rb.stats[2];
+            out.colors_encoded += rb.stats[4];
             out.passes++;
             first = last;
         }
@@ -449,7 +449,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             out.num_voxels += rb.stats[0] + copy_voxels(edits + i, 1);
             out.new_words += rb.stats[1];
             out.leaves_rebuilt += rb.counters[4];
-            out.colors_encoded += rb.stats[2];
+            out.colors_encoded += rb.stats[4];
             out.passes++;
         }
     }
@@ -677,7 +677,7 @@ hd_status hd_frame_graph_launch(hd_frame_graph* g, const hd_edit* edits, uint32_
     if (st != HD_OK) return st;
     ctx->undo.push_back(std::move(frame));
     ctx->redo.clear();
-    if (stats) *stats = hd_edit_stats{rb.stats[0], rb.stats[1], rb.counters[4], rb.root, 1u, rb.stats[2]};
+    if (stats) *stats = hd_edit_stats{rb.stats[0], rb.stats[1], rb.counters[4], rb.root, 1u, rb.stats[4]};
     if (replayed) *replayed = 1;
     return HD_OK;
 }
