@@ -1,5 +1,6 @@
 // Device edit pipeline: shared structures (see hd_edit.hip).
 #pragma once
+#include <cstddef>
 
 #include <hip/hip_runtime.h>
 
@@ -47,6 +48,21 @@ struct DevEdit {
     uint32_t fill_lo[3], fill_dim[3];
     uint32_t pad2;
 };
+
+// The leading 64 bytes of a DevEdit: what the sphere / box editors' tests and colours read,
+// loaded with one wide load per edit.
+struct alignas(16) EditHead {
+    uint32_t kind;
+    float c[3];
+    float radius;
+    float r2;
+    float bmin[3], bmax[3];
+    uint32_t single;
+    float src[3];
+};
+static_assert(sizeof(EditHead) == 64 && sizeof(DevEdit) % 16 == 0, "edit head layout");
+static_assert(offsetof(DevEdit, single) == offsetof(EditHead, single) && offsetof(DevEdit, src) == offsetof(EditHead, src),
+              "edit head layout");
 
 // Device buffers of one prepared fill edit (owned by the host call that runs its pass).
 struct FillBuffers {

@@ -98,7 +98,8 @@ __device__ __forceinline__ float sqf(float f) { return f * f; }
 __device__ __forceinline__ float tmaxf2(float a, float b) { return a > b ? a : b; }
 
 // hash_dag_editors.h:76-91 / 112-126
-__device__ __forceinline__ bool e_touch(const DevEdit& e, float x0, float y0, float z0, float x1, float y1, float z1) {
+template <class E>
+__device__ __forceinline__ bool e_touch(const E& e, float x0, float y0, float z0, float x1, float y1, float z1) {
     if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE || e.kind == EDIT_COPY || e.kind == EDIT_FILL) {
         return !(e.bmin[0] >= x1 || e.bmin[1] >= y1 || e.bmin[2] >= z1 || e.bmax[0] <= x0 || e.bmax[1] <= y0 ||
                  e.bmax[2] <= z0 || x0 >= e.bmax[0] || y0 >= e.bmax[1] || z0 >= e.bmax[2] || x1 <= e.bmin[0] ||
@@ -114,7 +115,8 @@ __device__ __forceinline__ bool e_touch(const DevEdit& e, float x0, float y0, fl
     return dist > 0;
 }
 // hash_dag_editors.h:233-252 / 280-295: 1 = full (add covers), 2 = empty (carve covers), 0 = no
-__device__ __forceinline__ uint32_t e_cover(const DevEdit& e, float x0, float y0, float z0, float x1, float y1,
+template <class E>
+__device__ __forceinline__ uint32_t e_cover(const E& e, float x0, float y0, float z0, float x1, float y1,
                                             float z1) {
     if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE) {
         const bool in = e.bmin[0] <= x0 && e.bmin[1] <= y0 && e.bmin[2] <= z0 && e.bmax[0] >= x1 && e.bmax[1] >= y1 &&
@@ -157,7 +159,8 @@ __device__ __forceinline__ void unpack_rgb(uint64_t c, float& r, float& g, float
     b = (float)((bits >> 11) & 0x1Fu) / 31.0f * omf + (float)((bits >> 27) & 0x1Fu) / 31.0f * f;
 }
 // get_new_color of the editors (hash_dag_editors.h:264-269, 307-312, 323-329)
-__device__ __forceinline__ uint64_t e_color(const DevEdit& e, float px, float py, float pz, uint64_t prev) {
+template <class E>
+__device__ __forceinline__ uint64_t e_color(const E& e, float px, float py, float pz, uint64_t prev) {
     if (e.kind != EDIT_SPHERE_PAINT) return pack_color(e.single, 0, 0);
     float r, g, b;
     unpack_rgb(prev, r, g, b);
@@ -167,8 +170,13 @@ __device__ __forceinline__ uint64_t e_color(const DevEdit& e, float px, float py
     const float omk = 1 - k;
     return pack_color(to101210(r * omk + 0.0f * k, g * omk + 0.0f * k, b * omk + 1.0f * k), 0, 0);
 }
-__device__ __forceinline__ bool e_value(const DevEdit& e, bool old) {
+template <class E>
+__device__ __forceinline__ bool e_value(const E& e, bool old) {
     return e.kind == EDIT_SPHERE_PAINT ? old : (e.kind == EDIT_SPHERE_ADD || e.kind == EDIT_BOX_ADD);
+}
+
+__device__ __forceinline__ EditHead load_head(const DevEdit* edits, uint32_t ei) {
+    return *reinterpret_cast<const EditHead*>(edits + ei);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -300,7 +308,8 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
         const bool active = t < total;
         const uint32_t i = (uint32_t)(t >> 3), c = (uint32_t)(t & 7u);
         const uint32_t slot = i * 8 + c;
-        uint32_t kind = KIND_KEEP, m = 0, lastCover = ~0u, coverKind = 0, es = 0, ec = 0;
+        uint32_t kind = KIND_KEEP, m = 0, lastCover = ~0u, coverKind = 0, coverColor = 0, es = 0, ec = 0;
+        uint64_t touch = 0;
         uint32_t cx = 0, cy = 0, cz = 0, childOld = kInvalidNode, childOrig = kInvalidNode;
         float x0 = 0, y0 = 0, z0 = 0, x1 = 0, y1 = 0, z1 = 0;
         unsigned long long coveredVoxels = 0;
@@ -328,18 +337,29 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
             z1 = z0 + side;
             es = I.estart[i];
             ec = I.ecount[i];
-            // filter the edit list; remember the last cover at levels >= ctl
-            for (uint32_t k = 0; k < ec; ++k) {
-                const DevEdit& e = P.edits[P.elist[l][es + k]];
-                if (!e_touch(e, x0, y0, z0, x1, y1, z1)) continue;
-                if (l + 1 >= ctl) {
-                    const uint32_t cv = e_cover(e, x0, y0, z0, x1, y1, z1);
-                    if (cv) {
-                        lastCover = m;
-                        coverKind = cv;
+            // filter the edit list, two edit heads in flight; remember the last cover at levels
+            // >= ctl (and its colour) and which of the first 64 edits touch
+            for (uint32_t k = 0; k < ec; k += 2) {
+                const bool two = k + 1 < ec;
+                const uint32_t e0 = P.elist[l][es + k];
+                const uint32_t e1 = two ? P.elist[l][es + k + 1] : e0;
+                const EditHead h0 = load_head(P.edits, e0), h1 = load_head(P.edits, e1);
+#pragma unroll
+                for (uint32_t u = 0; u < 2; ++u) {
+                    if (u == 1 && !two) break;
+                    const EditHead& h = u ? h1 : h0;
+                    if (!e_touch(h, x0, y0, z0, x1, y1, z1)) continue;
+                    if (k + u < 64) touch |= 1ull << (k + u);
+                    if (l + 1 >= ctl) {
+                        const uint32_t cv = e_cover(h, x0, y0, z0, x1, y1, z1);
+                        if (cv) {
+                            lastCover = m;
+                            coverKind = cv;
+                            coverColor = h.single;
+                        }
                     }
+                    m++;
                 }
-                m++;
             }
             // old voxel count of the effective old child (colour offsets, "keep" counts)
             uint32_t oldCount = 0;
@@ -371,19 +391,8 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
                 }
                 // numVoxels of a plain edit: a full / empty early exit counts the whole node
                 if (covered && rest == 0) coveredVoxels = 1ull << (3 * depth);
-                if (covered) {
-                    // colour of a full cover (add_large_single_color of the covering editor)
-                    uint32_t q = 0;
-                    for (uint32_t k = 0; k < ec; ++k) {
-                        const DevEdit& e = P.edits[P.elist[l][es + k]];
-                        if (!e_touch(e, x0, y0, z0, x1, y1, z1)) continue;
-                        if (q == lastCover) {
-                            I.ccolor[slot] = e.single;
-                            break;
-                        }
-                        q++;
-                    }
-                }
+                // colour of a full cover (add_large_single_color of the covering editor)
+                if (covered) I.ccolor[slot] = coverColor;
             }
         }
         const bool isItem = active && kind == KIND_ITEM;
@@ -409,8 +418,7 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
             uint32_t q = 0, w = 0;
             for (uint32_t k = 0; k < ec; ++k) {
                 const uint32_t ei = P.elist[l][es + k];
-                const DevEdit& e = P.edits[ei];
-                if (!e_touch(e, x0, y0, z0, x1, y1, z1)) continue;
+                if (k < 64 ? !((touch >> k) & 1ull) : !e_touch(load_head(P.edits, ei), x0, y0, z0, x1, y1, z1)) continue;
                 if (lastCover == ~0u || q > lastCover) P.elist[l + 1][eo + w++] = ei;
                 q++;
             }
@@ -932,7 +940,9 @@ __global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
     }
 }
 
-// leaf items: fold each voxel's value and colour through the edit list
+// leaf items: fold each voxel's value and colour through the edit list. A wavefront takes a run
+// of consecutive items (neighbours share edits, so their edit heads stay cached); an item's
+// edit indices are read into the lanes at once and its edit heads two at a time.
 __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
     const uint32_t L = P.levels - 2;
     const uint32_t n = min(P.count[L], P.cap[L]);
@@ -940,72 +950,96 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
     const Items I = P.items[L];
     const Items T = P.items[P.ctl];
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t i = wave_index(); i < n; i += wave_count()) {
-        const uint32_t c1 = lane >> 3, c2 = lane & 7u;
-        const uint32_t x = (I.px[i] << 2) | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u);
-        const uint32_t y = (I.py[i] << 2) | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u);
-        const uint32_t z = (I.pz[i] << 2) | ((c1 & 1u) << 1) | (c2 & 1u);
-        uint64_t oldMask = 0;
-        if (I.old_[i] != kInvalidNode) oldMask = *reinterpret_cast<const uint64_t*>(P.vpool + I.old_[i]);
-        const bool prev = (oldMask >> lane) & 1u;
-        const uint32_t oldRank = __popcll(oldMask & ((1ull << lane) - 1ull));
-        const uint32_t task = I.aux[i];
-        const LeafDesc d = task_leaf(P, T.handle[task]);
-        bool v = prev;
-        uint64_t col = 0;
-        bool haveCol = false;
-        const float fx = (float)x, fy = (float)y, fz = (float)z;
-        for (uint32_t k = 0; k < I.ecount[i]; ++k) {
-            const DevEdit& e = P.edits[P.elist[L][I.estart[i] + k]];
-            if (!e_touch(e, fx, fy, fz, fx + 1.0f, fy + 1.0f, fz + 1.0f)) continue;
-            if (e.kind == EDIT_FILL) {
-                // FillEditorColors::get_new_color: a voxel the flood fill set takes its colour
-                if (!v) {
-                    uint64_t idx = 0;
-                    const uint32_t st = fill_state_at(e, fx, fy, fz, idx);
-                    if (st & 1u) {
-                        if (st & 2u) {
-                            col = fill_color_at(e, idx);
-                            haveCol = true;
-                        }
-                        v = true;
-                    }
-                }
-                continue;
-            }
-            if (e.kind == EDIT_COPY) {
-                // get_new_value / get_new_color: an empty voxel takes its source's value and colour
-                if (!v) {
-                    uint32_t sx, sy, sz;
-                    copy_source(e, fx, fy, fz, sx, sy, sz);
-                    const uint32_t srcRoot = P.items[0].orig[0];
-                    const uint64_t sl = source_leaf(P.vpool, srcRoot, P.levels, sx >> 2, sy >> 2, sz >> 2);
-                    if ((sl >> leaf_bit(sx, sy, sz)) & 1ull) {
-                        col = source_color(P, srcRoot, sx, sy, sz);
-                        haveCol = true;
-                        v = true;
-                    }
-                }
-                continue;
-            }
-            const bool nv = e_value(e, v);
-            if (nv) {
-                uint64_t pc = 0;
-                if (e.kind == EDIT_SPHERE_PAINT && v) {
-                    if (!haveCol) col = prev ? old_color(P, I.cmode[i], I.cmodeval[i], d, I.ostart[i] + oldRank) : 0ull;
-                    pc = col;
-                }
-                col = e_color(e, fx, fy, fz, pc);
-                haveCol = true;
-            }
-            v = nv;
-        }
-        const uint64_t newMask = __ballot(v);
-        if (v) {
+    const uint32_t run = max(1u, min(32u, n / (4u * wave_count())));
+    for (uint32_t i0 = wave_index() * run; i0 < n; i0 += wave_count() * run) {
+        const uint32_t iEnd = min(i0 + run, n);
+        for (uint32_t i = i0; i < iEnd; ++i) {
+            const uint32_t es = I.estart[i], ec = I.ecount[i];
+            const uint32_t myEdit = lane < ec ? P.elist[L][es + lane] : 0u;
+            const uint32_t c1 = lane >> 3, c2 = lane & 7u;
+            const uint32_t x = (I.px[i] << 2) | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u);
+            const uint32_t y = (I.py[i] << 2) | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u);
+            const uint32_t z = (I.pz[i] << 2) | ((c1 & 1u) << 1) | (c2 & 1u);
+            uint64_t oldMask = 0;
+            if (I.old_[i] != kInvalidNode) oldMask = *reinterpret_cast<const uint64_t*>(P.vpool + I.old_[i]);
+            const bool prev = (oldMask >> lane) & 1u;
+            const uint32_t oldRank = __popcll(oldMask & ((1ull << lane) - 1ull));
+            const uint32_t task = I.aux[i];
+            const LeafDesc d = task_leaf(P, T.handle[task]);
+            const uint32_t cmode = I.cmode[i], cmodeval = I.cmodeval[i];
+            const uint64_t oldIndex = I.ostart[i] + oldRank;
             // oldColor() of a voxel that was empty is CompressedColor{} (hash_dag_edits.h:520-540)
-            if (!haveCol) col = prev ? old_color(P, I.cmode[i], I.cmodeval[i], d, I.ostart[i] + oldRank) : 0ull;
-            const uint32_t newRank = __popcll(newMask & ((1ull << lane) - 1ull));
-            P.colors[T.cbase[task] + I.nstart[i] + newRank] = col;
+            auto voxel_old_color = [&]() -> uint64_t {
+                if (!prev) return 0ull;
+                if (cmode == 1) return pack_color(cmodeval, 0, 0);
+                return leaf_valid_index(d, oldIndex) ? leaf_decode(P.arena, d, oldIndex) : 0ull;
+            };
+            bool v = prev;
+            uint64_t col = 0;
+            bool haveCol = false;
+            const float fx = (float)x, fy = (float)y, fz = (float)z;
+            auto apply = [&](const EditHead& h, uint32_t ei) {
+                if (!e_touch(h, fx, fy, fz, fx + 1.0f, fy + 1.0f, fz + 1.0f)) return;
+                if (h.kind == EDIT_FILL) {
+                    // FillEditorColors::get_new_color: a voxel the flood fill set takes its colour
+                    if (!v) {
+                        const DevEdit& e = P.edits[ei];
+                        uint64_t idx = 0;
+                        const uint32_t st = fill_state_at(e, fx, fy, fz, idx);
+                        if (st & 1u) {
+                            if (st & 2u) {
+                                col = fill_color_at(e, idx);
+                                haveCol = true;
+                            }
+                            v = true;
+                        }
+                    }
+                    return;
+                }
+                if (h.kind == EDIT_COPY) {
+                    // get_new_value / get_new_color: an empty voxel takes its source's value and colour
+                    if (!v) {
+                        const DevEdit& e = P.edits[ei];
+                        uint32_t sx, sy, sz;
+                        copy_source(e, fx, fy, fz, sx, sy, sz);
+                        const uint32_t srcRoot = P.items[0].orig[0];
+                        const uint64_t sl = source_leaf(P.vpool, srcRoot, P.levels, sx >> 2, sy >> 2, sz >> 2);
+                        if ((sl >> leaf_bit(sx, sy, sz)) & 1ull) {
+                            col = source_color(P, srcRoot, sx, sy, sz);
+                            haveCol = true;
+                            v = true;
+                        }
+                    }
+                    return;
+                }
+                const bool nv = e_value(h, v);
+                if (nv) {
+                    uint64_t pc = 0;
+                    if (h.kind == EDIT_SPHERE_PAINT && v) {
+                        if (!haveCol) col = voxel_old_color();
+                        pc = col;
+                    }
+                    col = e_color(h, fx, fy, fz, pc);
+                    haveCol = true;
+                }
+                v = nv;
+            };
+            for (uint32_t k = 0; k < ec; k += 2) {
+                const uint32_t e0 = k < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)myEdit, (int)k) : P.elist[L][es + k];
+                const bool two = k + 1 < ec;
+                const uint32_t e1 = !two ? e0
+                                         : (k + 1 < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)myEdit, (int)(k + 1))
+                                                       : P.elist[L][es + k + 1]);
+                const EditHead h0 = load_head(P.edits, e0), h1 = load_head(P.edits, e1);
+                apply(h0, e0);
+                if (two) apply(h1, e1);
+            }
+            const uint64_t newMask = __ballot(v);
+            if (v) {
+                if (!haveCol) col = voxel_old_color();
+                const uint32_t newRank = __popcll(newMask & ((1ull << lane) - 1ull));
+                P.colors[T.cbase[task] + I.nstart[i] + newRank] = col;
+            }
         }
     }
 }
