@@ -40,49 +40,62 @@ __global__ void k_gc_seed(uint32_t* frontier0, uint32_t* counts, const uint32_t*
         frontier0[i] = roots[i];
 }
 
-// mark children of the live nodes of `level` (an interior level): 8 lanes per node, one per
-// child slot; a wavefront appends the children it marked first with one atomic
+// mark children of the live nodes of `level` (an interior level): one lane per node, its
+// children's pointers, bitmap words and atomics issued together; a wavefront appends the
+// children it marked first with one atomic (in node order, child order within a node)
 __global__ __launch_bounds__(256) void k_gc_mark(const uint32_t* __restrict__ vpool, uint32_t* __restrict__ bitmap,
                                                  const uint32_t* __restrict__ frontier, uint32_t* __restrict__ next,
                                                  uint32_t* counts, uint32_t level, uint32_t capCur, uint32_t cap,
                                                  uint64_t numVirtual, uint32_t* errors) {
     const uint32_t n = min(counts[level], capCur);
-    const uint64_t total = (uint64_t)n * 8;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < total; t0 += stride) {
-        const uint64_t t = t0 + lane;
-        uint32_t child = 0;
-        bool first = false;
-        if (t < total) {
-            const uint32_t va = frontier[t >> 3], c = (uint32_t)(t & 7u);
-            const uint32_t mask = vpool[va] & 0xFFu;
-            if (mask & (1u << c)) {
-                child = vpool[va + child_off(mask, c)];
-                if (child >= numVirtual) {  // not an address of this table
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += stride) {
+        const uint32_t i = t0 + lane;
+        uint32_t kid[8], nk = 0, firstBits = 0;
+        if (i < n) {
+            const uint32_t va = frontier[i];
+            nk = __popc(vpool[va] & 0xFFu);
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) kid[k] = k < nk ? vpool[va + 1 + k] : 0u;
+            uint32_t w[8];
+            // shared children are marked by many parents: read first, so only the first few
+            // pay for the atomic on a hot bitmap word
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k)
+                w[k] = k < nk && kid[k] < numVirtual
+                           ? __hip_atomic_load(&bitmap[kid[k] >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : ~0u;
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) {
+                if (k >= nk) continue;
+                if (kid[k] >= numVirtual) {  // not an address of this table
                     atomicOr(errors, GC_ERR_MAP);
-                } else {
-                    // shared children are marked by many parents: read first, so only the
-                    // first few pay for the atomic on a hot bitmap word
-                    const uint32_t bit = 1u << (child & 31u);
-                    uint32_t* word = &bitmap[child >> 5];
-                    first = !(__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit) &&
-                            !(atomicOr(word, bit) & bit);
+                    continue;
                 }
+                const uint32_t bit = 1u << (kid[k] & 31u);
+                if (!(w[k] & bit) && !(atomicOr(&bitmap[kid[k] >> 5], bit) & bit)) firstBits |= 1u << k;
             }
         }
-        const uint64_t m = __ballot(first);
-        if (m) {
-            const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&counts[level + 1], (uint32_t)__popcll(m));
-            base = __shfl(base, (int)leader, 64);
-            if (first) {
-                const uint32_t slot =
-                    base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                if (slot >= cap) atomicOr(errors, GC_ERR_CAP);
-                else next[slot] = child;
-            }
+        // wave scan of the marked counts, one atomic for the wavefront
+        const uint32_t cnt = __popc(firstBits);
+        uint32_t incl = cnt;
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        const uint32_t total = __shfl(incl, 63, 64);
+        if (total == 0) continue;
+        uint32_t base = 0;
+        if (lane == 63) base = atomicAdd(&counts[level + 1], total);
+        base = __shfl(base, 63, 64) + incl - cnt;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            if (!((firstBits >> k) & 1u)) continue;
+            if (base >= cap) atomicOr(errors, GC_ERR_CAP);
+            else next[base] = kid[k];
+            ++base;
         }
     }
 }
@@ -107,7 +120,8 @@ __device__ __forceinline__ uint32_t map_find(const uint32_t* __restrict__ keys, 
     return kInvalidNode;
 }
 
-// gather live nodes of `level` with remapped children into tmp (9 words per node)
+// gather live nodes of `level` with remapped children into tmp (9 words per node); the
+// children's map probes run side by side
 __global__ __launch_bounds__(256) void k_gc_gather(const uint32_t* __restrict__ vpool,
                                                    const uint32_t* __restrict__ frontier, const uint32_t* counts,
                                                    uint32_t level, uint32_t leafLevel, uint32_t maxLevel, uint32_t cap,
@@ -126,15 +140,44 @@ __global__ __launch_bounds__(256) void k_gc_gather(const uint32_t* __restrict__ 
         }
         const uint32_t hdr = vpool[va];
         const uint32_t nk = __popc(hdr & 0xFFu);
-        out[0] = hdr;
-        for (uint32_t k = 1; k <= nk; ++k) {
-            uint32_t c = vpool[va + k];
-            if (level != maxLevel) {
-                c = map_find(mkeys, mvals, mmask, c);
-                if (c == kInvalidNode) atomicOr(errors, GC_ERR_MAP);
+        uint32_t c[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) c[k] = k < nk ? vpool[va + 1 + k] : 0u;
+        if (level != maxLevel) {
+            // map_find of every child: probe slots until the key or an empty slot (at most
+            // mmask + 1 probes each)
+            uint32_t slot[8], open = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) {
+                slot[k] = map_hash(c[k]) & mmask;
+                if (k < nk) open |= 1u << k;
             }
-            out[k] = c;
+            for (uint32_t probes = 0; open && probes <= mmask; ++probes) {
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k) {
+                    if (!((open >> k) & 1u)) continue;
+                    const uint32_t key = mkeys[slot[k]], val = mvals[slot[k]];
+                    if (key == c[k] + 1) {
+                        c[k] = val;
+                        open &= ~(1u << k);
+                    } else if (key == 0) {
+                        c[k] = kInvalidNode;
+                        open &= ~(1u << k);
+                    } else {
+                        slot[k] = (slot[k] + 1) & mmask;
+                    }
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) {
+                if ((open >> k) & 1u) c[k] = kInvalidNode;  // probed every slot
+                if (k < nk && c[k] == kInvalidNode) atomicOr(errors, GC_ERR_MAP);
+            }
         }
+        out[0] = hdr;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k)
+            if (k < nk) out[1 + k] = c[k];
         tlen[i] = nk + 1;
     }
 }
