@@ -87,6 +87,35 @@ __device__ __forceinline__ uint64_t wave_alloc64(unsigned long long* ctr, uint64
     return b + incl - v;
 }
 __device__ __forceinline__ void wave_add64(unsigned long long* ctr, uint64_t v) { (void)wave_alloc64(ctr, v); }
+// Block-wide form of wave_alloc32 for two counters: one atomic per counter per block (a counter
+// every wavefront of a large grid hits is limited by how fast one address takes atomics). Every
+// thread of the block calls it, in block-uniform control flow; `lds` holds 2 * (waves + 1) words.
+__device__ __forceinline__ void block_alloc32x2(uint32_t* lds, uint32_t* ctrA, uint32_t a, uint32_t* ctrB, uint32_t b,
+                                                uint32_t& outA, uint32_t& outB) {
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t ia = wave_scan32(a), ib = wave_scan32(b);
+    if (lane == 63) {
+        lds[wv] = ia;
+        lds[nw + 1 + wv] = ib;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ta = 0, tb = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint32_t xa = lds[w], xb = lds[nw + 1 + w];
+            lds[w] = ta;  // exclusive prefix of the wavefront totals
+            lds[nw + 1 + w] = tb;
+            ta += xa;
+            tb += xb;
+        }
+        lds[nw] = ta ? atomicAdd(ctrA, ta) : 0u;
+        lds[2 * nw + 1] = tb ? atomicAdd(ctrB, tb) : 0u;
+    }
+    __syncthreads();
+    outA = lds[nw] + lds[wv] + ia - a;
+    outB = lds[2 * nw + 1] + lds[nw + 1 + wv] + ib - b;
+    __syncthreads();  // the words are reused by the next call
+}
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
     return ((uint64_t)__shfl((uint32_t)(v >> 32), src, 64) << 32) | __shfl((uint32_t)v, src, 64);
 }
@@ -293,18 +322,22 @@ __device__ __forceinline__ void set_error(uint32_t* flags, uint32_t bit) { atomi
 // expand: items of level l -> child classification and items of level l+1
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
+    __shared__ uint32_t sAlloc[2 * (256 / 64 + 1)];
+    __shared__ uint32_t sErr;
     const uint32_t n = min(P.count[l], P.cap[l]);
-    if (*P.errors) return;
+    if (threadIdx.x == 0) sErr = *P.errors;
+    __syncthreads();
+    if (sErr) return;  // block-uniform: the loop below synchronises the block
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
     const uint32_t D = P.levels, L = D - 2, ctl = P.ctl;
     const uint32_t depth = D - (l + 1);  // child cube side = 2^depth
     const float side = (float)(1u << depth);
     const uint64_t total = (uint64_t)n * 8;
-    // wave-uniform loop: child items are allocated with one atomic per wavefront
-    for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < total;
-         t0 += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t t = t0 + (threadIdx.x & 63u);
+    // block-uniform loop: child items and edit-list words are allocated with one atomic each
+    // per block
+    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < total; b0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = b0 + threadIdx.x;
         const bool active = t < total;
         const uint32_t i = (uint32_t)(t >> 3), c = (uint32_t)(t & 7u);
         const uint32_t slot = i * 8 + c;
@@ -398,8 +431,8 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
         const bool isItem = active && kind == KIND_ITEM;
         const uint32_t rest = isItem ? (lastCover == ~0u ? m : m - lastCover - 1) : 0u;
         if (P.single_edit) wave_add64(P.stat_voxels, coveredVoxels);
-        const uint32_t ci = wave_alloc32(&P.count[l + 1], isItem ? 1u : 0u);
-        const uint32_t eo = wave_alloc32(&P.ecount[l + 1], rest);
+        uint32_t ci, eo;
+        block_alloc32x2(sAlloc, &P.count[l + 1], isItem ? 1u : 0u, &P.ecount[l + 1], rest, ci, eo);
         if (!active) continue;
         if (isItem && (ci >= P.cap[l + 1] || eo + rest > P.ecap[l + 1])) {
             set_error(P.errors, ERR_SCRATCH);

@@ -47,11 +47,13 @@ __global__ __launch_bounds__(256) void k_gc_mark(const uint32_t* __restrict__ vp
                                                  const uint32_t* __restrict__ frontier, uint32_t* __restrict__ next,
                                                  uint32_t* counts, uint32_t level, uint32_t capCur, uint32_t cap,
                                                  uint64_t numVirtual, uint32_t* errors) {
+    __shared__ uint32_t sTot[256 / 64 + 1];
     const uint32_t n = min(counts[level], capCur);
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += stride) {
-        const uint32_t i = t0 + lane;
+    // block-uniform loop: the next frontier is appended with one atomic per block
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+        const uint32_t i = b0 + threadIdx.x;
         uint32_t kid[8], nk = 0, firstBits = 0;
         if (i < n) {
             const uint32_t va = frontier[i];
@@ -77,7 +79,7 @@ __global__ __launch_bounds__(256) void k_gc_mark(const uint32_t* __restrict__ vp
                 if (!(w[k] & bit) && !(atomicOr(&bitmap[kid[k] >> 5], bit) & bit)) firstBits |= 1u << k;
             }
         }
-        // wave scan of the marked counts, one atomic for the wavefront
+        // block scan of the marked counts, one atomic for the block
         const uint32_t cnt = __popc(firstBits);
         uint32_t incl = cnt;
 #pragma unroll
@@ -85,11 +87,20 @@ __global__ __launch_bounds__(256) void k_gc_mark(const uint32_t* __restrict__ vp
             const uint32_t t = __shfl_up(incl, off, 64);
             if (lane >= off) incl += t;
         }
-        const uint32_t total = __shfl(incl, 63, 64);
-        if (total == 0) continue;
-        uint32_t base = 0;
-        if (lane == 63) base = atomicAdd(&counts[level + 1], total);
-        base = __shfl(base, 63, 64) + incl - cnt;
+        if (lane == 63) sTot[wv] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (uint32_t w = 0; w < nw; ++w) {
+                const uint32_t x = sTot[w];
+                sTot[w] = tot;
+                tot += x;
+            }
+            sTot[nw] = tot ? atomicAdd(&counts[level + 1], tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t base = sTot[nw] + sTot[wv] + incl - cnt;
+        __syncthreads();  // sTot is rewritten by the next iteration
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k) {
             if (!((firstBits >> k) & 1u)) continue;
