@@ -19,7 +19,7 @@ constexpr uint32_t EDIT_SPHERE_ADD = 0, EDIT_SPHERE_CARVE = 1, EDIT_BOX_ADD = 2,
 // error bits reported by a pass
 constexpr uint32_t ERR_SCRATCH = 1, ERR_BUCKET = 2, ERR_POOL = 4, ERR_HEADER = 8, ERR_COLOR_SCRATCH = 16,
                    ERR_COLOR_ARENA = 32, ERR_COLOR_TREE = 64, ERR_UNDO = 128, ERR_EMPTY = 256,
-                   ERR_LEAF_SIZE = 512;
+                   ERR_LEAF_SIZE = 512, ERR_KIND = 1024;
 
 constexpr uint32_t kMaxEditLevels = 17;
 constexpr uint32_t kUndoCap = 1u << 20;

@@ -485,7 +485,8 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
 // leaf level: one thread per leaf item, its 64 voxels as bit masks (bit = child1 * 8 + child2).
 // Thread-per-item keeps many items' loads in flight; the per-voxel editor tests are cheap.
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t leaf_touch_mask(const DevEdit& e, uint32_t bx, uint32_t by, uint32_t bz) {
+template <class E>
+__device__ __forceinline__ uint64_t leaf_touch_mask(const E& e, uint32_t bx, uint32_t by, uint32_t bz) {
     uint64_t tm = 0;
 #pragma unroll 8
     for (uint32_t b = 0; b < 64; ++b) {
@@ -498,6 +499,8 @@ __device__ __forceinline__ uint64_t leaf_touch_mask(const DevEdit& e, uint32_t b
     return tm;
 }
 
+// kSpecial: the pass may hold copy / fill editors (their code is built in only then)
+template <bool kSpecial>
 __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
     const uint32_t L = P.levels - 2;
     const uint32_t n = min(P.count[L], P.cap[L]);
@@ -516,9 +519,15 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
             uint32_t cx = ~0u, cy = ~0u, cz = ~0u;  // copies: last source leaf looked up
             uint64_t cachedLeaf = 0;
             for (uint32_t k = 0; k < ec; ++k) {
-                const DevEdit& e = P.edits[P.elist[L][es + k]];
-                const uint64_t tm = leaf_touch_mask(e, bx, by, bz);
-                if (e.kind == EDIT_COPY) {
+                const uint32_t ei = P.elist[L][es + k];
+                const EditHead h = load_head(P.edits, ei);
+                if (!kSpecial && (h.kind == EDIT_COPY || h.kind == EDIT_FILL)) {
+                    set_error(P.errors, ERR_KIND);
+                    break;
+                }
+                const uint64_t tm = leaf_touch_mask(h, bx, by, bz);
+                const DevEdit& e = P.edits[ei];
+                if (kSpecial && h.kind == EDIT_COPY) {
                     // get_new_value: an empty touched voxel takes its source voxel's value
                     const uint32_t srcRoot = P.items[0].orig[0];
                     uint64_t cand = tm & ~v, add = 0;
@@ -542,7 +551,7 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
                     v |= add;
                     continue;  // a copy's "num voxels" is its box volume, counted on the host
                 }
-                if (e.kind == EDIT_FILL) {
+                if (kSpecial && h.kind == EDIT_FILL) {
                     // BufferEditor::get_new_value: set where the buffer voxel is set
                     uint64_t cand = tm & ~v, add = 0;
                     while (cand) {
@@ -559,8 +568,8 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
                     touched |= tm;
                     continue;
                 }
-                if (e.kind == EDIT_SPHERE_ADD || e.kind == EDIT_BOX_ADD) v |= tm;
-                else if (e.kind != EDIT_SPHERE_PAINT) v &= ~tm;
+                if (h.kind == EDIT_SPHERE_ADD || h.kind == EDIT_BOX_ADD) v |= tm;
+                else if (h.kind != EDIT_SPHERE_PAINT) v &= ~tm;
                 touched |= tm;
             }
             I.ncount[i] = __popcll(v);
@@ -973,9 +982,78 @@ __global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
     }
 }
 
+// keep / full slots of the items at level L - 1 (their children are leaves: at most 64 colours
+// a slot): one wavefront per item, the colours of all its keep / full slots laid out over the
+// lanes (lane j of round r takes colour r * 64 + j of the item's slots in slot order).
+__global__ __launch_bounds__(256) void k_color_copy_leaves(EditPass P) {
+    const uint32_t l = P.levels - 3;
+    const uint32_t n = min(P.count[l], P.cap[l]);
+    if (*P.errors) return;
+    const Items I = P.items[l];
+    const Items T = P.items[P.ctl];
+    const uint32_t lane = threadIdx.x & 63u, s = lane & 7u;
+    for (uint32_t i = wave_index(); i < n; i += wave_count()) {
+        const uint32_t slot = i * 8 + s;
+        const uint32_t kind = I.ckind[slot];
+        const uint32_t cnt = (kind == KIND_KEEP || kind == KIND_FULL) ? I.cnew[slot] : 0u;
+        // exclusive prefix of the slot counts over lanes 0..7 (lanes 8.. repeat them)
+        uint32_t off = cnt;
+#pragma unroll
+        for (uint32_t d = 1; d < 8; d <<= 1) {
+            const uint32_t t = __shfl_up(off, d, 8);
+            if (s >= d) off += t;
+        }
+        const uint32_t total = __shfl(off, 7, 64);
+        off -= cnt;
+        if (total == 0) continue;
+        const uint32_t cs = I.cstart[slot], cos = I.costart[slot], ccol = I.ccolor[slot];
+        const uint32_t task = I.aux[i];
+        const uint64_t cb = T.cbase[task];
+        const LeafDesc d = task_leaf(P, T.handle[task]);
+        const uint32_t mode = I.cmode[i], mv = I.cmodeval[i];
+        for (uint32_t r = 0; r < total; r += 64) {
+            const uint32_t idx = r + lane;
+            // the slot holding colour idx: the last one starting at or before it
+            uint32_t j = 0;
+#pragma unroll
+            for (uint32_t step = 4; step > 0; step >>= 1)
+                if (__shfl(off, j + step, 64) <= idx) j += step;
+            const uint32_t k = idx - __shfl(off, j, 64);
+            const uint32_t kj = __shfl(kind, j, 64), csj = __shfl(cs, j, 64), cosj = __shfl(cos, j, 64);
+            const uint32_t ccj = __shfl(ccol, j, 64);
+            if (idx >= total) continue;
+            uint64_t col;
+            if (kj == KIND_FULL) col = pack_color(ccj, 0, 0);
+            else if (mode == 1) col = pack_color(mv, 0, 0);
+            else col = leaf_valid_index(d, (uint64_t)cosj + k) ? leaf_decode(P.arena, d, (uint64_t)cosj + k) : 0ull;
+            P.colors[cb + csj + k] = col;
+        }
+    }
+}
+
+// A leaf item's fields that k_color_leaf reads first.
+struct LeafMeta {
+    uint32_t es, ec, px, py, pz, old, aux, cmode, cmodeval, ostart, nstart;
+};
+__device__ __forceinline__ LeafMeta load_meta(const Items& I, uint32_t i) {
+    return LeafMeta{I.estart[i], I.ecount[i], I.px[i],    I.py[i],     I.pz[i],    I.old_[i],
+                    I.aux[i],    I.cmode[i],  I.cmodeval[i], I.ostart[i], I.nstart[i]};
+}
+// Edit head of lane k of a wavefront (the leading words only: kind .. single).
+__device__ __forceinline__ EditHead head_of_lane(const EditHead& mh, uint32_t k) {
+    EditHead h;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&mh);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&h);
+#pragma unroll
+    for (int w = 0; w < 13; ++w) dst[w] = (uint32_t)__builtin_amdgcn_readlane((int)src[w], (int)k);
+    return h;
+}
+
 // leaf items: fold each voxel's value and colour through the edit list. A wavefront takes a run
-// of consecutive items (neighbours share edits, so their edit heads stay cached); an item's
-// edit indices are read into the lanes at once and its edit heads two at a time.
+// of consecutive items (neighbours share edits, so their edit heads stay cached), reads the
+// next item's fields while it works on one, and loads an item's edit heads into the lanes at
+// once (lane k: edit k), each edit then read out of its lane.
+template <bool kSpecial>
 __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
     const uint32_t L = P.levels - 2;
     const uint32_t n = min(P.count[L], P.cap[L]);
@@ -986,21 +1064,23 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
     const uint32_t run = max(1u, min(32u, n / (4u * wave_count())));
     for (uint32_t i0 = wave_index() * run; i0 < n; i0 += wave_count() * run) {
         const uint32_t iEnd = min(i0 + run, n);
+        LeafMeta next = load_meta(I, i0);
         for (uint32_t i = i0; i < iEnd; ++i) {
-            const uint32_t es = I.estart[i], ec = I.ecount[i];
-            const uint32_t myEdit = lane < ec ? P.elist[L][es + lane] : 0u;
+            const LeafMeta m = next;
+            if (i + 1 < iEnd) next = load_meta(I, i + 1);
+            const uint32_t es = m.es, ec = m.ec;
             const uint32_t c1 = lane >> 3, c2 = lane & 7u;
-            const uint32_t x = (I.px[i] << 2) | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u);
-            const uint32_t y = (I.py[i] << 2) | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u);
-            const uint32_t z = (I.pz[i] << 2) | ((c1 & 1u) << 1) | (c2 & 1u);
+            const uint32_t x = (m.px << 2) | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u);
+            const uint32_t y = (m.py << 2) | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u);
+            const uint32_t z = (m.pz << 2) | ((c1 & 1u) << 1) | (c2 & 1u);
             uint64_t oldMask = 0;
-            if (I.old_[i] != kInvalidNode) oldMask = *reinterpret_cast<const uint64_t*>(P.vpool + I.old_[i]);
+            if (m.old != kInvalidNode) oldMask = *reinterpret_cast<const uint64_t*>(P.vpool + m.old);
             const bool prev = (oldMask >> lane) & 1u;
             const uint32_t oldRank = __popcll(oldMask & ((1ull << lane) - 1ull));
-            const uint32_t task = I.aux[i];
+            const uint32_t task = m.aux;
             const LeafDesc d = task_leaf(P, T.handle[task]);
-            const uint32_t cmode = I.cmode[i], cmodeval = I.cmodeval[i];
-            const uint64_t oldIndex = I.ostart[i] + oldRank;
+            const uint32_t cmode = m.cmode, cmodeval = m.cmodeval;
+            const uint64_t oldIndex = m.ostart + oldRank;
             // oldColor() of a voxel that was empty is CompressedColor{} (hash_dag_edits.h:520-540)
             auto voxel_old_color = [&]() -> uint64_t {
                 if (!prev) return 0ull;
@@ -1013,7 +1093,11 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
             const float fx = (float)x, fy = (float)y, fz = (float)z;
             auto apply = [&](const EditHead& h, uint32_t ei) {
                 if (!e_touch(h, fx, fy, fz, fx + 1.0f, fy + 1.0f, fz + 1.0f)) return;
-                if (h.kind == EDIT_FILL) {
+                if (!kSpecial && (h.kind == EDIT_COPY || h.kind == EDIT_FILL)) {
+                    set_error(P.errors, ERR_KIND);
+                    return;
+                }
+                if (kSpecial && h.kind == EDIT_FILL) {
                     // FillEditorColors::get_new_color: a voxel the flood fill set takes its colour
                     if (!v) {
                         const DevEdit& e = P.edits[ei];
@@ -1029,7 +1113,7 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
                     }
                     return;
                 }
-                if (h.kind == EDIT_COPY) {
+                if (kSpecial && h.kind == EDIT_COPY) {
                     // get_new_value / get_new_color: an empty voxel takes its source's value and colour
                     if (!v) {
                         const DevEdit& e = P.edits[ei];
@@ -1057,21 +1141,19 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
                 }
                 v = nv;
             };
-            for (uint32_t k = 0; k < ec; k += 2) {
-                const uint32_t e0 = k < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)myEdit, (int)k) : P.elist[L][es + k];
-                const bool two = k + 1 < ec;
-                const uint32_t e1 = !two ? e0
-                                         : (k + 1 < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)myEdit, (int)(k + 1))
-                                                       : P.elist[L][es + k + 1]);
-                const EditHead h0 = load_head(P.edits, e0), h1 = load_head(P.edits, e1);
-                apply(h0, e0);
-                if (two) apply(h1, e1);
+            for (uint32_t k0 = 0; k0 < ec; k0 += 64) {
+                const uint32_t myEdit = k0 + lane < ec ? P.elist[L][es + k0 + lane] : 0u;
+                EditHead mh{};
+                if (k0 + lane < ec) mh = load_head(P.edits, myEdit);
+                const uint32_t kn = min(64u, ec - k0);
+                for (uint32_t k = 0; k < kn; ++k)
+                    apply(head_of_lane(mh, k), (uint32_t)__builtin_amdgcn_readlane((int)myEdit, (int)k));
             }
             const uint64_t newMask = __ballot(v);
             if (v) {
                 if (!haveCol) col = voxel_old_color();
                 const uint32_t newRank = __popcll(newMask & ((1ull << lane) - 1ull));
-                P.colors[T.cbase[task] + I.nstart[i] + newRank] = col;
+                P.colors[T.cbase[task] + m.nstart + newRank] = col;
             }
         }
     }
@@ -1683,7 +1765,8 @@ static void fill_pass(hd_ctx* ctx, EditPass& P) {
 
 // Enqueues one pass over edits [first, first + n) of the uploaded edit array.
 // dN (optional): the edit count is read from device memory (at most n), for graph replays.
-hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s, const uint32_t* dN) {
+hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s, const uint32_t* dN,
+                       bool special) {
     EditPass P{};
     fill_pass(ctx, P);
     P.single_edit = (n == 1 && !dN) ? 1u : 0u;
@@ -1693,7 +1776,8 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
     hipMemcpyAsync(&P.items[0].old_[0], ctx->d_root, 4, hipMemcpyDeviceToDevice, s);
     hipMemcpyAsync(&P.items[0].orig[0], ctx->d_root, 4, hipMemcpyDeviceToDevice, s);
     for (uint32_t l = 0; l < L; ++l) k_expand<<<kGrid, 256, 0, s>>>(P, l);
-    k_leaf<<<kGrid, 256, 0, s>>>(P);
+    if (special) k_leaf<true><<<kGrid, 256, 0, s>>>(P);
+    else k_leaf<false><<<kGrid, 256, 0, s>>>(P);
     for (int l = (int)L; l >= 0; --l) {
         if (l < (int)L) k_build<<<kGrid, 256, 0, s>>>(P, (uint32_t)l);
         k_fill<<<256, 256, 0, s>>>(P.hset, P.hcap, 0u);
@@ -1706,8 +1790,12 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
         for (uint32_t l = P.ctl; l < L; ++l) k_color_starts<<<kGrid, 256, 0, s>>>(P, l);
         k_color_alloc<<<kGrid, 256, 0, s>>>(P);
         k_color_tree<<<kGrid, 256, 0, s>>>(P);
-        for (uint32_t l = P.ctl; l < L; ++l) k_color_copy<<<kGrid, 256, 0, s>>>(P, l);
-        k_color_leaf<<<kGrid, 256, 0, s>>>(P);
+        for (uint32_t l = P.ctl; l < L; ++l) {
+            if (l + 1 == L) k_color_copy_leaves<<<kGrid, 256, 0, s>>>(P);
+            else k_color_copy<<<kGrid, 256, 0, s>>>(P, l);
+        }
+        if (special) k_color_leaf<true><<<kGrid, 256, 0, s>>>(P);
+        else k_color_leaf<false><<<kGrid, 256, 0, s>>>(P);
         k_encode<<<kGrid, kEncBlock, 0, s>>>(P);
     }
     k_finish<<<1, 64, 0, s>>>(P, ctx->d_root);

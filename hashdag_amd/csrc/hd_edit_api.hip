@@ -15,7 +15,7 @@ namespace hd {
 hd_status edit_reserve(hd_ctx* ctx, const uint32_t* items, const uint32_t* elist, uint64_t colors);
 void edit_free(hd_ctx* ctx);
 hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s,
-                       const uint32_t* dN = nullptr);
+                       const uint32_t* dN = nullptr, bool special = true);
 hd_status edit_ensure_edits(hd_ctx* ctx, uint32_t n);
 hd_status enqueue_frame_from_device(hd_ctx* ctx, const hd_ray_params* d_params, uint32_t W, uint32_t H,
                                     uint32_t* d_paths, uint32_t* d_colors, float bias, float fog, hipStream_t s);
@@ -268,14 +268,16 @@ static hd_status edit_prepare(hd_ctx* ctx) {
     return edit_reserve_undo(ctx, kUndoCap);
 }
 
-static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s, Readback& rb, UndoFrame* frame) {
+// `special`: the pass holds a copy or a fill editor (leaf kernels built with their code)
+static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s, Readback& rb, UndoFrame* frame,
+                          bool special) {
     const bool colors = ctx->colors_loaded && ctx->g.leaf_level() > ctx->g.color_tree_levels;
     for (int attempt = 0; attempt < 40; ++attempt) {
         hd_status st = grow_arena(ctx, 1u << 22);
         if (st != HD_OK) return st;
         EditScratch& S = ctx->edit;
         k_reset<<<1, 1, 0, s>>>(ctx->counters, S.dstats);
-        st = enqueue_pass(ctx, first, n, colors, s);
+        st = enqueue_pass(ctx, first, n, colors, s, nullptr, special);
         if (st != HD_OK) return st;
         if (n > 1) {
             k_count_voxels<<<std::min<uint32_t>(n, 4096), 256, 0, s>>>(S.edits_dev + first, n, nullptr,
@@ -345,6 +347,7 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
             continue;
         }
         if (err & ERR_EMPTY) return fail(ctx, HD_ERR_INVALID, "edit would leave an empty scene (old root kept)");
+        if (err & ERR_KIND) return fail(ctx, HD_ERR_INVALID, "copy or fill editor in a pass built without them");
         ctx->arena_top = rb.stats[3];
         ctx->n_cnodes = rb.counters[2];
         ctx->n_cleaves = rb.counters[3];
@@ -419,7 +422,8 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             Readback rb{};
             st = prepare_fill(first);
             if (st != HD_OK) return st;
-            st = run_pass(ctx, first, last - first, s, rb, &frame);
+            st = run_pass(ctx, first, last - first, s, rb, &frame,
+                          edits[first].kind == EDIT_COPY || edits[first].kind == EDIT_FILL);
             if (st != HD_OK) return st;
             ctx->root = rb.root;
             frame.new_root = rb.root;
@@ -440,7 +444,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             Readback rb{};
             st = prepare_fill(i);
             if (st != HD_OK) return st;
-            st = run_pass(ctx, i, 1, s, rb, &frame);
+            st = run_pass(ctx, i, 1, s, rb, &frame, edits[i].kind == EDIT_COPY || edits[i].kind == EDIT_FILL);
             if (st != HD_OK) return st;
             ctx->root = rb.root;
             frame.new_root = rb.root;
@@ -558,7 +562,7 @@ static hd_status graph_capture(hd_frame_graph* g) {
     hipMemcpyAsync(S.edits_dev, g->h_edits, (uint64_t)g->max_edits * sizeof(DevEdit), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(g->d_params, g->h_params, sizeof(hd_ray_params), hipMemcpyHostToDevice, s);
     k_reset<<<1, 1, 0, s>>>(ctx->counters, S.dstats);
-    hd_status st = enqueue_pass(ctx, 0, g->max_edits, colors, s, g->d_n);
+    hd_status st = enqueue_pass(ctx, 0, g->max_edits, colors, s, g->d_n, false);  // kinds checked at capture
     k_count_voxels<<<std::min<uint32_t>(g->max_edits, 4096), 256, 0, s>>>(
         S.edits_dev, g->max_edits, g->d_n, 1u << ctx->g.levels, reinterpret_cast<unsigned long long*>(S.dstats + 0));
     if (st == HD_OK) st = enqueue_frame_from_device(ctx, g->d_params, g->width, g->height, g->paths, g->colors, g->bias,
