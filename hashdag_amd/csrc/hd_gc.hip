@@ -215,26 +215,116 @@ __global__ void k_gc_full(TableRef T, uint32_t level, uint32_t* full_nodes, uint
     else full_nodes[level] = va;
 }
 
-__global__ __launch_bounds__(256) void k_gc_add(TableRef T, uint32_t level, const uint32_t* counts, uint32_t cap,
-                                                const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ tmp,
-                                                const uint32_t* __restrict__ tlen, uint32_t* __restrict__ mkeys,
-                                                uint32_t* __restrict__ mvals, uint32_t mmask, uint32_t* errors) {
+// The live nodes of a level are placed bucket by bucket: each node takes a rank within its
+// bucket (k_gc_bucket), the buckets' ranges of a level-wide order follow from a scan of the
+// bucket counts (k_gc_scan, k_gc_order), one lane per bucket lays its nodes out in rank order
+// with the append rule of table_append (a node that does not fit in the rest of its page starts
+// the next page; k_gc_layout), and every node is then written where it was placed
+// (k_gc_write). This is what appending the nodes one at a time in rank order gives, without a
+// compare-and-swap loop on each bucket's size.
+__global__ __launch_bounds__(256) void k_gc_bucket(Geom g, uint32_t level, const uint32_t* counts, uint32_t cap,
+                                                   const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ tlen,
+                                                   uint32_t* __restrict__ bkt, uint32_t* __restrict__ rank,
+                                                   uint32_t* bcount) {
     const uint32_t n = min(counts[level], cap);
-    const bool leaf = level == T.g.leaf_level();
+    const bool leaf = level == g.leaf_level();
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t* node = tmp + (uint64_t)i * 9;
-        const uint32_t len = tlen[i];
-        const uint32_t h = leaf ? hash_leaf((uint64_t)node[0] | ((uint64_t)node[1] << 32)) : hash_words(node, len);
-        const uint32_t va = table_append(T, level, T.g.bucket_from_hash(level, h), node, len);
-        if (va == kAppendBucketFull) {
-            atomicOr(errors, GC_ERR_BUCKET);
-            continue;
+        const uint32_t h = leaf ? hash_leaf((uint64_t)node[0] | ((uint64_t)node[1] << 32)) : hash_words(node, tlen[i]);
+        const uint32_t b = g.bucket_from_hash(level, h);
+        bkt[i] = b;
+        rank[i] = atomicAdd(&bcount[b], 1u);
+    }
+}
+
+// exclusive scan of the bucket counts of one level (one block)
+__global__ __launch_bounds__(1024) void k_gc_scan(const uint32_t* __restrict__ bcount, uint32_t* __restrict__ bstart,
+                                                  uint32_t nb) {
+    __shared__ uint32_t sums[1024];
+    const uint32_t per = (nb + 1023u) / 1024u;
+    const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
+    uint32_t acc = 0;
+    for (uint32_t b = b0; b < b1; ++b) acc += bcount[b];
+    sums[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t t = 0; t < 1024; ++t) {
+            const uint32_t x = sums[t];
+            sums[t] = run;
+            run += x;
         }
-        if (va == kAppendPoolFull) {
+    }
+    __syncthreads();
+    acc = sums[threadIdx.x];
+    for (uint32_t b = b0; b < b1; ++b) {
+        bstart[b] = acc;
+        acc += bcount[b];
+    }
+}
+
+// node lengths in level-wide (bucket, rank) order
+__global__ __launch_bounds__(256) void k_gc_order(uint32_t level, const uint32_t* counts, uint32_t cap,
+                                                  const uint32_t* __restrict__ tlen, const uint32_t* __restrict__ bkt,
+                                                  const uint32_t* __restrict__ rank, const uint32_t* __restrict__ bstart,
+                                                  uint32_t* __restrict__ olen) {
+    const uint32_t n = min(counts[level], cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        olen[bstart[bkt[i]] + rank[i]] = tlen[i];
+}
+
+// one lane per bucket: positions of its nodes in rank order, after what the bucket holds
+// already (the level's full node); ~0 for nodes that no longer fit
+__global__ __launch_bounds__(256) void k_gc_layout(Geom g, uint32_t level, uint32_t* bucket_sizes,
+                                                   const uint32_t* __restrict__ bcount,
+                                                   const uint32_t* __restrict__ bstart,
+                                                   const uint32_t* __restrict__ olen, uint32_t* __restrict__ opos,
+                                                   uint32_t* errors) {
+    const uint32_t nb = g.buckets_per_level(level), ps = g.page_size, capB = g.bucket_size(level);
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+        uint32_t* bsPtr = &bucket_sizes[g.bucket_global(level, b)];
+        uint32_t cur = *bsPtr;
+        const uint32_t k0 = bstart[b], k1 = k0 + bcount[b];
+        bool full = false;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t len = olen[k];
+            const uint32_t left = ps - cur % ps;
+            const uint32_t start = (left != ps && left < len) ? cur + left : cur;
+            if (full || start + len > capB) {
+                full = true;
+                opos[k] = ~0u;
+                continue;
+            }
+            opos[k] = start;
+            cur = start + len;
+        }
+        if (full) atomicOr(errors, GC_ERR_BUCKET);
+        *bsPtr = cur;
+    }
+}
+
+// every node written where it was placed; old address -> new address into the map
+__global__ __launch_bounds__(256) void k_gc_write(TableRef T, uint32_t level, const uint32_t* counts, uint32_t cap,
+                                                  const uint32_t* __restrict__ frontier,
+                                                  const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ tlen,
+                                                  const uint32_t* __restrict__ bkt, const uint32_t* __restrict__ rank,
+                                                  const uint32_t* __restrict__ bstart,
+                                                  const uint32_t* __restrict__ opos, uint32_t* __restrict__ mkeys,
+                                                  uint32_t* __restrict__ mvals, uint32_t mmask, uint32_t* errors) {
+    const uint32_t n = min(counts[level], cap);
+    const uint32_t ps = T.g.page_size;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t b = bkt[i];
+        const uint32_t start = opos[bstart[b] + rank[i]];
+        if (start == ~0u) continue;  // the bucket overflowed (error set by the layout)
+        const uint32_t va = T.g.make_ptr(level, b, start);
+        if (start % ps == 0 && !table_map_page(T, va / ps)) {
             atomicOr(errors, GC_ERR_POOL);
             continue;
         }
-        // old address -> new address
+        const uint32_t* node = tmp + (uint64_t)i * 9;
+        const uint32_t len = tlen[i];
+        for (uint32_t k = 0; k < len; ++k) T.vpool[va + k] = node[k];
         const uint32_t key = frontier[i];
         uint32_t s = map_hash(key) & mmask;
         for (;;) {
@@ -334,6 +424,17 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
     for (uint32_t l = 0; l <= max_level && e == hipSuccess; ++l) e = alloc((void**)&frontier[l], (uint64_t)fcap(l) * 4);
     if (e == hipSuccess) e = alloc((void**)&tmp, (uint64_t)cap * 36);
     if (e == hipSuccess) e = alloc((void**)&tlen, (uint64_t)cap * 4);
+    // placement scratch: bucket and rank per node, lengths and positions in placement order,
+    // count and start per bucket of a level
+    uint32_t maxBuckets = 1;
+    for (uint32_t l = 0; l <= max_level; ++l) maxBuckets = std::max(maxBuckets, g.buckets_per_level(l));
+    uint32_t *bkt, *rank, *olen, *opos, *bcount, *bstart;
+    if (e == hipSuccess) e = alloc((void**)&bkt, (uint64_t)cap * 4);
+    if (e == hipSuccess) e = alloc((void**)&rank, (uint64_t)cap * 4);
+    if (e == hipSuccess) e = alloc((void**)&olen, (uint64_t)cap * 4);
+    if (e == hipSuccess) e = alloc((void**)&opos, (uint64_t)cap * 4);
+    if (e == hipSuccess) e = alloc((void**)&bcount, (uint64_t)maxBuckets * 4);
+    if (e == hipSuccess) e = alloc((void**)&bstart, (uint64_t)maxBuckets * 4);
     for (int k = 0; k < 2 && e == hipSuccess; ++k) {
         e = alloc((void**)&mk[k], (uint64_t)mcap * 4);
         if (e == hipSuccess) e = alloc((void**)&mv[k], (uint64_t)mcap * 4);
@@ -374,7 +475,14 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
         k_gc_fill<<<256, 256, 0, s>>>(ctx->bucket_sizes + b0, g.buckets_per_level(l), 0u);
         k_gc_full<<<1, 1, 0, s>>>(T, l, ctx->full_nodes, errors);
         k_gc_fill<<<256, 256, 0, s>>>(mk[cur], mcap, 0u);
-        k_gc_add<<<kGcGrid, 256, 0, s>>>(T, l, counts, cap, frontier[l], tmp, tlen, mk[cur], mv[cur], mcap - 1, errors);
+        const uint32_t nb = g.buckets_per_level(l);
+        k_gc_fill<<<256, 256, 0, s>>>(bcount, nb, 0u);
+        k_gc_bucket<<<kGcGrid, 256, 0, s>>>(g, l, counts, cap, tmp, tlen, bkt, rank, bcount);
+        k_gc_scan<<<1, 1024, 0, s>>>(bcount, bstart, nb);
+        k_gc_order<<<kGcGrid, 256, 0, s>>>(l, counts, cap, tlen, bkt, rank, bstart, olen);
+        k_gc_layout<<<(nb + 255) / 256, 256, 0, s>>>(g, l, ctx->bucket_sizes, bcount, bstart, olen, opos, errors);
+        k_gc_write<<<kGcGrid, 256, 0, s>>>(T, l, counts, cap, frontier[l], tmp, tlen, bkt, rank, bstart, opos, mk[cur],
+                                           mv[cur], mcap - 1, errors);
         cur ^= 1;
     }
     k_gc_roots<<<(nRoots + 255) / 256, 256, 0, s>>>(mk[cur ^ 1], mv[cur ^ 1], mcap - 1, droots, nRoots, errors);
