@@ -5,10 +5,12 @@
 //            the next level's frontier (the reference's per-level unordered_sets)
 //   rebuild  levels maxLevel .. 0: gather every live node with its children remapped through
 //            the old -> new map of the level below, reset the level's bucket sizes, append the
-//            full node first (ADD_FULL_NODES_FIRST), append every live node, record old -> new
+//            full node first (ADD_FULL_NODES_FIRST), place every live node in its bucket with
+//            the append rule (in the order of the ranks the nodes draw in their bucket), record
+//            old -> new
 //
 // Pages stay mapped (poolTop never shrinks), as in the reference. The result is canonically
-// identical to the DAG before GC; node order within buckets is the append order.
+// identical to the DAG before GC; node order within a bucket is the rank order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
