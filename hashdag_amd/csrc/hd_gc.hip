@@ -113,34 +113,13 @@ __global__ __launch_bounds__(256) void k_gc_mark(const uint32_t* __restrict__ vp
     }
 }
 
-__device__ __forceinline__ uint32_t map_hash(uint32_t k) {
-    k ^= k >> 16;
-    k *= 0x7feb352du;
-    k ^= k >> 15;
-    k *= 0x846ca68bu;
-    return k ^ (k >> 16);
-}
-
-__device__ __forceinline__ uint32_t map_find(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                             uint32_t mask, uint32_t key) {
-    uint32_t s = map_hash(key) & mask;
-    for (uint32_t probes = 0; probes <= mask; ++probes) {
-        const uint32_t k = keys[s];
-        if (k == key + 1) return vals[s];
-        if (k == 0) return kInvalidNode;
-        s = (s + 1) & mask;
-    }
-    return kInvalidNode;
-}
-
-// gather live nodes of `level` with remapped children into tmp (9 words per node); the
-// children's map probes run side by side
+// gather live nodes of `level` with their children's new addresses (the forwarding array of
+// the level below) into tmp (9 words per node)
 __global__ __launch_bounds__(256) void k_gc_gather(const uint32_t* __restrict__ vpool,
                                                    const uint32_t* __restrict__ frontier, const uint32_t* counts,
                                                    uint32_t level, uint32_t leafLevel, uint32_t maxLevel, uint32_t cap,
-                                                   const uint32_t* __restrict__ mkeys, const uint32_t* __restrict__ mvals,
-                                                   uint32_t mmask, uint32_t* __restrict__ tmp, uint32_t* __restrict__ tlen,
-                                                   uint32_t* errors) {
+                                                   const uint32_t* __restrict__ fwd, uint32_t* __restrict__ tmp,
+                                                   uint32_t* __restrict__ tlen) {
     const uint32_t n = min(counts[level], cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t va = frontier[i];
@@ -157,35 +136,9 @@ __global__ __launch_bounds__(256) void k_gc_gather(const uint32_t* __restrict__ 
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k) c[k] = k < nk ? vpool[va + 1 + k] : 0u;
         if (level != maxLevel) {
-            // map_find of every child: probe slots until the key or an empty slot (at most
-            // mmask + 1 probes each)
-            uint32_t slot[8], open = 0;
 #pragma unroll
-            for (uint32_t k = 0; k < 8; ++k) {
-                slot[k] = map_hash(c[k]) & mmask;
-                if (k < nk) open |= 1u << k;
-            }
-            for (uint32_t probes = 0; open && probes <= mmask; ++probes) {
-#pragma unroll
-                for (uint32_t k = 0; k < 8; ++k) {
-                    if (!((open >> k) & 1u)) continue;
-                    const uint32_t key = mkeys[slot[k]], val = mvals[slot[k]];
-                    if (key == c[k] + 1) {
-                        c[k] = val;
-                        open &= ~(1u << k);
-                    } else if (key == 0) {
-                        c[k] = kInvalidNode;
-                        open &= ~(1u << k);
-                    } else {
-                        slot[k] = (slot[k] + 1) & mmask;
-                    }
-                }
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < 8; ++k) {
-                if ((open >> k) & 1u) c[k] = kInvalidNode;  // probed every slot
-                if (k < nk && c[k] == kInvalidNode) atomicOr(errors, GC_ERR_MAP);
-            }
+            for (uint32_t k = 0; k < 8; ++k)
+                if (k < nk) c[k] = fwd[c[k]];
         }
         out[0] = hdr;
 #pragma unroll
@@ -305,14 +258,14 @@ __global__ __launch_bounds__(256) void k_gc_layout(Geom g, uint32_t level, uint3
     }
 }
 
-// every node written where it was placed; old address -> new address into the map
+// every node written where it was placed; its new address into the forwarding array
 __global__ __launch_bounds__(256) void k_gc_write(TableRef T, uint32_t level, const uint32_t* counts, uint32_t cap,
                                                   const uint32_t* __restrict__ frontier,
                                                   const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ tlen,
                                                   const uint32_t* __restrict__ bkt, const uint32_t* __restrict__ rank,
                                                   const uint32_t* __restrict__ bstart,
-                                                  const uint32_t* __restrict__ opos, uint32_t* __restrict__ mkeys,
-                                                  uint32_t* __restrict__ mvals, uint32_t mmask, uint32_t* errors) {
+                                                  const uint32_t* __restrict__ opos, uint32_t* __restrict__ fwd,
+                                                  uint32_t* errors) {
     const uint32_t n = min(counts[level], cap);
     const uint32_t ps = T.g.page_size;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -327,28 +280,15 @@ __global__ __launch_bounds__(256) void k_gc_write(TableRef T, uint32_t level, co
         const uint32_t* node = tmp + (uint64_t)i * 9;
         const uint32_t len = tlen[i];
         for (uint32_t k = 0; k < len; ++k) T.vpool[va + k] = node[k];
-        const uint32_t key = frontier[i];
-        uint32_t s = map_hash(key) & mmask;
-        for (;;) {
-            const uint32_t prev = atomicCAS(&mkeys[s], 0u, key + 1);
-            if (prev == 0) {
-                mvals[s] = va;
-                break;
-            }
-            s = (s + 1) & mmask;
-        }
+        fwd[frontier[i]] = va;
     }
 }
 
 // every kept root -> its address after GC
-__global__ void k_gc_roots(const uint32_t* mkeys, const uint32_t* mvals, uint32_t mmask, uint32_t* roots,
-                           uint32_t n_roots, uint32_t* errors) {
+__global__ void k_gc_roots(const uint32_t* __restrict__ fwd, uint32_t* roots, uint32_t n_roots, uint32_t* errors) {
     if (*errors) return;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_roots; i += gridDim.x * blockDim.x) {
-        const uint32_t r = map_find(mkeys, mvals, mmask, roots[i]);
-        if (r == kInvalidNode) atomicOr(errors, GC_ERR_MAP);
-        else roots[i] = r;
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_roots; i += gridDim.x * blockDim.x)
+        roots[i] = fwd[roots[i]];
 }
 
 }  // namespace
@@ -403,8 +343,6 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
     for (uint32_t l = 0; l <= max_level; ++l) capMax = std::max<uint64_t>(capMax, fcap(l));
     if (capMax >= (1u << 30)) return fail(ctx, HD_ERR_INVALID, "level too large for GC scratch");
     const uint32_t cap = (uint32_t)capMax;
-    uint32_t mcap = 1;
-    while (mcap < 2 * cap) mcap <<= 1;
 
     std::vector<void*> allocs;
     auto alloc = [&](void** p, uint64_t bytes) -> hipError_t {
@@ -417,7 +355,7 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
         for (void* p : allocs) hipFree(p);
     };
     const uint64_t bitmapWords = (g.total_virtual() + 31) / 32;
-    uint32_t *bitmap, *counts, *errors, *tmp, *tlen, *mk[2], *mv[2], *droots;
+    uint32_t *bitmap, *counts, *errors, *tmp, *tlen, *fwd, *droots;
     std::vector<uint32_t*> frontier(max_level + 1, nullptr);
     hipError_t e = alloc((void**)&bitmap, bitmapWords * 4);
     if (e == hipSuccess) e = alloc((void**)&counts, 64 * 4);
@@ -437,10 +375,9 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
     if (e == hipSuccess) e = alloc((void**)&opos, (uint64_t)cap * 4);
     if (e == hipSuccess) e = alloc((void**)&bcount, (uint64_t)maxBuckets * 4);
     if (e == hipSuccess) e = alloc((void**)&bstart, (uint64_t)maxBuckets * 4);
-    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-        e = alloc((void**)&mk[k], (uint64_t)mcap * 4);
-        if (e == hipSuccess) e = alloc((void**)&mv[k], (uint64_t)mcap * 4);
-    }
+    // old address -> new address of every live node: one word per virtual address (written
+    // for live nodes only, read only for them)
+    if (e == hipSuccess) e = alloc((void**)&fwd, (uint64_t)g.total_virtual() * 4);
     if (e != hipSuccess) {
         release();
         return hip_fail(ctx, e, "GC scratch");
@@ -467,27 +404,22 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
         }
     }
     TableRef T{g, ctx->vpool, ctx->page_table, ctx->bucket_sizes, ctx->counters + 0, ctx->cfg.pool_pages};
-    int cur = 0;
     for (uint32_t l = max_level; l != (uint32_t)-1; --l) {
-        const int prev = cur ^ 1;  // map of level l + 1
-        k_gc_gather<<<kGcGrid, 256, 0, s>>>(ctx->vpool, frontier[l], counts, l, L, max_level, cap, mk[prev], mv[prev],
-                                            mcap - 1, tmp, tlen, errors);
+        k_gc_gather<<<kGcGrid, 256, 0, s>>>(ctx->vpool, frontier[l], counts, l, L, max_level, cap, fwd, tmp, tlen);
         // reset the level's buckets, then the full node first, then the live nodes
         const uint32_t b0 = g.bucket_global(l, 0);
         k_gc_fill<<<256, 256, 0, s>>>(ctx->bucket_sizes + b0, g.buckets_per_level(l), 0u);
         k_gc_full<<<1, 1, 0, s>>>(T, l, ctx->full_nodes, errors);
-        k_gc_fill<<<256, 256, 0, s>>>(mk[cur], mcap, 0u);
         const uint32_t nb = g.buckets_per_level(l);
         k_gc_fill<<<256, 256, 0, s>>>(bcount, nb, 0u);
         k_gc_bucket<<<kGcGrid, 256, 0, s>>>(g, l, counts, cap, tmp, tlen, bkt, rank, bcount);
         k_gc_scan<<<1, 1024, 0, s>>>(bcount, bstart, nb);
         k_gc_order<<<kGcGrid, 256, 0, s>>>(l, counts, cap, tlen, bkt, rank, bstart, olen);
         k_gc_layout<<<(nb + 255) / 256, 256, 0, s>>>(g, l, ctx->bucket_sizes, bcount, bstart, olen, opos, errors);
-        k_gc_write<<<kGcGrid, 256, 0, s>>>(T, l, counts, cap, frontier[l], tmp, tlen, bkt, rank, bstart, opos, mk[cur],
-                                           mv[cur], mcap - 1, errors);
-        cur ^= 1;
+        k_gc_write<<<kGcGrid, 256, 0, s>>>(T, l, counts, cap, frontier[l], tmp, tlen, bkt, rank, bstart, opos, fwd,
+                                           errors);
     }
-    k_gc_roots<<<(nRoots + 255) / 256, 256, 0, s>>>(mk[cur ^ 1], mv[cur ^ 1], mcap - 1, droots, nRoots, errors);
+    k_gc_roots<<<(nRoots + 255) / 256, 256, 0, s>>>(fwd, droots, nRoots, errors);
     uint32_t err = 0;
     std::vector<uint32_t> newRoots(nRoots);
     std::vector<uint32_t> cnt(max_level + 1);
