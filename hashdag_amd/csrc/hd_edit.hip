@@ -1031,6 +1031,163 @@ __global__ __launch_bounds__(256) void k_color_copy_leaves(EditPass P) {
     }
 }
 
+// leaf_decode of ascending colour indices of one leaf. An index in the macro block of the
+// previous one walks forward from the previous block (blocks start at ascending indices, so
+// the last block starting at or before the index is reached by that walk); another macro block
+// starts a binary search for that last block, which is the block leaf_decode's search finds.
+struct LeafCursor {
+    const uint64_t* arena;
+    LeafDesc d;
+    uint64_t m = ~0ull, macroBit = 0, blk = 0;
+    uint32_t pos = 0, hiM = 0;
+    __device__ uint64_t get(uint64_t index) {
+        if (!leaf_valid_index(d, index)) return 0;  // CompressedColor{}
+        const uint64_t G = index + (d.offset != ~0ull ? d.offset : 0ull);
+        const uint64_t* blocks = arena + d.blocks;
+        const uint64_t* macro = arena + d.macro;
+        const uint64_t mm = G >> 14;
+        const uint32_t local = (uint32_t)(G & 16383u);
+        if (mm != m) {
+            m = mm;
+            macroBit = macro[2 * mm + 1];
+            uint32_t lo = (uint32_t)macro[2 * mm];
+            hiM = 2 * (mm + 1) < d.n_macro ? (uint32_t)macro[2 * (mm + 1)] - 1u : (uint32_t)d.n_blocks - 1u;
+            uint32_t hi = hiM;  // blocks[lo] starts at index 0 of the macro block
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo + 1) / 2;
+                if (hdr_index((uint32_t)blocks[mid]) <= local) lo = mid;
+                else hi = mid - 1;
+            }
+            pos = lo;
+            blk = blocks[pos];
+        } else {
+            while (pos < hiM) {
+                const uint64_t nb = blocks[pos + 1];
+                if (hdr_index((uint32_t)nb) > local) break;
+                ++pos;
+                blk = nb;
+            }
+        }
+        const uint32_t h = (uint32_t)blk, bits = (uint32_t)(blk >> 32), bpw = hdr_bpw(h);
+        if (bpw == 0) return pack_color(bits, 0, 0);
+        const uint64_t bit = macroBit + (h >> 16) + (uint64_t)((local - hdr_index(h)) * bpw);
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(arena) + d.weights32;
+        const uint64_t byte = bit >> 3;
+        const uint32_t w0 = w[byte >> 2], sh = (uint32_t)(byte & 3u) * 8u;
+        const uint32_t b0 = (w0 >> sh) & 0xFFu;
+        const uint32_t b1 = (byte & 3u) == 3u ? (w[(byte >> 2) + 1] & 0xFFu) : ((w0 >> (sh + 8)) & 0xFFu);
+        const uint32_t dst = (b0 << 8) | b1;
+        const uint32_t wq = (dst >> (16u - bpw - (uint32_t)(bit & 7u))) & ((1u << bpw) - 1u);
+        return pack_color(bits, wq, bpw);
+    }
+};
+
+// Block-wide wave_alloc32 (one atomic per block); block-uniform control flow, `lds` holds
+// waves + 1 words.
+__device__ __forceinline__ uint32_t block_alloc32(uint32_t* lds, uint32_t* ctr, uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t incl = wave_scan32(v);
+    if (lane == 63) lds[wv] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint32_t x = lds[w];
+            lds[w] = tot;
+            tot += x;
+        }
+        lds[nw] = tot ? atomicAdd(ctr, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t out = lds[nw] + lds[wv] + incl - v;
+    __syncthreads();
+    return out;
+}
+
+// Leaf items of a pass without copy / fill editors, one thread per item. The voxel masks
+// follow k_leaf. A voxel that is set at the end took the colour of the last add editor that
+// touched it (an add sets the voxel and its colour, a carve clears it, so the last editor
+// touching a voxel that ends set is an add), else it kept its old colour. So the editors are
+// walked backwards, each add claiming the set voxels it touches that no later add claimed;
+// the unclaimed set voxels read their old colours in ascending order through a LeafCursor.
+// Items whose list holds a paint editor (its colour depends on the colour before it) are
+// listed in `paintList` for the wavefront-per-item kernel.
+__global__ __launch_bounds__(256) void k_color_leaf_items(EditPass P, uint32_t* paintList, uint32_t* paintCount) {
+    __shared__ uint32_t sAlloc[256 / 64 + 1];
+    __shared__ uint32_t sErr;
+    const uint32_t L = P.levels - 2;
+    const uint32_t n = min(P.count[L], P.cap[L]);
+    if (threadIdx.x == 0) sErr = *P.errors;
+    __syncthreads();
+    if (sErr) return;  // block-uniform: the loop below synchronises the block
+    const Items I = P.items[L];
+    const Items T = P.items[P.ctl];
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+        const uint32_t i = b0 + threadIdx.x;
+        bool paint = false;
+        uint32_t es = 0, ec = 0;
+        if (i < n) {
+            es = I.estart[i];
+            ec = I.ecount[i];
+            for (uint32_t k = 0; k < ec; ++k) {
+                const uint32_t kind = P.edits[P.elist[L][es + k]].kind;
+                if (kind == EDIT_SPHERE_PAINT) paint = true;
+                if (kind == EDIT_COPY || kind == EDIT_FILL) set_error(P.errors, ERR_KIND);
+            }
+        }
+        const uint32_t at = block_alloc32(sAlloc, paintCount, paint ? 1u : 0u);
+        if (i >= n) continue;
+        if (paint) {
+            paintList[at] = i;
+            continue;
+        }
+        const uint32_t bx = I.px[i] << 2, by = I.py[i] << 2, bz = I.pz[i] << 2;
+        const uint32_t old = I.old_[i];
+        const uint64_t oldMask = old != kInvalidNode ? *reinterpret_cast<const uint64_t*>(P.vpool + old) : 0ull;
+        uint64_t v = oldMask;
+        for (uint32_t k = 0; k < ec; ++k) {
+            const EditHead h = load_head(P.edits, P.elist[L][es + k]);
+            const uint64_t tm = leaf_touch_mask(h, bx, by, bz);
+            if (h.kind == EDIT_SPHERE_ADD || h.kind == EDIT_BOX_ADD) v |= tm;
+            else v &= ~tm;
+        }
+        if (!v) continue;
+        const uint32_t task = I.aux[i];
+        uint64_t* out = P.colors + T.cbase[task] + I.nstart[i];
+        uint64_t claimed = 0;
+        for (uint32_t k = ec; k-- > 0 && (v & ~claimed);) {
+            const EditHead h = load_head(P.edits, P.elist[L][es + k]);
+            if (h.kind != EDIT_SPHERE_ADD && h.kind != EDIT_BOX_ADD) continue;
+            uint64_t fresh = leaf_touch_mask(h, bx, by, bz) & v & ~claimed;
+            claimed |= fresh;
+            const uint64_t col = pack_color(h.single, 0, 0);
+            while (fresh) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(fresh);
+                fresh &= fresh - 1;
+                out[__popcll(v & ((1ull << b) - 1ull))] = col;
+            }
+        }
+        uint64_t kept = v & ~claimed;  // set before the pass (an unclaimed set voxel was not touched by an add)
+        if (!kept) continue;
+        const uint32_t ostart = I.ostart[i];
+        if (I.cmode[i] == 1) {
+            const uint64_t col = pack_color(I.cmodeval[i], 0, 0);
+            while (kept) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(kept);
+                kept &= kept - 1;
+                out[__popcll(v & ((1ull << b) - 1ull))] = col;
+            }
+            continue;
+        }
+        LeafCursor cur{P.arena, task_leaf(P, T.handle[task])};
+        while (kept) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(kept);
+            kept &= kept - 1;
+            out[__popcll(v & ((1ull << b) - 1ull))] = cur.get(ostart + __popcll(oldMask & ((1ull << b) - 1ull)));
+        }
+    }
+}
+
 // A leaf item's fields that k_color_leaf reads first.
 struct LeafMeta {
     uint32_t es, ec, px, py, pz, old, aux, cmode, cmodeval, ostart, nstart;
@@ -1053,10 +1210,11 @@ __device__ __forceinline__ EditHead head_of_lane(const EditHead& mh, uint32_t k)
 // of consecutive items (neighbours share edits, so their edit heads stay cached), reads the
 // next item's fields while it works on one, and loads an item's edit heads into the lanes at
 // once (lane k: edit k), each edit then read out of its lane.
+// `list` (optional): the items to take, *listCount of them (else every leaf item).
 template <bool kSpecial>
-__global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
+__global__ __launch_bounds__(256) void k_color_leaf(EditPass P, const uint32_t* list, const uint32_t* listCount) {
     const uint32_t L = P.levels - 2;
-    const uint32_t n = min(P.count[L], P.cap[L]);
+    const uint32_t n = list ? min(*listCount, P.cap[L]) : min(P.count[L], P.cap[L]);
     if (*P.errors) return;
     const Items I = P.items[L];
     const Items T = P.items[P.ctl];
@@ -1064,10 +1222,10 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
     const uint32_t run = max(1u, min(32u, n / (4u * wave_count())));
     for (uint32_t i0 = wave_index() * run; i0 < n; i0 += wave_count() * run) {
         const uint32_t iEnd = min(i0 + run, n);
-        LeafMeta next = load_meta(I, i0);
+        LeafMeta next = load_meta(I, list ? list[i0] : i0);
         for (uint32_t i = i0; i < iEnd; ++i) {
             const LeafMeta m = next;
-            if (i + 1 < iEnd) next = load_meta(I, i + 1);
+            if (i + 1 < iEnd) next = load_meta(I, list ? list[i + 1] : i + 1);
             const uint32_t es = m.es, ec = m.ec;
             const uint32_t c1 = lane >> 3, c2 = lane & 7u;
             const uint32_t x = (m.px << 2) | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u);
@@ -1747,6 +1905,7 @@ static void fill_pass(hd_ctx* ctx, EditPass& P) {
     P.color_top = reinterpret_cast<unsigned long long*>(S.dstats + 2);
     P.arena_top = reinterpret_cast<unsigned long long*>(S.dstats + 3);
     P.stat_colors = reinterpret_cast<unsigned long long*>(S.dstats + 4);
+    P.paint_count = reinterpret_cast<uint32_t*>(S.dstats + 6);  // zeroed with the stats
     P.colors = S.colors;
     P.cmarks = S.cmarks;
     P.color_cap = S.color_cap;
@@ -1794,8 +1953,16 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
             if (l + 1 == L) k_color_copy_leaves<<<kGrid, 256, 0, s>>>(P);
             else k_color_copy<<<kGrid, 256, 0, s>>>(P, l);
         }
-        if (special) k_color_leaf<true><<<kGrid, 256, 0, s>>>(P);
-        else k_color_leaf<false><<<kGrid, 256, 0, s>>>(P);
+        if (special) {
+            k_color_leaf<true><<<kGrid, 256, 0, s>>>(P, nullptr, nullptr);
+        } else {
+            // items with a paint editor are listed (in the level's hash words, free by now)
+            // for the wavefront-per-item kernel
+            uint32_t* paintList = P.items[L].hash;
+            uint32_t* paintCount = P.paint_count;
+            k_color_leaf_items<<<kGrid, 256, 0, s>>>(P, paintList, paintCount);
+            k_color_leaf<false><<<kGrid, 256, 0, s>>>(P, paintList, paintCount);
+        }
         k_encode<<<kGrid, kEncBlock, 0, s>>>(P);
     }
     k_finish<<<1, 64, 0, s>>>(P, ctx->d_root);
