@@ -11,6 +11,7 @@ SMALL = {
     "d8": (8, 256, 4),
     "d8s": (8, 64, 4),
     "d10": (10, 512, 6),
+    "d10w": (10, 512, 3),   # colour subtrees of 128^3: leaves of several 16384-colour macro blocks
     "d13": (13, 256, 10),   # reference colour-tree depth (10) with a small footprint
 }
 

@@ -69,7 +69,7 @@ def assert_same(dag, sc, od, oc, what):
     assert np.array_equal(goc.canonical(god), oc.canonical(od)), f"{what}: colours differ"
 
 
-@pytest.mark.parametrize("name", ["d8", "d13"])
+@pytest.mark.parametrize("name", ["d8", "d13", "d10w"])
 def test_plain_edits_match_oracle(name):
     p, sc, g, dag, od, oc = make(name)
     for k, (kind, center, r) in enumerate(random_edits(p, 10, seed=7)):
@@ -79,7 +79,7 @@ def test_plain_edits_match_oracle(name):
         assert_same(dag, sc, od, oc, f"edit {k} {(kind, center, r)}")
 
 
-@pytest.mark.parametrize("name", ["d8", "d13"])
+@pytest.mark.parametrize("name", ["d8", "d13", "d10w"])
 def test_fused_batch_matches_sequential_oracle(name):
     p, sc, g, dag, od, oc = make(name)
     edits = random_edits(p, 40, seed=11)
