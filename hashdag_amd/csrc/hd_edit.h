@@ -127,7 +127,6 @@ struct EditPass {
     unsigned long long* color_top;
     unsigned long long* arena_top;
     unsigned long long* stat_colors;  // colours staged (color_top also counts the chunk padding)
-    uint32_t* paint_count;            // leaf items listed for the wavefront colour kernel
     uint64_t* colors;
     uint64_t* cmarks;  // per 64-colour chunk of `colors`: kUniformChunk | colour, or 0 (explicit)
     uint64_t color_cap;

@@ -1082,71 +1082,35 @@ struct LeafCursor {
     }
 };
 
-// Block-wide wave_alloc32 (one atomic per block); block-uniform control flow, `lds` holds
-// waves + 1 words.
-__device__ __forceinline__ uint32_t block_alloc32(uint32_t* lds, uint32_t* ctr, uint32_t v) {
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const uint32_t incl = wave_scan32(v);
-    if (lane == 63) lds[wv] = incl;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (uint32_t w = 0; w < nw; ++w) {
-            const uint32_t x = lds[w];
-            lds[w] = tot;
-            tot += x;
-        }
-        lds[nw] = tot ? atomicAdd(ctr, tot) : 0u;
-    }
-    __syncthreads();
-    const uint32_t out = lds[nw] + lds[wv] + incl - v;
-    __syncthreads();
-    return out;
-}
-
 // Leaf items of a pass without copy / fill editors, one thread per item. The voxel masks
-// follow k_leaf. A voxel that is set at the end took the colour of the last add editor that
-// touched it (an add sets the voxel and its colour, a carve clears it, so the last editor
-// touching a voxel that ends set is an add), else it kept its old colour. So the editors are
-// walked backwards, each add claiming the set voxels it touches that no later add claimed;
-// the unclaimed set voxels read their old colours in ascending order through a LeafCursor.
-// Items whose list holds a paint editor (its colour depends on the colour before it) are
-// listed in `paintList` for the wavefront-per-item kernel.
-__global__ __launch_bounds__(256) void k_color_leaf_items(EditPass P, uint32_t* paintList, uint32_t* paintCount) {
-    __shared__ uint32_t sAlloc[256 / 64 + 1];
-    __shared__ uint32_t sErr;
+// follow k_leaf (a paint editor leaves them as they are). Without a paint editor in the list, a
+// voxel that is set at the end took the colour of the last add editor that touched it (an add
+// sets the voxel and its colour, a carve clears it, so the last editor touching a voxel that
+// ends set is an add), else it kept its old colour: the editors are walked backwards, each add
+// claiming the set voxels it touches that no later add claimed, and the unclaimed set voxels
+// read their old colours in ascending order through a LeafCursor. With a paint editor (whose
+// colour depends on the colour before it) every set voxel follows the edit list on its own,
+// as a lane of k_color_leaf does.
+__global__ __launch_bounds__(256) void k_color_leaf_items(EditPass P) {
     const uint32_t L = P.levels - 2;
     const uint32_t n = min(P.count[L], P.cap[L]);
-    if (threadIdx.x == 0) sErr = *P.errors;
-    __syncthreads();
-    if (sErr) return;  // block-uniform: the loop below synchronises the block
+    if (*P.errors) return;
     const Items I = P.items[L];
     const Items T = P.items[P.ctl];
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
-        const uint32_t i = b0 + threadIdx.x;
-        bool paint = false;
-        uint32_t es = 0, ec = 0;
-        if (i < n) {
-            es = I.estart[i];
-            ec = I.ecount[i];
-            for (uint32_t k = 0; k < ec; ++k) {
-                const uint32_t kind = P.edits[P.elist[L][es + k]].kind;
-                if (kind == EDIT_SPHERE_PAINT) paint = true;
-                if (kind == EDIT_COPY || kind == EDIT_FILL) set_error(P.errors, ERR_KIND);
-            }
-        }
-        const uint32_t at = block_alloc32(sAlloc, paintCount, paint ? 1u : 0u);
-        if (i >= n) continue;
-        if (paint) {
-            paintList[at] = i;
-            continue;
-        }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t es = I.estart[i], ec = I.ecount[i];
         const uint32_t bx = I.px[i] << 2, by = I.py[i] << 2, bz = I.pz[i] << 2;
         const uint32_t old = I.old_[i];
         const uint64_t oldMask = old != kInvalidNode ? *reinterpret_cast<const uint64_t*>(P.vpool + old) : 0ull;
         uint64_t v = oldMask;
+        bool paint = false;
         for (uint32_t k = 0; k < ec; ++k) {
             const EditHead h = load_head(P.edits, P.elist[L][es + k]);
+            if (h.kind == EDIT_COPY || h.kind == EDIT_FILL) set_error(P.errors, ERR_KIND);
+            if (h.kind == EDIT_SPHERE_PAINT) {
+                paint = true;
+                continue;
+            }
             const uint64_t tm = leaf_touch_mask(h, bx, by, bz);
             if (h.kind == EDIT_SPHERE_ADD || h.kind == EDIT_BOX_ADD) v |= tm;
             else v &= ~tm;
@@ -1154,6 +1118,47 @@ __global__ __launch_bounds__(256) void k_color_leaf_items(EditPass P, uint32_t* 
         if (!v) continue;
         const uint32_t task = I.aux[i];
         uint64_t* out = P.colors + T.cbase[task] + I.nstart[i];
+        if (paint) {
+            const uint32_t ostart = I.ostart[i], cmode = I.cmode[i], cmodeval = I.cmodeval[i];
+            LeafCursor cur{P.arena, task_leaf(P, T.handle[task])};
+            uint64_t rest = v;
+            while (rest) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(rest);
+                rest &= rest - 1;
+                const uint32_t c1 = b >> 3, c2 = b & 7u;
+                const float fx = (float)(bx | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u));
+                const float fy = (float)(by | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u));
+                const float fz = (float)(bz | ((c1 & 1u) << 1) | (c2 & 1u));
+                const bool prev = (oldMask >> b) & 1ull;
+                // oldColor() of a voxel that was empty is CompressedColor{} (hash_dag_edits.h:520-540);
+                // read at most once per voxel, in ascending voxel order
+                auto old_of_voxel = [&]() -> uint64_t {
+                    if (!prev) return 0ull;
+                    if (cmode == 1) return pack_color(cmodeval, 0, 0);
+                    return cur.get(ostart + __popcll(oldMask & ((1ull << b) - 1ull)));
+                };
+                bool vb = prev, have = false;
+                uint64_t col = 0;
+                for (uint32_t k = 0; k < ec; ++k) {
+                    const EditHead h = load_head(P.edits, P.elist[L][es + k]);
+                    if (!e_touch(h, fx, fy, fz, fx + 1.0f, fy + 1.0f, fz + 1.0f)) continue;
+                    const bool nv = e_value(h, vb);
+                    if (nv) {
+                        uint64_t pc = 0;
+                        if (h.kind == EDIT_SPHERE_PAINT && vb) {
+                            if (!have) col = old_of_voxel();
+                            pc = col;
+                        }
+                        col = e_color(h, fx, fy, fz, pc);
+                        have = true;
+                    }
+                    vb = nv;
+                }
+                if (!have) col = old_of_voxel();
+                out[__popcll(v & ((1ull << b) - 1ull))] = col;
+            }
+            continue;
+        }
         uint64_t claimed = 0;
         for (uint32_t k = ec; k-- > 0 && (v & ~claimed);) {
             const EditHead h = load_head(P.edits, P.elist[L][es + k]);
@@ -1206,15 +1211,14 @@ __device__ __forceinline__ EditHead head_of_lane(const EditHead& mh, uint32_t k)
     return h;
 }
 
-// leaf items: fold each voxel's value and colour through the edit list. A wavefront takes a run
-// of consecutive items (neighbours share edits, so their edit heads stay cached), reads the
-// next item's fields while it works on one, and loads an item's edit heads into the lanes at
-// once (lane k: edit k), each edit then read out of its lane.
-// `list` (optional): the items to take, *listCount of them (else every leaf item).
-template <bool kSpecial>
-__global__ __launch_bounds__(256) void k_color_leaf(EditPass P, const uint32_t* list, const uint32_t* listCount) {
+// Leaf items of a pass with a copy or fill editor: one lane per voxel folds the voxel's value
+// and colour through the edit list. A wavefront takes a run of consecutive items (neighbours
+// share edits, so their edit heads stay cached), reads the next item's fields while it works
+// on one, and loads an item's edit heads into the lanes at once (lane k: edit k), each edit
+// then read out of its lane.
+__global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
     const uint32_t L = P.levels - 2;
-    const uint32_t n = list ? min(*listCount, P.cap[L]) : min(P.count[L], P.cap[L]);
+    const uint32_t n = min(P.count[L], P.cap[L]);
     if (*P.errors) return;
     const Items I = P.items[L];
     const Items T = P.items[P.ctl];
@@ -1222,10 +1226,10 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P, const uint32_t* 
     const uint32_t run = max(1u, min(32u, n / (4u * wave_count())));
     for (uint32_t i0 = wave_index() * run; i0 < n; i0 += wave_count() * run) {
         const uint32_t iEnd = min(i0 + run, n);
-        LeafMeta next = load_meta(I, list ? list[i0] : i0);
+        LeafMeta next = load_meta(I, i0);
         for (uint32_t i = i0; i < iEnd; ++i) {
             const LeafMeta m = next;
-            if (i + 1 < iEnd) next = load_meta(I, list ? list[i + 1] : i + 1);
+            if (i + 1 < iEnd) next = load_meta(I, i + 1);
             const uint32_t es = m.es, ec = m.ec;
             const uint32_t c1 = lane >> 3, c2 = lane & 7u;
             const uint32_t x = (m.px << 2) | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u);
@@ -1251,11 +1255,7 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P, const uint32_t* 
             const float fx = (float)x, fy = (float)y, fz = (float)z;
             auto apply = [&](const EditHead& h, uint32_t ei) {
                 if (!e_touch(h, fx, fy, fz, fx + 1.0f, fy + 1.0f, fz + 1.0f)) return;
-                if (!kSpecial && (h.kind == EDIT_COPY || h.kind == EDIT_FILL)) {
-                    set_error(P.errors, ERR_KIND);
-                    return;
-                }
-                if (kSpecial && h.kind == EDIT_FILL) {
+                if (h.kind == EDIT_FILL) {
                     // FillEditorColors::get_new_color: a voxel the flood fill set takes its colour
                     if (!v) {
                         const DevEdit& e = P.edits[ei];
@@ -1271,7 +1271,7 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P, const uint32_t* 
                     }
                     return;
                 }
-                if (kSpecial && h.kind == EDIT_COPY) {
+                if (h.kind == EDIT_COPY) {
                     // get_new_value / get_new_color: an empty voxel takes its source's value and colour
                     if (!v) {
                         const DevEdit& e = P.edits[ei];
@@ -1905,7 +1905,6 @@ static void fill_pass(hd_ctx* ctx, EditPass& P) {
     P.color_top = reinterpret_cast<unsigned long long*>(S.dstats + 2);
     P.arena_top = reinterpret_cast<unsigned long long*>(S.dstats + 3);
     P.stat_colors = reinterpret_cast<unsigned long long*>(S.dstats + 4);
-    P.paint_count = reinterpret_cast<uint32_t*>(S.dstats + 6);  // zeroed with the stats
     P.colors = S.colors;
     P.cmarks = S.cmarks;
     P.color_cap = S.color_cap;
@@ -1953,16 +1952,8 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
             if (l + 1 == L) k_color_copy_leaves<<<kGrid, 256, 0, s>>>(P);
             else k_color_copy<<<kGrid, 256, 0, s>>>(P, l);
         }
-        if (special) {
-            k_color_leaf<true><<<kGrid, 256, 0, s>>>(P, nullptr, nullptr);
-        } else {
-            // items with a paint editor are listed (in the level's hash words, free by now)
-            // for the wavefront-per-item kernel
-            uint32_t* paintList = P.items[L].hash;
-            uint32_t* paintCount = P.paint_count;
-            k_color_leaf_items<<<kGrid, 256, 0, s>>>(P, paintList, paintCount);
-            k_color_leaf<false><<<kGrid, 256, 0, s>>>(P, paintList, paintCount);
-        }
+        if (special) k_color_leaf<<<kGrid, 256, 0, s>>>(P);
+        else k_color_leaf_items<<<kGrid, 256, 0, s>>>(P);
         k_encode<<<kGrid, kEncBlock, 0, s>>>(P);
     }
     k_finish<<<1, 64, 0, s>>>(P, ctx->d_root);
