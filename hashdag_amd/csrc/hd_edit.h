@@ -127,6 +127,7 @@ struct EditPass {
     unsigned long long* color_top;
     unsigned long long* arena_top;
     unsigned long long* stat_colors;  // colours staged (color_top also counts the chunk padding)
+    uint32_t* find_count;             // per level: representatives listed for k_find
     uint64_t* colors;
     uint64_t* cmarks;  // per 64-colour chunk of `colors`: kUniformChunk | colour, or 0 (explicit)
     uint64_t color_cap;

@@ -87,6 +87,27 @@ __device__ __forceinline__ uint64_t wave_alloc64(unsigned long long* ctr, uint64
     return b + incl - v;
 }
 __device__ __forceinline__ void wave_add64(unsigned long long* ctr, uint64_t v) { (void)wave_alloc64(ctr, v); }
+// Block-wide wave_alloc32 (one atomic per block); block-uniform control flow, `lds` holds
+// waves + 1 words.
+__device__ __forceinline__ uint32_t block_alloc32(uint32_t* lds, uint32_t* ctr, uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t incl = wave_scan32(v);
+    if (lane == 63) lds[wv] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint32_t x = lds[w];
+            lds[w] = tot;
+            tot += x;
+        }
+        lds[nw] = tot ? atomicAdd(ctr, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t out = lds[nw] + lds[wv] + incl - v;
+    __syncthreads();
+    return out;
+}
 // Block-wide form of wave_alloc32 for two counters: one atomic per counter per block (a counter
 // every wavefront of a large grid hits is limited by how fast one address takes atomics). Every
 // thread of the block calls it, in block-uniform control flow; `lds` holds 2 * (waves + 1) words.
@@ -661,32 +682,44 @@ __device__ __forceinline__ bool same_content(const Items& I, uint32_t a, uint32_
     return true;
 }
 
+// Representatives (the first item of each distinct content) are listed for k_find, in the
+// level's nstart words (free until the colour phase), with one atomic per block.
 __global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
+    __shared__ uint32_t sAlloc[256 / 64 + 1];
+    __shared__ uint32_t sErr;
     const uint32_t n = min(P.count[l], P.cap[l]);
-    if (*P.errors) return;
+    if (threadIdx.x == 0) sErr = *P.errors;
+    __syncthreads();
+    if (sErr) return;  // block-uniform: the loop below synchronises the block
     const Items I = P.items[l];
     const uint32_t hmask = P.hcap - 1;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (I.state[i] != 1) continue;
-        uint32_t s = I.hash[i] & hmask;
-        for (;;) {
-            // read first: many items can share one content (and so one slot), and a CAS per
-            // item on one address would serialise them all
-            uint32_t cur = __atomic_load_n(&P.hset[s], __ATOMIC_RELAXED);
-            if (cur == 0) {
-                cur = atomicCAS(&P.hset[s], 0u, i + 1);
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+        const uint32_t i = b0 + threadIdx.x;
+        bool isRep = false;
+        if (i < n && I.state[i] == 1) {
+            uint32_t s = I.hash[i] & hmask;
+            for (;;) {
+                // read first: many items can share one content (and so one slot), and a CAS per
+                // item on one address would serialise them all
+                uint32_t cur = __atomic_load_n(&P.hset[s], __ATOMIC_RELAXED);
                 if (cur == 0) {
-                    I.rep[i] = i;
+                    cur = atomicCAS(&P.hset[s], 0u, i + 1);
+                    if (cur == 0) {
+                        I.rep[i] = i;
+                        isRep = true;
+                        break;
+                    }
+                }
+                const uint32_t j = cur - 1;
+                if (I.hash[j] == I.hash[i] && same_content(I, i, j)) {
+                    I.rep[i] = j;
                     break;
                 }
+                s = (s + 1) & hmask;
             }
-            const uint32_t j = cur - 1;
-            if (I.hash[j] == I.hash[i] && same_content(I, i, j)) {
-                I.rep[i] = j;
-                break;
-            }
-            s = (s + 1) & hmask;
         }
+        const uint32_t at = block_alloc32(sAlloc, &P.find_count[l], isRep ? 1u : 0u);
+        if (isRep) I.nstart[at] = i;
     }
 }
 
@@ -700,8 +733,10 @@ __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
     const Geom g = P.g;
     const uint32_t lane = threadIdx.x & 63u;
     const bool leafLevel = l == g.leaf_level();
-    for (uint32_t i = wave_index(); i < n; i += wave_count()) {
-        if (I.state[i] != 1 || I.rep[i] != i) continue;
+    // the representatives listed by k_dedup
+    const uint32_t nReps = min(P.find_count[l], n);
+    for (uint32_t r = wave_index(); r < nReps; r += wave_count()) {
+        const uint32_t i = I.nstart[r];
         const uint32_t bucket = g.bucket_from_hash(l, I.hash[i]);
         const uint32_t base = g.make_ptr(l, bucket, 0);
         const uint32_t bs = P.bucket_sizes[g.bucket_global(l, bucket)];
@@ -1905,6 +1940,7 @@ static void fill_pass(hd_ctx* ctx, EditPass& P) {
     P.color_top = reinterpret_cast<unsigned long long*>(S.dstats + 2);
     P.arena_top = reinterpret_cast<unsigned long long*>(S.dstats + 3);
     P.stat_colors = reinterpret_cast<unsigned long long*>(S.dstats + 4);
+    P.find_count = reinterpret_cast<uint32_t*>(S.dstats + 16);  // 32 words, zeroed with the stats
     P.colors = S.colors;
     P.cmarks = S.cmarks;
     P.color_cap = S.color_cap;

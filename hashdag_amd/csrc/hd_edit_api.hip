@@ -93,7 +93,7 @@ __global__ void k_reset(uint32_t* counters, uint64_t* stats) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         counters[1] = 0;  // errors
         counters[4] = 0;  // undo log
-        for (int i = 0; i < 8; ++i) stats[i] = 0;
+        for (int i = 0; i < 32; ++i) stats[i] = 0;  // stats, and the per-level find-list counts
         stats[3] = *reinterpret_cast<const uint64_t*>(counters + kArenaTopWord);  // committed arena top
     }
 }
