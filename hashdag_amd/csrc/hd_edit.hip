@@ -1117,15 +1117,15 @@ struct LeafCursor {
     }
 };
 
-// Leaf items of a pass without copy / fill editors, one thread per item. The voxel masks
-// follow k_leaf (a paint editor leaves them as they are). Without a paint editor in the list, a
-// voxel that is set at the end took the colour of the last add editor that touched it (an add
-// sets the voxel and its colour, a carve clears it, so the last editor touching a voxel that
-// ends set is an add), else it kept its old colour: the editors are walked backwards, each add
-// claiming the set voxels it touches that no later add claimed, and the unclaimed set voxels
-// read their old colours in ascending order through a LeafCursor. With a paint editor (whose
-// colour depends on the colour before it) every set voxel follows the edit list on its own,
-// as a lane of k_color_leaf does.
+// Leaf items of a pass without copy / fill editors, one thread per item. The voxels set at the
+// end are those of the leaf node the item resolved to (k_leaf's masks). Without a paint editor
+// in the list, a voxel that is set at the end took the colour of the last add editor that
+// touched it (an add sets the voxel and its colour, a carve clears it, so the last editor
+// touching a voxel that ends set is an add), else it kept its old colour: the editors are
+// walked backwards, each add claiming the set voxels it touches that no later add claimed, and
+// the unclaimed set voxels read their old colours in ascending order through a LeafCursor.
+// With a paint editor (whose colour depends on the colour before it) every set voxel follows
+// the edit list on its own, as a lane of k_color_leaf does.
 __global__ __launch_bounds__(256) void k_color_leaf_items(EditPass P) {
     const uint32_t L = P.levels - 2;
     const uint32_t n = min(P.count[L], P.cap[L]);
@@ -1137,18 +1137,14 @@ __global__ __launch_bounds__(256) void k_color_leaf_items(EditPass P) {
         const uint32_t bx = I.px[i] << 2, by = I.py[i] << 2, bz = I.pz[i] << 2;
         const uint32_t old = I.old_[i];
         const uint64_t oldMask = old != kInvalidNode ? *reinterpret_cast<const uint64_t*>(P.vpool + old) : 0ull;
-        uint64_t v = oldMask;
+        // the voxels k_leaf set: the leaf node the item resolved to
+        const uint32_t leaf = I.nidx[i];
+        const uint64_t v = leaf != kInvalidNode ? *reinterpret_cast<const uint64_t*>(P.vpool + leaf) : 0ull;
         bool paint = false;
         for (uint32_t k = 0; k < ec; ++k) {
-            const EditHead h = load_head(P.edits, P.elist[L][es + k]);
-            if (h.kind == EDIT_COPY || h.kind == EDIT_FILL) set_error(P.errors, ERR_KIND);
-            if (h.kind == EDIT_SPHERE_PAINT) {
-                paint = true;
-                continue;
-            }
-            const uint64_t tm = leaf_touch_mask(h, bx, by, bz);
-            if (h.kind == EDIT_SPHERE_ADD || h.kind == EDIT_BOX_ADD) v |= tm;
-            else v &= ~tm;
+            const uint32_t kind = P.edits[P.elist[L][es + k]].kind;
+            if (kind == EDIT_COPY || kind == EDIT_FILL) set_error(P.errors, ERR_KIND);
+            paint |= kind == EDIT_SPHERE_PAINT;
         }
         if (!v) continue;
         const uint32_t task = I.aux[i];
