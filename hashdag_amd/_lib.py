@@ -151,6 +151,12 @@ _SIGS = {
     "hd_write_basic_dag_file": (C.c_int32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint32, u32p,
                                             C.c_uint64]),
     "hd_read_basic_dag_file": (C.c_int32, [C.c_char_p, C.POINTER(C.c_double), u32p, u32p, C.c_uint64, u64p]),
+    "hd_write_compressed_colors_file": (C.c_int32, [C.c_char_p, C.c_uint32, u64p, C.c_uint64, u32p, C.c_uint64, u64p,
+                                                    C.c_uint64, u64p, C.c_uint64]),
+    "hd_read_compressed_colors_file": (C.c_int32, [C.c_char_p, u32p, u64p, u64p, u32p, u64p, u64p]),
+    "hd_basic_dag_counts": (C.c_int32, [u32p, C.c_uint64, C.c_uint32, C.c_uint32, u32p, u64p, u64p]),
+    "hd_colors_from_basic_dag": (C.c_int32, [u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, u64p, C.c_uint64,
+                                             u32p, u32p, u64p, u64p, u64p]),
     "hd_save_hash_dag_file": (C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hd_load_hash_dag_file": (C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
     "hd_checkpoint_save": (C.c_int32, [C.c_void_p, C.c_char_p]),

@@ -8,12 +8,21 @@
 //               then poolTop pages of the physical pool and the page table, without count prefixes.
 //               The dump holds no bucket sizes and no colours, so it is loaded by walking the DAG
 //               from its root into BasicDAG words and building the table from those.
+//   colours     BasicDAGFactory::save/load_compressed_colors_to/from_file (basic_dag.cpp:34-89):
+//               topLevels (uint32), enclosedLeaves, weights, blocks, macro blocks (uint64 count +
+//               data each); the weight words are stored byte-swapped against their in-memory order
+//               (ColorUtils::swap_byte_order, color_utils.h:85-105). Such a file pairs with a
+//               BasicDAG whose headers count voxels at every level, through enclosedLeaves above
+//               topLevels (BasicDAGColorsBase::get_leaves_count, basic_dag.h:60-72); the colour
+//               tree over it is built as create_hash_dag_colors builds it
+//               (hash_dag_factory.cpp:101-150).
 //   checkpoint  this project's own format: the whole table image (with bucket sizes and full
 //               nodes) and the colour state (tree, shared offsets, unique leaves, arena), so a
 //               context can be saved and restored exactly; the undo history is not saved.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -105,6 +114,31 @@ struct BasicFromPool {
         memo.emplace(key, at);
         if (out.size() >= (1ull << 32)) throw std::runtime_error("DAG too large for BasicDAG words");
         return at;
+    }
+};
+
+// Voxel counts of a BasicDAG's nodes (children after their parent, leaves 2 words).
+struct DagCounts {
+    const uint32_t* words;
+    uint64_t n;
+    uint32_t leafLevel;
+    std::unordered_map<uint32_t, uint64_t> memo;  // word index -> voxels
+    uint32_t at(uint64_t i) const {
+        if (i >= n) throw std::runtime_error("BasicDAG index out of range");
+        return words[i];
+    }
+    uint64_t count(uint32_t level, uint32_t idx) {
+        auto it = memo.find(idx);
+        if (it != memo.end()) return it->second;
+        uint64_t c = 0;
+        if (level == leafLevel) {
+            c = (uint64_t)__builtin_popcount(at(idx)) + (uint64_t)__builtin_popcount(at((uint64_t)idx + 1));
+        } else {
+            const uint32_t nk = (uint32_t)__builtin_popcount(at(idx) & 0xFFu);
+            for (uint32_t k = 0; k < nk; ++k) c += count(level + 1, at((uint64_t)idx + 1 + k));
+        }
+        memo.emplace(idx, c);
+        return c;
     }
 };
 
@@ -343,6 +377,165 @@ hd_status hd_checkpoint_load(hd_ctx* ctx, const char* path) {
     ctx->n_cleaves = leaves.size();
     ctx->colors_loaded = true;
     ctx->ptr_gen++;
+    return HD_OK;
+}
+
+hd_status hd_write_compressed_colors_file(const char* path, uint32_t top_levels, const uint64_t* enclosed,
+                                          uint64_t n_enclosed, const uint32_t* weights, uint64_t n_weights,
+                                          const uint64_t* blocks, uint64_t n_blocks, const uint64_t* macro,
+                                          uint64_t n_macro) {
+    if (!path || (!enclosed && n_enclosed) || (!weights && n_weights) || (!blocks && n_blocks) || (!macro && n_macro))
+        return fail(nullptr, HD_ERR_INVALID, "bad arguments");
+    File w(path, "wb");
+    if (!w.ok) return fail(nullptr, HD_ERR_INVALID, std::string("cannot open ") + path);
+    w.put<uint32_t>(top_levels);
+    w.put_array(enclosed, n_enclosed);
+    std::vector<uint32_t> swapped(weights, weights + n_weights);
+    for (uint32_t& v : swapped) v = __builtin_bswap32(v);
+    w.put_array(swapped.data(), n_weights);
+    w.put_array(blocks, n_blocks);
+    w.put_array(macro, n_macro);
+    if (!w.ok) return fail(nullptr, HD_ERR_INVALID, std::string("write failed: ") + path);
+    return HD_OK;
+}
+
+hd_status hd_read_compressed_colors_file(const char* path, uint32_t* top_levels, uint64_t sizes[4], uint64_t* enclosed,
+                                         uint32_t* weights, uint64_t* blocks, uint64_t* macro) {
+    if (!path || !sizes) return fail(nullptr, HD_ERR_INVALID, "bad arguments");
+    File r(path, "rb");
+    if (!r.ok) return fail(nullptr, HD_ERR_INVALID, std::string("cannot open ") + path);
+    const bool fill = enclosed && weights && blocks && macro;
+    const uint32_t top = r.get<uint32_t>();
+    std::vector<uint64_t> e, b, m;
+    std::vector<uint32_t> wv;
+    const uint64_t limit = 1ull << 40;
+    if (!(r.get_array(e, limit) && r.get_array(wv, limit) && r.get_array(b, limit) && r.get_array(m, limit)))
+        return fail(nullptr, HD_ERR_INVALID, "truncated compressed colours file");
+    const uint64_t n[4] = {e.size(), wv.size(), b.size(), m.size()};
+    if (fill && (sizes[0] < n[0] || sizes[1] < n[1] || sizes[2] < n[2] || sizes[3] < n[3]))
+        return fail(nullptr, HD_ERR_INVALID, "colour buffers too small");
+    if (top_levels) *top_levels = top;
+    for (int k = 0; k < 4; ++k) sizes[k] = n[k];
+    if (!fill) return HD_OK;
+    std::memcpy(enclosed, e.data(), n[0] * 8);
+    for (uint64_t k = 0; k < n[1]; ++k) weights[k] = __builtin_bswap32(wv[k]);
+    std::memcpy(blocks, b.data(), n[2] * 8);
+    std::memcpy(macro, m.data(), n[3] * 8);
+    return HD_OK;
+}
+
+hd_status hd_basic_dag_counts(const uint32_t* words, uint64_t n_words, uint32_t levels, uint32_t top_levels,
+                              uint32_t* out_words, uint64_t* enclosed, uint64_t* n_enclosed) {
+    if (!words || !n_words || !out_words || !n_enclosed || levels < 2 || levels > 32 || top_levels > levels - 2)
+        return fail(nullptr, HD_ERR_INVALID, "bad arguments");
+    try {
+        DagCounts dc{words, n_words, levels - 2, {}};
+        std::memcpy(out_words, words, n_words * 4);
+        std::vector<uint64_t> encl;
+        std::unordered_map<uint32_t, uint32_t> slot;  // node -> enclosedLeaves index
+        std::vector<uint8_t> seen(n_words, 0);
+        // every interior node once: its header counts its voxels (levels >= top_levels) or
+        // holds its index into enclosedLeaves
+        std::function<void(uint32_t, uint32_t)> walk = [&](uint32_t level, uint32_t idx) {
+            if (level == dc.leafLevel || seen[idx]) return;
+            seen[idx] = 1;
+            const uint32_t hdr = dc.at(idx), nk = (uint32_t)__builtin_popcount(hdr & 0xFFu);
+            const uint64_t c = dc.count(level, idx);
+            uint64_t up = c;
+            if (level < top_levels) {
+                up = encl.size();
+                encl.push_back(c);
+            }
+            if (up >= (1u << 24)) throw std::runtime_error("a voxel count does not fit a node header: raise top_levels");
+            out_words[idx] = (hdr & 0xFFu) | (uint32_t)(up << 8);
+            for (uint32_t k = 0; k < nk; ++k) walk(level + 1, dc.at((uint64_t)idx + 1 + k));
+        };
+        walk(0, 0);
+        const uint64_t cap = *n_enclosed;
+        *n_enclosed = encl.size();
+        if (enclosed) {
+            if (cap < encl.size()) return fail(nullptr, HD_ERR_INVALID, "enclosedLeaves buffer too small");
+            std::memcpy(enclosed, encl.data(), encl.size() * 8);
+        }
+    } catch (const std::exception& ex) {
+        return fail(nullptr, HD_ERR_INVALID, ex.what());
+    }
+    return HD_OK;
+}
+
+hd_status hd_colors_from_basic_dag(const uint32_t* words, uint64_t n_words, uint32_t levels, uint32_t color_tree_levels,
+                                   uint32_t top_levels, const uint64_t* enclosed, uint64_t n_enclosed,
+                                   uint32_t* scene_words, uint32_t* nodes, uint64_t* n_nodes, uint64_t* offsets,
+                                   uint64_t* n_offsets) {
+    const uint32_t ctl = color_tree_levels;
+    if (!words || !n_words || !n_nodes || !n_offsets || levels < 3 || levels > 32 || ctl < 1 || ctl >= levels - 2 ||
+        top_levels > ctl || (!enclosed && n_enclosed))
+        return fail(nullptr, HD_ERR_INVALID, "bad arguments (the colour tree must end above the leaves' parents, "
+                                             "and topLevels may not exceed it)");
+    try {
+        const uint32_t leafLevel = levels - 2;
+        auto at = [&](uint64_t i) -> uint32_t {
+            if (i >= n_words) throw std::runtime_error("BasicDAG index out of range");
+            return words[i];
+        };
+        auto leaves_count = [&](uint32_t level, uint32_t node) -> uint64_t {
+            const uint32_t up = node >> 8;
+            if (level < top_levels) {
+                if (up >= n_enclosed) throw std::runtime_error("enclosedLeaves index out of range");
+                return enclosed[up];
+            }
+            return up;
+        };
+        // handle 0 stays "no leaf", so leaf k of the reference's builder is handle k + 1
+        std::vector<uint32_t> tree;
+        std::vector<uint64_t> leafOffsets{0};
+        std::function<uint32_t(uint32_t, uint32_t, uint64_t)> build = [&](uint32_t level, uint32_t idx,
+                                                                         uint64_t leavesCount) -> uint32_t {
+            const uint32_t node = at(idx), mask = node & 0xFFu;
+            const uint32_t colorIndex = (uint32_t)tree.size();
+            uint32_t rank = 0;
+            if (level == ctl - 1) {
+                for (uint32_t i = 0; i < 8; ++i) {
+                    tree.push_back((uint32_t)leafOffsets.size());
+                    leafOffsets.push_back(leavesCount);
+                    if (mask & (1u << i)) leavesCount += leaves_count(level + 1, at((uint64_t)at((uint64_t)idx + 1 + rank++)));
+                }
+            } else {
+                tree.resize(tree.size() + 8, 0u);
+                for (uint32_t i = 0; i < 8; ++i) {
+                    if (!(mask & (1u << i))) continue;
+                    const uint32_t child = at((uint64_t)idx + 1 + rank++);
+                    const uint32_t ci = build(level + 1, child, leavesCount);
+                    tree[colorIndex + i] = ci;
+                    leavesCount += leaves_count(level + 1, at(child));
+                }
+            }
+            return colorIndex;
+        };
+        build(0, 0, 0);
+        const uint64_t capN = *n_nodes, capO = *n_offsets;
+        *n_nodes = tree.size();
+        *n_offsets = leafOffsets.size();
+        if (!nodes && !offsets && !scene_words) return HD_OK;
+        if (!nodes || !offsets || !scene_words || capN < tree.size() || capO < leafOffsets.size())
+            return fail(nullptr, HD_ERR_INVALID, "colour tree buffers too small");
+        std::memcpy(nodes, tree.data(), tree.size() * 4);
+        std::memcpy(offsets, leafOffsets.data(), leafOffsets.size() * 8);
+        // the scene's headers count voxels below the colour tree only (levels >= ctl)
+        std::memcpy(scene_words, words, n_words * 4);
+        std::vector<uint8_t> seen(n_words, 0);
+        std::function<void(uint32_t, uint32_t)> norm = [&](uint32_t level, uint32_t idx) {
+            if (level >= ctl || seen[idx]) return;
+            seen[idx] = 1;
+            const uint32_t hdr = at(idx), nk = (uint32_t)__builtin_popcount(hdr & 0xFFu);
+            scene_words[idx] = hdr & 0xFFu;
+            for (uint32_t k = 0; k < nk; ++k) norm(level + 1, at((uint64_t)idx + 1 + k));
+        };
+        norm(0, 0);
+        (void)leafLevel;
+    } catch (const std::exception& ex) {
+        return fail(nullptr, HD_ERR_INVALID, ex.what());
+    }
     return HD_OK;
 }
 

@@ -323,6 +323,35 @@ hd_status hd_read_basic_dag_file(const char* path, double bounds[6], uint32_t* l
  * context's table from it (the same DAG, re-interned). */
 hd_status hd_save_hash_dag_file(hd_ctx* ctx, const char* path, const double bounds_min[3], const double bounds_max[3]);
 hd_status hd_load_hash_dag_file(hd_ctx* ctx, const char* path, double bounds[6]);
+/* BasicDAGFactory::save_compressed_colors_to_file / load_compressed_colors_from_file
+ * (basic_dag.cpp:34-89): topLevels (uint32), then enclosedLeaves (uint64), weights (uint32),
+ * blocks (uint64), macro blocks (uint64), each as a uint64 count and the data. Weight words are
+ * given and returned in memory order and stored byte-swapped (ColorUtils::swap_byte_order,
+ * color_utils.h:85-105). sizes[4] = counts of enclosedLeaves, weights, blocks, macro: on reading
+ * with any array NULL the counts only, else the capacities in and the counts out. */
+hd_status hd_write_compressed_colors_file(const char* path, uint32_t top_levels, const uint64_t* enclosed,
+                                          uint64_t n_enclosed, const uint32_t* weights, uint64_t n_weights,
+                                          const uint64_t* blocks, uint64_t n_blocks, const uint64_t* macro,
+                                          uint64_t n_macro);
+hd_status hd_read_compressed_colors_file(const char* path, uint32_t* top_levels, uint64_t sizes[4], uint64_t* enclosed,
+                                         uint32_t* weights, uint64_t* blocks, uint64_t* macro);
+/* The voxel-count headers the reference's compressed colours pair with
+ * (BasicDAGColorsBase::get_leaves_count, basic_dag.h:60-72): out_words = words with every
+ * interior header's upper 24 bits set to its subtree's voxel count, or, for levels < top_levels,
+ * to an index into `enclosed` holding the count. *n_enclosed: capacity in (enclosed may be NULL
+ * for the count only), count out. */
+hd_status hd_basic_dag_counts(const uint32_t* words, uint64_t n_words, uint32_t levels, uint32_t top_levels,
+                              uint32_t* out_words, uint64_t* enclosed, uint64_t* n_enclosed);
+/* create_hash_dag_colors (hash_dag_factory.cpp:101-150) over a BasicDAG with such headers: the
+ * colour tree (8 entries per node, pre-order; at level color_tree_levels - 1 every child slot
+ * gets a leaf whose offset is the voxel count before it) as this library loads it
+ * (hd_load_colors: handle 0 is "no leaf", so the builder's leaf k is handle k + 1, offsets[0] = 0),
+ * and scene_words = the words with the headers above the colour tree reduced to the child mask.
+ * *n_nodes / *n_offsets: capacities in (all three arrays NULL for the counts only), counts out. */
+hd_status hd_colors_from_basic_dag(const uint32_t* words, uint64_t n_words, uint32_t levels, uint32_t color_tree_levels,
+                                   uint32_t top_levels, const uint64_t* enclosed, uint64_t n_enclosed,
+                                   uint32_t* scene_words, uint32_t* nodes, uint64_t* n_nodes, uint64_t* offsets,
+                                   uint64_t* n_offsets);
 /* Checkpoint of a context, no counterpart in the reference: the table image (with bucket sizes
  * and full nodes) and the colour state, restored exactly into a context of the same geometry.
  * The undo history is not saved and is cleared on load. */

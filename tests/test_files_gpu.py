@@ -89,3 +89,26 @@ def test_hash_dag_dump_layout_and_reload(tmp_path):
     assert bounds == ((0.0, 0.0, 0.0), (8192.0, 8192.0, 8192.0))
     from oracle import OracleDAG
     assert np.array_equal(OracleDAG.from_image(back.download(), back.pool_pages).canonical(), od.canonical())
+
+
+@pytest.mark.parametrize("name,top", [("d10w", 3), ("d13", 5)])
+def test_reference_scene_files_trace_like_the_scene(tmp_path, name, top):
+    """The reference's BasicDAG + compressed-colours files of a scene, loaded through the colour
+    tree create_hash_dag_colors builds, trace to the same frame as the scene itself."""
+    from hashdag_amd import HashDAG
+    from hashdag_amd.files import load_reference_scene
+    from tests.test_files import reference_format_files
+    p, sc, g, od, canon, words, enclosed, dag_path, col_path = reference_format_files(tmp_path, name, top)
+    back = load_reference_scene(dag_path, col_path, g.color_tree_levels)
+    a = HashDAG(g, pool_pages=POOL_PAGES)
+    a.load_scene(sc)
+    b = HashDAG(g, pool_pages=POOL_PAGES)
+    b.load_scene(back)
+    assert frame(b, g, p) == frame(a, g, p)
+    # and edits on it give the oracle's DAG and colours
+    from oracle import OracleColors
+    oc = OracleColors(back.color_nodes, back.color_offsets, back.blocks, back.weights, back.macro)
+    for kind, center, r in random_edits(p, 6, seed=61):
+        b.edit(kind, center, r)
+        od.edit(kind, center, r, oc)
+    assert_same(b, back, od, oc, "edits on the loaded reference files")
