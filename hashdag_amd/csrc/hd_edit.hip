@@ -854,30 +854,49 @@ __global__ __launch_bounds__(256) void k_resolve(EditPass P, uint32_t l) {
 // ---------------------------------------------------------------------------------------------
 // colours
 // ---------------------------------------------------------------------------------------------
-// new / old colour starts of the child items (relative to the task's colour array)
+// new / old colour starts of the child items (relative to the task's colour array): one lane per
+// child slot, the 8 slots of an item in 8 consecutive lanes (their starts by a scan over the 8)
 __global__ __launch_bounds__(256) void k_color_starts(EditPass P, uint32_t l) {
     const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (l == P.ctl) {
-            I.nstart[i] = 0;
-            I.ostart[i] = 0;
-        }
-        uint32_t ns = I.nstart[i], os = I.ostart[i];
-        for (uint32_t c = 0; c < 8; ++c) {
-            const uint32_t slot = i * 8 + c;
-            if (I.ckind[slot] == KIND_ITEM) {
-                const uint32_t ci = I.cval[slot];
-                C.nstart[ci] = ns;
-                C.ostart[ci] = os;
+    const uint64_t total = (uint64_t)n * 8;
+    // the loop bound is a multiple of 8 per step, so the 8 lanes of an item step together
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ((total + 63) & ~63ull);
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const bool active = t < total;
+        const uint32_t slot = (uint32_t)t, i = slot >> 3, c = slot & 7u;
+        const uint32_t cn = active ? I.cnew[slot] : 0u, co = active ? I.cold[slot] : 0u;
+        uint32_t sn = cn, so = co;  // inclusive scans over the item's 8 slots
+#pragma unroll
+        for (uint32_t d = 1; d < 8; d <<= 1) {
+            const uint32_t xn = __shfl_up(sn, d, 8), xo = __shfl_up(so, d, 8);
+            if (c >= d) {
+                sn += xn;
+                so += xo;
             }
-            I.cstart[slot] = ns;
-            I.costart[slot] = os;
-            ns += I.cnew[slot];
-            os += I.cold[slot];
         }
+        if (!active) continue;
+        uint32_t ns = 0, os = 0;
+        if (l == P.ctl) {
+            if (c == 0) {
+                I.nstart[i] = 0;
+                I.ostart[i] = 0;
+            }
+        } else {
+            ns = I.nstart[i];
+            os = I.ostart[i];
+        }
+        ns += sn - cn;
+        os += so - co;
+        if (I.ckind[slot] == KIND_ITEM) {
+            const uint32_t ci = I.cval[slot];
+            C.nstart[ci] = ns;
+            C.ostart[ci] = os;
+        }
+        I.cstart[slot] = ns;
+        I.costart[slot] = os;
     }
 }
 
