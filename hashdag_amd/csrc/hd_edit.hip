@@ -1457,10 +1457,10 @@ __global__ __launch_bounds__(256) void k_gather_leaves(const LeafDesc* leaves, c
         out[k] = leaves[handles[k] & 0x7FFFFFFFu];
 }
 
-// One wavefront per task: encode its colour array into a new unique leaf in the arena.
-// Pass 1 counts blocks and weight bits; pass 2 walks the colours in chunks of 64 with wave
-// scans carried across chunks. Weight bits of a chunk are gathered in LDS and each weight
-// word is written once (the word a chunk leaves partial is carried to the next chunk).
+// Colour leaf encoder (k_encode): per task, a counting walk over the colours (blocks and weight
+// bits), then a writing walk in chunks of 64 with wave scans carried across chunks. Weight bits
+// of a chunk are gathered in LDS and each weight word is written once (the word a chunk leaves
+// partial is carried to the next chunk).
 constexpr uint32_t kEncBlock = 256;
 constexpr uint32_t kEncWaves = kEncBlock / 64;
 
@@ -1602,9 +1602,12 @@ __device__ __forceinline__ void encode_colors(const uint64_t* __restrict__ cols,
     }
 }
 
-// One wavefront per task: encode its colour array into a new unique leaf in the arena. Tasks
-// are taken in rounds of 64 per wavefront (lane j records task j's sizes in the counting
-// walk), so the arena is reserved with one atomic per round.
+// One wavefront per task: encode its colour array into a new unique leaf in the arena. The grid
+// holds a wavefront per task (up to a cap, then a stride), so the hardware hands out tasks to
+// free wavefronts as they finish: task sizes vary by orders of magnitude, and a fixed share of
+// tasks per wavefront would leave the kernel's time to the wavefront with the largest share.
+// Each task reserves its arena words with one atomic once its blocks and weight bits are
+// counted.
 __global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
     const uint32_t ctl = P.ctl;
     const uint32_t n = min(P.count[ctl], P.cap[ctl]);
@@ -1613,62 +1616,44 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
     __shared__ uint32_t sBits[kEncWaves][12];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     uint32_t* buf = sBits[wv];
-    const uint32_t nW = gridDim.x * kEncWaves;
-    for (uint32_t r = blockIdx.x * kEncWaves + wv; r < n; r += 64 * nW) {
-        // blocks and weight bits of each task of the round
-        uint32_t myBlocks = 0, myBits = 0, myCnt = 0;
-        for (uint32_t j = 0; j < 64; ++j) {
-            const uint32_t t = r + j * nW;
-            if (t >= n) break;
-            const uint32_t cnt = T.ncount[t];
-            if (cnt == 0) continue;
-            if ((cnt + 16383u) / 16384u > 160) {  // more than 160 * 16384 colours in one subtree
-                if (lane == 0) set_error(P.errors, ERR_LEAF_SIZE);
-                continue;
-            }
-            const uint64_t cb = T.cbase[t];
-            uint32_t nb, bits;
-            encode_colors<false>(P.colors + cb, P.cmarks + cb / 64, cnt, lane, buf, nullptr, nullptr, nullptr, nb,
-                                 bits);
-            if (lane == j) {
-                myBlocks = nb;
-                myBits = bits;
-                myCnt = cnt;
-            }
+    for (uint32_t t = blockIdx.x * kEncWaves + wv; t < n; t += gridDim.x * kEncWaves) {
+        const uint32_t cnt = T.ncount[t];
+        if (cnt == 0) continue;
+        if ((cnt + 16383u) / 16384u > 160) {  // more than 160 * 16384 colours in one subtree
+            if (lane == 0) set_error(P.errors, ERR_LEAF_SIZE);
+            continue;
         }
-        const uint32_t myMacro = (myCnt + 16383u) / 16384u;
-        const uint64_t myWords = myCnt ? (uint64_t)myBlocks + 2ull * myMacro + ((myBits + 31) / 32 + 1) / 2 + 1 : 0ull;
-        const uint64_t myBase = wave_alloc64(P.arena_top, myWords);
-        if (myCnt && myBase + myWords > P.arena_cap) set_error(P.errors, ERR_COLOR_ARENA);
-        for (uint32_t j = 0; j < 64; ++j) {
-            const uint32_t t = r + j * nW;
-            if (t >= n) break;
-            const uint32_t cnt = __shfl(myCnt, j, 64);
-            if (cnt == 0) continue;
-            const uint64_t base = shfl64(myBase, j), words = shfl64(myWords, j);
-            if (base + words > P.arena_cap) continue;
-            const uint32_t nBlocks = __shfl(myBlocks, j, 64), nBits = __shfl(myBits, j, 64);
-            const uint32_t nMacro = (cnt + 16383u) / 16384u;
-            const uint32_t nWeights = (nBits + 31) / 32;
-            uint64_t* blocks = P.arena + base;
-            uint64_t* macro = blocks + nBlocks;
-            uint32_t* weights = reinterpret_cast<uint32_t*>(macro + 2 * nMacro);
-            const uint64_t cb = T.cbase[t];
-            uint32_t wb, wbits;
-            encode_colors<true>(P.colors + cb, P.cmarks + cb / 64, cnt, lane, buf, blocks, macro, weights, wb, wbits);
-            if (lane < 2) weights[nWeights + lane] = 0u;  // padding read by the 16-bit window
-            if (lane == 0) {
-                LeafDesc d;
-                d.blocks = base;
-                d.n_blocks = nBlocks;
-                d.macro = base + nBlocks;
-                d.n_macro = 2ull * nMacro;
-                d.weights32 = 2ull * (base + nBlocks + 2ull * nMacro);
-                d.n_weights = nWeights;
-                d.offset = ~0ull;
-                d.pad = 0;
-                P.leaves[T.newhandle[t] & 0x7FFFFFFFu] = d;
-            }
+        const uint64_t cb = T.cbase[t];
+        uint32_t nBlocks, nBits;
+        encode_colors<false>(P.colors + cb, P.cmarks + cb / 64, cnt, lane, buf, nullptr, nullptr, nullptr, nBlocks,
+                             nBits);
+        const uint32_t nMacro = (cnt + 16383u) / 16384u;
+        const uint32_t nWeights = (nBits + 31) / 32;
+        const uint64_t words = (uint64_t)nBlocks + 2ull * nMacro + (nWeights + 1) / 2 + 1;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(P.arena_top, (unsigned long long)words);
+        base = shfl64(base, 0);
+        if (base + words > P.arena_cap) {
+            if (lane == 0) set_error(P.errors, ERR_COLOR_ARENA);
+            continue;
+        }
+        uint64_t* blocks = P.arena + base;
+        uint64_t* macro = blocks + nBlocks;
+        uint32_t* weights = reinterpret_cast<uint32_t*>(macro + 2 * nMacro);
+        uint32_t wb, wbits;
+        encode_colors<true>(P.colors + cb, P.cmarks + cb / 64, cnt, lane, buf, blocks, macro, weights, wb, wbits);
+        if (lane < 2) weights[nWeights + lane] = 0u;  // padding read by the 16-bit window
+        if (lane == 0) {
+            LeafDesc d;
+            d.blocks = base;
+            d.n_blocks = nBlocks;
+            d.macro = base + nBlocks;
+            d.n_macro = 2ull * nMacro;
+            d.weights32 = 2ull * (base + nBlocks + 2ull * nMacro);
+            d.n_weights = nWeights;
+            d.offset = ~0ull;
+            d.pad = 0;
+            P.leaves[T.newhandle[t] & 0x7FFFFFFFu] = d;
         }
     }
 }
@@ -2005,7 +1990,10 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
         }
         if (special) k_color_leaf<<<kGrid, 256, 0, s>>>(P);
         else k_color_leaf_items<<<kGrid, 256, 0, s>>>(P);
-        k_encode<<<kGrid, kEncBlock, 0, s>>>(P);
+        // a wavefront per task slot (the task count is on the device), at most 2^18 of them
+        const uint32_t encBlocks = std::min<uint32_t>(std::max<uint32_t>((P.cap[P.ctl] + kEncWaves - 1) / kEncWaves, 1u),
+                                                      1u << 16);
+        k_encode<<<encBlocks, kEncBlock, 0, s>>>(P);
     }
     k_finish<<<1, 64, 0, s>>>(P, ctx->d_root);
     const hipError_t e = hipGetLastError();
