@@ -674,6 +674,21 @@ __global__ void k_fill(uint32_t* p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Slots of the dedup hash set a level uses: the power of two at or above twice its item
+// count (load factor at most 1/2), at most the whole set.
+__device__ __forceinline__ uint32_t hset_slots(const EditPass& P, uint32_t l) {
+    const uint32_t n = min(P.count[l], P.cap[l]);
+    uint32_t m = 64;
+    while (m < P.hcap && m < 2u * n) m <<= 1;
+    return min(m, P.hcap);
+}
+
+// clears the slots of the dedup hash set that level l will use
+__global__ void k_hset_clear(EditPass P, uint32_t l) {
+    const uint32_t m = hset_slots(P, l);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) P.hset[i] = 0u;
+}
+
 __device__ __forceinline__ bool same_content(const Items& I, uint32_t a, uint32_t b) {
     const uint32_t n = I.clen[a];
     if (n != I.clen[b]) return false;
@@ -692,7 +707,7 @@ __global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
     __syncthreads();
     if (sErr) return;  // block-uniform: the loop below synchronises the block
     const Items I = P.items[l];
-    const uint32_t hmask = P.hcap - 1;
+    const uint32_t hmask = hset_slots(P, l) - 1;
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
         const uint32_t i = b0 + threadIdx.x;
         bool isRep = false;
@@ -1974,7 +1989,7 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
     else k_leaf<false><<<kGrid, 256, 0, s>>>(P);
     for (int l = (int)L; l >= 0; --l) {
         if (l < (int)L) k_build<<<kGrid, 256, 0, s>>>(P, (uint32_t)l);
-        k_fill<<<256, 256, 0, s>>>(P.hset, P.hcap, 0u);
+        k_hset_clear<<<256, 256, 0, s>>>(P, (uint32_t)l);
         k_dedup<<<kGrid, 256, 0, s>>>(P, (uint32_t)l);
         k_find<<<kGrid, 256, 0, s>>>(P, (uint32_t)l);
         k_add<<<kGrid, 256, 0, s>>>(P, (uint32_t)l);
