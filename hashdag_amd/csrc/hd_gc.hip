@@ -114,20 +114,22 @@ __global__ __launch_bounds__(256) void k_gc_mark(const uint32_t* __restrict__ vp
 }
 
 // gather live nodes of `level` with their children's new addresses (the forwarding array of
-// the level below) into tmp (9 words per node)
-__global__ __launch_bounds__(256) void k_gc_gather(const uint32_t* __restrict__ vpool,
+// the level below) into tmp (9 words per node), and each node's bucket
+__global__ __launch_bounds__(256) void k_gc_gather(Geom g, const uint32_t* __restrict__ vpool,
                                                    const uint32_t* __restrict__ frontier, const uint32_t* counts,
-                                                   uint32_t level, uint32_t leafLevel, uint32_t maxLevel, uint32_t cap,
+                                                   uint32_t level, uint32_t maxLevel, uint32_t cap,
                                                    const uint32_t* __restrict__ fwd, uint32_t* __restrict__ tmp,
-                                                   uint32_t* __restrict__ tlen) {
+                                                   uint32_t* __restrict__ tlen, uint32_t* __restrict__ bkt) {
     const uint32_t n = min(counts[level], cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t va = frontier[i];
         uint32_t* out = tmp + (uint64_t)i * 9;
-        if (level == leafLevel) {
-            out[0] = vpool[va];
-            out[1] = vpool[va + 1];
+        if (level == g.leaf_level()) {
+            const uint32_t w0 = vpool[va], w1 = vpool[va + 1];
+            out[0] = w0;
+            out[1] = w1;
             tlen[i] = 2;
+            bkt[i] = g.bucket_from_hash(level, hash_leaf((uint64_t)w0 | ((uint64_t)w1 << 32)));
             continue;
         }
         const uint32_t hdr = vpool[va];
@@ -140,11 +142,16 @@ __global__ __launch_bounds__(256) void k_gc_gather(const uint32_t* __restrict__ 
             for (uint32_t k = 0; k < 8; ++k)
                 if (k < nk) c[k] = fwd[c[k]];
         }
+        uint32_t node[9];
+        node[0] = hdr;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) node[1 + k] = c[k];
         out[0] = hdr;
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k)
             if (k < nk) out[1 + k] = c[k];
         tlen[i] = nk + 1;
+        bkt[i] = g.bucket_from_hash(level, hash_words(node, nk + 1));
     }
 }
 
@@ -171,25 +178,18 @@ __global__ void k_gc_full(TableRef T, uint32_t level, uint32_t* full_nodes, uint
 }
 
 // The live nodes of a level are placed bucket by bucket: each node takes a rank within its
-// bucket (k_gc_bucket), the buckets' ranges of a level-wide order follow from a scan of the
+// bucket (the bucket from k_gc_gather, the rank from k_gc_bucket), the buckets' ranges of a level-wide order follow from a scan of the
 // bucket counts (k_gc_scan, k_gc_order), one lane per bucket lays its nodes out in rank order
 // with the append rule of table_append (a node that does not fit in the rest of its page starts
 // the next page; k_gc_layout), and every node is then written where it was placed
 // (k_gc_write). This is what appending the nodes one at a time in rank order gives, without a
 // compare-and-swap loop on each bucket's size.
-__global__ __launch_bounds__(256) void k_gc_bucket(Geom g, uint32_t level, const uint32_t* counts, uint32_t cap,
-                                                   const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ tlen,
-                                                   uint32_t* __restrict__ bkt, uint32_t* __restrict__ rank,
+__global__ __launch_bounds__(256) void k_gc_bucket(uint32_t level, const uint32_t* counts, uint32_t cap,
+                                                   const uint32_t* __restrict__ bkt, uint32_t* __restrict__ rank,
                                                    uint32_t* bcount) {
     const uint32_t n = min(counts[level], cap);
-    const bool leaf = level == g.leaf_level();
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t* node = tmp + (uint64_t)i * 9;
-        const uint32_t h = leaf ? hash_leaf((uint64_t)node[0] | ((uint64_t)node[1] << 32)) : hash_words(node, tlen[i]);
-        const uint32_t b = g.bucket_from_hash(level, h);
-        bkt[i] = b;
-        rank[i] = atomicAdd(&bcount[b], 1u);
-    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        rank[i] = atomicAdd(&bcount[bkt[i]], 1u);
 }
 
 // exclusive scan of the bucket counts of one level (one block)
@@ -405,14 +405,14 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
     }
     TableRef T{g, ctx->vpool, ctx->page_table, ctx->bucket_sizes, ctx->counters + 0, ctx->cfg.pool_pages};
     for (uint32_t l = max_level; l != (uint32_t)-1; --l) {
-        k_gc_gather<<<kGcGrid, 256, 0, s>>>(ctx->vpool, frontier[l], counts, l, L, max_level, cap, fwd, tmp, tlen);
+        k_gc_gather<<<kGcGrid, 256, 0, s>>>(g, ctx->vpool, frontier[l], counts, l, max_level, cap, fwd, tmp, tlen, bkt);
         // reset the level's buckets, then the full node first, then the live nodes
         const uint32_t b0 = g.bucket_global(l, 0);
         k_gc_fill<<<256, 256, 0, s>>>(ctx->bucket_sizes + b0, g.buckets_per_level(l), 0u);
         k_gc_full<<<1, 1, 0, s>>>(T, l, ctx->full_nodes, errors);
         const uint32_t nb = g.buckets_per_level(l);
         k_gc_fill<<<256, 256, 0, s>>>(bcount, nb, 0u);
-        k_gc_bucket<<<kGcGrid, 256, 0, s>>>(g, l, counts, cap, tmp, tlen, bkt, rank, bcount);
+        k_gc_bucket<<<kGcGrid, 256, 0, s>>>(l, counts, cap, bkt, rank, bcount);
         k_gc_scan<<<1, 1024, 0, s>>>(bcount, bstart, nb);
         k_gc_order<<<kGcGrid, 256, 0, s>>>(l, counts, cap, tlen, bkt, rank, bstart, olen);
         k_gc_layout<<<(nb + 255) / 256, 256, 0, s>>>(g, l, ctx->bucket_sizes, bcount, bstart, olen, opos, errors);
