@@ -506,16 +506,61 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
 // leaf level: one thread per leaf item, its 64 voxels as bit masks (bit = child1 * 8 + child2).
 // Thread-per-item keeps many items' loads in flight; the per-voxel editor tests are cheap.
 // ---------------------------------------------------------------------------------------------
+// Voxel bit of a leaf for the offsets (xi, yi, zi) in 0..3: child1 * 8 + child2, child1 from
+// the offsets' high bits and child2 from their low bits (x, y, z from bit 2 down).
+__device__ __forceinline__ constexpr uint32_t leaf_voxel_bit(uint32_t xi, uint32_t yi, uint32_t zi) {
+    return ((((xi >> 1) << 2) | ((yi >> 1) << 1) | (zi >> 1)) << 3) | (((xi & 1u) << 2) | ((yi & 1u) << 1) | (zi & 1u));
+}
+// The voxels of the leaf at (bx, by, bz) * 4 that e_touch accepts. Both of its tests split by
+// axis: the box test is an overlap test per axis, and the sphere test subtracts a term per axis
+// from r² in x, y, z order, each term depending on that axis's coordinate only (a term that is
+// not subtracted is 0, and dist - 0 is dist). So the x terms are taken once for the 4 x
+// offsets, then the y terms for the 16 (x, y) pairs, then the z terms for the 64 voxels: the
+// same FP32 operations per voxel in the same order as e_touch.
 template <class E>
 __device__ __forceinline__ uint64_t leaf_touch_mask(const E& e, uint32_t bx, uint32_t by, uint32_t bz) {
+    const uint32_t base[3] = {bx, by, bz};
+    if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE || e.kind == EDIT_COPY || e.kind == EDIT_FILL) {
+        uint32_t in[3];  // per axis: bit i = offset i overlaps
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            in[a] = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < 4; ++i) {
+                const float v0 = (float)(base[a] | i), v1 = v0 + 1.0f;
+                const bool out = e.bmin[a] >= v1 || e.bmax[a] <= v0 || v0 >= e.bmax[a] || v1 <= e.bmin[a];
+                in[a] |= out ? 0u : (1u << i);
+            }
+        }
+        uint64_t tm = 0;
+#pragma unroll
+        for (uint32_t xi = 0; xi < 4; ++xi)
+#pragma unroll
+            for (uint32_t yi = 0; yi < 4; ++yi)
+#pragma unroll
+                for (uint32_t zi = 0; zi < 4; ++zi)
+                    if (((in[0] >> xi) & (in[1] >> yi) & (in[2] >> zi)) & 1u) tm |= 1ull << leaf_voxel_bit(xi, yi, zi);
+        return tm;
+    }
+    float term[3][4];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            const float v0 = (float)(base[a] | i), v1 = v0 + 1.0f;
+            term[a][i] = e.c[a] < v0 ? sqf(e.c[a] - v0) : (e.c[a] > v1 ? sqf(e.c[a] - v1) : 0.0f);
+        }
     uint64_t tm = 0;
-#pragma unroll 8
-    for (uint32_t b = 0; b < 64; ++b) {
-        const uint32_t c1 = b >> 3, c2 = b & 7u;
-        const float fx = (float)(bx | (((c1 >> 2) & 1u) << 1) | ((c2 >> 2) & 1u));
-        const float fy = (float)(by | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u));
-        const float fz = (float)(bz | ((c1 & 1u) << 1) | (c2 & 1u));
-        if (e_touch(e, fx, fy, fz, fx + 1.0f, fy + 1.0f, fz + 1.0f)) tm |= 1ull << b;
+#pragma unroll
+    for (uint32_t xi = 0; xi < 4; ++xi) {
+        const float dx = e.r2 - term[0][xi];
+#pragma unroll
+        for (uint32_t yi = 0; yi < 4; ++yi) {
+            const float dxy = dx - term[1][yi];
+#pragma unroll
+            for (uint32_t zi = 0; zi < 4; ++zi)
+                if (dxy - term[2][zi] > 0) tm |= 1ull << leaf_voxel_bit(xi, yi, zi);
+        }
     }
     return tm;
 }
