@@ -49,6 +49,7 @@ _SIGS = {
     "orc_colors_leaf_sizes": (C.c_int, [C.c_void_p, C.c_uint32, u64p]),
     "orc_colors_leaf_copy": (C.c_int, [C.c_void_p, C.c_uint32, u64p, u32p, u64p]),
     "orc_color_canonical": (C.c_uint64, [C.c_void_p, C.c_void_p, u32p, C.c_uint64]),
+    "orc_color_leaf_bytes": (C.c_uint64, [C.c_void_p, C.c_void_p, u32p, C.c_uint64]),
     "orc_encode_colors": (C.c_int, [u32p, u8p, u8p, C.c_uint64, u64p, u32p, u64p, u64p, C.c_uint64, C.c_uint64,
                                     C.c_uint64]),
     "orc_decode_color": (C.c_int, [u64p, C.c_uint64, u32p, C.c_uint64, u64p, C.c_uint64, C.c_uint64, C.c_uint64,
@@ -68,6 +69,7 @@ _SIGS = {
                                    C.c_int, C.c_float, C.c_void_p]),
     "orc_get_color": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
     "orc_dag_reachable": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "orc_find_in_bucket": (C.c_uint32, [C.c_void_p, C.c_uint32, C.c_uint32, u32p, C.c_uint32]),
 }
 
 
@@ -250,6 +252,15 @@ class OracleColors:
         out = np.zeros(n, np.uint32)
         if n:
             orc.orc_color_canonical(dag.h, self.h, _p(out), n)
+        return out
+
+    def leaf_bytes(self, dag: OracleDAG) -> np.ndarray:
+        """The stored words of every colour subtree's leaf in DFS order (encoder byte choices
+        included), see orc_color_leaf_bytes."""
+        n = orc.orc_color_leaf_bytes(dag.h, self.h, None, 0)
+        out = np.zeros(n, np.uint32)
+        if n:
+            orc.orc_color_leaf_bytes(dag.h, self.h, _p(out), n)
         return out
 
 

@@ -252,6 +252,65 @@ static void color_canon_rec(const DAG& dag, const HashDAGColors& colors, u32 lev
     }
 }
 
+// The raw words of every level-(color_tree_levels) subtree's colour leaf in DFS order:
+// [x, y, z, kind] then, for a unique leaf (kind 1), n_blocks, n_weights, n_macro and the
+// blocks / weights / macro words as stored; for a shared leaf (kind 2) its offset (lo, hi);
+// kind 0 = no leaf. Compares the encoder's byte choices, not only the decoded colours.
+static void color_bytes_rec(const DAG& dag, const HashDAGColors& colors, u32 level, u32 va, u32 colorNode, Path path,
+                            std::vector<u32>& out) {
+    const u32 ctl = dag.g().color_tree_levels;
+    const u32 hdr = dag.node(va);
+    const u8 m = child_mask(hdr);
+    for (u8 c = 0; c < 8; ++c) {
+        if (!(m & (1u << c))) continue;
+        Path cp = path;
+        cp.descend(c);
+        const u32 child = dag.child_index(va, m, c);
+        const u32 cn = colors.nodes[colorNode + c];
+        if (level + 1 == ctl) {
+            out.push_back(cp.p.x);
+            out.push_back(cp.p.y);
+            out.push_back(cp.p.z);
+            const LeafView leaf = colors.get_leaf(cn);
+            if (!leaf.d) {
+                out.push_back(0);
+            } else if (leaf.is_shared()) {
+                out.push_back(2);
+                out.push_back((u32)leaf.offset);
+                out.push_back((u32)(leaf.offset >> 32));
+            } else {
+                const ColorLeaf& l = *leaf.d;
+                out.push_back(1);
+                out.push_back((u32)l.blocks.size());
+                out.push_back((u32)l.weights.size());
+                out.push_back((u32)l.macro.size());
+                for (u64 b : l.blocks) {
+                    out.push_back((u32)b);
+                    out.push_back((u32)(b >> 32));
+                }
+                out.insert(out.end(), l.weights.begin(), l.weights.end());
+                for (u64 b : l.macro) {
+                    out.push_back((u32)b);
+                    out.push_back((u32)(b >> 32));
+                }
+            }
+        } else {
+            if (cn == 0 && level > 0) continue;
+            color_bytes_rec(dag, colors, level + 1, child, cn, cp, out);
+        }
+    }
+}
+
+uint64_t orc_color_leaf_bytes(void* d, void* c, uint32_t* out, uint64_t cap) {
+    GUARD_BEGIN
+    std::vector<u32> v;
+    color_bytes_rec(*static_cast<DAG*>(d), *static_cast<HashDAGColors*>(c), 0, static_cast<DAG*>(d)->root, 0,
+                    Path(0, 0, 0), v);
+    if (out && cap >= v.size()) std::copy(v.begin(), v.end(), out);
+    return v.size();
+    GUARD_END(0)
+}
+
 uint64_t orc_color_canonical(void* d, void* c, uint32_t* out, uint64_t cap) {
     GUARD_BEGIN
     std::vector<u32> v;
@@ -455,6 +514,15 @@ int orc_dag_reachable(void* d, uint32_t max_level, uint64_t* out) {
     for (uint32_t l = 0; l <= max_level; ++l) out[l] = valid[l].size();
     return 0;
     GUARD_END(-1)
+}
+
+// HashTable::find_interior_node_in_bucket / find_leaf_node_in_bucket (hash_table.h:196-357) on
+// one bucket of the table as it stands (leaf level: words = the two leaf words)
+uint32_t orc_find_in_bucket(void* d, uint32_t level, uint32_t bucket, const uint32_t* words, uint32_t n) {
+    DAG& dag = *static_cast<DAG*>(d);
+    const uint32_t bs = dag.t.bucket_sizes[dag.g().bucket_global(level, bucket)];
+    if (level == dag.g().leaf_level()) return dag.t.find_leaf_in_bucket(bs, bucket, (u64)words[0] | ((u64)words[1] << 32));
+    return dag.t.find_interior_in_bucket(bs, level, bucket, n, words);
 }
 
 }  // extern "C"

@@ -86,6 +86,10 @@ int orc_colors_leaf_copy(void* colors, uint32_t i, uint64_t* blocks, uint32_t* w
 /* decoded colour sequence per level-(color_tree_levels) subtree in DFS order: for each subtree
  * [x, y, z, n, then n x (colorBits, weight | bpw << 8)]; returns required length */
 uint64_t orc_color_canonical(void* dag, void* colors, uint32_t* out, uint64_t cap);
+/* raw leaf words per level-(color_tree_levels) subtree in DFS order: [x, y, z, kind], then for
+ * kind 1 (unique) n_blocks, n_weights, n_macro and the stored words, for kind 2 (shared) the
+ * offset as two words; returns required length */
+uint64_t orc_color_leaf_bytes(void* dag, void* colors, uint32_t* out, uint64_t cap);
 
 /* colour codec round trip: encode n colours (colorBits[i], weight[i], bpw[i]) with the
  * reference encoder; sizes out = n_blocks, n_weights, n_macro */
@@ -128,6 +132,8 @@ int orc_get_color(void* dag, void* colors, uint32_t x, uint32_t y, uint32_t z, u
 int orc_gc(void* dag, uint32_t max_level);
 /* distinct node addresses reachable from the root, per level 0..max_level */
 int orc_dag_reachable(void* dag, uint32_t max_level, uint64_t* out);
+/* the reference's in-bucket find on one bucket (hash_table.h:196-357): address or 0xFFFFFFFF */
+uint32_t orc_find_in_bucket(void* dag, uint32_t level, uint32_t bucket, const uint32_t* words, uint32_t n);
 
 #ifdef __cplusplus
 }
