@@ -161,6 +161,8 @@ _SIGS = {
     "hd_load_hash_dag_file": (C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
     "hd_checkpoint_save": (C.c_int32, [C.c_void_p, C.c_char_p]),
     "hd_checkpoint_load": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "hd_validate_image": (C.c_int32, [C.POINTER(hd_config), u32p, u32p, u32p, u32p, C.c_uint32, C.c_uint32,
+                                      C.c_char_p, C.c_uint32]),
     "hd_replay_parse": (C.c_int32, [C.c_char_p, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "hd_replay_actions": (C.c_int32, [C.c_void_p, C.POINTER(C.POINTER(hd_replay_action)), u64p,
                                       C.POINTER(C.c_uint32)]),

@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libhashdag_hip.so")
 
 HIP_SOURCES = ["hd_context.hip", "hd_trace.hip", "hd_stream.hip", "hd_edit.hip", "hd_edit_api.hip", "hd_gc.hip",
                "hd_serve.hip", "hd_replay.hip", "hd_files.hip"]
-CPP_SOURCES = ["hd_scene.cpp", "hd_factory.cpp"]
+CPP_SOURCES = ["hd_scene.cpp", "hd_factory.cpp", "hd_validate.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

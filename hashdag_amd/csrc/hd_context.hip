@@ -129,6 +129,11 @@ hd_status hd_create(const hd_config* cfg, hd_ctx** out) {
     if (cfg->page_size == 0 || (cfg->page_size & (cfg->page_size - 1)) || cfg->bucket_size_top % cfg->page_size ||
         cfg->bucket_size_low % cfg->page_size)
         return fail(nullptr, HD_ERR_INVALID, "bucket sizes must be multiples of a power-of-two page size");
+    // a full node's header counts 8^(levels - level) voxels in 24 bits from color_tree_levels
+    // down (hash_table.h:749-789; the reference aborts beyond that, hash_table.h:772); a colour
+    // tree at or below the leaf level means no colour leaves
+    if (cfg->color_tree_levels + 7 < cfg->levels || cfg->color_tree_levels == 0)
+        return fail(nullptr, HD_ERR_INVALID, "color_tree_levels must be at least levels - 7");
     auto* ctx = new hd_ctx();
     ctx->cfg = *cfg;
     ctx->g = Geom{cfg->levels,          cfg->page_size,       cfg->num_top_levels, cfg->bucket_bits_top,
@@ -270,6 +275,9 @@ extern "C" {
 hd_status hd_upload_image(hd_ctx* ctx, const uint32_t* pool, const uint32_t* page_table, const uint32_t* bucket_sizes,
                           const uint32_t* full_nodes, uint32_t pool_top, uint32_t root) {
     if (!ctx || !pool || !page_table || !bucket_sizes || !full_nodes) return fail(ctx, HD_ERR_INVALID, "null argument");
+    const std::string why =
+        validate_image(ctx->g, ctx->cfg.pool_pages, pool, page_table, bucket_sizes, full_nodes, pool_top, root);
+    if (!why.empty()) return fail(ctx, HD_ERR_INVALID, "invalid image: " + why);
     return upload_image_impl(ctx, pool, page_table, bucket_sizes, full_nodes, pool_top, root);
 }
 

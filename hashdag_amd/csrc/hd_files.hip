@@ -330,6 +330,11 @@ hd_status hd_checkpoint_load(hd_ctx* ctx, const char* path) {
     r.get(magic, 8);
     if (!r.ok || std::memcmp(magic, kCkptMagic, 8) != 0 || !r.at_end())
         return fail(ctx, HD_ERR_INVALID, "checkpoint does not end where its sizes say");
+    if (hasColors) {
+        const std::string why = validate_colors(g, nodes.data(), nodes.size(), offsets.size(), leaves.data(),
+                                                leaves.size(), arena.size(), main);
+        if (!why.empty()) return fail(ctx, HD_ERR_INVALID, "invalid checkpoint colours: " + why);
+    }
     hd_status st = hd_upload_image(ctx, pool.data(), pt.data(), bs.data(), full.data(), poolTop, root);
     if (st != HD_OK) return st;
     ctx->undo.clear();

@@ -138,4 +138,12 @@ struct HostImage {
 // "bucket"/"pool" in the message on overflow.
 void load_from_basic_dag(HostImage& img, const uint32_t* data, uint64_t n);
 
+// Checks of external inputs before they reach the device (hd_validate.cpp); "" when valid,
+// else the reason. pool holds physical pages 0 .. pool_top - 1.
+std::string validate_image(const Geom& g, uint32_t pool_pages, const uint32_t* pool, const uint32_t* page_table,
+                           const uint32_t* bucket_sizes, const uint32_t* full_nodes, uint32_t pool_top,
+                           uint32_t root);
+std::string validate_colors(const Geom& g, const uint32_t* nodes, uint64_t n_nodes, uint64_t n_offsets,
+                            const LeafDesc* leaves, uint64_t n_leaves, uint64_t arena_words, const LeafDesc& main);
+
 }  // namespace hd

@@ -25,6 +25,13 @@ class Geometry:
     bucket_size_low: int = 4096
     color_tree_levels: int = 10
 
+    def __post_init__(self):
+        # full-node headers count 8^(levels - level) voxels in 24 bits from the colour-tree
+        # level down (hash_table.h:749-789)
+        if self.color_tree_levels < 1 or self.color_tree_levels + 7 < self.levels:
+            raise ValueError(f"color_tree_levels {self.color_tree_levels} must be at least levels - 7 "
+                             f"({self.levels - 7}) and positive")
+
     @property
     def leaf_level(self) -> int:
         return self.levels - 2

@@ -90,6 +90,17 @@ hd_status hd_download_image(hd_ctx* ctx, uint32_t* pool, uint32_t* page_table, u
 hd_status hd_upload_image(hd_ctx* ctx, const uint32_t* pool, const uint32_t* page_table, const uint32_t* bucket_sizes,
                           const uint32_t* full_nodes, uint32_t pool_top, uint32_t root);
 
+/* Checks a reference-layout image the way hd_upload_image does before any copy (it rejects
+ * a failing image with HD_ERR_INVALID): every page-table entry maps a distinct physical page
+ * below pool_top, every bucket size fits its bucket with its used pages mapped, and every node
+ * reachable from the root or from the full nodes of levels 1 .. leaf level lies in its level's
+ * used bucket range with child pointers inside the table. `pool` holds pages 0 .. pool_top-1.
+ * Needs no device. why (may be NULL): the reason, NUL-terminated within why_len bytes.
+ * No counterpart in the reference, whose loader reads a dump as is (hash_dag_factory.cpp:225-246). */
+hd_status hd_validate_image(const hd_config* cfg, const uint32_t* pool, const uint32_t* page_table,
+                            const uint32_t* bucket_sizes, const uint32_t* full_nodes, uint32_t pool_top, uint32_t root,
+                            char* why, uint32_t why_len);
+
 /* HashDAG::firstNodeIndex get/set (undo/redo = root history, hash_dag.h:68-120). */
 hd_status hd_get_root(hd_ctx* ctx, uint32_t* root);
 hd_status hd_set_root(hd_ctx* ctx, uint32_t root);
