@@ -63,8 +63,11 @@ _SIGS = {
     "orc_trace_shadows": (C.c_int, [C.c_void_p, dblp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p,
                                     C.c_float, C.c_float, u64p, C.c_int]),
     "orc_edit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, fltp, C.c_float, u64p]),
+    "orc_edit_threads": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, fltp, C.c_float, C.c_int, u64p]),
     "orc_gc": (C.c_int, [C.c_void_p, C.c_uint32]),
     "orc_edit_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "orc_edit_copy_kind": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_float, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]),
     "orc_edit_copy_tf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_int, C.c_float, C.c_void_p]),
     "orc_get_color": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
@@ -169,20 +172,24 @@ class OracleDAG:
     def get_value(self, x, y, z) -> bool:
         return bool(orc.orc_get_value(self.h, x, y, z))
 
-    def edit(self, kind: int, center, radius: float, colors: "OracleColors | None" = None):
+    def edit(self, kind: int, center, radius: float, colors: "OracleColors | None" = None, threads: int = 1):
         c = (C.c_float * 3)(*center)
         stats = np.zeros(3, np.uint64)
-        r = orc.orc_edit(self.h, colors.h if colors else None, kind, c, radius, _p(stats, C.c_uint64))
+        r = orc.orc_edit_threads(self.h, colors.h if colors else None, kind, c, radius, threads,
+                                 _p(stats, C.c_uint64))
         if r != 0:
             raise RuntimeError(_err())
         return [int(v) for v in stats]
 
     def edit_copy(self, center, radius: float, source, dest, colors: "OracleColors", transform=None,
-                  swirl_period=None):
-        """CopyEditor; transform: 3x3 row-major matrix (FP64), swirl_period: None = no swirl."""
+                  swirl_period=None, decompress=False):
+        """CopyEditor; transform: 3x3 row-major matrix (FP64), swirl_period: None = no swirl;
+        decompress: CopyEditorWithDecompression (no transform or swirl)."""
         c, s, d = (C.c_float * 3)(*center), (C.c_float * 3)(*source), (C.c_float * 3)(*dest)
         stats = np.zeros(3, np.uint64)
-        if transform is None and swirl_period is None:
+        if decompress:
+            rc = orc.orc_edit_copy_kind(self.h, colors.h, 7, c, radius, s, d, stats.ctypes.data)
+        elif transform is None and swirl_period is None:
             rc = orc.orc_edit_copy(self.h, colors.h, c, radius, s, d, stats.ctypes.data)
         else:
             t = np.ascontiguousarray(np.eye(3) if transform is None else transform, np.float64).reshape(9)

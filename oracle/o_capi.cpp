@@ -5,6 +5,7 @@
 #include <atomic>
 #include <exception>
 #include <set>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -426,12 +427,17 @@ int orc_trace_shadows(void* d, const double params[12], uint32_t W, uint32_t H, 
 }
 
 int orc_edit(void* d, void* c, uint32_t kind, const float center[3], float radius, uint64_t stats[3]) {
+    return orc_edit_threads(d, c, kind, center, radius, 1, stats);
+}
+
+int orc_edit_threads(void* d, void* c, uint32_t kind, const float center[3], float radius, int threads,
+                     uint64_t stats[3]) {
     GUARD_BEGIN
     Edit e{};
     e.kind = kind;
     e.center = {center[0], center[1], center[2]};
     e.radius = radius;
-    const EditStats s = apply_edit(*static_cast<DAG*>(d), static_cast<HashDAGColors*>(c), e, nullptr);
+    const EditStats s = apply_edit(*static_cast<DAG*>(d), static_cast<HashDAGColors*>(c), e, nullptr, threads);
     if (stats) {
         stats[0] = s.numNodes;
         stats[1] = s.numVoxels;
@@ -443,9 +449,15 @@ int orc_edit(void* d, void* c, uint32_t kind, const float center[3], float radiu
 
 int orc_edit_copy(void* d, void* c, const float center[3], float radius, const float source[3], const float dest[3],
                   uint64_t stats[3]) {
+    return orc_edit_copy_kind(d, c, COPY, center, radius, source, dest, stats);
+}
+
+int orc_edit_copy_kind(void* d, void* c, uint32_t kind, const float center[3], float radius, const float source[3],
+                       const float dest[3], uint64_t stats[3]) {
     GUARD_BEGIN
+    if (kind != COPY && kind != COPY_DEC) throw std::runtime_error("oracle: not a copy editor kind");
     Edit e{};
-    e.kind = COPY;
+    e.kind = kind;
     e.center = {center[0], center[1], center[2]};
     e.radius = radius;
     e.source = {source[0], source[1], source[2]};

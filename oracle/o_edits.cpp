@@ -5,6 +5,10 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <map>
 #include <stdexcept>
@@ -109,6 +113,11 @@ struct Editor {
     std::vector<CompressedColor> fillColors;
     F3 fmin{};
     u32 fsize = 0;
+    // copy with decompression (hash_dag_editors.h:437-616): the source box's float and integer
+    // minimum and the side of its voxel buffer
+    F3 copyMin{};
+    U3 intCopyMin{0, 0, 0};
+    u32 dsize = 0;
 
     explicit Editor(const Edit& ed) : e(ed) {
         F3 c = e.center;
@@ -136,11 +145,56 @@ struct Editor {
             const double s = 1 / (v1 - v2 + v3);
             for (int i = 0; i < 9; ++i) inv[i] = a[i] * s;
         }
+        if (e.kind == COPY_DEC) {
+            // BoxEditorBase(center - source + dest, radius); a buffer of 2 ceil(radius) voxels
+            // per axis from truncate(center - radius) (hash_dag_editors.h:455-466)
+            c = e.center - e.source + e.dest;
+            copyMin = e.center - f3(e.radius);
+            intCopyMin = U3{to_u32(copyMin.x), to_u32(copyMin.y), to_u32(copyMin.z)};
+            dsize = 2 * (u32)std::ceil(e.radius);
+        }
         bmin = c - f3(radius);
         bmax = c + f3(radius);
         r2 = e.radius * e.radius;
     }
-    bool is_box() const { return e.kind == BOX_ADD || e.kind == BOX_CARVE || e.kind == COPY || e.kind == FILL; }
+    bool is_box() const {
+        return e.kind == BOX_ADD || e.kind == BOX_CARVE || e.kind == COPY || e.kind == FILL || e.kind == COPY_DEC;
+    }
+
+    // ---- CopyEditorWithDecompression ----
+    // buffer voxel of a destination voxel corner: truncate(position - dest + source - copyMin)
+    // (hash_dag_editors.h:588, 601); false outside the buffer
+    bool dec_voxel(F3 pos, Path& src) const {
+        const F3 q = pos - e.dest + e.source - copyMin;
+        const u32 x = to_u32(q.x), y = to_u32(q.y), z = to_u32(q.z);
+        if (!(x < dsize && y < dsize && z < dsize)) return false;
+        src = Path(intCopyMin.x + x, intCopyMin.y + y, intCopyMin.z + z);
+        return true;
+    }
+    // DAGUtils::is_empty (dag_utils.h:174-252) on the DAG before the edit: no node of maxLevel
+    // overlaps [start, start + size) in the reference's inclusive-max test
+    bool src_empty_rec(u32 level, u32 va, Path path, u32 maxLevel, U3 start, U3 size) const {
+        if (level == maxLevel) return false;
+        const Geometry& g = srcTable->g;
+        const u32 shift = g.levels - level;
+        const U3 mn{path.p.x << shift, path.p.y << shift, path.p.z << shift};
+        const U3 mx{mn.x + ((1u << shift) - 1), mn.y + ((1u << shift) - 1), mn.z + ((1u << shift) - 1)};
+        const U3 inMax{start.x + size.x, start.y + size.y, start.z + size.z};
+        const bool overlaps = !(mn.x >= inMax.x || mn.y >= inMax.y || mn.z >= inMax.z || mx.x <= start.x ||
+                                mx.y <= start.y || mx.z <= start.z || start.x >= mx.x || start.y >= mx.y ||
+                                start.z >= mx.z || inMax.x <= mn.x || inMax.y <= mn.y || inMax.z <= mn.z);
+        if (level < g.leaf_level() && !overlaps) return true;
+        const u32 hdr = *srcTable->ptr(va);
+        const u8 m = child_mask(hdr);
+        for (u8 c = 0; c < 8; ++c) {
+            if (!(m & (1u << c))) continue;
+            Path np = path;
+            np.descend(c);
+            if (!src_empty_rec(level + 1, *srcTable->ptr(va + child_offset(m, c)), np, maxLevel, start, size))
+                return false;
+        }
+        return true;
+    }
 
     // ---- BufferEditor (hash_dag_editors.h:144-209) ----
     bool fvalid(u32 x, u32 y, u32 z) const { return x < fsize && y < fsize && z < fsize; }
@@ -251,10 +305,24 @@ struct Editor {
         if (e.kind == SPHERE_CARVE) return inside_sphere(mn, mx);
         return false;
     }
-    // Editor::should_edit/is_full/is_empty (hash_dag_editors.h:12-27)
+    // Editor::should_edit/is_full/is_empty (hash_dag_editors.h:12-27); the decompressing copy
+    // also skips nodes above levels - 4 whose source region is empty (COPY_EMPTY_CHECKS,
+    // hash_dag_editors.h:561-582)
     bool should_edit(const Path& p, u32 depth) const {
         const F3 s = p.as_position(depth);
-        return should_edit_impl(s, s + f3((float)(1u << depth)));
+        if (!should_edit_impl(s, s + f3((float)(1u << depth)))) return false;
+        if (e.kind == COPY_DEC) {
+            const u32 levels = srcTable->g.levels, level = levels - depth;
+            if (level < levels - 4) {
+                const F3 off = s - e.dest + e.source;
+                if (off.x >= 0 && off.y >= 0 && off.z >= 0) {
+                    const U3 start{to_u32(off.x), to_u32(off.y), to_u32(off.z)};
+                    const u32 n = 1u << depth;
+                    if (src_empty_rec(0, srcRoot, Path(0, 0, 0), level, start, U3{n, n, n})) return false;
+                }
+            }
+        }
+        return true;
     }
     bool is_full(const Path& p, u32 depth) const {
         const F3 s = p.as_position(depth);
@@ -268,6 +336,11 @@ struct Editor {
     // get_new_value (hash_dag_editors.h:263, 306, 322, 416-424)
     bool new_value(bool old, F3 pos) const {
         if (e.kind == COPY) return old ? old : value_at(*srcTable, srcRoot, source_of(pos));
+        if (e.kind == COPY_DEC) {  // hash_dag_editors.h:582-592
+            Path src;
+            if (old || !dec_voxel(pos, src)) return old;
+            return value_at(*srcTable, srcRoot, src);
+        }
         if (e.kind == FILL) {  // BufferEditor::get_new_value (hash_dag_editors.h:211-220)
             const F3 fp = pos - fmin;
             const u32 x = to_u32(fp.x), y = to_u32(fp.y), z = to_u32(fp.z);
@@ -283,6 +356,11 @@ struct Editor {
         if (e.kind == COPY) {
             if (oldValue == newValue || !newValue) return oldColor();
             return color_at(*srcTable, srcRoot, *srcColors, source_of(pos));
+        }
+        if (e.kind == COPY_DEC) {  // hash_dag_editors.h:593-611 (colours of the DAG before the edit)
+            Path src;
+            if (oldValue == newValue || !newValue || !dec_voxel(pos, src)) return oldColor();
+            return color_at(*srcTable, srcRoot, *srcColors, src);
         }
         if (e.kind == FILL) {  // FillEditorColors::get_new_color (hash_dag_editors.h:737-754)
             if (newValue && !oldValue) {
@@ -552,12 +630,13 @@ CompressedColor get_color_at(const DAG& dag, const HashDAGColors& colors, Path p
 }
 
 // hash_dag.h:255-405 (tasks run in creation order; undo frame kept when requested)
-EditStats apply_edit(DAG& dag, HashDAGColors* colors, const Edit& e, UndoFrame* undo) {
+EditStats apply_edit(DAG& dag, HashDAGColors* colors, const Edit& e, UndoFrame* undo, int threads) {
     Editor ed(e);
     ed.srcTable = &dag.t;  // the copy reads the DAG as it was before this edit
     ed.srcRoot = dag.root;
     ed.srcColors = colors;
-    if ((e.kind == COPY || e.kind == FILL) && !colors) throw std::runtime_error("oracle: copy and fill need colours");
+    if ((e.kind == COPY || e.kind == COPY_DEC || e.kind == FILL) && !colors)
+        throw std::runtime_error("oracle: copy and fill need colours");
     if (e.kind == FILL) ed.build_fill();
     EditCtx ctx{dag.t, colors, ed, {}, {}, {}, {}};
     if (undo) {
@@ -565,15 +644,44 @@ EditStats apply_edit(DAG& dag, HashDAGColors* colors, const Edit& e, UndoFrame* 
         undo->leaves.clear();
     }
     edit_threaded(true, 0, dag.root, 0, Path(0, 0, 0), ctx);
-    for (Task* task : ctx.taskOrder) {
+    // the tasks (hash_dag.h:330-356): in creation order, or taken by `threads` threads from a
+    // shared index as the reference's thread pool takes them (NUM_THREADS, typedefs.h:54-56)
+    auto run_task = [&](Task* task) {
         LeafCtx lc{dag.t, task->existing, task->builder.get(), 0, task->stats};
         task->finalIndex = edit_leaf(dag.g().color_tree_levels, task->childNodeIndex, task->path, ed, lc).nodeIndex;
+    };
+    if (threads <= 1 || ctx.taskOrder.size() < 2) {
+        for (Task* task : ctx.taskOrder) run_task(task);
+    } else {
+        std::vector<std::mutex> locks(4096);
+        dag.t.locks = &locks;
+        std::atomic<size_t> next{0};
+        std::exception_ptr failure;
+        std::mutex failMu;
+        std::vector<std::thread> pool;
+        for (int k = 0; k < threads; ++k)
+            pool.emplace_back([&]() {
+                for (size_t i; (i = next.fetch_add(1)) < ctx.taskOrder.size();) {
+                    try {
+                        run_task(ctx.taskOrder[i]);
+                    } catch (...) {
+                        std::lock_guard<std::mutex> g(failMu);
+                        failure = std::current_exception();
+                        next = ctx.taskOrder.size();
+                    }
+                }
+            });
+        for (auto& th : pool) th.join();
+        dag.t.locks = nullptr;
+        if (failure) std::rethrow_exception(failure);
+    }
+    for (Task* task : ctx.taskOrder) {
         ctx.stats.numNodes += task->stats.numNodes;
         ctx.stats.numVoxels += task->stats.numVoxels;
     }
     dag.root = edit_threaded(false, 0, dag.root, 0, Path(0, 0, 0), ctx).nodeIndex;
     ctx.stats.leavesRebuilt = ctx.toBuild.size();
-    if (e.kind == COPY) ctx.stats.numVoxels = ed.edit_count();  // "num voxels" (hash_dag.h:374-378)
+    if (e.kind == COPY || e.kind == COPY_DEC) ctx.stats.numVoxels = ed.edit_count();  // "num voxels" (hash_dag.h:374-378)
     for (auto& tb : ctx.toBuild) {
         ColorLeaf& ref = colors->leaves[tb.newLeafIndex & 0x7FFFFFFFu];
         // The undo frame keeps the leaf the subtree used before the edit: the shared main

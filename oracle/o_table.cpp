@@ -63,6 +63,8 @@ void HashTable::create(const Geometry& geom, u32 pool_pages) {
 
 // hash_table.h:794-809
 void HashTable::allocate_page(u32 page) {
+    std::unique_lock<std::mutex> lk(pool_lock, std::defer_lock);
+    if (locks) lk.lock();
     if (pool_top >= pool_max_pages) {
         overflow = true;
         throw std::runtime_error("oracle: pool exhausted");
@@ -153,6 +155,8 @@ u32 HashTable::add_interior(u32 level, u32 n, const u32* node, u32 hash) {
 u32 HashTable::find_or_add_leaf(u64 leaf) {
     const u32 hash = hash_leaf(leaf);
     const u32 bucket = g.bucket_from_hash(g.leaf_level(), hash);
+    std::unique_lock<std::mutex> lk;
+    if (locks) lk = std::unique_lock<std::mutex>((*locks)[g.bucket_global(g.leaf_level(), bucket) % locks->size()]);
     const u32 bs = bucket_sizes[g.bucket_global(g.leaf_level(), bucket)];
     u32 r = find_leaf_in_bucket(bs, bucket, leaf);
     if (r == NOT_FOUND) r = add_leaf(leaf, hash);
@@ -163,6 +167,8 @@ u32 HashTable::find_or_add_leaf(u64 leaf) {
 u32 HashTable::find_or_add_interior(u32 level, u32 n, const u32* node) {
     const u32 hash = hash_interior(n, node);
     const u32 bucket = g.bucket_from_hash(level, hash);
+    std::unique_lock<std::mutex> lk;
+    if (locks) lk = std::unique_lock<std::mutex>((*locks)[g.bucket_global(level, bucket) % locks->size()]);
     const u32 bs = bucket_sizes[g.bucket_global(level, bucket)];
     u32 r = find_interior_in_bucket(bs, level, bucket, n, node);
     if (r == NOT_FOUND) r = add_interior(level, n, node, hash);

@@ -119,10 +119,19 @@ int orc_trace_shadows(void* dag, const double params[12], uint32_t W, uint32_t H
 /* edits: kind 0 sphere add, 1 sphere carve, 2 box add, 3 box carve, 4 sphere paint, 6 fill.
  * stats out: num nodes, num voxels, colour leaves rebuilt. colors may be NULL. */
 int orc_edit(void* dag, void* colors, uint32_t kind, const float center[3], float radius, uint64_t stats[3]);
+/* the same with the edit tasks (one per touched colour subtree, hash_dag.h:330-356) run on
+ * `threads` threads under striped bucket locks; canonically the same result */
+int orc_edit_threads(void* dag, void* colors, uint32_t kind, const float center[3], float radius, int threads,
+                     uint64_t stats[3]);
 /* copy editor (CopyEditorWithoutDecompression, identity transform): box of `radius` around
  * dest + (center - source), filled from source + (p - dest) of the DAG before the edit */
 int orc_edit_copy(void* dag, void* colors, const float center[3], float radius, const float source[3],
                   const float dest[3], uint64_t stats[3]);
+/* kind 5 = the copy above, 7 = CopyEditorWithDecompression (hash_dag_editors.h:437-616): the
+ * source box read through a voxel buffer of 2 ceil(radius) per axis from truncate(center -
+ * radius), nodes above levels - 4 whose source region is empty left unedited */
+int orc_edit_copy_kind(void* dag, void* colors, uint32_t kind, const float center[3], float radius,
+                       const float source[3], const float dest[3], uint64_t stats[3]);
 /* the same with the copy's Matrix3x3 transform (row-major, FP64) and swirl (swirl != 0: the
  * relative position turns about y by angle y / swirl_period, hash_dag_editors.h:364-379) */
 int orc_edit_copy_tf(void* dag, void* colors, const float center[3], float radius, const float source[3],

@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstring>
 #include <cmath>
+#include <mutex>
 #include <vector>
 #include <string>
 #include <unordered_map>
@@ -154,8 +155,9 @@ constexpr u32 C_IgnoreColorIndex = 0xFFFFFFFu;
 constexpr u32 C_RemoveColorIndex = 0xEFFFFFFu;
 
 // Paged bucketed hash table, CPU layout: pool of physical pages + page table
-// (hash_table.h:137-898, hash_table.cpp:5-71). Single-threaded: no locks, no Bloom filter
-// (the Bloom filter is result-neutral, hash_table.h:563-659).
+// (hash_table.h:137-898, hash_table.cpp:5-71). No Bloom filter (result-neutral,
+// hash_table.h:563-659). find_or_add takes a bucket's lock when `locks` is set (threaded edit
+// tasks, as the reference's per-bucket mutexes); pages are then numbered under `pool_lock`.
 struct HashTable {
     Geometry g;
     std::vector<u32> pool;         // pool_max_pages * page_size
@@ -165,6 +167,8 @@ struct HashTable {
     u32 pool_top = 1;              // page 0 is never used (hash_table.h:821)
     u32 pool_max_pages = 0;
     bool overflow = false;         // bucket or pool overflow observed (the reference does not check)
+    std::vector<std::mutex>* locks = nullptr;  // striped bucket locks while edit tasks run threaded
+    std::mutex pool_lock;
 
     void create(const Geometry& geom, u32 pool_pages);
     u32* ptr(u32 va) { return &pool[(u64)page_table[va / g.page_size] * g.page_size + va % g.page_size]; }
@@ -290,7 +294,10 @@ F3 sun_direction();
 // ---------------------------------------------------------------------------------------------
 // Editors (hash_dag_editors.h) and edits (hash_dag_edits.h, hash_dag.h:255-405)
 // ---------------------------------------------------------------------------------------------
-enum EditKind : u32 { SPHERE_ADD = 0, SPHERE_CARVE = 1, BOX_ADD = 2, BOX_CARVE = 3, SPHERE_PAINT = 4, COPY = 5, FILL = 6 };
+enum EditKind : u32 {
+    SPHERE_ADD = 0, SPHERE_CARVE = 1, BOX_ADD = 2, BOX_CARVE = 3, SPHERE_PAINT = 4, COPY = 5, FILL = 6,
+    COPY_DEC = 7  // CopyEditorWithDecompression (hash_dag_editors.h:437-616)
+};
 struct Edit {
     u32 kind;
     F3 center;
@@ -308,7 +315,7 @@ struct EditStats { u64 numNodes = 0, numVoxels = 0, leavesRebuilt = 0; };
 struct UndoFrame { u32 root; std::vector<std::pair<u32, std::pair<ColorLeaf, ColorLeaf>>> leaves; };
 
 // One edit on (dag, colors); colors may be null (EDITS_ENABLE_COLORS=0).
-EditStats apply_edit(DAG& dag, HashDAGColors* colors, const Edit& e, UndoFrame* undo);
+EditStats apply_edit(DAG& dag, HashDAGColors* colors, const Edit& e, UndoFrame* undo, int threads = 1);
 
 // GC (hash_dag.h:407-547); deterministic sorted iteration instead of unordered_set order.
 void remove_stale_nodes(DAG& dag, u32 maxLevel);

@@ -382,3 +382,24 @@ def test_color_lookup_matches_canonical_order():
         assert ok and (bits, wb) == cols[v]
     ok, bits, wb = od.color(oc, 0, 127, 0)  # empty (above the terrain)
     assert not ok and (bits, wb) == (0, 0)
+
+
+def test_threaded_edit_tasks_same_result():
+    """The edit tasks run on several threads under striped bucket locks (the reference's thread
+    pool with per-bucket mutexes, hash_dag.h:330-356) give the canonical DAG, colours and counts
+    of the single-threaded run."""
+    p, sc, g = scene("d10")
+    runs = []
+    for threads in (1, 6):
+        od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+        oc = OracleColors.from_scene(sc)
+        rng = np.random.default_rng(5)
+        stats = []
+        for _ in range(8):
+            kind = int(rng.integers(0, 5))
+            center = tuple(float(v) for v in rng.integers(60, 450, size=3))
+            stats.append(od.edit(kind, center, float(rng.integers(6, 60)), oc, threads=threads)[1:])
+        runs.append((od.canonical(), oc.canonical(od), oc.leaf_bytes(od), stats))
+    a, b = runs
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert a[3] == b[3]
