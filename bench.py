@@ -175,8 +175,10 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"2^{D} coloured simplex terrain, {W}x{H} primary paths + colours + 1 shadow ray per pixel, "
-                        f"{len(cams)} seeded cameras (one camera per step)",
-            "levels": D, "width": W, "height": H, "cameras": len(cams), "seed": args.seed,
+                        f"one camera per step: cameras 0..{min(args.steps, len(cams)) - 1} of {len(cams)} seeded "
+                        f"cameras" + (f" (cycled {args.steps / len(cams):.2f} times)" if args.steps > len(cams) else ""),
+            "levels": D, "width": W, "height": H, "cameras_timed": min(args.steps, len(cams)),
+            "cameras_generated": len(cams), "seed": args.seed,
             "voxels": int(scene.num_voxels), "tile": args.tile,
             "bucket_geometry": {"num_top_levels": geom.num_top_levels,
                                 "top": f"{1 << geom.bucket_bits_top} x {geom.bucket_size_top} words",
@@ -479,21 +481,46 @@ def c5_bench(args):
 
 
 def cpu_edit_baseline(od, oc, params, args):
-    """The oracle's single-threaded edit path (a C++ restatement of edit_threads with colour
-    rebuild) on the first edits of the same list, until the time budget is used."""
-    from hashdag_amd.workload import mixed_edits
+    """The oracle's edit path (a C++ restatement of edit_threads) on the CPU-plain workload: the
+    2^12 binary terrain (colour tree down to the leaf level: no colour leaves), 100 sphere edits
+    (50 add / 50 carve, radius uniform 8-64) at seeded surface points, one at a time, timed like
+    the reference's "edits" stat (hash_dag.h:262, 368). The edit tasks run on one thread and on
+    all host cores (the reference's thread pool, NUM_THREADS, typedefs.h:54-56); both runs must
+    reach the same canonical DAG. The sample is bounded by --edit-cpu-seconds."""
+    import numpy as np
 
-    edits = mixed_edits(params, args.edits, seed=args.seed + 1)
-    voxels, spent, n = 0, 0.0, 0
-    for kind, center, r in edits:
-        if spent >= args.edit_cpu_seconds:
-            break
-        t = time.perf_counter()
-        voxels += od.edit(kind, center, r, oc)[1]
-        spent += time.perf_counter() - t
-        n += 1
-    return {"value": round(voxels / spent, 1), "unit": "edited voxels/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} edits of the same list on the 2^{params.levels} scene ({voxels} voxels) in {spent:.1f}s"}
+    from hashdag_amd import Geometry, TerrainParams, generate_terrain
+    from hashdag_amd.workload import sphere_edits
+    from oracle import OracleDAG
+
+    p1 = TerrainParams(levels=12, seed=args.seed, colors=False, color_tree_levels=10)
+    sc = generate_terrain(p1)
+    g = Geometry.fit(12, sc.level_words, color_tree_levels=10)
+    edits = sphere_edits(p1, 100, seed=args.seed + 9)
+    threads = len(os.sched_getaffinity(0))
+    out = {}
+    canon = []
+    for th in (1, threads):
+        od1 = OracleDAG.from_basic(g, sc.dag, 1 << 19)
+        voxels, spent, n = 0, 0.0, 0
+        for kind, center, r in edits:
+            if spent >= args.edit_cpu_seconds / 2:
+                break
+            t = time.perf_counter()
+            voxels += od1.edit(kind, center, r, None, threads=th)[1]
+            spent += time.perf_counter() - t
+            n += 1
+        out[th] = (voxels / spent, n, voxels, spent)
+        if n == len(edits):
+            canon.append(od1.canonical())
+        del od1
+    same = len(canon) == 2 and bool(np.array_equal(canon[0], canon[1]))
+    v1, vn = out[1], out[threads]
+    return {"value": round(vn[0], 1), "unit": "edited voxels/s", "cores": threads, "kind": "port",
+            "single_thread_value": round(v1[0], 1),
+            "sample": f"C1: 2^12 binary terrain, {vn[1]} of 100 sphere edits (50 add / 50 carve, radius 8-64), "
+                      f"{vn[2]} voxels in {vn[3]:.2f}s on {threads} threads; 1 thread: {v1[1]} edits, {v1[2]} voxels "
+                      f"in {v1[3]:.2f}s" + ("; both reach the same canonical DAG" if same else "")}
 
 
 def measured_traffic(kernel: str, levels: int, W: int, H: int):
