@@ -156,6 +156,8 @@ hd_status hd_create(const hd_config* cfg, hd_ctx** out) {
     if ((e = hipMalloc(&ctx->vpool, vwords * 4)) != hipSuccess) return cleanup(hip_fail(nullptr, e, "vpool"));
     if ((e = hipMalloc(&ctx->page_table, g.total_pages() * 4)) != hipSuccess) return cleanup(hip_fail(nullptr, e, "page table"));
     if ((e = hipMalloc(&ctx->bucket_sizes, g.total_buckets() * 4)) != hipSuccess) return cleanup(hip_fail(nullptr, e, "buckets"));
+    if ((e = hipMalloc(&ctx->bucket_snapshot, g.total_buckets() * 4)) != hipSuccess)
+        return cleanup(hip_fail(nullptr, e, "bucket snapshot"));
     if ((e = hipMalloc(&ctx->full_nodes, g.levels * 4)) != hipSuccess) return cleanup(hip_fail(nullptr, e, "full nodes"));
     if ((e = hipMalloc(&ctx->counters, 64 * 4)) != hipSuccess) return cleanup(hip_fail(nullptr, e, "counters"));
     if ((e = hipMalloc(&ctx->byte_counters, 8 * sizeof(unsigned long long))) != hipSuccess)
@@ -180,6 +182,7 @@ hd_status hd_destroy(hd_ctx* ctx) {
     hipFree(ctx->vpool);
     hipFree(ctx->page_table);
     hipFree(ctx->bucket_sizes);
+    hipFree(ctx->bucket_snapshot);
     hipFree(ctx->full_nodes);
     hipFree(ctx->counters);
     hipFree(ctx->byte_counters);

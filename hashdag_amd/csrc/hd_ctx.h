@@ -32,6 +32,7 @@ struct hd_ctx {
     uint32_t* vpool = nullptr;          // total_virtual + kVpoolPad words
     uint32_t* page_table = nullptr;     // total_pages
     uint32_t* bucket_sizes = nullptr;   // total_buckets
+    uint32_t* bucket_snapshot = nullptr;  // total_buckets: sizes before the running edit pass
     uint32_t* full_nodes = nullptr;     // levels
     uint32_t* counters = nullptr;       // [0] pool_top, [1] error flags, [2..] scratch counters
     uint32_t root = 0;

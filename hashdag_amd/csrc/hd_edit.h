@@ -15,6 +15,9 @@ namespace hd {
 
 constexpr uint32_t EDIT_SPHERE_ADD = 0, EDIT_SPHERE_CARVE = 1, EDIT_BOX_ADD = 2, EDIT_BOX_CARVE = 3,
                    EDIT_SPHERE_PAINT = 4, EDIT_COPY = 5, EDIT_FILL = 6;
+// kind of the C ABI's decompressing copy (hd_edit.kind); its DevEdit has kind EDIT_COPY, dec = 1
+constexpr uint32_t EDIT_COPY_DEC = 7;
+inline bool is_copy_kind(uint32_t kind) { return kind == EDIT_COPY || kind == EDIT_COPY_DEC; }
 
 // error bits reported by a pass
 constexpr uint32_t ERR_SCRATCH = 1, ERR_BUCKET = 2, ERR_POOL = 4, ERR_HEADER = 8, ERR_COLOR_SCRATCH = 16,
@@ -46,7 +49,13 @@ struct DevEdit {
     const uint64_t* fill_colors;
     uint32_t fill_size;
     uint32_t fill_lo[3], fill_dim[3];
-    uint32_t pad2;
+    // copy with decompression (kind COPY with dec = 1, hash_dag_editors.h:437-616): the source
+    // box's float minimum center - radius, its truncation, and the side of its voxel buffer
+    uint32_t dec;
+    float cmin[3];
+    uint32_t icmin[3];
+    uint32_t dsize;
+    uint32_t pad3;
 };
 
 // The leading 64 bytes of a DevEdit: what the sphere / box editors' tests and colours read,

@@ -298,6 +298,62 @@ __device__ __forceinline__ void copy_source(const DevEdit& e, float px, float py
     sz = f2u_x86(e.src[2] + tz);
 }
 
+// source voxel of a destination voxel corner for either copy: destination_to_source for the
+// copy without decompression; for the decompressing copy the voxel buffer's cell
+// truncate(position - dest + source - copyMin), read at intCopyMin + cell, and false outside
+// the buffer (get_new_value / get_new_color keep the old voxel, hash_dag_editors.h:582-611)
+__device__ __forceinline__ bool copy_source_ok(const DevEdit& e, float px, float py, float pz, uint32_t& sx,
+                                               uint32_t& sy, uint32_t& sz) {
+    if (!e.dec) {
+        copy_source(e, px, py, pz, sx, sy, sz);
+        return true;
+    }
+    const uint32_t x = f2u_x86(px - e.dst[0] + e.src[0] - e.cmin[0]);
+    const uint32_t y = f2u_x86(py - e.dst[1] + e.src[1] - e.cmin[1]);
+    const uint32_t z = f2u_x86(pz - e.dst[2] + e.src[2] - e.cmin[2]);
+    if (!(x < e.dsize && y < e.dsize && z < e.dsize)) return false;
+    sx = e.icmin[0] + x;
+    sy = e.icmin[1] + y;
+    sz = e.icmin[2] + z;
+    return true;
+}
+
+// DAGUtils::is_empty (dag_utils.h:174-252) on the DAG before the pass, for the region
+// [start, start + size) and a node level maxLevel >= 1: the recursion keeps the nodes above
+// maxLevel whose inclusive box [min, min + 2^shift - 1] passes the overlap test (min < start +
+// size and min + 2^shift - 1 > start per axis) and reports "not empty" as soon as it reaches a
+// node of maxLevel. Every ancestor of an overlapping node overlaps too, so the region is empty
+// exactly when no overlapping node of level maxLevel - 1 exists with a child. Per axis those
+// nodes are a run of at most two indices, found directly; each candidate is one root-to-node
+// walk.
+__device__ bool source_region_empty(const uint32_t* __restrict__ vpool, uint32_t root, uint32_t levels,
+                                    uint32_t maxLevel, uint32_t sx, uint32_t sy, uint32_t sz, uint32_t size) {
+    const uint32_t k = maxLevel - 1, shift = levels - k;
+    const uint32_t start[3] = {sx, sy, sz};
+    uint32_t lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = (uint32_t)(((uint64_t)start[a] + 1) >> shift);
+        const uint64_t h = ((uint64_t)start[a] + size - 1) >> shift;
+        hi[a] = (uint32_t)min<uint64_t>(h, (1ull << k) - 1);
+        if (lo[a] > hi[a]) return true;
+    }
+    for (uint32_t ax = lo[0]; ax <= hi[0]; ++ax)
+        for (uint32_t ay = lo[1]; ay <= hi[1]; ++ay)
+            for (uint32_t az = lo[2]; az <= hi[2]; ++az) {
+                uint32_t va = root;
+                bool exists = true;
+                for (uint32_t level = 0; level < k && exists; ++level) {
+                    const uint32_t b = k - level - 1;
+                    const uint32_t c = (((ax >> b) & 1u) << 2) | (((ay >> b) & 1u) << 1) | ((az >> b) & 1u);
+                    const uint32_t m = vpool[va] & 0xFFu;
+                    if (!(m & (1u << c))) exists = false;
+                    else va = vpool[va + __popc(m & ((1u << c) - 1u)) + 1u];
+                }
+                if (exists && (vpool[va] & 0xFFu) != 0u) return false;
+            }
+    return true;
+}
+
 // fill buffer voxel of a voxel corner: truncate(position - min) (hash_dag_editors.h:213-214);
 // returns the state byte (0 outside the buffer)
 __device__ __forceinline__ uint32_t fill_state_at(const DevEdit& e, float px, float py, float pz, uint64_t& idx) {
@@ -338,6 +394,20 @@ __device__ __forceinline__ uint32_t leaf_bit(uint32_t x, uint32_t y, uint32_t z)
 __device__ __forceinline__ uint32_t child_off(uint32_t mask, uint32_t c) { return __popc(mask & ((1u << c) - 1u)) + 1u; }
 
 __device__ __forceinline__ void set_error(uint32_t* flags, uint32_t bit) { atomicOr(flags, bit); }
+
+// CopyEditorWithDecompression::should_edit's empty check (COPY_EMPTY_CHECKS,
+// hash_dag_editors.h:555-577): a node above levels - 4 whose source region (its box moved by
+// source - dest, when that lies at non-negative coordinates) is empty in the DAG before the
+// pass is not edited
+__device__ __forceinline__ bool copy_source_empty(const EditPass& P, uint32_t ei, float x0, float y0, float z0,
+                                                  uint32_t level) {
+    const DevEdit& e = P.edits[ei];
+    if (!e.dec) return false;
+    const float ox = x0 - e.dst[0] + e.src[0], oy = y0 - e.dst[1] + e.src[1], oz = z0 - e.dst[2] + e.src[2];
+    if (!(ox >= 0 && oy >= 0 && oz >= 0)) return false;
+    return source_region_empty(P.vpool, P.items[0].orig[0], P.levels, level, f2u_x86(ox), f2u_x86(oy), f2u_x86(oz),
+                               1u << (P.levels - level));
+}
 
 // ---------------------------------------------------------------------------------------------
 // expand: items of level l -> child classification and items of level l+1
@@ -403,6 +473,8 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
                     if (u == 1 && !two) break;
                     const EditHead& h = u ? h1 : h0;
                     if (!e_touch(h, x0, y0, z0, x1, y1, z1)) continue;
+                    if (h.kind == EDIT_COPY && l + 1 + 4 < D && copy_source_empty(P, u ? e1 : e0, x0, y0, z0, l + 1))
+                        continue;
                     if (k + u < 64) touch |= 1ull << (k + u);
                     if (l + 1 >= ctl) {
                         const uint32_t cv = e_cover(h, x0, y0, z0, x1, y1, z1);
@@ -605,7 +677,7 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
                         const float fy = (float)(by | (((c1 >> 1) & 1u) << 1) | ((c2 >> 1) & 1u));
                         const float fz = (float)(bz | ((c1 & 1u) << 1) | (c2 & 1u));
                         uint32_t sx, sy, sz;
-                        copy_source(e, fx, fy, fz, sx, sy, sz);
+                        if (!copy_source_ok(e, fx, fy, fz, sx, sy, sz)) continue;
                         if ((sx >> 2) != cx || (sy >> 2) != cy || (sz >> 2) != cz) {
                             cx = sx >> 2;
                             cy = sy >> 2;
@@ -1386,7 +1458,7 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
                     if (!v) {
                         const DevEdit& e = P.edits[ei];
                         uint32_t sx, sy, sz;
-                        copy_source(e, fx, fy, fz, sx, sy, sz);
+                        if (!copy_source_ok(e, fx, fy, fz, sx, sy, sz)) return;
                         const uint32_t srcRoot = P.items[0].orig[0];
                         const uint64_t sl = source_leaf(P.vpool, srcRoot, P.levels, sx >> 2, sy >> 2, sz >> 2);
                         if ((sl >> leaf_bit(sx, sy, sz)) & 1ull) {

@@ -113,7 +113,19 @@ DevEdit make_edit(const hd_edit& e) {
     d.r2 = e.radius * e.radius;
     float box[3] = {e.center[0], e.center[1], e.center[2]};
     float boxRadius = e.radius;
-    if (e.kind == EDIT_COPY) {
+    if (e.kind == EDIT_COPY_DEC) {
+        // BoxEditorBase(center - source + dest, radius); values read through a buffer of
+        // 2 ceil(radius) voxels per axis from truncate(center - radius) (hash_dag_editors.h:455-466)
+        d.kind = EDIT_COPY;
+        d.dec = 1;
+        for (int i = 0; i < 3; ++i) {
+            box[i] = e.center[i] - e.source[i] + e.dest[i];
+            d.cmin[i] = e.center[i] - e.radius;
+            d.icmin[i] = f2u_host(d.cmin[i]);
+            d.inv[4 * i] = 1.0;  // identity: the transform and swirl do not apply
+        }
+        d.dsize = 2u * (uint32_t)std::ceil(e.radius);
+    } else if (e.kind == EDIT_COPY) {
         // BoxEditorBase(dest + float3(T * (center - source)), radius * max|T * (1,1,1)|)
         // (hash_dag_editors.h:350-351; Matrix3x3 in FP64, Matrix3x3.h:190-243, 399-406)
         double T[9];
@@ -155,7 +167,7 @@ DevEdit make_edit(const hd_edit& e) {
 uint64_t copy_voxels(const hd_edit* e, uint32_t n) {
     uint64_t sum = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        if (e[i].kind != EDIT_COPY) continue;
+        if (!is_copy_kind(e[i].kind)) continue;
         const DevEdit d = make_edit(e[i]);
         sum += (uint32_t)(uint32_t)(d.bmax[0] - d.bmin[0]) * (uint32_t)(d.bmax[1] - d.bmin[1]) *
                (uint32_t)(d.bmax[2] - d.bmin[2]);
@@ -276,6 +288,8 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
         hd_status st = grow_arena(ctx, 1u << 22);
         if (st != HD_OK) return st;
         EditScratch& S = ctx->edit;
+        const uint64_t bucketBytes = (uint64_t)ctx->g.total_buckets() * 4;
+        HD_HIP(ctx, hipMemcpyAsync(ctx->bucket_snapshot, ctx->bucket_sizes, bucketBytes, hipMemcpyDeviceToDevice, s));
         k_reset<<<1, 1, 0, s>>>(ctx->counters, S.dstats);
         st = enqueue_pass(ctx, first, n, colors, s, nullptr, special);
         if (st != HD_OK) return st;
@@ -292,9 +306,20 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
         const uint32_t err = rb.counters[1];
         if (err) {
             // The root was not switched. Leaf slots the pass already re-pointed get their
-            // previous leaves back; nodes it appended stay as unreachable garbage.
+            // previous leaves back. The buckets get their sizes from before the pass: what it
+            // appended is dropped (pages it mapped stay mapped, as after GC), so a retry or a
+            // later find never meets a node of a failed pass.
+            HD_HIP(ctx, hipMemcpyAsync(ctx->bucket_sizes, ctx->bucket_snapshot, bucketBytes, hipMemcpyDeviceToDevice, s));
+            HD_HIP(ctx, hipStreamSynchronize(s));
             if (rb.counters[4]) {
                 edit_restore_leaves(ctx, s);
+                HD_HIP(ctx, hipStreamSynchronize(s));
+            }
+            if (rb.counters[0] > ctx->cfg.pool_pages) {
+                // page allocations that failed still advanced the page counter: bring it back to
+                // the budget, which every page below it is part of
+                const uint32_t top = (uint32_t)ctx->cfg.pool_pages;
+                HD_HIP(ctx, hipMemcpyAsync(ctx->counters, &top, 4, hipMemcpyHostToDevice, s));
                 HD_HIP(ctx, hipStreamSynchronize(s));
             }
         }
@@ -372,9 +397,11 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
         return HD_OK;
     }
     for (uint32_t i = 0; i < n; ++i)
-        if (edits[i].kind > EDIT_FILL || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
+        if (edits[i].kind > EDIT_COPY_DEC || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
             return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");
     for (uint32_t i = 0; i < n; ++i) {
+        if (edits[i].kind == EDIT_COPY_DEC && !(edits[i].radius < 65536.f))
+            return fail(ctx, HD_ERR_INVALID, "copy radius too large for its voxel buffer");
         if (edits[i].kind != EDIT_COPY) continue;
         // Matrix3x3::IsInvertible (Matrix3x3.h:245-248)
         double T[9];
@@ -416,14 +443,14 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
         // pass; every other run of editors fuses into one pass.
         for (uint32_t first = 0; first < n;) {
             uint32_t last = first + 1;
-            while (last < n && edits[last].kind != EDIT_COPY && edits[last].kind != EDIT_FILL) ++last;
+            while (last < n && !is_copy_kind(edits[last].kind) && edits[last].kind != EDIT_FILL) ++last;
             UndoFrame frame;
             frame.prev_root = ctx->root;
             Readback rb{};
             st = prepare_fill(first);
             if (st != HD_OK) return st;
             st = run_pass(ctx, first, last - first, s, rb, &frame,
-                          edits[first].kind == EDIT_COPY || edits[first].kind == EDIT_FILL);
+                          is_copy_kind(edits[first].kind) || edits[first].kind == EDIT_FILL);
             if (st != HD_OK) return st;
             ctx->root = rb.root;
             frame.new_root = rb.root;
@@ -444,7 +471,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             Readback rb{};
             st = prepare_fill(i);
             if (st != HD_OK) return st;
-            st = run_pass(ctx, i, 1, s, rb, &frame, edits[i].kind == EDIT_COPY || edits[i].kind == EDIT_FILL);
+            st = run_pass(ctx, i, 1, s, rb, &frame, is_copy_kind(edits[i].kind) || edits[i].kind == EDIT_FILL);
             if (st != HD_OK) return st;
             ctx->root = rb.root;
             frame.new_root = rb.root;
@@ -561,6 +588,7 @@ static hd_status graph_capture(hd_frame_graph* g) {
     hipMemcpyAsync(g->d_n, g->h_n, 4, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(S.edits_dev, g->h_edits, (uint64_t)g->max_edits * sizeof(DevEdit), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(g->d_params, g->h_params, sizeof(hd_ray_params), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(ctx->bucket_snapshot, ctx->bucket_sizes, (uint64_t)ctx->g.total_buckets() * 4, hipMemcpyDeviceToDevice, s);
     k_reset<<<1, 1, 0, s>>>(ctx->counters, S.dstats);
     hd_status st = enqueue_pass(ctx, 0, g->max_edits, colors, s, g->d_n, false);  // kinds checked at capture
     k_count_voxels<<<std::min<uint32_t>(g->max_edits, 4096), 256, 0, s>>>(
@@ -636,6 +664,7 @@ hd_status hd_frame_graph_launch(hd_frame_graph* g, const hd_edit* edits, uint32_
                                 hd_edit_stats* stats, uint32_t* replayed) {
     if (!g || !params || (n && !edits)) return fail(g ? g->ctx : nullptr, HD_ERR_INVALID, "bad arguments");
     hd_ctx* ctx = g->ctx;
+    if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
     if (n > g->max_edits) return fail(ctx, HD_ERR_INVALID, "more edits than the graph was built for");
     for (uint32_t i = 0; i < n; ++i)
         if (edits[i].kind > EDIT_SPHERE_PAINT || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
@@ -655,7 +684,16 @@ hd_status hd_frame_graph_launch(hd_frame_graph* g, const hd_edit* edits, uint32_
     HD_HIP(ctx, hipStreamSynchronize(g->stream));
     const Readback& rb = *g->h_rb;
     if (rb.counters[1]) {
-        // a capacity ran out: roll the leaf slots back and run the frame eagerly (which grows it)
+        // a capacity ran out: drop what the pass appended, roll the leaf slots back and run the
+        // frame eagerly (which grows it)
+        HD_HIP(ctx, hipMemcpyAsync(ctx->bucket_sizes, ctx->bucket_snapshot, (uint64_t)ctx->g.total_buckets() * 4,
+                                   hipMemcpyDeviceToDevice, g->stream));
+        HD_HIP(ctx, hipStreamSynchronize(g->stream));
+        if (rb.counters[0] > ctx->cfg.pool_pages) {
+            const uint32_t top = (uint32_t)ctx->cfg.pool_pages;
+            HD_HIP(ctx, hipMemcpyAsync(ctx->counters, &top, 4, hipMemcpyHostToDevice, g->stream));
+            HD_HIP(ctx, hipStreamSynchronize(g->stream));
+        }
         if (rb.counters[4]) {
             edit_restore_leaves(ctx, g->stream);
             HD_HIP(ctx, hipStreamSynchronize(g->stream));

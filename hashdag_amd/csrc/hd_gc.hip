@@ -428,9 +428,16 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
     HD_HIP(ctx, hipMemcpyAsync(cnt.data(), counts, cnt.size() * 4, hipMemcpyDeviceToHost, s));
     e = hipStreamSynchronize(s);
     release();
-    if (e != hipSuccess) return hip_fail(ctx, e, "GC");
-    if (err) return fail(ctx, err & GC_ERR_POOL ? HD_ERR_POOL_FULL : HD_ERR_BUCKET_FULL,
-                         "GC failed (capacity); the table is no longer consistent");
+    if (e != hipSuccess || err) {
+        // levels were rewritten in place up to the failure: the table no longer holds the DAG,
+        // so every call on this context refuses until a DAG is loaded again
+        ctx->loaded = false;
+        ctx->undo.clear();
+        ctx->redo.clear();
+        if (e != hipSuccess) return hip_fail(ctx, e, "GC");
+        return fail(ctx, err & GC_ERR_POOL ? HD_ERR_POOL_FULL : HD_ERR_BUCKET_FULL,
+                    "GC failed (capacity); the table is no longer consistent: load a DAG again");
+    }
     ctx->root = remap(roots, newRoots, ctx->root);
     HD_HIP(ctx, hipMemcpyAsync(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice, s));
     for (uint32_t i = 0; i < n_extra; ++i) extra[i] = remap(roots, newRoots, extra[i]);

@@ -206,6 +206,12 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
     std::deque<uint32_t> queue;  // arrived, not started (arrival order)
     uint32_t finished = 0, batch = 0;
     hd_status result = HD_OK;
+    // a HIP error ends the run with its status; cleanup() below still runs
+    auto hip_ok = [&](hipError_t err, const char* what) {
+        if (err == hipSuccess) return true;
+        if (result == HD_OK) result = hip_fail(ctx, err, what);
+        return false;
+    };
 
     auto poll_slots = [&](bool wait_one) {
         for (;;) {
@@ -252,9 +258,10 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
             // finishes first, everything after starts after
             queue.pop_front();
             const double start = now_s();
-            HD_HIP(ctx, hipStreamSynchronize(editStream));
+            bool ok = hip_ok(hipStreamSynchronize(editStream), "serving: edit stream");
             for (Slot& sl : slots)
-                if (sl.req >= 0) HD_HIP(ctx, hipStreamSynchronize(sl.stream));
+                if (ok && sl.req >= 0) ok = hip_ok(hipStreamSynchronize(sl.stream), "serving: trace slot");
+            if (!ok) break;
             poll_slots(false);
             const hd_status st = reqs[head].kind == HD_REQ_UNDO ? hd_undo(ctx) : hd_redo(ctx);
             out[head].start_s = start;
@@ -278,8 +285,10 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
                 queue.pop_front();
             }
             // the pass waits on the device for the traces queued before it
+            bool ok = true;
             for (Slot& s : slots)
-                if (s.req >= 0) HD_HIP(ctx, hipStreamWaitEvent(editStream, s.done, 0));
+                if (ok && s.req >= 0) ok = hip_ok(hipStreamWaitEvent(editStream, s.done, 0), "serving: wait for traces");
+            if (!ok) break;
             const double start = now_s();
             for (uint32_t m : members) {
                 out[m].start_s = start;
@@ -287,14 +296,18 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
             }
             // admission by free pages: collect stale nodes below the watermark; reject a batch
             // whose estimate does not fit even after that
-            HD_HIP(ctx, hipStreamSynchronize(editStream));
+            if (!hip_ok(hipStreamSynchronize(editStream), "serving: edit stream")) break;
             hd_mem_info_t mem;
             hd_status st = hd_mem_info(ctx, &mem);
             const uint64_t need = estimate_pages(fused.data(), (uint32_t)fused.size(), ctx->g.page_size);
             uint32_t gcRan = 0;
+            // a GC that fails leaves the table unusable (the context refuses further calls), so
+            // it ends the run with its status rather than rejecting one batch
+            hd_status gcSt = HD_OK;
             if (st == HD_OK && mem.free_pages < need + cfg->reserve_pages) {
-                st = hd_gc_roots(ctx, ctx->g.leaf_level(), cfg->keep_undo, nullptr, 0, nullptr, editStream);
+                gcSt = hd_gc_roots(ctx, ctx->g.leaf_level(), cfg->keep_undo, nullptr, 0, nullptr, editStream);
                 gcRan = 1;
+                st = gcSt;
                 if (st == HD_OK) st = hd_mem_info(ctx, &mem);
                 if (st == HD_OK && mem.free_pages < need) st = HD_ERR_POOL_FULL;
             }
@@ -303,8 +316,9 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
             if ((st == HD_ERR_BUCKET_FULL || st == HD_ERR_POOL_FULL) && !gcRan) {
                 // a full bucket or page budget: collect stale nodes once and retry
                 gcRan = 1;
-                if (hd_gc_roots(ctx, ctx->g.leaf_level(), cfg->keep_undo, nullptr, 0, nullptr, editStream) == HD_OK)
-                    st = hd_edit_batch(ctx, fused.data(), (uint32_t)fused.size(), 1, &es, editStream);
+                gcSt = hd_gc_roots(ctx, ctx->g.leaf_level(), cfg->keep_undo, nullptr, 0, nullptr, editStream);
+                if (gcSt == HD_OK) st = hd_edit_batch(ctx, fused.data(), (uint32_t)fused.size(), 1, &es, editStream);
+                else st = gcSt;
             }
             if (st == HD_OK && !cfg->keep_undo) hd_undo_clear(ctx);
             const double done = now_s();
@@ -316,7 +330,8 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
                 out[m].gc = gcRan;
                 ++finished;
             }
-            if (st != HD_OK && st != HD_ERR_BUCKET_FULL && st != HD_ERR_POOL_FULL) result = st;
+            if (gcSt != HD_OK) result = gcSt;
+            else if (st != HD_OK && st != HD_ERR_BUCKET_FULL && st != HD_ERR_POOL_FULL) result = st;
             ++batch;
         } else {
             // run the traces at the head of the queue, one per free slot
@@ -347,8 +362,10 @@ hd_status hd_serve_run(hd_ctx* ctx, const hd_request* reqs, uint32_t n, const hd
                 k_zero64<<<1, 64, 0, free->stream>>>(free->sum);
                 k_checksum<<<(uint32_t)std::min<uint64_t>((px + 255) / 256, 2048), 256, 0, free->stream>>>(free->colors, px,
                                                                                                            free->sum);
-                HD_HIP(ctx, hipMemcpyAsync(free->hsum, free->sum, 8, hipMemcpyDeviceToHost, free->stream));
-                HD_HIP(ctx, hipEventRecord(free->done, free->stream));
+                if (!hip_ok(hipMemcpyAsync(free->hsum, free->sum, 8, hipMemcpyDeviceToHost, free->stream),
+                            "serving: checksum readback") ||
+                    !hip_ok(hipEventRecord(free->done, free->stream), "serving: trace event"))
+                    break;
                 out[q].voxels = px;
                 free->req = q;
                 launched = true;

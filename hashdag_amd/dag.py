@@ -230,10 +230,18 @@ class HashDAG:
     def edit(self, kind: int, center, radius: float) -> hd_edit_stats:
         return self.edit_batch([(kind, center, radius)])
 
-    def copy(self, center, radius: float, source, dest, transform=None, swirl_period=None) -> hd_edit_stats:
+    def copy(self, center, radius: float, source, dest, transform=None, swirl_period=None,
+             decompress: bool = False) -> hd_edit_stats:
         """CopyEditor (hash_dag_editors.h:332-435): the box of radius * max|T (1,1,1)| around
         dest + T (center - source) gets the voxels (and colours) at source + T^-1 (p - dest),
-        the relative position first swirled about y when swirl_period is given."""
+        the relative position first swirled about y when swirl_period is given.
+
+        decompress=True: CopyEditorWithDecompression (hash_dag_editors.h:437-616), no transform
+        or swirl; the source is read through its voxel buffer at truncate(center - radius)."""
+        if decompress:
+            if transform is not None or swirl_period is not None:
+                raise ValueError("the decompressing copy takes no transform or swirl")
+            return self.edit_batch([(7, center, radius, source, dest)])
         return self.edit_batch([(5, center, radius, source, dest, transform, swirl_period)])
 
     def remove_stale_nodes(self, max_level: int | None = None, stream=None, keep_undo: bool = False,

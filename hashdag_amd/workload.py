@@ -42,6 +42,7 @@ def surface_points(params: TerrainParams, n: int, seed: int = 2) -> np.ndarray:
 
 # editor kinds of the C ABI (hd_edit.kind)
 SPHERE_ADD, SPHERE_CARVE, BOX_ADD, BOX_CARVE, SPHERE_PAINT, COPY = range(6)
+FILL, COPY_DEC = 6, 7
 
 
 def _centre(rng, params: TerrainParams):
@@ -99,15 +100,16 @@ def instances(params: TerrainParams, n: int, seed: int = 6, hmin: float = 16.0, 
     return out
 
 
-def copy_edits(params: TerrainParams, n: int, seed: int = 7, radius: float = 256.0):
+def copy_edits(params: TerrainParams, n: int, seed: int = 7, radius: float = 256.0, decompress: bool = False):
     """n copies of the (2 radius)^3 region around a seeded surface point to another seeded
-    surface point: [(COPY, source, radius, source, dest), ...] (translation only)."""
+    surface point: [(COPY, source, radius, source, dest), ...] (translation only); with
+    decompress the kind is COPY_DEC (CopyEditorWithDecompression)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
         src = _centre(rng, params)
         dst = _centre(rng, params)
-        out.append((COPY, src, float(radius), src, dst))
+        out.append((COPY_DEC if decompress else COPY, src, float(radius), src, dst))
     return out
 
 

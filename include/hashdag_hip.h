@@ -173,6 +173,12 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
 #define HD_EDIT_SPHERE_PAINT 4u
 #define HD_EDIT_COPY 5u  /* CopyEditorWithoutDecompression with its transform and swirl (hash_dag_editors.h:332-435) */
 #define HD_EDIT_FILL 6u  /* FillEditorColors: flood fill of the empty voxels around center (hash_dag_editors.h:624-768) */
+/* CopyEditorWithDecompression (hash_dag_editors.h:437-616, the copy the checked-in
+ * script_definitions.h:2 selects): the box of radius around center - source + dest reads the
+ * voxels of the box of radius around center through a buffer of 2 ceil(radius) voxels per axis
+ * from truncate(center - radius); nodes above levels - 4 whose source region is empty are left
+ * unedited (COPY_EMPTY_CHECKS). No transform or swirl; radius below 65536. */
+#define HD_EDIT_COPY_DEC 7u
 typedef struct hd_edit {
     uint32_t kind;
     float center[3];

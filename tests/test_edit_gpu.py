@@ -64,9 +64,13 @@ def make(name):
 
 
 def assert_same(dag, sc, od, oc, what):
+    """Canonical DAG, decoded colours per subtree, and the stored words of every subtree's leaf
+    (blocks, weights, macro blocks of rebuilt leaves; the shared offset of the others)."""
     god, goc = gpu_state_in_oracle(dag, sc)
     assert np.array_equal(god.canonical(), od.canonical()), f"{what}: DAG differs"
     assert np.array_equal(goc.canonical(god), oc.canonical(od)), f"{what}: colours differ"
+    gb, ob = goc.leaf_bytes(god), oc.leaf_bytes(od)
+    assert gb.size == ob.size and np.array_equal(gb, ob), f"{what}: colour leaf words differ"
 
 
 @pytest.mark.parametrize("name", ["d8", "d13", "d10w"])
@@ -76,6 +80,7 @@ def test_plain_edits_match_oracle(name):
         st = dag.edit(kind, center, r)
         ost = od.edit(kind, center, r, oc)
         assert st.num_voxels == ost[1], f"edit {k}: voxel count {st.num_voxels} != {ost[1]}"
+        assert st.leaves_rebuilt == ost[2], f"edit {k}: leaves rebuilt {st.leaves_rebuilt} != {ost[2]}"
         assert_same(dag, sc, od, oc, f"edit {k} {(kind, center, r)}")
 
 
@@ -200,6 +205,37 @@ def test_copy_matches_oracle(fused):
     if fused:
         assert st.passes == 4  # each copy starts a new pass: [o o o] [C o o o] [C o o o] [C]
     assert_same(dag, sc, od, oc, f"copies (fused={fused})")
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_decompressing_copy_matches_oracle(fused):
+    """CopyEditorWithDecompression (hash_dag_editors.h:437-616) between other editors: the
+    buffer read from truncate(center - radius) with fractional radii and centres off the source,
+    and the empty-source checks above levels - 4 (copies reaching into the air above the
+    terrain and past the scene's low corner)."""
+    p, sc, g, dag, od, oc = make("d13")
+    from hashdag_amd.workload import COPY_DEC
+    others = random_edits(p, 6, seed=41)
+    copies = [(COPY_DEC, c, 36.0, s, d) for _, c, _, s, d in copy_edits(p, 3, seed=43)]
+    # a fractional radius, a centre off the source point, and a source box past the low corner
+    _, c0, _, s0, d0 = copies[0]
+    copies[0] = (COPY_DEC, (c0[0] + 3.0, c0[1] + 17.5, c0[2] - 2.0), 20.5, s0, d0)
+    _, c2, _, _, d2 = copies[2]
+    copies[2] = (COPY_DEC, (10.0, c2[1], 12.0), 33.25, (10.0, c2[1], 12.0), d2)
+    edits = []
+    for k in range(3):
+        edits += others[2 * k:2 * k + 2] + [copies[k]]
+    st = dag.edit_batch(edits, fused=fused)
+    total = 0
+    for ed in edits:
+        if ed[0] == COPY_DEC:
+            total += od.edit_copy(ed[1], ed[2], ed[3], ed[4], oc, decompress=True)[1]
+        else:
+            total += od.edit(ed[0], ed[1], ed[2], oc)[1]
+    assert st.num_voxels == total
+    if fused:
+        assert st.passes == 4
+    assert_same(dag, sc, od, oc, f"decompressing copies (fused={fused})")
 
 
 def _rotation(deg, axis):

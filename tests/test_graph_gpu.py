@@ -1,5 +1,7 @@
 """HIP-graph replays of edit batch + trace: every replay (with eager work and synchronisation
-between replays) must give the same frame, voxel count and final DAG as the eager calls."""
+between replays) must give the same frame, voxel count and final DAG as the eager calls, and
+the oracle applying the same edits one by one must trace the same frames and reach the same DAG
+and colour leaves."""
 import numpy as np
 import pytest
 
@@ -21,7 +23,11 @@ def test_graph_replays_match_eager(tiny_scratch):
     from hashdag_amd import DAGInfo, DAGTracer, HashDAG, ray_params
     from hashdag_amd.graph import FrameGraph
     from hashdag_amd.workload import cameras
+    from oracle import Oracle, OracleColors, OracleDAG
+    from tests.test_edit_gpu import assert_same
     p, sc, g = scene("d13")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
     W, H = 120, 80
     info = DAGInfo.unit(g.levels)
     ga = HashDAG(g, pool_pages=POOL_PAGES)
@@ -45,10 +51,16 @@ def test_graph_replays_match_eager(tiny_scratch):
         tb.resolve_shadows(cam, info, eb, 1.0, 0.0, params=prm)
         torch.cuda.synchronize()
         assert np.array_equal(ta.colors.cpu().numpy(), tb.colors.cpu().numpy()), f"frame {k} differs"
+        ovox = sum(od.edit(kind, c, r, oc)[1] for kind, c, r in edits)
         if sb is not None:
-            assert st.num_voxels == sb.num_voxels
+            assert st.num_voxels == sb.num_voxels == ovox
+        op = Oracle.ray_params(cam, info, g.levels, W, H)
+        opaths = Oracle.trace_paths(od, op, W, H)
+        oframe = Oracle.trace_shadows(od, op, opaths, Oracle.trace_colors(od, oc, opaths, W, H), W, H, 1.0, 0.0)
+        assert np.array_equal(ta.colors.cpu().numpy().view(np.uint32), oframe), f"frame {k} differs from the oracle"
     assert replays >= (3 if tiny_scratch else len(cams))
     assert np.array_equal(canonical(ga), canonical(eb))
+    assert_same(ga, sc, od, oc, "after the graph replays")
     # undo through graph-committed frames restores the scene
     for _ in range(ga.undo_depth()[0]):
         ga.undo()
