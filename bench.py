@@ -157,7 +157,20 @@ def main() -> int:
     e1.record(stream)
     torch.cuda.synchronize()
     stream_gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del src, dst
+    # and a read-only pass over 4 GiB (the tracer kernels read far more than they write)
+    rsrc = torch.cat([src, dst])
+    fold = torch.empty(1 << 16, dtype=torch.int32, device="cuda")
+    for _ in range(3):
+        lib.hd_stream_read(C.c_void_p(rsrc.data_ptr()), 2 * nbytes, C.c_void_p(fold.data_ptr()),
+                           C.c_void_p(stream.cuda_stream))
+    e0.record(stream)
+    for _ in range(reps):
+        lib.hd_stream_read(C.c_void_p(rsrc.data_ptr()), 2 * nbytes, C.c_void_p(fold.data_ptr()),
+                           C.c_void_p(stream.cuda_stream))
+    e1.record(stream)
+    torch.cuda.synchronize()
+    read_gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst, rsrc, fold
 
     value = W * H * args.steps / elapsed / 1e6
     result = {
@@ -198,6 +211,7 @@ def main() -> int:
             "traffic": measured_traffic(names[dom], D, W, H),
             "traffic_source": "profiles/trace_kernels_2p15_1080p.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "achievable_stream_gbs": round(stream_gbs, 1),
+            "achievable_read_gbs": round(read_gbs, 1),
         },
     }
 

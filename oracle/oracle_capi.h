@@ -17,8 +17,16 @@
  *   trace_paths       brute-force DDA nearest hit agrees except where the reference's DFS
  *                     order/epsilon differ (counted), mask properties, root rejection
  *                                                                tests/test_oracle_tracer.py
+ *                     exact hit order: a reference-order DFS over the dense occupancy
+ *                     pyramid with exact FP64 slab tests gives the same voxel on every ray
+ *                     away from a decision boundary              tests/test_oracle_pins.py
  *   trace_colors      colour index == Morton rank of the voxel in its level-10 subtree
  *                     (dense-grid brute force)                   tests/test_oracle_tracer.py
+ *                     sky / magenta / checker fallbacks in float32 tests/test_oracle_pins.py
+ *   encoder bytes     blocks, weight words and macro words == a Python encoder written from
+ *                     the format                                 tests/test_oracle_pins.py
+ *   paint colours     the quantised blend of each touched voxel's old colour, written out
+ *                     in float32                                 tests/test_oracle_pins.py
  *   trace_shadows     any-hit result == brute-force segment/voxel test on a dense grid;
  *                     the same result with either any-hit child order  tests/test_oracle_tracer.py
  *   edits             dense-grid brute force of sphere/box/paint voxel sets and colours, and of
@@ -30,8 +38,11 @@
  *                     unpinned (read from hash_dag_editors.h:364-379 only)
  *   GC                canonical form unchanged, idempotent, used words == reachable words +
  *                     full node + padding        tests/test_oracle_table.py, test_oracle_edits.py
- *   interior find     the reference's walk rule (start + n < min(bucket size, page end)) is
- *                     pinned only by reading hash_table.h:276-357: parity unpinned
+ *   interior find     the walk rule (start + n < min(bucket size, page end)) on synthetic
+ *                     buckets whose nodes end at page and bucket ends, against a walk written
+ *                     from hash_table.h:276-357                   tests/test_oracle_pins.py
+ *   decompressing copy dense-grid brute force with the buffer mapping and the empty-source
+ *                     checks                                     tests/test_oracle_copy_dec.py
  */
 #pragma once
 #include <stdint.h>

@@ -341,3 +341,76 @@ def test_interior_find_walk(seed, fill, pages):
         found += want is not None
     # the rule leaves some nodes unfound (the last node of a page, the bucket's tail)
     assert 0 < found < len(nodes)
+
+
+# ---------------------------------------------------------------------------------------------
+# SpherePaintEditor's quantised colour (hash_dag_editors.h:315-330, color_utils.h:9-60)
+# ---------------------------------------------------------------------------------------------
+
+
+def decode_compressed(bits, wb):
+    """CompressedColor::get_color (variable_weight_size_colors.h:144-154) in float32:
+    rgb101210 for 0 bits per weight, else lerp(rgb565 min, rgb565 max, w / (2^bpw - 1))."""
+    f = np.float32
+    w, bpw = wb & 0xFF, wb >> 8
+    if bpw == 0:
+        return [f(bits & 0x3FF) / f(1023), f((bits >> 10) & 0xFFF) / f(4095), f((bits >> 22) & 0x3FF) / f(1023)]
+    lo = [f(bits & 0x1F) / f(31), f((bits >> 5) & 0x3F) / f(63), f((bits >> 11) & 0x1F) / f(31)]
+    hi = [f((bits >> 16) & 0x1F) / f(31), f((bits >> 21) & 0x3F) / f(63), f((bits >> 27) & 0x1F) / f(31)]
+    t = f(w) / f((1 << bpw) - 1)
+    return [a * (f(1) - t) + b * t for a, b in zip(lo, hi)]
+
+
+def paint_expected(old_bits, old_wb, pos, center, radius):
+    """set_single_color(lerp(old, (0, 0, 1), clamp(1 - length(pos - center) / radius, 0, 0.1)))
+    with lerp(a, b, f) = a (1 - f) + b f and float3_to_rgb101210's clamp and truncation."""
+    f = np.float32
+    d = [f(p) - f(c) for p, c in zip(pos, center)]
+    length = np.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], dtype=np.float32)
+    k = f(1) - length / f(radius)
+    k = f(0) if k < f(0) else (f(0.1) if k > f(0.1) else k)
+    old = decode_compressed(old_bits, old_wb)
+    new = [a * (f(1) - k) + b * k for a, b in zip(old, (f(0), f(0), f(1)))]
+    cl = [min(max(c, f(0)), f(1)) for c in new]
+    return int(np.uint32(cl[0] * f(1023))) | (int(np.uint32(cl[1] * f(4095))) << 10) | \
+        (int(np.uint32(cl[2] * f(1023))) << 22)
+
+
+def test_paint_quantisation():
+    """Every voxel a paint sphere touches gets the quantised blend of its old colour (single or
+    rgb565 min / max with weights); the others keep theirs. Paint edits twice so the second
+    blends colours the first one wrote."""
+    from hashdag_amd import terrain_height
+    p, sc, g = scene("d8")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    grid = od.dense()
+    x, z = 120, 130
+    y = terrain_height(p, x, z)
+    checked = blended = 0
+    for center, radius in (((float(x), float(y), float(z)), 9.0), ((float(x + 3), float(y + 1), float(z - 2)), 6.5)):
+        lo = [int(math.floor(c - radius)) - 1 for c in center]
+        hi = [int(math.ceil(c + radius)) + 1 for c in center]
+        box = [tuple(v) for v in np.argwhere(grid[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]) + lo]
+        before = {v: od.color(oc, *v) for v in box}
+        od.edit(4, center, radius, oc)
+        for v in box:
+            s0, bits0, wb0 = before[v]
+            s1, bits1, wb1 = od.color(oc, *v)
+            assert s0 and s1
+            # should_edit on the voxel's unit box (SphereEditorBase, hash_dag_editors.h:112-126)
+            f = np.float32
+            dist = f(radius) * f(radius)
+            for a in range(3):
+                c, v0, v1 = f(center[a]), f(v[a]), f(v[a] + 1)
+                if c < v0:
+                    dist -= (c - v0) * (c - v0)
+                elif c > v1:
+                    dist -= (c - v1) * (c - v1)
+            if dist > 0:
+                assert (bits1, wb1) == (paint_expected(bits0, wb0, v, center, radius), 0), v
+                blended += int(bits1 != bits0 or wb0 != 0)
+            else:
+                assert (bits1, wb1) == (bits0, wb0), v
+            checked += 1
+    assert checked > 500 and blended > 200
