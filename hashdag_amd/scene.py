@@ -8,6 +8,7 @@ load_colors_from_DAG).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -49,7 +50,36 @@ class Scene:
     level_words: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
 
 
-def generate_terrain(params: TerrainParams) -> Scene:
+def _cache_path(params: TerrainParams, cache_dir: str) -> str:
+    key = (f"t{params.levels}_f{params.footprint}_s{params.seed}_o{params.octaves}_a{params.amplitude!r}"
+           f"_q{params.frequency!r}_c{params.color_tree_levels}_{int(params.colors)}_k{params.color_seed}")
+    return os.path.join(cache_dir, key + ".npz")
+
+
+def generate_terrain(params: TerrainParams, cache_dir: str | None = None) -> Scene:
+    """Builds the scene of `params`. cache_dir (default: $HD_SCENE_CACHE, unset = no cache)
+    keeps generated scenes as .npz files keyed by every parameter that shapes the scene, so
+    later processes (profiler passes of one benchmark) load instead of regenerating."""
+    cache_dir = cache_dir if cache_dir is not None else os.environ.get("HD_SCENE_CACHE")
+    if cache_dir:
+        path = _cache_path(params, cache_dir)
+        if os.path.exists(path):
+            with np.load(path) as z:
+                return Scene(params.levels, params.color_tree_levels, z["dag"], z["nodes"], z["offsets"],
+                             z["blocks"], z["weights"], z["macro"], int(z["num_voxels"]), z["level_nodes"], params,
+                             z["level_words"])
+        sc = _generate(params)
+        os.makedirs(cache_dir, exist_ok=True)
+        tmp = f"{path}.{os.getpid()}.tmp.npz"
+        np.savez(tmp, dag=sc.dag, nodes=sc.color_nodes, offsets=sc.color_offsets, blocks=sc.blocks,
+                 weights=sc.weights, macro=sc.macro, num_voxels=np.uint64(sc.num_voxels),
+                 level_nodes=sc.level_nodes, level_words=sc.level_words)
+        os.replace(tmp, path)
+        return sc
+    return _generate(params)
+
+
+def _generate(params: TerrainParams) -> Scene:
     p = params.to_c()
     handle = C.c_void_p()
     check(lib.hd_scene_generate(C.byref(p), C.byref(handle)))
