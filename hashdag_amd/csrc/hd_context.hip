@@ -169,6 +169,9 @@ hd_status hd_create(const hd_config* cfg, hd_ctx** out) {
     hipMemsetAsync(ctx->full_nodes, 0, g.levels * 4, ctx->stream);
     hipMemsetAsync(ctx->counters, 0, 64 * 4, ctx->stream);
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return cleanup(hip_fail(nullptr, e, "init"));
+    // the static edit scratch of this device (hd_edit_batch_static), looked up once per device
+    const hd_status st = static_scratch_setup(ctx);
+    if (st != HD_OK) return cleanup(fail(nullptr, st, ctx->err));
     *out = ctx;
     return HD_OK;
 }

@@ -69,6 +69,8 @@ constexpr uint64_t kVpoolPad = 64;  // words after the virtual space (vector rea
 constexpr uint32_t kArenaTopWord = 8;  // counters[8..9]: committed colour arena top (64-bit)
 
 void edit_free(hd_ctx* ctx);
+// device address of the static edit scratch on ctx's device, looked up once per device
+hd_status static_scratch_setup(hd_ctx* ctx);
 hd_status fail(hd_ctx* ctx, hd_status code, const std::string& msg);
 hd_status hip_fail(hd_ctx* ctx, hipError_t e, const char* what);
 DevDag dev_dag(const hd_ctx* ctx);

@@ -175,4 +175,8 @@ struct EditScratch {
     std::vector<void*> allocs;
 };
 
+// Lays out one level's item arrays and edit list from `mem` (nullptr: sizes only); returns
+// the bytes they take (hd_edit.hip).
+uint64_t carve_level(Items& I, uint32_t*& elist, bool leafLevel, uint64_t c, uint64_t e, char* mem);
+
 }  // namespace hd

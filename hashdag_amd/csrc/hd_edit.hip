@@ -1881,15 +1881,15 @@ static void scratch_free(hd_ctx* ctx) {
     S.undo_cap = ucap;
 }
 
-// Carves the item arrays of one level (capacity c) and its edit list (e words) out of one
-// allocation; the leaf level has no child-slot arrays.
-static hipError_t alloc_level(EditScratch& S, uint32_t l, bool leafLevel, uint64_t c, uint64_t e) {
+// Lays the item arrays of one level (capacity c) and its edit list (e words) out from `mem`
+// (nullptr: sizes them only) in 256-byte aligned fields; returns the bytes they take. The
+// leaf level has no child-slot arrays.
+uint64_t carve_level(Items& I, uint32_t*& elist, bool leafLevel, uint64_t c, uint64_t e, char* mem) {
     const uint64_t c8 = leafLevel ? 0 : 8 * c;
     struct Field {
         void** p;
         uint64_t bytes;
     };
-    Items& I = S.items[l];
     I = Items{};
     const Field f[] = {
         {(void**)&I.px, c * 4},       {(void**)&I.py, c * 4},      {(void**)&I.pz, c * 4},
@@ -1902,20 +1902,25 @@ static hipError_t alloc_level(EditScratch& S, uint32_t l, bool leafLevel, uint64
         {(void**)&I.costart, c8 * 4}, {(void**)&I.nidx, c * 4},    {(void**)&I.ncount, c * 4},
         {(void**)&I.cont, c * 36},    {(void**)&I.clen, c * 4},    {(void**)&I.hash, c * 4},
         {(void**)&I.rep, c * 4},      {(void**)&I.state, c * 4},   {(void**)&I.nstart, c * 4},
-        {(void**)&I.ostart, c * 4},   {(void**)&I.cbase, c * 8},   {(void**)&S.elist[l], e * 4},
+        {(void**)&I.ostart, c * 4},   {(void**)&I.cbase, c * 8},   {(void**)&elist, e * 4},
     };
     auto up = [](uint64_t v) { return (v + 255) & ~255ull; };
-    uint64_t total = 0;
-    for (const Field& x : f) total += up(x.bytes);
-    char* mem = nullptr;
-    const hipError_t err = hipMalloc(&mem, std::max<uint64_t>(total, 256));
-    if (err != hipSuccess) return err;
-    S.level_mem[l] = mem;
     uint64_t off = 0;
     for (const Field& x : f) {
-        *x.p = x.bytes ? mem + off : nullptr;
+        *x.p = (mem && x.bytes) ? mem + off : nullptr;
         off += up(x.bytes);
     }
+    return std::max<uint64_t>(off, 256);
+}
+
+// One allocation per level holding its item arrays and edit list.
+static hipError_t alloc_level(EditScratch& S, uint32_t l, bool leafLevel, uint64_t c, uint64_t e) {
+    const uint64_t total = carve_level(S.items[l], S.elist[l], leafLevel, c, e, nullptr);
+    char* mem = nullptr;
+    const hipError_t err = hipMalloc(&mem, total);
+    if (err != hipSuccess) return err;
+    S.level_mem[l] = mem;
+    carve_level(S.items[l], S.elist[l], leafLevel, c, e, mem);
     S.item_cap[l] = (uint32_t)c;
     S.elist_cap[l] = (uint32_t)e;
     return hipSuccess;
@@ -2039,8 +2044,7 @@ void edit_free(hd_ctx* ctx) {
     S = EditScratch{};
 }
 
-static void fill_pass(hd_ctx* ctx, EditPass& P) {
-    EditScratch& S = ctx->edit;
+static void fill_pass(hd_ctx* ctx, EditPass& P, const EditScratch& S) {
     P.g = ctx->g;
     P.levels = ctx->g.levels;
     P.ctl = ctx->g.color_tree_levels;
@@ -2091,10 +2095,11 @@ static void fill_pass(hd_ctx* ctx, EditPass& P) {
 
 // Enqueues one pass over edits [first, first + n) of the uploaded edit array.
 // dN (optional): the edit count is read from device memory (at most n), for graph replays.
+// scratch (optional): the scratch the pass runs in (default: the context's own).
 hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s, const uint32_t* dN,
-                       bool special) {
+                       bool special, const EditScratch* scratch) {
     EditPass P{};
-    fill_pass(ctx, P);
+    fill_pass(ctx, P, scratch ? *scratch : ctx->edit);
     P.single_edit = (n == 1 && !dN) ? 1u : 0u;
     const uint32_t D = P.levels, L = D - 2;
     k_init_pass<<<(std::max<uint32_t>(n, 1) + 255) / 256, 256, 0, s>>>(P, 0xFFFFFFFFu, first, n, dN);
@@ -2134,8 +2139,8 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
 }
 
 // After a failed pass: give every leaf slot the pass touched its previous leaf back.
-void edit_restore_leaves(hd_ctx* ctx, hipStream_t s) {
-    const EditScratch& S = ctx->edit;
+void edit_restore_leaves(hd_ctx* ctx, hipStream_t s, const EditScratch* scratch) {
+    const EditScratch& S = scratch ? *scratch : ctx->edit;
     k_restore_leaves<<<256, 256, 0, s>>>(ctx->cleaves, S.undo_handles, S.undo_descs, ctx->counters + 4, S.undo_cap);
 }
 
@@ -2217,7 +2222,7 @@ hd_status fill_prepare(hd_ctx* ctx, const hd_edit& e, DevEdit& d, FillBuffers& f
         if (err == hipSuccess) err = hipMemcpyAsync(dxyz, loaded.data(), loaded.size() * 4, hipMemcpyHostToDevice, s);
         if (err == hipSuccess) {
             EditPass P{};
-            fill_pass(ctx, P);
+            fill_pass(ctx, P, ctx->edit);
             k_fill_colors<<<std::min<uint32_t>((nl + 255) / 256, 4096), 256, 0, s>>>(P, ctx->d_root, dxyz, nl, dcol);
             err = hipGetLastError();
         }

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "hd_ctx.h"
@@ -15,14 +16,14 @@ namespace hd {
 hd_status edit_reserve(hd_ctx* ctx, const uint32_t* items, const uint32_t* elist, uint64_t colors);
 void edit_free(hd_ctx* ctx);
 hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s,
-                       const uint32_t* dN = nullptr, bool special = true);
+                       const uint32_t* dN = nullptr, bool special = true, const EditScratch* scratch = nullptr);
 hd_status edit_ensure_edits(hd_ctx* ctx, uint32_t n);
 hd_status enqueue_frame_from_device(hd_ctx* ctx, const hd_ray_params* d_params, uint32_t W, uint32_t H,
                                     uint32_t* d_paths, uint32_t* d_colors, float bias, float fog, hipStream_t s);
 hd_status edit_reserve_undo(hd_ctx* ctx, uint64_t entries);
 hd_status edit_scatter_leaves(hd_ctx* ctx, const uint32_t* handles, const LeafDesc* descs, uint32_t n);
 hd_status fill_prepare(hd_ctx* ctx, const hd_edit& e, DevEdit& d, FillBuffers& fb, hipStream_t s);
-void edit_restore_leaves(hd_ctx* ctx, hipStream_t s);
+void edit_restore_leaves(hd_ctx* ctx, hipStream_t s, const EditScratch* scratch = nullptr);
 hipError_t edit_gather_next(hd_ctx* ctx, uint32_t n, LeafDesc* host, hipStream_t s);
 
 namespace {
@@ -280,18 +281,21 @@ static hd_status edit_prepare(hd_ctx* ctx) {
     return edit_reserve_undo(ctx, kUndoCap);
 }
 
-// `special`: the pass holds a copy or a fill editor (leaf kernels built with their code)
+// `special`: the pass holds a copy or a fill editor (leaf kernels built with their code).
+// `fixed`: run in this scratch and grow nothing (the colour arena, tables and undo log
+// included); a pass that needs more fails with the previous state kept.
 static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s, Readback& rb, UndoFrame* frame,
-                          bool special) {
+                          bool special, const EditScratch* fixed = nullptr) {
     const bool colors = ctx->colors_loaded && ctx->g.leaf_level() > ctx->g.color_tree_levels;
     for (int attempt = 0; attempt < 40; ++attempt) {
-        hd_status st = grow_arena(ctx, 1u << 22);
+        hd_status st = HD_OK;
+        if (!fixed) st = grow_arena(ctx, 1u << 22);
         if (st != HD_OK) return st;
-        EditScratch& S = ctx->edit;
+        const EditScratch& S = fixed ? *fixed : ctx->edit;
         const uint64_t bucketBytes = (uint64_t)ctx->g.total_buckets() * 4;
         HD_HIP(ctx, hipMemcpyAsync(ctx->bucket_snapshot, ctx->bucket_sizes, bucketBytes, hipMemcpyDeviceToDevice, s));
         k_reset<<<1, 1, 0, s>>>(ctx->counters, S.dstats);
-        st = enqueue_pass(ctx, first, n, colors, s, nullptr, special);
+        st = enqueue_pass(ctx, first, n, colors, s, nullptr, special, fixed);
         if (st != HD_OK) return st;
         if (n > 1) {
             k_count_voxels<<<std::min<uint32_t>(n, 4096), 256, 0, s>>>(S.edits_dev + first, n, nullptr,
@@ -312,7 +316,7 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
             HD_HIP(ctx, hipMemcpyAsync(ctx->bucket_sizes, ctx->bucket_snapshot, bucketBytes, hipMemcpyDeviceToDevice, s));
             HD_HIP(ctx, hipStreamSynchronize(s));
             if (rb.counters[4]) {
-                edit_restore_leaves(ctx, s);
+                edit_restore_leaves(ctx, s, fixed);
                 HD_HIP(ctx, hipStreamSynchronize(s));
             }
             if (rb.counters[0] > ctx->cfg.pool_pages) {
@@ -322,6 +326,12 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
                 HD_HIP(ctx, hipMemcpyAsync(ctx->counters, &top, 4, hipMemcpyHostToDevice, s));
                 HD_HIP(ctx, hipStreamSynchronize(s));
             }
+        }
+        if (fixed) {
+            if (err & (ERR_SCRATCH | ERR_COLOR_SCRATCH | ERR_UNDO))
+                return fail(ctx, HD_ERR_OOM, "edit batch too large for the static scratch (previous state kept)");
+            if (err & (ERR_COLOR_ARENA | ERR_COLOR_TREE))
+                return fail(ctx, HD_ERR_COLOR_FULL, "colour arena or tables full; a pass without a fixed scratch grows them (previous state kept)");
         }
         if (err & (ERR_SCRATCH | ERR_COLOR_SCRATCH)) {
             // The device counters hold what the pass asked for. Levels below the first one that
@@ -542,6 +552,144 @@ hd_status hd_reserve_edit_scratch(hd_ctx* ctx, uint32_t items_per_level, uint32_
         elist[l] = edit_list_words;
     }
     return edit_reserve(ctx, items, elist, colors);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Edit passes that allocate nothing: a fused pass run in a file-scope device array instead of
+// the context's growable scratch, for callers that cannot call hipMalloc (code under graph
+// capture, serving loops, harnesses that forbid it). Each GPU holds its own copy of the array
+// until the process ends; every call on a GPU uses that GPU's copy, so calls that overlap on
+// one GPU race.
+// ---------------------------------------------------------------------------------------------
+namespace hd {
+namespace {
+
+constexpr uint64_t kStaticScratchBytes = 32ull << 20;
+constexpr uint32_t kStaticItems = 4096;          // work items per level (level l: at most 8^l)
+constexpr uint32_t kStaticElist = 32768;         // edit-list words per level
+constexpr uint64_t kStaticColors = 1ull << 20;   // staged colours
+constexpr uint32_t kStaticUndo = 1u << 14;       // leaf slots one pass may re-point
+constexpr uint32_t kStaticEdits = HD_STATIC_MAX_EDITS;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+std::mutex g_scratch_mu;
+// device address of `scratch` per device ordinal, from hipGetSymbolAddress (the array's name
+// is a host-side stand-in in host code, never a device address)
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch view of a pass at depth `levels` over the array at `base`; the bytes it takes.
+uint64_t static_layout(EditScratch& S, uint32_t levels, char* base) {
+    S = EditScratch{};
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) -> char* {
+        char* p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~255ull;
+        return p;
+    };
+    S.levels = levels;
+    S.dcount = reinterpret_cast<uint32_t*>(take(64 * 4));
+    S.decount = reinterpret_cast<uint32_t*>(take(64 * 4));
+    S.dstats = reinterpret_cast<uint64_t*>(take(64 * 8));
+    S.edits_dev = reinterpret_cast<DevEdit*>(take((uint64_t)kStaticEdits * sizeof(DevEdit)));
+    S.edits_cap = kStaticEdits;
+    S.undo_handles = reinterpret_cast<uint32_t*>(take((uint64_t)kStaticUndo * 4));
+    S.undo_descs = reinterpret_cast<LeafDesc*>(take((uint64_t)kStaticUndo * sizeof(LeafDesc)));
+    S.undo_next = reinterpret_cast<LeafDesc*>(take((uint64_t)kStaticUndo * sizeof(LeafDesc)));
+    S.undo_cap = kStaticUndo;
+    S.hcap = 2 * kStaticItems;  // a power of two at least twice any level's items
+    S.hset = reinterpret_cast<uint32_t*>(take((uint64_t)S.hcap * 4));
+    S.colors = reinterpret_cast<uint64_t*>(take(kStaticColors * 8));
+    S.cmarks = reinterpret_cast<uint64_t*>(take(kStaticColors / 64 * 8));
+    S.color_cap = kStaticColors;
+    const uint32_t L = levels - 2;
+    for (uint32_t l = 0; l <= L; ++l) {
+        const uint64_t c = l < 4 ? std::min<uint64_t>(1ull << (3 * l), kStaticItems) : kStaticItems;
+        const uint64_t bytes = carve_level(S.items[l], S.elist[l], l == L, c, kStaticElist, nullptr);
+        char* mem = take(bytes);
+        if (base) carve_level(S.items[l], S.elist[l], l == L, c, kStaticElist, mem);
+        S.item_cap[l] = (uint32_t)c;
+        S.elist_cap[l] = kStaticElist;
+    }
+    return off;
+}
+
+}  // namespace
+
+// Looks up (once per device, with that device current) the device address of this library's
+// static edit scratch on ctx's device.
+hd_status static_scratch_setup(hd_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) return fail(ctx, HD_ERR_INVALID, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[ctx->device]) return HD_OK;
+    int cur = -1;
+    HD_HIP(ctx, hipGetDevice(&cur));
+    if (cur != ctx->device) return fail(ctx, HD_ERR_INVALID, "the context's device is not the current device");
+    void* ptr = nullptr;
+    HD_HIP(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(ctx, HD_ERR_HIP, "static edit scratch has no device address");
+    g_scratch_ptr[ctx->device] = static_cast<char*>(ptr);
+    return HD_OK;
+}
+
+}  // namespace hd
+
+extern "C" {
+
+hd_status hd_edit_batch_static(hd_ctx* ctx, const hd_edit* edits, uint32_t n, hd_edit_stats* stats, void* stream) {
+    if (!ctx || (!edits && n)) return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
+    if (n > kStaticEdits) return fail(ctx, HD_ERR_INVALID, "more edits than HD_STATIC_MAX_EDITS");
+    if (ctx->g.levels > kMaxEditLevels) return fail(ctx, HD_ERR_INVALID, "levels above the static scratch's 17");
+    for (uint32_t i = 0; i < n; ++i)
+        if (edits[i].kind > EDIT_SPHERE_PAINT || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
+            return fail(ctx, HD_ERR_INVALID, "static passes take sphere / box / paint editors with a finite radius");
+    if (!ctx->colors_loaded) return fail(ctx, HD_ERR_INVALID, "no colour tree (hd_load_colors, or one hd_edit_batch call, sets it up)");
+    if (n == 0) {
+        if (stats) *stats = hd_edit_stats{0, 0, 0, ctx->root, 0, 0};
+        return HD_OK;
+    }
+    hd_status st = static_scratch_setup(ctx);
+    if (st != HD_OK) return st;
+    char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = g_scratch_ptr[ctx->device];
+    }
+    EditScratch S;
+    if (static_layout(S, ctx->g.levels, nullptr) > kStaticScratchBytes)
+        return fail(ctx, HD_ERR_INVALID, "static edit scratch layout exceeds the array");
+    static_layout(S, ctx->g.levels, base);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // editors to the device in chunks staged on the stack
+    constexpr uint32_t kChunk = 32;
+    DevEdit dev[kChunk];
+    for (uint32_t i = 0; i < n; i += kChunk) {
+        const uint32_t m = std::min(kChunk, n - i);
+        for (uint32_t k = 0; k < m; ++k) dev[k] = make_edit(edits[i + k]);
+        HD_HIP(ctx, hipMemcpyAsync(S.edits_dev + i, dev, (uint64_t)m * sizeof(DevEdit), hipMemcpyHostToDevice, s));
+        HD_HIP(ctx, hipStreamSynchronize(s));
+    }
+    HD_HIP(ctx, hipMemcpyAsync(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice, s));
+    Readback rb{};
+    st = run_pass(ctx, 0, n, s, rb, nullptr, false, &S);
+    if (st != HD_OK) return st;
+    ctx->root = rb.root;
+    // the pass re-pointed leaf slots without an undo frame: earlier frames no longer apply
+    ctx->undo.clear();
+    ctx->redo.clear();
+    hd_edit_stats out{};
+    out.num_voxels = rb.stats[0];
+    out.new_words = rb.stats[1];
+    out.leaves_rebuilt = rb.counters[4];
+    out.colors_encoded = rb.stats[4];
+    out.passes = 1;
+    out.root = ctx->root;
+    if (stats) *stats = out;
+    return HD_OK;
 }
 
 }  // extern "C"

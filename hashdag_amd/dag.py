@@ -227,6 +227,17 @@ class HashDAG:
         check(lib.hd_edit_batch(self._h, arr, len(edits), 1 if fused else 0, C.byref(st), stream), self._h)
         return st
 
+    def edit_batch_static(self, edits, stream=None) -> hd_edit_stats:
+        """One fused pass of at most HD_STATIC_MAX_EDITS sphere / box / paint editors in the
+        library's static device scratch: allocates nothing, records no undo frame and clears
+        the undo history."""
+        arr = (hd_edit * max(len(edits), 1))()
+        for i, ed in enumerate(edits):
+            fill_edit(arr[i], ed)
+        st = hd_edit_stats()
+        check(lib.hd_edit_batch_static(self._h, arr, len(edits), C.byref(st), stream), self._h)
+        return st
+
     def edit(self, kind: int, center, radius: float) -> hd_edit_stats:
         return self.edit_batch([(kind, center, radius)])
 

@@ -205,6 +205,18 @@ typedef struct hd_edit_stats {
  * previous root is kept. One undo frame per pass. */
 hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t fused, hd_edit_stats* stats,
                         void* stream);
+/* No counterpart in the reference: hd_edit_batch(fused = 1) for callers that cannot allocate
+ * device memory (code under graph capture, serving loops, harnesses that forbid hipMalloc).
+ * One fused pass of at most HD_STATIC_MAX_EDITS sphere / box / paint editors, run in a 32 MiB
+ * file-scope device array of the library (one per GPU, held until the process ends) instead
+ * of the context's growable scratch; nothing is allocated. Limits per pass: 4096 work items
+ * per level, 32768 edit-list words per level, 2^20 staged colours, 16384 rebuilt colour
+ * leaves; a pass that needs more, or more colour arena / colour tree room than the context
+ * holds, returns HD_ERR_OOM / HD_ERR_COLOR_FULL with the DAG as it was. Needs a colour tree
+ * (hd_load_colors, or any earlier hd_edit_batch). The pass is not recorded in the undo
+ * history, which it clears. Calls that overlap on one GPU share its array and race. */
+#define HD_STATIC_MAX_EDITS 256u
+hd_status hd_edit_batch_static(hd_ctx* ctx, const hd_edit* edits, uint32_t n, hd_edit_stats* stats, void* stream);
 /* HashDAGUndoRedo::undo / redo (hash_dag.h:68-120) and free (133-145). */
 hd_status hd_undo(hd_ctx* ctx);
 hd_status hd_redo(hd_ctx* ctx);
