@@ -44,6 +44,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=str, default="auto", help="wavefront tile WxH, or auto (autotuned)")
+    ap.add_argument("--schedule", type=str, default="auto",
+                    help="tracer schedule: tile (one wavefront per tile), persist[:N] (persistent wavefronts "
+                         "refilled at N idle lanes), or auto (the autotuner's pick with --tile auto, else tile)")
     ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
     ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
     ap.add_argument("--no-edits", action="store_true")
@@ -97,6 +100,14 @@ def main() -> int:
     else:
         tw, th = (int(v) for v in args.tile.split("x"))
         tracer.set_tile(dag, tw, th)
+    if args.schedule.startswith("persist"):
+        refill = int(args.schedule.split(":")[1]) if ":" in args.schedule else 16
+        tracer.set_schedule(dag, True, refill)
+    elif args.schedule == "tile":
+        tracer.set_schedule(dag, False)
+    from hashdag_amd.tracer import launch_config
+    launch = launch_config(dag)
+    log(f"tracer launch: {launch}")
 
     stream = torch.cuda.current_stream()
 
@@ -192,7 +203,8 @@ def main() -> int:
                         f"cameras" + (f" (cycled {args.steps / len(cams):.2f} times)" if args.steps > len(cams) else ""),
             "levels": D, "width": W, "height": H, "cameras_timed": min(args.steps, len(cams)),
             "cameras_generated": len(cams), "seed": args.seed,
-            "voxels": int(scene.num_voxels), "tile": args.tile,
+            "voxels": int(scene.num_voxels), "tile": args.tile, "fetch": launch["fetch"],
+            "schedule": launch["schedule"],
             "bucket_geometry": {"num_top_levels": geom.num_top_levels,
                                 "top": f"{1 << geom.bucket_bits_top} x {geom.bucket_size_top} words",
                                 "low": f"{1 << geom.bucket_bits_low} x {geom.bucket_size_low} words",

@@ -203,6 +203,8 @@ _SIGS = {
                                       C.c_void_p, C.c_void_p, C.c_uint32, u32p, u32p, C.POINTER(C.c_float)]),
     "hd_set_trace_tile": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "hd_set_trace_fetch": (C.c_int32, [C.c_void_p, C.c_uint32]),
+    "hd_set_trace_schedule": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "hd_get_trace_schedule": (C.c_int32, [C.c_void_p, u32p]),
     "hd_get_trace_launch": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "hd_trace_count_bytes": (C.c_int32, [C.c_void_p, C.POINTER(hd_ray_params), C.c_uint32, C.c_uint32, C.c_void_p,
                                          C.c_void_p, C.c_float, C.c_float, u64p]),

@@ -94,6 +94,8 @@ static hd_status upload_image_impl(hd_ctx* ctx, const uint32_t* pool, const uint
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(dphys);
     if (e != hipSuccess) return hip_fail(ctx, e, "upload image");
+    const hd_status st = leaf_tags_rebuild(ctx, ctx->stream);
+    if (st != HD_OK) return st;
     ctx->root = root;
     ctx->loaded = true;
     return HD_OK;
@@ -163,6 +165,8 @@ hd_status hd_create(const hd_config* cfg, hd_ctx** out) {
     if ((e = hipMalloc(&ctx->byte_counters, 8 * sizeof(unsigned long long))) != hipSuccess)
         return cleanup(hip_fail(nullptr, e, "byte counters"));
     if ((e = hipMalloc(&ctx->d_root, 64)) != hipSuccess) return cleanup(hip_fail(nullptr, e, "root"));
+    if ((e = hipMalloc(&ctx->leaf_tags, std::max<uint64_t>(leaf_tag_bytes(g), 4))) != hipSuccess)
+        return cleanup(hip_fail(nullptr, e, "leaf tags"));
     hipMemsetAsync(ctx->vpool, 0, vwords * 4, ctx->stream);
     hipMemsetAsync(ctx->page_table, 0, g.total_pages() * 4, ctx->stream);
     hipMemsetAsync(ctx->bucket_sizes, 0, g.total_buckets() * 4, ctx->stream);
@@ -189,6 +193,8 @@ hd_status hd_destroy(hd_ctx* ctx) {
     hipFree(ctx->full_nodes);
     hipFree(ctx->counters);
     hipFree(ctx->byte_counters);
+    trace_scratch_free(ctx);
+    hipFree(ctx->leaf_tags);
     hipFree(ctx->cnodes);
     hipFree(ctx->coffsets);
     hipFree(ctx->cleaves);

@@ -12,6 +12,17 @@
 #include "hd_internal.h"
 
 namespace hd {
+// Scratch of one stream's persistent traces: tile queue heads, and per pixel tile the packed
+// shadow rays (origin + pixel), their lit / shadowed colours and their count.
+struct TraceScratch {
+    hipStream_t stream = nullptr;
+    uint32_t* queues = nullptr;
+    uint4* rec = nullptr;
+    uint2* shades = nullptr;
+    uint32_t* cnt = nullptr;
+    uint64_t tiles_cap = 0;
+};
+
 // One undo step (HashDAGUndoRedo::Frame, hash_dag.h:29-45): the roots before / after and the
 // colour-leaf descriptors the pass replaced (old leaf data stays in the colour arena).
 struct UndoFrame {
@@ -35,6 +46,9 @@ struct hd_ctx {
     uint32_t* bucket_snapshot = nullptr;  // total_buckets: sizes before the running edit pass
     uint32_t* full_nodes = nullptr;     // levels
     uint32_t* counters = nullptr;       // [0] pool_top, [1] error flags, [2..] scratch counters
+    // leaf level: one tag byte per 2-word slot of every bucket (bits 24-31 of the leaf's hash),
+    // valid below each bucket's size; finds scan the tags before comparing leaves
+    uint8_t* leaf_tags = nullptr;
     uint32_t root = 0;
     bool loaded = false;
 
@@ -55,6 +69,13 @@ struct hd_ctx {
     uint32_t tile_w = 8, tile_h = 8;
     uint32_t fetch_mode = 0;  // tracer node fetch: 0 = pointer then header, 1 = whole node
     unsigned long long* byte_counters = nullptr;  // 3 counters for instrumented launches
+    // tracer schedule: 0 = one wavefront per pixel tile, 1 = persistent wavefronts fed from
+    // tile queues, refilled once refill_at lanes are idle
+    uint32_t trace_persist = 0, refill_at = 16;
+    uint32_t persist_blocks[4] = {};   // resident blocks per persistent kernel (0: not measured)
+    // scratch of the persistent schedule, one set per stream traced on (traces on different
+    // streams of one context may overlap)
+    std::vector<hd::TraceScratch> trace_scratch;
 
     // edits
     uint32_t* d_root = nullptr;  // device copy of the root (edit passes chain through it)
@@ -71,6 +92,16 @@ constexpr uint32_t kArenaTopWord = 8;  // counters[8..9]: committed colour arena
 void edit_free(hd_ctx* ctx);
 // device address of the static edit scratch on ctx's device, looked up once per device
 hd_status static_scratch_setup(hd_ctx* ctx);
+// the persistent tracer's scratch for stream s, grown to hold a W x H frame (allocates: not
+// under stream capture); trace_scratch_free releases every stream's set
+hd_status trace_reserve(hd_ctx* ctx, hipStream_t s, uint32_t W, uint32_t H, TraceScratch** out = nullptr);
+void trace_scratch_free(hd_ctx* ctx);
+// recomputes the leaf tags of every leaf-level bucket from the table (after uploads and GC)
+hd_status leaf_tags_rebuild(hd_ctx* ctx, hipStream_t s);
+// bytes of the leaf tag array of a geometry
+inline uint64_t leaf_tag_bytes(const Geom& g) {
+    return (uint64_t)g.buckets_per_level(g.leaf_level()) * (g.bucket_size(g.leaf_level()) / 2);
+}
 hd_status fail(hd_ctx* ctx, hd_status code, const std::string& msg);
 hd_status hip_fail(hd_ctx* ctx, hipError_t e, const char* what);
 DevDag dev_dag(const hd_ctx* ctx);

@@ -17,7 +17,13 @@ constexpr uint32_t EDIT_SPHERE_ADD = 0, EDIT_SPHERE_CARVE = 1, EDIT_BOX_ADD = 2,
                    EDIT_SPHERE_PAINT = 4, EDIT_COPY = 5, EDIT_FILL = 6;
 // kind of the C ABI's decompressing copy (hd_edit.kind); its DevEdit has kind EDIT_COPY, dec = 1
 constexpr uint32_t EDIT_COPY_DEC = 7;
+// SphereNoiseEditor adding / carving (hd_edit.kind); both DevEdits have kind EDIT_NOISE (the
+// buffer they read holds the voxels' values after the edit)
+constexpr uint32_t EDIT_NOISE = 8, EDIT_NOISE_ADD = 8, EDIT_NOISE_CARVE = 9;
 inline bool is_copy_kind(uint32_t kind) { return kind == EDIT_COPY || kind == EDIT_COPY_DEC; }
+inline bool is_noise_kind(uint32_t kind) { return kind == EDIT_NOISE_ADD || kind == EDIT_NOISE_CARVE; }
+// editors that read a voxel buffer built from the DAG before their pass (each starts a pass)
+inline bool is_buffer_kind(uint32_t kind) { return kind == EDIT_FILL || is_noise_kind(kind); }
 
 // error bits reported by a pass
 constexpr uint32_t ERR_SCRATCH = 1, ERR_BUCKET = 2, ERR_POOL = 4, ERR_HEADER = 8, ERR_COLOR_SCRATCH = 16,
@@ -42,9 +48,9 @@ struct DevEdit {
     float swirl_period;
     uint32_t pad;
     double inv[9];       // copy: inverse of the copy's transform (row-major, FP64)
-    // fill: per buffer voxel (x + size * (y + size * z), buffer origin src[]) bit 0 = the voxel
-    // is set after the fill, bit 1 = the flood fill set it; colours of the fill's bounding box
-    // (lo, dim) for the voxels it set
+    // fill and noise: per buffer voxel (x + size * (y + size * z), buffer origin src[]) bit 0 =
+    // the voxel is set after the edit, bit 1 = the flood fill set it; colours of the fill's
+    // bounding box (lo, dim) for the voxels it set
     const uint8_t* fill_state;
     const uint64_t* fill_colors;
     uint32_t fill_size;
@@ -77,11 +83,14 @@ static_assert(offsetof(DevEdit, single) == offsetof(EditHead, single) && offseto
 struct FillBuffers {
     uint8_t* state = nullptr;
     uint64_t* colors = nullptr;
+    uint8_t* tables = nullptr;  // noise: the permutation tables
     void release() {
         hipFree(state);
         hipFree(colors);
+        hipFree(tables);
         state = nullptr;
         colors = nullptr;
+        tables = nullptr;
     }
     ~FillBuffers() { release(); }
 };
@@ -118,6 +127,7 @@ struct EditPass {
     const uint32_t* full_nodes;
     uint32_t* pool_top;
     uint32_t* errors;
+    uint8_t* leaf_tags;  // tag byte per leaf slot (leaf level), kept by k_add, read by k_find
     uint32_t* n_cnodes;
     uint32_t* n_leaves;
     uint32_t* undo_count;

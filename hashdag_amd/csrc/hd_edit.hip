@@ -28,6 +28,8 @@
 // round trip and can be captured in a HIP graph.
 #include <hip/hip_runtime.h>
 
+#include <random>
+
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -150,7 +152,8 @@ __device__ __forceinline__ float tmaxf2(float a, float b) { return a > b ? a : b
 // hash_dag_editors.h:76-91 / 112-126
 template <class E>
 __device__ __forceinline__ bool e_touch(const E& e, float x0, float y0, float z0, float x1, float y1, float z1) {
-    if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE || e.kind == EDIT_COPY || e.kind == EDIT_FILL) {
+    if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE || e.kind == EDIT_COPY || e.kind == EDIT_FILL ||
+        e.kind == EDIT_NOISE) {
         return !(e.bmin[0] >= x1 || e.bmin[1] >= y1 || e.bmin[2] >= z1 || e.bmax[0] <= x0 || e.bmax[1] <= y0 ||
                  e.bmax[2] <= z0 || x0 >= e.bmax[0] || y0 >= e.bmax[1] || z0 >= e.bmax[2] || x1 <= e.bmin[0] ||
                  y1 <= e.bmin[1] || z1 <= e.bmin[2]);
@@ -362,6 +365,14 @@ __device__ __forceinline__ uint32_t fill_state_at(const DevEdit& e, float px, fl
     if (x >= S || y >= S || z >= S) return 0u;
     idx = (uint64_t)x + (uint64_t)S * y + (uint64_t)S * S * z;
     return e.fill_state[idx];
+}
+// SphereNoiseEditor::get_new_color (hash_dag_editors.h:794-808): rgb565 min / max (precomputed
+// in `single`), 3 bits of weight |p - c| / boxRadius - 0.3 (radius holds the box radius 1.5 r)
+__device__ __forceinline__ uint64_t noise_color(const DevEdit& e, float px, float py, float pz) {
+    const float dx = px - e.c[0], dy = py - e.c[1], dz = pz - e.c[2];
+    const float len = __fsqrt_rn(dx * dx + dy * dy + dz * dz);
+    const int32_t w = (int32_t)((__fdiv_rn(len, e.radius) - 0.3f) * 7.0f);
+    return pack_color(e.single, (uint32_t)(w < 0 ? 0 : (w > 7 ? 7 : w)), 3u);
 }
 __device__ __forceinline__ uint64_t fill_color_at(const DevEdit& e, uint64_t idx) {
     const uint64_t S = e.fill_size;
@@ -592,7 +603,8 @@ __device__ __forceinline__ constexpr uint32_t leaf_voxel_bit(uint32_t xi, uint32
 template <class E>
 __device__ __forceinline__ uint64_t leaf_touch_mask(const E& e, uint32_t bx, uint32_t by, uint32_t bz) {
     const uint32_t base[3] = {bx, by, bz};
-    if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE || e.kind == EDIT_COPY || e.kind == EDIT_FILL) {
+    if (e.kind == EDIT_BOX_ADD || e.kind == EDIT_BOX_CARVE || e.kind == EDIT_COPY || e.kind == EDIT_FILL ||
+        e.kind == EDIT_NOISE) {
         uint32_t in[3];  // per axis: bit i = offset i overlaps
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -659,7 +671,7 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
             for (uint32_t k = 0; k < ec; ++k) {
                 const uint32_t ei = P.elist[L][es + k];
                 const EditHead h = load_head(P.edits, ei);
-                if (!kSpecial && (h.kind == EDIT_COPY || h.kind == EDIT_FILL)) {
+                if (!kSpecial && (h.kind == EDIT_COPY || h.kind == EDIT_FILL || h.kind == EDIT_NOISE)) {
                     set_error(P.errors, ERR_KIND);
                     break;
                 }
@@ -689,7 +701,7 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
                     v |= add;
                     continue;  // a copy's "num voxels" is its box volume, counted on the host
                 }
-                if (kSpecial && h.kind == EDIT_FILL) {
+                if (kSpecial && (h.kind == EDIT_FILL || h.kind == EDIT_NOISE)) {
                     // BufferEditor::get_new_value: set where the buffer voxel is set
                     uint64_t cand = tm & ~v, add = 0;
                     while (cand) {
@@ -874,7 +886,26 @@ __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
         const uint32_t bs = P.bucket_sizes[g.bucket_global(l, bucket)];
         const uint32_t nw = I.clen[i];
         uint32_t found = ~0u;
-        if (leafLevel) {
+        if (leafLevel && P.leaf_tags) {
+            // slots (2 words each) whose tag byte matches, lowest first, confirmed on the leaf;
+            // 4 tags per lane, 256 slots per wavefront step
+            const uint64_t key = (uint64_t)I.cont[i * 9] | ((uint64_t)I.cont[i * 9 + 1] << 32);
+            const uint32_t want = I.hash[i] >> 24, nslots = bs / 2;
+            const uint8_t* tags = P.leaf_tags + (uint64_t)bucket * (g.bucket_size(l) / 2);
+            for (uint32_t s0 = 0; s0 < nslots && found == ~0u; s0 += 256) {
+                const uint32_t sl = s0 + 4 * lane;
+                uint32_t hitSlot = ~0u;
+                if (sl < nslots) {
+                    const uint32_t four = *reinterpret_cast<const uint32_t*>(tags + sl);
+                    for (uint32_t k = 0; k < 4 && hitSlot == ~0u; ++k)
+                        if (sl + k < nslots && ((four >> (8 * k)) & 0xFFu) == want &&
+                            *reinterpret_cast<const uint64_t*>(P.vpool + base + 2 * (sl + k)) == key)
+                            hitSlot = sl + k;
+                }
+                const uint64_t b = __ballot(hitSlot != ~0u);
+                if (b) found = 2u * (uint32_t)__shfl((int)hitSlot, __ffsll((long long)b) - 1, 64);
+            }
+        } else if (leafLevel) {
             const uint64_t key = (uint64_t)I.cont[i * 9] | ((uint64_t)I.cont[i * 9 + 1] << 32);
             for (uint32_t p0 = 0; p0 < bs && found == ~0u; p0 += 128) {
                 const uint32_t pos = p0 + 2 * lane;
@@ -966,6 +997,8 @@ __global__ __launch_bounds__(256) void k_add(EditPass P, uint32_t l) {
                 const uint32_t va = g.make_ptr(l, bucket, start);
                 if (start % ps != 0 || alloc_page(P, va / ps)) {
                     for (uint32_t k = 0; k < nw; ++k) P.vpool[va + k] = I.cont[i * 9 + k];
+                    if (P.leaf_tags && l == g.leaf_level())
+                        P.leaf_tags[(uint64_t)bucket * (cap / 2) + start / 2] = (uint8_t)(I.hash[i] >> 24);
                     I.nidx[i] = va;
                     added = nw;
                 }
@@ -1294,7 +1327,7 @@ __global__ __launch_bounds__(256) void k_color_leaf_items(EditPass P) {
         bool paint = false;
         for (uint32_t k = 0; k < ec; ++k) {
             const uint32_t kind = P.edits[P.elist[L][es + k]].kind;
-            if (kind == EDIT_COPY || kind == EDIT_FILL) set_error(P.errors, ERR_KIND);
+            if (kind == EDIT_COPY || kind == EDIT_FILL || kind == EDIT_NOISE) set_error(P.errors, ERR_KIND);
             paint |= kind == EDIT_SPHERE_PAINT;
         }
         if (!v) continue;
@@ -1448,6 +1481,19 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
                                 col = fill_color_at(e, idx);
                                 haveCol = true;
                             }
+                            v = true;
+                        }
+                    }
+                    return;
+                }
+                if (h.kind == EDIT_NOISE) {
+                    // BufferEditor::get_new_value, SphereNoiseEditor::get_new_color: a voxel that
+                    // becomes set takes the editor's colour
+                    if (!v) {
+                        uint64_t idx = 0;
+                        if (fill_state_at(P.edits[ei], fx, fy, fz, idx) & 1u) {
+                            col = noise_color(P.edits[ei], fx, fy, fz);
+                            haveCol = true;
                             v = true;
                         }
                     }
@@ -1805,6 +1851,83 @@ __global__ __launch_bounds__(256) void k_fill_load(const uint32_t* __restrict__ 
     }
 }
 
+// ---- noise editor preparation (SphereNoiseEditor, hash_dag_editors.h:762-792) ----
+// FastNoise's 3D Perlin fractal with the editor's settings (FastNoise.cpp:195-205, 328-333,
+// 760-797, 841-888): float, quintic interpolation, 3 octaves of lacunarity 2 and gain 0.5;
+// tables = m_perm (512 bytes) then m_perm12 (512 bytes), staged in LDS
+__device__ __forceinline__ int noise_floor(float f) { return f >= 0 ? (int)f : (int)f - 1; }
+__device__ __forceinline__ float noise_lerp(float a, float b, float t) { return a + t * (b - a); }
+__device__ __forceinline__ float noise_quintic(float t) { return t * t * t * (t * (t * 6 - 15) + 10); }
+__device__ __forceinline__ float noise_grad(const uint8_t* perm, const uint8_t* perm12, uint32_t offset, int x, int y,
+                                            int z, float dx, float dy, float dz) {
+    // the 12 gradients (FastNoise.cpp:48-66): x = +-1 for the first 8, y = +-1 for 0-3 and 8-11,
+    // z = +-1 from 4 on
+    const uint32_t k = perm12[(x & 0xff) + perm[(y & 0xff) + perm[(z & 0xff) + offset]]];
+    const float gx = k < 8 ? ((k & 1u) ? -1.f : 1.f) : 0.f;
+    const float gy = k < 4 ? ((k & 2u) ? -1.f : 1.f) : (k < 8 ? 0.f : ((k & 1u) ? -1.f : 1.f));
+    const float gz = k < 4 ? 0.f : ((k & 2u) ? -1.f : 1.f);
+    return dx * gx + dy * gy + dz * gz;
+}
+__device__ float perlin_single(const uint8_t* perm, const uint8_t* perm12, uint32_t offset, float x, float y,
+                               float z) {
+    const int x0 = noise_floor(x), y0 = noise_floor(y), z0 = noise_floor(z);
+    const int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+    const float xs = noise_quintic(x - (float)x0), ys = noise_quintic(y - (float)y0), zs = noise_quintic(z - (float)z0);
+    const float xd0 = x - (float)x0, yd0 = y - (float)y0, zd0 = z - (float)z0;
+    const float xd1 = xd0 - 1, yd1 = yd0 - 1, zd1 = zd0 - 1;
+    const float xf00 = noise_lerp(noise_grad(perm, perm12, offset, x0, y0, z0, xd0, yd0, zd0),
+                                  noise_grad(perm, perm12, offset, x1, y0, z0, xd1, yd0, zd0), xs);
+    const float xf10 = noise_lerp(noise_grad(perm, perm12, offset, x0, y1, z0, xd0, yd1, zd0),
+                                  noise_grad(perm, perm12, offset, x1, y1, z0, xd1, yd1, zd0), xs);
+    const float xf01 = noise_lerp(noise_grad(perm, perm12, offset, x0, y0, z1, xd0, yd0, zd1),
+                                  noise_grad(perm, perm12, offset, x1, y0, z1, xd1, yd0, zd1), xs);
+    const float xf11 = noise_lerp(noise_grad(perm, perm12, offset, x0, y1, z1, xd0, yd1, zd1),
+                                  noise_grad(perm, perm12, offset, x1, y1, z1, xd1, yd1, zd1), xs);
+    return noise_lerp(noise_lerp(xf00, xf10, ys), noise_lerp(xf01, xf11, ys), zs);
+}
+
+// Buffer voxel values after the noise edit: set to `adding` where the distance to the buffer
+// centre (uint3 difference, FP64) is below r * (1 + noise), else the DAG's value at the world
+// voxel truncate(min + p) (BufferEditor::get_value, hash_dag_editors.h:185-195)
+__global__ __launch_bounds__(256) void k_noise_state(const uint32_t* __restrict__ vpool, const uint32_t* d_root,
+                                                     uint32_t levels, float mx, float my, float mz, uint32_t S,
+                                                     float r, float frequency, float bounding,
+                                                     const uint8_t* __restrict__ tables, uint32_t adding,
+                                                     uint8_t* __restrict__ out) {
+    __shared__ uint8_t sTab[1024];
+    for (uint32_t i = threadIdx.x; i < 1024; i += blockDim.x) sTab[i] = tables[i];
+    __syncthreads();
+    const uint8_t* perm = sTab;
+    const uint8_t* perm12 = sTab + 512;
+    const uint64_t n = (uint64_t)S * S * S;
+    const uint32_t root = *d_root, c0 = S / 2;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(i % S), y = (uint32_t)(i / S % S), z = (uint32_t)(i / ((uint64_t)S * S));
+        // GetPerlinFractal(float(x), float(y), float(z)) with FBM (FastNoise.cpp:760-797)
+        float px = (float)x * frequency, py = (float)y * frequency, pz = (float)z * frequency;
+        float sum = perlin_single(perm, perm12, perm[0], px, py, pz), amp = 1;
+        for (uint32_t o = 1; o < 3; ++o) {
+            px *= 2.f;
+            py *= 2.f;
+            pz *= 2.f;
+            amp *= 0.5f;
+            sum += perlin_single(perm, perm12, perm[o], px, py, pz) * amp;
+        }
+        const float noise = sum * bounding;
+        const uint32_t dx = c0 - x, dy = c0 - y, dz = c0 - z;
+        const double distance = __dsqrt_rn((double)(dx * dx + dy * dy + dz * dz));
+        uint32_t v;
+        if (distance < (double)r * (1. + (double)noise)) {
+            v = adding;
+        } else {
+            const uint32_t wx = f2u_x86(mx + (float)x), wy = f2u_x86(my + (float)y), wz = f2u_x86(mz + (float)z);
+            const uint64_t lf = source_leaf(vpool, root, levels, wx >> 2, wy >> 2, wz >> 2);
+            v = (uint32_t)((lf >> leaf_bit(wx, wy, wz)) & 1ull);
+        }
+        out[i] = (uint8_t)v;
+    }
+}
+
 // DAGUtils::get_color of the given voxels (the solid voxels the flood fill loaded)
 __global__ __launch_bounds__(256) void k_fill_colors(EditPass P, const uint32_t* d_root, const uint32_t* __restrict__ xyz,
                                                      uint32_t n, uint64_t* __restrict__ out) {
@@ -2044,6 +2167,29 @@ void edit_free(hd_ctx* ctx) {
     S = EditScratch{};
 }
 
+namespace {
+// tags of the leaf slots below each leaf-level bucket's size: one block per bucket at a time
+__global__ __launch_bounds__(256) void k_leaf_tags(Geom g, const uint32_t* __restrict__ vpool,
+                                                   const uint32_t* __restrict__ bucket_sizes, uint8_t* __restrict__ tags) {
+    const uint32_t L = g.leaf_level(), nb = g.buckets_per_level(L), slots = g.bucket_size(L) / 2;
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint32_t n = min(bucket_sizes[g.bucket_global(L, b)], g.bucket_size(L)) / 2;
+        const uint32_t base = g.make_ptr(L, b, 0);
+        for (uint32_t s = threadIdx.x; s < n; s += blockDim.x) {
+            const uint64_t leaf = *reinterpret_cast<const uint64_t*>(vpool + base + 2 * s);
+            tags[(uint64_t)b * slots + s] = (uint8_t)(hash_leaf(leaf) >> 24);
+        }
+    }
+}
+}  // namespace
+
+hd_status leaf_tags_rebuild(hd_ctx* ctx, hipStream_t s) {
+    k_leaf_tags<<<4096, 256, 0, s>>>(ctx->g, ctx->vpool, ctx->bucket_sizes, ctx->leaf_tags);
+    HD_HIP(ctx, hipGetLastError());
+    HD_HIP(ctx, hipStreamSynchronize(s));
+    return HD_OK;
+}
+
 static void fill_pass(hd_ctx* ctx, EditPass& P, const EditScratch& S) {
     P.g = ctx->g;
     P.levels = ctx->g.levels;
@@ -2054,6 +2200,7 @@ static void fill_pass(hd_ctx* ctx, EditPass& P, const EditScratch& S) {
     P.full_nodes = ctx->full_nodes;
     P.pool_top = ctx->counters + 0;
     P.errors = ctx->counters + 1;
+    P.leaf_tags = ctx->leaf_tags;
     P.n_cnodes = ctx->counters + 2;
     P.n_leaves = ctx->counters + 3;
     P.undo_count = ctx->counters + 4;
@@ -2266,6 +2413,64 @@ hd_status fill_prepare(hd_ctx* ctx, const hd_edit& e, DevEdit& d, FillBuffers& f
         // FillEditorBase: boundsMin / boundsMax from the min corners of the filled voxels
         d.bmin[a] = fmin[a] + (float)lo[a];
         d.bmax[a] = fmin[a] + (float)hi[a];
+    }
+    return HD_OK;
+}
+
+// SphereNoiseEditor (hash_dag_editors.h:762-809): the buffer of a BufferEditor of radius 1.5 r
+// (2 ceil(1.5 r) voxels per axis from center - 1.5 r) holds each voxel's value after the edit;
+// the permutation is FastNoise::SetSeed's (FastNoise.cpp:207-225) for the editor's seed.
+hd_status noise_prepare(hd_ctx* ctx, const hd_edit& e, DevEdit& d, FillBuffers& fb, hipStream_t s) {
+    const float r = e.radius, br = 1.5f * r;
+    const float ce = std::ceil(br);
+    if (!(ce >= 1.0f) || ce > 512.0f) return fail(ctx, HD_ERR_INVALID, "noise radius must be in (0, 341]");
+    const uint32_t S = 2u * (uint32_t)ce;
+    const uint64_t n = (uint64_t)S * S * S;
+    const float fmin[3] = {e.center[0] - br, e.center[1] - br, e.center[2] - br};
+    const int seed = std::abs(1923 + (int32_t)e.center[0] + 23 * (int32_t)e.center[1] + 2938 * (int32_t)e.center[2]);
+    uint8_t tab[1024];
+    {
+        std::mt19937_64 gen((std::mt19937_64::result_type)(int64_t)seed);
+        uint8_t* perm = tab;
+        uint8_t* perm12 = tab + 512;
+        for (int i = 0; i < 256; ++i) perm[i] = (uint8_t)i;
+        for (int j = 0; j < 256; ++j) {
+            const int k = (int)(gen() % (uint64_t)(256 - j)) + j;
+            const uint8_t l = perm[j];
+            perm[j] = perm[j + 256] = perm[k];
+            perm[k] = l;
+            perm12[j] = perm12[j + 256] = (uint8_t)(perm[j] % 12);
+        }
+    }
+    // FastNoise::CalculateFractalBounding for 3 octaves of gain 0.5 (FastNoise.cpp:227-237)
+    float amp = 0.5f, ampFractal = 1.0f;
+    for (int i = 1; i < 3; ++i) {
+        ampFractal += amp;
+        amp *= 0.5f;
+    }
+    const float bounding = 1.0f / ampFractal;
+    fb.release();
+    HD_HIP(ctx, hipMalloc(&fb.state, n));
+    HD_HIP(ctx, hipMalloc(&fb.tables, sizeof(tab)));
+    HD_HIP(ctx, hipMemcpyAsync(fb.tables, tab, sizeof(tab), hipMemcpyHostToDevice, s));
+    k_noise_state<<<(uint32_t)std::min<uint64_t>((n + 255) / 256, 65536), 256, 0, s>>>(
+        ctx->vpool, ctx->d_root, ctx->g.levels, fmin[0], fmin[1], fmin[2], S, r, 1 / r, bounding, fb.tables,
+        e.kind == EDIT_NOISE_ADD ? 1u : 0u, fb.state);
+    HD_HIP(ctx, hipGetLastError());
+    HD_HIP(ctx, hipStreamSynchronize(s));
+    d.kind = EDIT_NOISE;
+    d.radius = br;  // BoxEditorBase::radius, which get_new_color divides by
+    d.fill_state = fb.state;
+    d.fill_size = S;
+    auto rgb565 = [](float rr, float gg, float bb) {
+        return (uint32_t)(uint16_t)(((uint16_t)(rr * 31.0f) << 0) | ((uint16_t)(gg * 63.0f) << 5) | ((uint16_t)(bb * 31.0f) << 11));
+    };
+    d.single = rgb565(94 / 255.f, 30 / 255.f, 0 / 255.f) | (rgb565(125 / 255.f, 100 / 255.f, 89 / 255.f) << 16);
+    for (int a = 0; a < 3; ++a) {
+        d.c[a] = e.center[a];
+        d.src[a] = fmin[a];
+        d.bmin[a] = fmin[a];
+        d.bmax[a] = e.center[a] + br;
     }
     return HD_OK;
 }

@@ -23,6 +23,7 @@ hd_status enqueue_frame_from_device(hd_ctx* ctx, const hd_ray_params* d_params, 
 hd_status edit_reserve_undo(hd_ctx* ctx, uint64_t entries);
 hd_status edit_scatter_leaves(hd_ctx* ctx, const uint32_t* handles, const LeafDesc* descs, uint32_t n);
 hd_status fill_prepare(hd_ctx* ctx, const hd_edit& e, DevEdit& d, FillBuffers& fb, hipStream_t s);
+hd_status noise_prepare(hd_ctx* ctx, const hd_edit& e, DevEdit& d, FillBuffers& fb, hipStream_t s);
 void edit_restore_leaves(hd_ctx* ctx, hipStream_t s, const EditScratch* scratch = nullptr);
 hipError_t edit_gather_next(hd_ctx* ctx, uint32_t n, LeafDesc* host, hipStream_t s);
 
@@ -54,7 +55,7 @@ __global__ void k_count_voxels(const DevEdit* __restrict__ edits, uint32_t n, co
     if (dN) n = min(n, *dN);  // graph replays read the edit count from device memory
     for (uint32_t ei = blockIdx.x; ei < n; ei += gridDim.x) {
         const DevEdit e = edits[ei];
-        if (e.kind == EDIT_COPY || e.kind == EDIT_FILL) continue;  // counted on the host
+        if (e.kind == EDIT_COPY || e.kind == EDIT_FILL || e.kind == EDIT_NOISE) continue;  // counted on the host
         const float r = e.radius;
         const int64_t x0 = std::max<int64_t>(0, (int64_t)floorf(e.c[0] - r) - 1);
         const int64_t x1 = std::min<int64_t>((int64_t)E - 1, (int64_t)ceilf(e.c[0] + r) + 1);
@@ -407,7 +408,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
         return HD_OK;
     }
     for (uint32_t i = 0; i < n; ++i)
-        if (edits[i].kind > EDIT_COPY_DEC || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
+        if (edits[i].kind > EDIT_NOISE_CARVE || !(edits[i].radius > 0.f) || !std::isfinite(edits[i].radius))
             return fail(ctx, HD_ERR_INVALID, "unsupported editor kind or radius");
     for (uint32_t i = 0; i < n; ++i) {
         if (edits[i].kind == EDIT_COPY_DEC && !(edits[i].radius < 65536.f))
@@ -433,11 +434,12 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
     if (n) HD_HIP(ctx, hipMemcpyAsync(ctx->edit.edits_dev, dev.data(), (uint64_t)n * sizeof(DevEdit), hipMemcpyHostToDevice, s));
     HD_HIP(ctx, hipMemcpyAsync(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice, s));
     hd_edit_stats out{};
-    // a fill's buffers are built from the DAG as it is when its pass starts
+    // a fill's or a noise editor's buffers are built from the DAG as it is when its pass starts
     FillBuffers fb;
     auto prepare_fill = [&](uint32_t i) -> hd_status {
-        if (edits[i].kind != EDIT_FILL) return HD_OK;
-        hd_status r = fill_prepare(ctx, edits[i], dev[i], fb, s);
+        if (!is_buffer_kind(edits[i].kind)) return HD_OK;
+        hd_status r = edits[i].kind == EDIT_FILL ? fill_prepare(ctx, edits[i], dev[i], fb, s)
+                                                 : noise_prepare(ctx, edits[i], dev[i], fb, s);
         if (r != HD_OK) return r;
         HD_HIP(ctx, hipMemcpyAsync(ctx->edit.edits_dev + i, &dev[i], sizeof(DevEdit), hipMemcpyHostToDevice, s));
         return HD_OK;
@@ -453,21 +455,22 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
         // pass; every other run of editors fuses into one pass.
         for (uint32_t first = 0; first < n;) {
             uint32_t last = first + 1;
-            while (last < n && !is_copy_kind(edits[last].kind) && edits[last].kind != EDIT_FILL) ++last;
+            while (last < n && !is_copy_kind(edits[last].kind) && !is_buffer_kind(edits[last].kind)) ++last;
             UndoFrame frame;
             frame.prev_root = ctx->root;
             Readback rb{};
             st = prepare_fill(first);
             if (st != HD_OK) return st;
             st = run_pass(ctx, first, last - first, s, rb, &frame,
-                          is_copy_kind(edits[first].kind) || edits[first].kind == EDIT_FILL);
+                          is_copy_kind(edits[first].kind) || is_buffer_kind(edits[first].kind));
             if (st != HD_OK) return st;
             ctx->root = rb.root;
             frame.new_root = rb.root;
             ctx->undo.push_back(std::move(frame));
             ctx->redo.clear();
             out.num_voxels += rb.stats[0] + copy_voxels(edits + first, last - first);
-            if (edits[first].kind == EDIT_FILL && last - first > 1) out.num_voxels += fill_voxels(dev[first], 1u << ctx->g.levels);
+            if (is_buffer_kind(edits[first].kind) && last - first > 1)
+                out.num_voxels += fill_voxels(dev[first], 1u << ctx->g.levels);
             out.new_words += rb.stats[1];
             out.leaves_rebuilt += rb.counters[4];
             out.colors_encoded += rb.stats[4];
@@ -481,7 +484,7 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             Readback rb{};
             st = prepare_fill(i);
             if (st != HD_OK) return st;
-            st = run_pass(ctx, i, 1, s, rb, &frame, is_copy_kind(edits[i].kind) || edits[i].kind == EDIT_FILL);
+            st = run_pass(ctx, i, 1, s, rb, &frame, is_copy_kind(edits[i].kind) || is_buffer_kind(edits[i].kind));
             if (st != HD_OK) return st;
             ctx->root = rb.root;
             frame.new_root = rb.root;
@@ -732,6 +735,10 @@ static hd_status graph_capture(hd_frame_graph* g) {
     EditScratch& S = ctx->edit;
     const bool colors = ctx->colors_loaded && ctx->g.leaf_level() > ctx->g.color_tree_levels;
     hipStream_t s = g->stream;
+    if (ctx->trace_persist) {
+        const hd_status rs = trace_reserve(ctx, s, g->width, g->height);
+        if (rs != HD_OK) return rs;
+    }
     HD_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipMemcpyAsync(g->d_n, g->h_n, 4, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(S.edits_dev, g->h_edits, (uint64_t)g->max_edits * sizeof(DevEdit), hipMemcpyHostToDevice, s);

@@ -440,6 +440,10 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
     }
     ctx->root = remap(roots, newRoots, ctx->root);
     HD_HIP(ctx, hipMemcpyAsync(ctx->d_root, &ctx->root, 4, hipMemcpyHostToDevice, s));
+    {
+        const hd_status ts = leaf_tags_rebuild(ctx, s);  // the leaf buckets were rewritten
+        if (ts != HD_OK) return ts;
+    }
     for (uint32_t i = 0; i < n_extra; ++i) extra[i] = remap(roots, newRoots, extra[i]);
     if (keep_undo) {
         for (auto* frames : {&ctx->undo, &ctx->redo})

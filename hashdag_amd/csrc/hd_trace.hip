@@ -10,6 +10,8 @@
 // words are read straight from the virtual-address pool (no page-table indirection).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "hd_ctx.h"
 
 namespace hd {
@@ -671,6 +673,225 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, co
     flush_count<kCount>(bytes, counter);
 }
 
+// ---- persistent wavefronts fed from tile queues ----
+//
+// The per-tile launch keeps a wavefront alive until its slowest ray ends, so lanes whose rays
+// ended (or, for shadows, that hold sky pixels) sit idle. Here every wavefront stays resident
+// and takes 64-pixel tiles (the context's tile shape) from one of kQueues queues, each a
+// contiguous run of the image's tiles; once `refillAt` or more of its lanes are idle it hands
+// them the untaken pixels of its current tile, then of the next one. The queue of a wavefront
+// is picked from the XCD its block runs on, so one XCD's L2 serves one region of the image; a
+// wavefront whose queue is empty moves on to the next queue. Each ray's walk is the per-tile
+// kernel's walk, so every pixel's result is the same.
+constexpr uint32_t kQueues = 64;
+constexpr uint32_t kQueueStride = 32;  // words between queue heads (one 128-byte line each)
+
+struct TileQ {
+    uint32_t* heads;  // kQueues * kQueueStride words, zeroed before each launch
+    uint32_t ntiles, tilesX;
+};
+
+__device__ __forceinline__ uint32_t queue_begin(uint32_t q, uint32_t ntiles) {
+    return (uint32_t)(((uint64_t)ntiles * q) / kQueues);
+}
+
+// Next tile for this wavefront (wave-uniform), moving from queue q to the following ones as they
+// run dry; ~0u once all kQueues queues have been found dry.
+__device__ __forceinline__ uint32_t grab_tile(const TileQ& Q, uint32_t& q, uint32_t& tried) {
+    while (tried < kQueues) {
+        const uint32_t n = queue_begin(q + 1, Q.ntiles) - queue_begin(q, Q.ntiles);
+        uint32_t* h = Q.heads + q * kQueueStride;
+        const uint32_t seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (seen < n) {
+            uint32_t t = 0;
+            if ((threadIdx.x & 63u) == 0) t = atomicAdd(h, 1u);
+            t = __builtin_amdgcn_readfirstlane(t);
+            if (t < n) return queue_begin(q, Q.ntiles) + t;
+        }
+        q = q + 1 == kQueues ? 0 : q + 1;
+        ++tried;
+    }
+    return ~0u;
+}
+
+// position of the n-th (from 0) set bit of m (n < popcount(m))
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t n) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t width = 32; width >= 1; width >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(m & ((1ull << width) - 1ull));
+        if (n >= c) {
+            n -= c;
+            m >>= width;
+            pos += width;
+        }
+    }
+    return pos;
+}
+
+__global__ void k_queue_reset(uint32_t* heads) {
+    for (uint32_t i = threadIdx.x; i < kQueues * kQueueStride; i += blockDim.x) heads[i] = 0;
+}
+
+// Shadow rays of a frame for the persistent walk (tracer.cu:589-697 up to the walk): per pixel
+// tile, the hit pixels' shadow origins and pixel indices packed to the front of the tile's 64
+// records (count in cnt), with the colour each gets lit and in shadow; sky pixels (null paths,
+// tracer.cu:644) are shaded here. One wavefront per tile, tiles numbered ty * tilesX + tx.
+template <bool kFog>
+__global__ __launch_bounds__(kBlock) void k_shadow_setup(RayP p, const RayP* __restrict__ pmem, uint32_t W, uint32_t H,
+                                                         const uint4* __restrict__ paths, uint32_t* __restrict__ colors,
+                                                         float bias, float fog, uint32_t tileW, uint32_t tileH,
+                                                         uint32_t tilesX, uint32_t tilesY, uint4* __restrict__ rec,
+                                                         uint2* __restrict__ shades, uint32_t* __restrict__ cnt) {
+    if (pmem) p = *pmem;
+    uint32_t x, y;
+    pixel_of(tileW, tileH, x, y);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t tx = blockIdx.x * 2u + (wave & 1u), ty = blockIdx.y * 2u + (wave >> 1);
+    const uint32_t t = ty * tilesX + tx;
+    bool valid = false;
+    uint4 r{};
+    uint2 sh{};
+    if (x < W && y < H) {
+        const uint32_t pix = y * W + x;
+        const uint4 pp = paths[pix];
+        const uint32_t colorIn = colors[pix];
+        F3 sun;
+        sun_direction(sun.x, sun.y, sun.z);
+        double cx, cy, cz;
+        if (null_path(pp)) {
+            if (kFog) {
+                camera_dir(p, x, H - 1 - y, cx, cy, cz);
+                colors[pix] = shade(colorIn, 1.f, 1e9, cx, cy, cz, fog, sun);
+            } else {
+                colors[pix] = shade_clear(colorIn, 1.f);
+            }
+        } else {
+            camera_dir(p, x, H - 1 - y, cx, cy, cz);
+            const F3 o = shadow_origin(p, pp, cx, cy, cz, bias, sun);
+            r = make_uint4(__float_as_uint(o.x), __float_as_uint(o.y), __float_as_uint(o.z), pix);
+            if (kFog) {
+                double distance, nx, ny, nz;
+                hit_distance(p, pp, distance, nx, ny, nz);
+                sh = make_uint2(shade(colorIn, 1.f, distance, nx, ny, nz, fog, sun),
+                                shade(colorIn, 0.f, distance, nx, ny, nz, fog, sun));
+            } else {
+                sh = make_uint2(shade_clear(colorIn, 1.f), shade_clear(colorIn, 0.f));
+            }
+            valid = true;
+        }
+    }
+    const uint64_t m = __ballot(valid);
+    if (valid) {
+        const uint32_t k = t * 64u + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        rec[k] = r;
+        shades[k] = sh;
+    }
+    if (lane == 0 && tx < tilesX && ty < tilesY) cnt[t] = (uint32_t)__popcll(m);
+}
+
+// kShadow = false: tracer.cu:145-252 per pixel of each tile (paths out). kShadow = true: the
+// any-hit walk of tracer.cu:453-543 for the records k_shadow_setup packed per tile, then the
+// pixel's lit or shadowed colour.
+template <bool kShadow, int kFetch>
+__global__ __launch_bounds__(kBlock) void k_trace_persist(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W,
+                                                          uint32_t H, uint4* __restrict__ paths,
+                                                          uint32_t* __restrict__ colors, const uint4* __restrict__ rec,
+                                                          const uint2* __restrict__ shades,
+                                                          const uint32_t* __restrict__ cnt, uint32_t tileW,
+                                                          uint32_t tileH, TileQ Q, uint32_t refillAt) {
+    const Stack st = stack_of(dag.levels);
+    if (!kShadow && pmem) p = *pmem;
+    if (dag.root_ptr) dag.root = *dag.root_ptr;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    // blocks go to the XCDs round robin: the 8 queues of XCD blockIdx % 8 cover one eighth of
+    // the image
+    uint32_t q = (blockIdx.x % 8u) * 8u + ((blockIdx.x / 8u) * 4u + wave) % 8u;
+    uint32_t tried = 0;
+    bool exhausted = false;
+    uint64_t cand = 0;      // lanes holding an untaken candidate of the current tile
+    uint32_t candWork = 0;  // this lane's candidate: pixel (paths) or record index (shadows)
+    bool active = false;
+    uint32_t pix = 0;
+    uint2 sh{};
+    Walk w;
+    F3 o{}, d{}, inv{};
+    uint32_t order = 0;
+    if (kShadow) {
+        sun_direction(d.x, d.y, d.z);
+        inv = F3{(float)(1.0 / (double)d.x), (float)(1.0 / (double)d.y), (float)(1.0 / (double)d.z)};
+        order = (d.x < 0.f ? 4u : 0u) + (d.y < 0.f ? 2u : 0u) + (d.z < 0.f ? 1u : 0u);
+    }
+    uint64_t unused = 0;
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        if (!exhausted && (uint32_t)__popcll(idle) >= refillAt) {
+            while (idle) {
+                if (!cand) {
+                    const uint32_t t = grab_tile(Q, q, tried);
+                    if (t == ~0u) {
+                        exhausted = true;
+                        break;
+                    }
+                    bool ok;
+                    if (kShadow) {
+                        const uint32_t n = __builtin_amdgcn_readfirstlane(cnt[t]);
+                        ok = lane < n;
+                        candWork = t * 64u + lane;
+                    } else {
+                        const uint32_t x = (t % Q.tilesX) * tileW + lane % tileW;
+                        const uint32_t y = (t / Q.tilesX) * tileH + lane / tileW;
+                        ok = x < W && y < H;
+                        candWork = y * W + x;
+                    }
+                    cand = __ballot(ok);
+                    continue;
+                }
+                // the idle lanes, in lane order, take the candidates in lane order
+                const uint32_t k = min((uint32_t)__popcll(idle), (uint32_t)__popcll(cand));
+                const uint32_t cr = (uint32_t)__popcll(cand & below), ir = (uint32_t)__popcll(idle & below);
+                const bool give = ((cand >> lane) & 1ull) && cr < k;
+                const bool take = ((idle >> lane) & 1ull) && ir < k;
+                // every lane reads a lane (its own when not taking): the ir-th candidate's
+                const uint32_t src = take ? nth_set_bit(cand, ir) : lane;
+                const uint32_t work = (uint32_t)__shfl((int)candWork, (int)src, 64);
+                cand &= ~__ballot(give);
+                idle &= ~__ballot(take);
+                if (take) {
+                    if (kShadow) {
+                        const uint4 r = rec[work];
+                        sh = shades[work];
+                        o = F3{__uint_as_float(r.x), __uint_as_float(r.y), __uint_as_float(r.z)};
+                        pix = r.w;
+                    } else {
+                        pix = work;
+                        primary_ray(p, pix % W, pix / W, o, d, inv, order);
+                    }
+                    walk_begin<false, kFetch>(dag, w, o, d, inv, unused);
+                    active = true;
+                }
+            }
+        }
+        if (!__ballot(active)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active) {
+            const uint32_t r = walk_step<false, kFetch>(dag, w, o, d, inv, order, st, unused);
+            if (r != kWalkOn) {
+                active = false;
+                if (kShadow) {
+                    colors[pix] = r == kWalkHit ? sh.y : sh.x;
+                } else {
+                    const uint32_t x = pix % W, y = pix / W;
+                    paths[(uint64_t)(H - 1 - y) * W + x] = make_uint4(w.px, w.py, w.pz, 0u);
+                }
+            }
+        }
+    }
+}
+
 inline RayP to_rayp(const hd_ray_params* p) {
     RayP r;
     for (int i = 0; i < 3; ++i) {
@@ -689,9 +910,54 @@ inline dim3 grid_of(const hd_ctx* ctx, uint32_t W, uint32_t H) {
 // The caller's stream as given; NULL is the HIP default stream (torch's default stream).
 inline hipStream_t stream_of(hd_ctx*, void* s) { return reinterpret_cast<hipStream_t>(s); }
 
-// Primary paths with the fetch strategy of the context; pm (device) overrides p when given.
+// Persistent schedule: the kernel of (shadow, fetch) with as many blocks as stay resident; shadows
+// first pack their rays per tile (k_shadow_setup).
+template <bool kShadow>
+hd_status launch_persist(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP* pm, uint32_t W, uint32_t H,
+                         uint4* paths, uint32_t* colors, float bias, float fog, hipStream_t s) {
+    const auto k = ctx->fetch_mode == kFetchNode ? k_trace_persist<kShadow, kFetchNode>
+                                                 : k_trace_persist<kShadow, kFetchPointer>;
+    const uint32_t smem = stack_bytes(ctx->g.levels);
+    uint32_t& blocks = ctx->persist_blocks[(kShadow ? 2 : 0) + (ctx->fetch_mode == kFetchNode ? 1 : 0)];
+    if (!blocks) {
+        int perCU = 0;
+        HD_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(k), kBlock, smem));
+        hipDeviceProp_t prop;
+        HD_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
+        blocks = (uint32_t)std::max(1, perCU) * (uint32_t)std::max(1, prop.multiProcessorCount);
+    }
+    TraceScratch* ts = nullptr;
+    const hd_status rs = trace_reserve(ctx, s, W, H, &ts);
+    if (rs != HD_OK) return rs;
+    TileQ Q;
+    Q.heads = ts->queues;
+    Q.tilesX = (W + ctx->tile_w - 1) / ctx->tile_w;
+    const uint32_t tilesY = (H + ctx->tile_h - 1) / ctx->tile_h;
+    Q.ntiles = Q.tilesX * tilesY;
+    uint4* rec = nullptr;
+    uint2* shades = nullptr;
+    uint32_t* cnt = nullptr;
+    if (kShadow) {
+        rec = ts->rec;
+        shades = ts->shades;
+        cnt = ts->cnt;
+        const auto setup = fog == 0.f ? k_shadow_setup<false> : k_shadow_setup<true>;
+        setup<<<grid_of(ctx, W, H), kBlock, 0, s>>>(p, pm, W, H, paths, colors, bias, fog, ctx->tile_w, ctx->tile_h,
+                                                    Q.tilesX, tilesY, rec, shades, cnt);
+        HD_HIP(ctx, hipGetLastError());
+    }
+    k_queue_reset<<<1, 256, 0, s>>>(Q.heads);
+    k<<<blocks, kBlock, smem, s>>>(dag, p, pm, W, H, paths, colors, rec, shades, cnt, ctx->tile_w, ctx->tile_h, Q,
+                                   std::max<uint32_t>(1, ctx->refill_at));
+    HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+
+// Primary paths with the fetch strategy and schedule of the context; pm (device) overrides p
+// when given.
 hd_status launch_paths(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP* pm, uint32_t W, uint32_t H,
                        uint4* out, hipStream_t s) {
+    if (ctx->trace_persist) return launch_persist<false>(ctx, dag, p, pm, W, H, out, nullptr, 0.f, 0.f, s);
     const auto k = ctx->fetch_mode == kFetchNode ? k_trace_paths<false, kFetchNode> : k_trace_paths<false, kFetchPointer>;
     k<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), s>>>(dag, p, pm, W, H, out, ctx->tile_w, ctx->tile_h,
                                                                      nullptr);
@@ -701,6 +967,8 @@ hd_status launch_paths(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP
 
 hd_status launch_shadows(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP* pm, uint32_t W, uint32_t H,
                          const uint4* paths, uint32_t* colors, float bias, float fog, hipStream_t s) {
+    if (ctx->trace_persist)
+        return launch_persist<true>(ctx, dag, p, pm, W, H, const_cast<uint4*>(paths), colors, bias, fog, s);
     const auto k = ctx->fetch_mode == kFetchNode ? k_trace_shadows<false, kFetchNode> : k_trace_shadows<false, kFetchPointer>;
     k<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), s>>>(dag, p, pm, W, H, paths, colors, bias, fog,
                                                                      ctx->tile_w, ctx->tile_h, nullptr);
@@ -712,6 +980,56 @@ hd_status launch_shadows(hd_ctx* ctx, const DevDag& dag, const RayP& p, const Ra
 }  // namespace hd
 
 namespace hd {
+hd_status trace_reserve(hd_ctx* ctx, hipStream_t s, uint32_t W, uint32_t H, TraceScratch** out) {
+    // tiles of the frame in any tile shape of 64 pixels: at most ceil(W / w) * ceil(H / h)
+    uint64_t tiles = 0;
+    for (uint32_t w = 1; w <= 64; w *= 2)
+        tiles = std::max<uint64_t>(tiles, (uint64_t)((W + w - 1) / w) * ((H + 64 / w - 1) / (64 / w)));
+    TraceScratch* ts = nullptr;
+    for (TraceScratch& t : ctx->trace_scratch)
+        if (t.stream == s) ts = &t;
+    if (ts && ts->tiles_cap >= tiles) {
+        if (out) *out = ts;
+        return HD_OK;
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HD_HIP(ctx, hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+        return fail(ctx, HD_ERR_INVALID, "persistent trace scratch must be reserved before stream capture");
+    if (!ts) {
+        ctx->trace_scratch.push_back(TraceScratch{});
+        ts = &ctx->trace_scratch.back();
+        ts->stream = s;
+        HD_HIP(ctx, hipMalloc(&ts->queues, 64 * 32 * 4));
+    } else {
+        HD_HIP(ctx, hipStreamSynchronize(s));
+        hipFree(ts->rec);
+        hipFree(ts->shades);
+        hipFree(ts->cnt);
+        ts->rec = nullptr;
+        ts->shades = nullptr;
+        ts->cnt = nullptr;
+        ts->tiles_cap = 0;
+    }
+    HD_HIP(ctx, hipMalloc(&ts->rec, tiles * 64 * sizeof(uint4)));
+    HD_HIP(ctx, hipMalloc(&ts->shades, tiles * 64 * sizeof(uint2)));
+    HD_HIP(ctx, hipMalloc(&ts->cnt, tiles * sizeof(uint32_t)));
+    ts->tiles_cap = tiles;
+    ctx->ptr_gen++;
+    if (out) *out = ts;
+    return HD_OK;
+}
+
+void trace_scratch_free(hd_ctx* ctx) {
+    for (TraceScratch& t : ctx->trace_scratch) {
+        hipFree(t.queues);
+        hipFree(t.rec);
+        hipFree(t.shades);
+        hipFree(t.cnt);
+    }
+    ctx->trace_scratch.clear();
+}
+
 // The three tracer kernels with the ray parameters and the root read from device memory, so a
 // captured graph traces whatever camera and DAG state the replay finds there.
 hd_status enqueue_frame_from_device(hd_ctx* ctx, const hd_ray_params* d_params, uint32_t W, uint32_t H,
@@ -785,6 +1103,23 @@ hd_status hd_set_trace_fetch(hd_ctx* ctx, uint32_t fetch_mode) {
     return HD_OK;
 }
 
+hd_status hd_set_trace_schedule(hd_ctx* ctx, uint32_t persistent, uint32_t refill_at) {
+    if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
+    if (persistent > 1 || (persistent && (refill_at == 0 || refill_at > 64)))
+        return fail(ctx, HD_ERR_INVALID, "schedule is 0 (per tile) or 1 (persistent, refill_at in 1..64)");
+    ctx->trace_persist = persistent;
+    if (persistent) ctx->refill_at = refill_at;
+    ctx->ptr_gen++;
+    return HD_OK;
+}
+
+hd_status hd_get_trace_schedule(hd_ctx* ctx, uint32_t out[2]) {
+    if (!ctx || !out) return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    out[0] = ctx->trace_persist;
+    out[1] = ctx->refill_at;
+    return HD_OK;
+}
+
 hd_status hd_get_trace_launch(hd_ctx* ctx, uint32_t out[3]) {
     if (!ctx || !out) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     out[0] = ctx->tile_w;
@@ -801,18 +1136,22 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
     if (!ctx->loaded || !ctx->colors_loaded) return fail(ctx, HD_ERR_INVALID, "no DAG or colours loaded");
     static const uint32_t shapes[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}, {2, 32}, {64, 1}};
     static const uint32_t fetches[] = {kFetchPointer, kFetchNode};
-    constexpr uint32_t kTrials = 6 * 2;  // tile shape x fetch strategy
+    // schedule: one wavefront per tile, or persistent wavefronts refilled at 8 / 24 idle lanes
+    static const uint32_t schedules[][2] = {{0, 16}, {1, 8}, {1, 24}};
+    constexpr uint32_t kTrials = 6 * 2 * 3;  // tile shape x fetch strategy x schedule
     hipEvent_t e0, e1;
     HD_HIP(ctx, hipEventCreate(&e0));
     HD_HIP(ctx, hipEventCreate(&e1));
     float bestMs = 1e30f;
-    uint32_t bw = ctx->tile_w, bh = ctx->tile_h, bf = ctx->fetch_mode;
+    uint32_t bw = ctx->tile_w, bh = ctx->tile_h, bf = ctx->fetch_mode, bp = ctx->trace_persist, br = ctx->refill_at;
     reps = std::max<uint32_t>(reps, 1);
     for (uint32_t trial = 0; trial < kTrials; ++trial) {
         const auto& sh = shapes[trial % 6];
         ctx->tile_w = sh[0];
         ctx->tile_h = sh[1];
-        ctx->fetch_mode = fetches[trial / 6];
+        ctx->fetch_mode = fetches[(trial / 6) % 2];
+        ctx->trace_persist = schedules[trial / 12][0];
+        ctx->refill_at = schedules[trial / 12][1];
         float total = 0;
         for (uint32_t rep = 0; rep <= reps; ++rep) {  // rep 0 warms up
             HD_HIP(ctx, hipEventRecord(e0, ctx->stream));
@@ -838,6 +1177,8 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
             bw = sh[0];
             bh = sh[1];
             bf = ctx->fetch_mode;
+            bp = ctx->trace_persist;
+            br = ctx->refill_at;
         }
     }
     hipEventDestroy(e0);
@@ -845,6 +1186,8 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
     ctx->tile_w = bw;
     ctx->tile_h = bh;
     ctx->fetch_mode = bf;
+    ctx->trace_persist = bp;
+    ctx->refill_at = br;
     ctx->ptr_gen++;
     if (best_w) *best_w = bw;
     if (best_h) *best_h = bh;

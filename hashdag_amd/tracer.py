@@ -102,10 +102,19 @@ class DAGTracer:
         check(lib.hd_set_trace_fetch(dag.handle, fetch), dag.handle)
 
 
+    def set_schedule(self, dag: HashDAG, persistent: bool, refill_at: int = 16) -> None:
+        """0: one wavefront per pixel tile; 1: persistent wavefronts fed from tile queues,
+        refilling idle lanes once refill_at of them are idle."""
+        check(lib.hd_set_trace_schedule(dag.handle, 1 if persistent else 0, refill_at), dag.handle)
+
+
 def launch_config(dag: HashDAG) -> dict:
     out = (C.c_uint32 * 3)()
     check(lib.hd_get_trace_launch(dag.handle, out), dag.handle)
-    return {"tile": f"{out[0]}x{out[1]}", "fetch": ["pointer", "node"][out[2]]}
+    sch = np.zeros(2, np.uint32)
+    check(lib.hd_get_trace_schedule(dag.handle, ptr(sch)), dag.handle)
+    schedule = f"persistent (refill at {int(sch[1])})" if sch[0] else "per tile"
+    return {"tile": f"{out[0]}x{out[1]}", "fetch": ["pointer", "node"][out[2]], "schedule": schedule}
 
 
 def autotune(tracer: "DAGTracer", dag: HashDAG, params_list, reps: int = 2) -> dict:

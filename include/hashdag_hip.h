@@ -147,9 +147,17 @@ hd_status hd_set_trace_tile(hd_ctx* ctx, uint32_t tile_w, uint32_t tile_h);
 hd_status hd_set_trace_fetch(hd_ctx* ctx, uint32_t fetch_mode);
 /* out: tile_w, tile_h, fetch mode */
 hd_status hd_get_trace_launch(hd_ctx* ctx, uint32_t out[3]);
+/* Tracer schedule, not in the reference (one thread per pixel, dag_tracer.cu:121-122):
+ * persistent = 0 runs one wavefront per pixel tile; 1 keeps every wavefront resident, taking
+ * tiles from per-region queues and handing new pixels to its idle lanes once refill_at
+ * (1..64) of them are idle. Paths and shadows; both give the same images. */
+hd_status hd_set_trace_schedule(hd_ctx* ctx, uint32_t persistent, uint32_t refill_at);
+/* out: persistent, refill_at */
+hd_status hd_get_trace_schedule(hd_ctx* ctx, uint32_t out[2]);
 
 /* Autotuner for the tracer launch shape: times full frames (paths + colours + shadows) over
- * n_params cameras for each wavefront tile shape (8x8, 16x4, 4x16, 32x2, 2x32, 64x1), keeps
+ * n_params cameras for each wavefront tile shape (8x8, 16x4, 4x16, 32x2, 2x32, 64x1), node
+ * fetch strategy and schedule (per tile; persistent refilled at 8 or 24 idle lanes), keeps
  * the fastest in ctx and returns it with its ms per frame. No counterpart in the reference
  * (fixed (4,64) blocks, dag_tracer.cu:121-122). */
 hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n_params, uint32_t width,
@@ -179,6 +187,13 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
  * from truncate(center - radius); nodes above levels - 4 whose source region is empty are left
  * unedited (COPY_EMPTY_CHECKS). No transform or swirl; radius below 65536. */
 #define HD_EDIT_COPY_DEC 7u
+/* SphereNoiseEditor(isAdding = true / false) (hash_dag_editors.h:762-809, engine.cpp:688): in
+ * the box of half side 1.5 radius, voxels within radius * (1 + FastNoise Perlin fractal) of the
+ * centre are set (adding); the carving form leaves values unchanged, as the reference's
+ * BufferEditor::get_new_value does. A voxel that becomes set takes the editor's rgb565 min/max
+ * colour with a 3-bit weight. Each starts its own pass; 1.5 radius at most 512. */
+#define HD_EDIT_NOISE_ADD 8u
+#define HD_EDIT_NOISE_CARVE 9u
 typedef struct hd_edit {
     uint32_t kind;
     float center[3];

@@ -73,6 +73,7 @@ _SIGS = {
     "orc_get_color": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
     "orc_dag_reachable": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "orc_find_in_bucket": (C.c_uint32, [C.c_void_p, C.c_uint32, C.c_uint32, u32p, C.c_uint32]),
+    "orc_perlin": (C.c_int, [C.c_int, C.c_float, C.c_int, C.c_uint32, fltp, fltp, u8p]),
 }
 
 
@@ -317,3 +318,14 @@ class Oracle:
         orc.orc_trace_shadows(dag.h, _p(params, C.c_double), W, H, y0, y1, _p(paths), _p(out), bias, fog,
                               _p(b, C.c_uint64) if count else None, threads)
         return (out, int(b[0])) if count else out
+
+
+def perlin_fractal(seed: int, frequency: float, octaves: int, xyz, with_perm: bool = False):
+    """The oracle's FastNoise Perlin fractal at points xyz (N x 3 float32); optionally the
+    permutation tables (m_perm, m_perm12)."""
+    pts = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    out = np.zeros(len(pts), np.float32)
+    perm = np.zeros(1024, np.uint8) if with_perm else None
+    orc.orc_perlin(seed, frequency, octaves, len(pts), _p(pts, C.c_float), _p(out, C.c_float),
+                   _p(perm, C.c_uint8) if with_perm else None)
+    return (out, perm[:512], perm[512:]) if with_perm else out

@@ -538,3 +538,16 @@ uint32_t orc_find_in_bucket(void* d, uint32_t level, uint32_t bucket, const uint
 }
 
 }  // extern "C"
+
+int orc_perlin(int seed, float frequency, int octaves, uint32_t n, const float* xyz, float* out, uint8_t* perm) {
+    PerlinFbm noise;
+    noise.set_seed(seed);
+    noise.frequency = frequency;
+    noise.set_octaves(octaves);
+    for (uint32_t i = 0; i < n; ++i) out[i] = noise.fractal(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    if (perm) {
+        std::memcpy(perm, noise.perm, 512);
+        std::memcpy(perm + 512, noise.perm12, 512);
+    }
+    return 0;
+}

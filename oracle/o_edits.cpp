@@ -145,6 +145,7 @@ struct Editor {
             const double s = 1 / (v1 - v2 + v3);
             for (int i = 0; i < 9; ++i) inv[i] = a[i] * s;
         }
+        if (e.kind == NOISE_ADD || e.kind == NOISE_CARVE) radius = 1.5f * e.radius;  // BufferEditor(center, 1.5 r)
         if (e.kind == COPY_DEC) {
             // BoxEditorBase(center - source + dest, radius); a buffer of 2 ceil(radius) voxels
             // per axis from truncate(center - radius) (hash_dag_editors.h:455-466)
@@ -155,10 +156,14 @@ struct Editor {
         }
         bmin = c - f3(radius);
         bmax = c + f3(radius);
+        boxRadius = radius;
         r2 = e.radius * e.radius;
     }
+    float boxRadius = 0;  // BoxEditorBase::radius
+    bool is_noise() const { return e.kind == NOISE_ADD || e.kind == NOISE_CARVE; }
     bool is_box() const {
-        return e.kind == BOX_ADD || e.kind == BOX_CARVE || e.kind == COPY || e.kind == FILL || e.kind == COPY_DEC;
+        return e.kind == BOX_ADD || e.kind == BOX_CARVE || e.kind == COPY || e.kind == FILL || e.kind == COPY_DEC ||
+               is_noise();
     }
 
     // ---- CopyEditorWithDecompression ----
@@ -244,6 +249,42 @@ struct Editor {
                     if (values[i] == LoadedTrue) color = color_at(*srcTable, srcRoot, *srcColors, world_path(x, y, z));
                     else if (values[i] == SetTrue) fillColors[i] = color;
                 }
+    }
+    // SphereNoiseEditor (hash_dag_editors.h:762-792): the buffer of a BufferEditor of radius
+    // 1.5 r; voxels whose distance to the buffer centre (integer, FP64) is below
+    // r * (1 + Perlin fractal) are set to isAdding, the others are loaded from the DAG on use
+    void build_noise() {
+        const float br = boxRadius;
+        fmin = e.center - f3(br);
+        fsize = 2 * (u32)std::ceil(br);
+        values.assign((u64)fsize * fsize * fsize, Unknown);
+        PerlinFbm noise;
+        noise.set_seed(std::abs(1923 + (int32_t)e.center.x + 23 * (int32_t)e.center.y + 2938 * (int32_t)e.center.z));
+        noise.frequency = 1 / e.radius;
+        noise.set_octaves(3);
+        const u32 c0 = fsize / 2;
+        const u8 set = e.kind == NOISE_ADD ? SetTrue : SetFalse;
+        for (u32 x = 0; x < fsize; ++x)
+            for (u32 y = 0; y < fsize; ++y)
+                for (u32 z = 0; z < fsize; ++z) {
+                    // length(editCenter - position) on uint3: 32-bit differences and dot product
+                    const u32 dx = c0 - x, dy = c0 - y, dz = c0 - z;
+                    const double distance = std::sqrt((double)(u32)(dx * dx + dy * dy + dz * dz));
+                    if (distance < e.radius * (1. + (double)noise.fractal((float)x, (float)y, (float)z)))
+                        values[findex(x, y, z)] = set;
+                }
+    }
+    // SphereNoiseEditor::get_new_color (hash_dag_editors.h:794-808)
+    CompressedColor noise_color(F3 pos) const {
+        CompressedColor c;
+        c.bitsPerWeight = 3;
+        const u16 mn = float3_to_rgb565(F3{94, 30, 0} / 255.f), mx = float3_to_rgb565(F3{125, 100, 89} / 255.f);
+        c.colorBits = (u32)mn | ((u32)mx << 16);
+        const F3 d = pos - e.center;
+        const float len = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+        const int32_t w = (int32_t)((len / boxRadius - 0.3f) * (float)((1 << 3) - 1));
+        c.weight = (u8)(w < 0 ? 0 : (w > 7 ? 7 : w));
+        return c;
     }
     // destination_to_source (hash_dag_editors.h:364-379): position - dest, swirled about the y
     // axis (host cosf / sinf), through the inverse transform in FP64, plus source
@@ -341,7 +382,7 @@ struct Editor {
             if (old || !dec_voxel(pos, src)) return old;
             return value_at(*srcTable, srcRoot, src);
         }
-        if (e.kind == FILL) {  // BufferEditor::get_new_value (hash_dag_editors.h:211-220)
+        if (e.kind == FILL || is_noise()) {  // BufferEditor::get_new_value (hash_dag_editors.h:211-220)
             const F3 fp = pos - fmin;
             const u32 x = to_u32(fp.x), y = to_u32(fp.y), z = to_u32(fp.z);
             if (!fvalid(x, y, z) || !fget(x, y, z)) return old;
@@ -362,6 +403,7 @@ struct Editor {
             if (oldValue == newValue || !newValue || !dec_voxel(pos, src)) return oldColor();
             return color_at(*srcTable, srcRoot, *srcColors, src);
         }
+        if (is_noise()) return oldValue == newValue ? oldColor() : noise_color(pos);
         if (e.kind == FILL) {  // FillEditorColors::get_new_color (hash_dag_editors.h:737-754)
             if (newValue && !oldValue) {
                 const F3 fp = pos - fmin;
@@ -638,6 +680,7 @@ EditStats apply_edit(DAG& dag, HashDAGColors* colors, const Edit& e, UndoFrame* 
     if ((e.kind == COPY || e.kind == COPY_DEC || e.kind == FILL) && !colors)
         throw std::runtime_error("oracle: copy and fill need colours");
     if (e.kind == FILL) ed.build_fill();
+    if (ed.is_noise()) ed.build_noise();
     EditCtx ctx{dag.t, colors, ed, {}, {}, {}, {}};
     if (undo) {
         undo->root = dag.root;

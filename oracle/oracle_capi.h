@@ -154,6 +154,10 @@ int orc_gc(void* dag, uint32_t max_level);
 int orc_dag_reachable(void* dag, uint32_t max_level, uint64_t* out);
 /* the reference's in-bucket find on one bucket (hash_table.h:196-357): address or 0xFFFFFFFF */
 uint32_t orc_find_in_bucket(void* dag, uint32_t level, uint32_t bucket, const uint32_t* words, uint32_t n);
+/* FastNoise::GetPerlinFractal (FastNoise.cpp:760-797) with SetSeed(seed), SetFrequency(frequency),
+ * SetFractalOctaves(octaves) at n points xyz[3 * i]; octaves = 1 with frequency 1 gives
+ * SinglePerlin(perm[0], x, y, z). perm (optional, 1024 bytes): m_perm then m_perm12. */
+int orc_perlin(int seed, float frequency, int octaves, uint32_t n, const float* xyz, float* out, uint8_t* perm);
 
 #ifdef __cplusplus
 }

@@ -296,7 +296,21 @@ F3 sun_direction();
 // ---------------------------------------------------------------------------------------------
 enum EditKind : u32 {
     SPHERE_ADD = 0, SPHERE_CARVE = 1, BOX_ADD = 2, BOX_CARVE = 3, SPHERE_PAINT = 4, COPY = 5, FILL = 6,
-    COPY_DEC = 7  // CopyEditorWithDecompression (hash_dag_editors.h:437-616)
+    COPY_DEC = 7,  // CopyEditorWithDecompression (hash_dag_editors.h:437-616)
+    NOISE_ADD = 8, NOISE_CARVE = 9  // SphereNoiseEditor(isAdding) (hash_dag_editors.h:762-809)
+};
+
+// FastNoise's 3D Perlin fractal (FN_DECIMAL = float, quintic interpolation, FBM) as
+// SphereNoiseEditor configures it: SetSeed, SetFrequency, SetFractalOctaves
+// (FastNoise.cpp:195-237, 328-333, 760-797, 841-888; defaults FastNoise.h:222-230).
+struct PerlinFbm {
+    u8 perm[512], perm12[512];
+    float frequency = 0.01f, lacunarity = 2.f, gain = 0.5f, bounding = 0.f;
+    int octaves = 3;
+    void set_seed(int seed);
+    void set_octaves(int n);
+    float single(u8 offset, float x, float y, float z) const;
+    float fractal(float x, float y, float z) const;  // GetPerlinFractal
 };
 struct Edit {
     u32 kind;

@@ -17,6 +17,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _frames_match(dag, od, oc, cams, levels, W, H, what):
+    """Each camera's paths, colours and shaded colours against the oracle, traced with one
+    wavefront per tile and with persistent wavefronts."""
     import torch
 
     from hashdag_amd import DAGInfo, DAGTracer, ray_params
@@ -25,28 +27,33 @@ def _frames_match(dag, od, oc, cams, levels, W, H, what):
     info = DAGInfo.unit(levels)
     tracer = DAGTracer(W, H)
     for k, cam in enumerate(cams):
-        print(f"[{what}] camera {k} {W}x{H}", flush=True)
         prm = ray_params(cam, info, levels, W, H)
         oparams = Oracle.ray_params(cam, info, levels, W, H)
         assert np.array_equal(np.array(prm.as_list()), oparams), f"{what}: ray setup differs"
-        tracer.resolve_paths(cam, info, dag, params=prm)
-        tracer.resolve_colors(dag)
-        torch.cuda.synchronize()
-        gpaths = tracer.paths.cpu().numpy().view(np.uint32)
-        gcols = tracer.colors.cpu().numpy().view(np.uint32)
-        tracer.resolve_shadows(cam, info, dag, 1.0, 0.0, params=prm)
-        torch.cuda.synchronize()
-        gshad = tracer.colors.cpu().numpy().view(np.uint32)
         opaths = Oracle.trace_paths(od, oparams, W, H, threads=16)
-        bad = (gpaths != opaths).any(-1)
-        assert not bad.any(), f"{what} camera {k}: {int(bad.sum())} paths differ, first {np.argwhere(bad)[:3]}"
         assert (opaths[..., :3] != 0).any(-1).mean() > 0.05, f"{what} camera {k} sees almost nothing"
         ocols = Oracle.trace_colors(od, oc, opaths, W, H, threads=16)
-        assert np.array_equal(gcols, ocols), f"{what} camera {k}: {int((gcols != ocols).sum())} colours differ"
         oshad = Oracle.trace_shadows(od, oparams, opaths, ocols, W, H, 1.0, 0.0, threads=16)
-        assert np.array_equal(gshad, oshad), f"{what} camera {k}: {int((gshad != oshad).sum())} shaded pixels differ"
-        del tracer
-        tracer = DAGTracer(W, H)
+        for persistent in (False, True):
+            print(f"[{what}] camera {k} {W}x{H} persistent={persistent}", flush=True)
+            tracer.set_schedule(dag, persistent, 16)
+            tracer.resolve_paths(cam, info, dag, params=prm)
+            tracer.resolve_colors(dag)
+            torch.cuda.synchronize()
+            gpaths = tracer.paths.cpu().numpy().view(np.uint32)
+            gcols = tracer.colors.cpu().numpy().view(np.uint32)
+            tracer.resolve_shadows(cam, info, dag, 1.0, 0.0, params=prm)
+            torch.cuda.synchronize()
+            gshad = tracer.colors.cpu().numpy().view(np.uint32)
+            sched = "persistent" if persistent else "per tile"
+            bad = (gpaths != opaths).any(-1)
+            assert not bad.any(), f"{what} camera {k} {sched}: {int(bad.sum())} paths differ, first {np.argwhere(bad)[:3]}"
+            assert np.array_equal(gcols, ocols), f"{what} camera {k} {sched}: {int((gcols != ocols).sum())} colours differ"
+            assert np.array_equal(gshad, oshad), \
+                f"{what} camera {k} {sched}: {int((gshad != oshad).sum())} shaded pixels differ"
+            del tracer
+            tracer = DAGTracer(W, H)
+    tracer.set_schedule(dag, False)
 
 
 def test_full_size_frames_bit_exact():

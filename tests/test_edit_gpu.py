@@ -320,3 +320,63 @@ def test_fill_matches_oracle(fused):
     if fused:
         assert st.passes == 5  # [F o o] [F] [F o] [F o] [F]
     assert_same(dag, sc, od, oc, f"fills (fused={fused})")
+
+
+def _leaf_words(dag, g):
+    """Words used in the leaf level's buckets."""
+    img = dag.download()
+    L = g.leaf_level
+    b0 = g.top() * (1 << g.bucket_bits_top) + (L - g.top()) * (1 << g.bucket_bits_low) if L >= g.top() \
+        else L * (1 << g.bucket_bits_top)
+    nb = (1 << g.bucket_bits_low) if L >= g.top() else (1 << g.bucket_bits_top)
+    return int(img.bucket_sizes[b0:b0 + nb].astype(np.uint64).sum())
+
+
+def test_leaf_find_reuses_existing_leaves():
+    """The leaf-level find (tag bytes per slot, then the leaf) finds every leaf the table
+    holds: re-running an undone batch, and re-applying a batch after GC (which rebuilds the
+    tags), appends no leaf words; the DAG matches the oracle throughout."""
+    p, sc, g, dag, od, oc = make("d13")
+    edits = [e for e in random_edits(p, 24, seed=31) if e[0] != SPHERE_PAINT]
+    dag.edit_batch(edits, fused=True)
+    for kind, center, r in edits:
+        od.edit(kind, center, r, oc)
+    after = _leaf_words(dag, g)
+    dag.undo()
+    dag.edit_batch(edits, fused=True)
+    assert _leaf_words(dag, g) == after, "re-running the batch appended leaves the table held"
+    assert_same(dag, sc, od, oc, "re-run batch")
+    dag.remove_stale_nodes()
+    after_gc = _leaf_words(dag, g)
+    dag.edit_batch(edits, fused=True)  # idempotent: every leaf it builds exists
+    for kind, center, r in edits:
+        od.edit(kind, center, r, oc)
+    assert _leaf_words(dag, g) == after_gc, "after GC the batch appended leaves the table held"
+    assert_same(dag, sc, od, oc, "batch after GC")
+
+
+NOISE_ADD, NOISE_CARVE = 8, 9
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_noise_matches_oracle(fused):
+    """SphereNoiseEditor adding and carving (each starts its own pass, reading the DAG the
+    editors before it left), mixed with sphere / paint editors: voxel counts, DAG, colours and
+    leaf words against the oracle."""
+    from hashdag_amd import terrain_height
+    p, sc, g, dag, od, oc = make("d13")
+    F = p.footprint
+    edits = []
+    for k, (x, z, r) in enumerate(((60, 70, 6.0), (140, 90, 9.5), (200, 180, 4.0), (90, 200, 12.0))):
+        y = float(terrain_height(p, x, z) + (3 if k % 2 == 0 else 0))
+        kind = NOISE_CARVE if k == 2 else NOISE_ADD
+        edits.append((kind, (float(x), y, float(z)), r))
+        edits.append((SPHERE_PAINT if k % 2 else SPHERE_ADD, (float(x + 5), y, float(z + 3)), 5.0))
+    assert all(0 < c[0] < F and 0 < c[2] < F for _, c, _ in edits)
+    st = dag.edit_batch(edits, fused=fused)
+    total = 0
+    for kind, center, r in edits:
+        total += od.edit(kind, center, r, oc)[1]
+    assert st.passes == (4 if fused else len(edits))
+    assert st.num_voxels == total
+    assert_same(dag, sc, od, oc, f"noise fused={fused}")
