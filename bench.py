@@ -46,7 +46,8 @@ def main() -> int:
     ap.add_argument("--tile", type=str, default="auto", help="wavefront tile WxH, or auto (autotuned)")
     ap.add_argument("--schedule", type=str, default="auto",
                     help="tracer schedule: tile (one wavefront per tile), persist[:N] (persistent wavefronts "
-                         "refilled at N idle lanes), or auto (the autotuner's pick with --tile auto, else tile)")
+                         "refilled at N idle lanes), beam[:N] (per tile, walks beyond N steps searched a wavefront "
+                         "per ray), or auto (the autotuner's pick with --tile auto, else tile)")
     ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
     ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
     ap.add_argument("--no-edits", action="store_true")
@@ -70,7 +71,7 @@ def main() -> int:
     import torch
 
     from hashdag_amd import DAGInfo, DAGTracer, Geometry, HashDAG, TerrainParams, generate_terrain, ray_params
-    from hashdag_amd._lib import lib
+    from hashdag_amd._lib import check, lib
     from hashdag_amd.workload import cameras
 
     W, H, D = args.width, args.height, args.levels
@@ -102,9 +103,12 @@ def main() -> int:
         tracer.set_tile(dag, tw, th)
     if args.schedule.startswith("persist"):
         refill = int(args.schedule.split(":")[1]) if ":" in args.schedule else 16
-        tracer.set_schedule(dag, True, refill)
+        tracer.set_schedule(dag, 1, refill)
+    elif args.schedule.startswith("beam"):
+        cap = int(args.schedule.split(":")[1]) if ":" in args.schedule else 128
+        tracer.set_schedule(dag, 2, cap)
     elif args.schedule == "tile":
-        tracer.set_schedule(dag, False)
+        tracer.set_schedule(dag, 0)
     from hashdag_amd.tracer import launch_config
     launch = launch_config(dag)
     log(f"tracer launch: {launch}")
@@ -159,12 +163,14 @@ def main() -> int:
     dst = torch.empty_like(src)
     import ctypes as C
     for _ in range(3):
-        lib.hd_stream_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), nbytes, C.c_void_p(stream.cuda_stream))
+        check(lib.hd_stream_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), nbytes,
+                                 C.c_void_p(stream.cuda_stream)))
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     reps = 10
     for _ in range(reps):
-        lib.hd_stream_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), nbytes, C.c_void_p(stream.cuda_stream))
+        check(lib.hd_stream_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), nbytes,
+                                 C.c_void_p(stream.cuda_stream)))
     e1.record(stream)
     torch.cuda.synchronize()
     stream_gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
@@ -172,12 +178,12 @@ def main() -> int:
     rsrc = torch.cat([src, dst])
     fold = torch.empty(1 << 16, dtype=torch.int32, device="cuda")
     for _ in range(3):
-        lib.hd_stream_read(C.c_void_p(rsrc.data_ptr()), 2 * nbytes, C.c_void_p(fold.data_ptr()),
-                           C.c_void_p(stream.cuda_stream))
+        check(lib.hd_stream_read(C.c_void_p(rsrc.data_ptr()), 2 * nbytes, C.c_void_p(fold.data_ptr()),
+                                 C.c_void_p(stream.cuda_stream)))
     e0.record(stream)
     for _ in range(reps):
-        lib.hd_stream_read(C.c_void_p(rsrc.data_ptr()), 2 * nbytes, C.c_void_p(fold.data_ptr()),
-                           C.c_void_p(stream.cuda_stream))
+        check(lib.hd_stream_read(C.c_void_p(rsrc.data_ptr()), 2 * nbytes, C.c_void_p(fold.data_ptr()),
+                                 C.c_void_p(stream.cuda_stream)))
     e1.record(stream)
     torch.cuda.synchronize()
     read_gbs = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
@@ -317,7 +323,18 @@ def edit_bench(scene, geom, params, W, H, args):
     log(f"fused: {st_f.num_voxels} voxels in {t_f * 1e3:.1f} ms ({st_f.passes} pass), then GC {gc_f * 1e3:.1f} ms "
         f"({gcs_f['words_before']} -> {gcs_f['words_after']} words)")
     img_f = frames(dag)
+    # algorithmic bytes of the same fused pass, counted on a replay after undo + GC (the counting
+    # kernels are slower, so the timed pass above runs without them)
+    for _ in range(dag.undo_depth()[0]):
+        dag.undo()
+    dag.remove_stale_nodes()
+    dag.count_edit_bytes(True)
+    dag.edit_batch(edits, fused=True)
+    ab = dag.count_edit_bytes(False)
+    torch.cuda.synchronize()
     dag.close()
+    abytes = sum(ab.values())
+    log(f"fused pass algorithmic bytes: {ab} = {abytes / 1e9:.3f} GB -> {abytes / t_f / 1e9:.1f} GB/s")
     plain = HashDAG(geom)
     plain.load_scene(scene)
     st_p, t_p, gc_p, _ = timed(plain, False, edits[:16])
@@ -343,6 +360,11 @@ def edit_bench(scene, geom, params, W, H, args):
         "batched_with_gc_voxels_per_s": round(st_f.num_voxels / (t_f + gc_f), 1),
         "batched_over_plain_with_gc": round((t_p + gc_p) / (t_f + gc_f), 2),
         "results_identical": same,
+        "roofline": {"bound": "hbm", "achieved": round(abytes / t_f / 1e9, 2), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(abytes / t_f / 8e12, 5), "traffic": None,
+                     "bytes": ab, "note": "algorithmic bytes of the fused pass (node words read by the descent, "
+                     "bucket bytes read by the finds, node words written, staged colours written and read, colour "
+                     "leaf words written) over its wall time including host steps"},
     }
 
 

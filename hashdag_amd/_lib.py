@@ -149,6 +149,7 @@ _SIGS = {
                                   C.c_void_p]),
     "hd_edit_batch_static": (C.c_int32, [C.c_void_p, C.POINTER(hd_edit), C.c_uint32, C.POINTER(hd_edit_stats),
                                          C.c_void_p]),
+    "hd_edit_count_bytes": (C.c_int32, [C.c_void_p, C.c_uint32, u64p]),
     "hd_gc": (C.c_int32, [C.c_void_p, C.c_uint32, u64p, C.c_void_p]),
     "hd_write_basic_dag_file": (C.c_int32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint32, u32p,
                                             C.c_uint64]),
@@ -209,6 +210,7 @@ _SIGS = {
     "hd_trace_count_bytes": (C.c_int32, [C.c_void_p, C.POINTER(hd_ray_params), C.c_uint32, C.c_uint32, C.c_void_p,
                                          C.c_void_p, C.c_float, C.c_float, u64p]),
     "hd_stream_copy": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "hd_stream_read": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
     "hd_scene_generate": (C.c_int32, [C.POINTER(hd_terrain_params), C.POINTER(C.c_void_p)]),
     "hd_scene_free": (None, [C.c_void_p]),
     "hd_scene_sizes": (C.c_int32, [C.c_void_p, u64p]),

@@ -167,6 +167,8 @@ hd_status hd_create(const hd_config* cfg, hd_ctx** out) {
     if ((e = hipMalloc(&ctx->d_root, 64)) != hipSuccess) return cleanup(hip_fail(nullptr, e, "root"));
     if ((e = hipMalloc(&ctx->leaf_tags, std::max<uint64_t>(leaf_tag_bytes(g), 4))) != hipSuccess)
         return cleanup(hip_fail(nullptr, e, "leaf tags"));
+    if ((e = hipMalloc(&ctx->edit_bytes, 8 * sizeof(unsigned long long))) != hipSuccess)
+        return cleanup(hip_fail(nullptr, e, "edit byte counters"));
     hipMemsetAsync(ctx->vpool, 0, vwords * 4, ctx->stream);
     hipMemsetAsync(ctx->page_table, 0, g.total_pages() * 4, ctx->stream);
     hipMemsetAsync(ctx->bucket_sizes, 0, g.total_buckets() * 4, ctx->stream);
@@ -195,6 +197,7 @@ hd_status hd_destroy(hd_ctx* ctx) {
     hipFree(ctx->byte_counters);
     trace_scratch_free(ctx);
     hipFree(ctx->leaf_tags);
+    hipFree(ctx->edit_bytes);
     hipFree(ctx->cnodes);
     hipFree(ctx->coffsets);
     hipFree(ctx->cleaves);

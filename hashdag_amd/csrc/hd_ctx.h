@@ -21,6 +21,12 @@ struct TraceScratch {
     uint2* shades = nullptr;
     uint32_t* cnt = nullptr;
     uint64_t tiles_cap = 0;
+    // rays handed from the capped per-tile walk to the wavefront-per-ray search, and from it
+    // back to the plain walk: pixel lists of pixels_cap entries, counts in long_count[0..1]
+    uint32_t* long_items = nullptr;
+    uint32_t* rest_items = nullptr;
+    uint32_t* long_count = nullptr;
+    uint64_t pixels_cap = 0;
 };
 
 // One undo step (HashDAGUndoRedo::Frame, hash_dag.h:29-45): the roots before / after and the
@@ -49,6 +55,11 @@ struct hd_ctx {
     // leaf level: one tag byte per 2-word slot of every bucket (bits 24-31 of the leaf's hash),
     // valid below each bucket's size; finds scan the tags before comparing leaves
     uint8_t* leaf_tags = nullptr;
+    // edit pass measurement (hd_edit_count_bytes): device counters of the kernels, host sums of
+    // the passes' own statistics
+    unsigned long long* edit_bytes = nullptr;
+    bool count_edit_bytes = false;
+    uint64_t edit_bytes_host[3] = {};
     uint32_t root = 0;
     bool loaded = false;
 
@@ -72,6 +83,9 @@ struct hd_ctx {
     // tracer schedule: 0 = one wavefront per pixel tile, 1 = persistent wavefronts fed from
     // tile queues, refilled once refill_at lanes are idle
     uint32_t trace_persist = 0, refill_at = 16;
+    // per-tile schedule: walks longer than long_steps steps (0: no cap) are finished by the
+    // wavefront-per-ray search
+    uint32_t long_steps = 0;
     uint32_t persist_blocks[4] = {};   // resident blocks per persistent kernel (0: not measured)
     // scratch of the persistent schedule, one set per stream traced on (traces on different
     // streams of one context may overlap)

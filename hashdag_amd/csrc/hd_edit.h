@@ -128,6 +128,9 @@ struct EditPass {
     uint32_t* pool_top;
     uint32_t* errors;
     uint8_t* leaf_tags;  // tag byte per leaf slot (leaf level), kept by k_add, read by k_find
+    // algorithmic bytes when measuring (else null): [0] node words read by the descent,
+    // [1] bucket bytes read by finds
+    unsigned long long* abytes;
     uint32_t* n_cnodes;
     uint32_t* n_leaves;
     uint32_t* undo_count;

@@ -436,6 +436,7 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
     const uint32_t depth = D - (l + 1);  // child cube side = 2^depth
     const float side = (float)(1u << depth);
     const uint64_t total = (uint64_t)n * 8;
+    uint64_t nodeBytes = 0;  // header + child pointers of the items' nodes (measurement)
     // block-uniform loop: child items and edit-list words are allocated with one atomic each
     // per block
     for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < total; b0 += (uint64_t)gridDim.x * blockDim.x) {
@@ -457,12 +458,14 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
             if (old != kInvalidNode) {
                 const uint32_t cm = P.vpool[old] & 0xFFu;
                 if (cm & (1u << c)) childOld = P.vpool[old + child_off(cm, c)];
+                if (c == 0) nodeBytes += 4u * (1u + (uint32_t)__popc(cm));
             }
             if (orig == old) {
                 childOrig = childOld;
             } else if (orig != kInvalidNode) {
                 const uint32_t cm = P.vpool[orig] & 0xFFu;
                 if (cm & (1u << c)) childOrig = P.vpool[orig + child_off(cm, c)];
+                if (c == 0) nodeBytes += 4u * (1u + (uint32_t)__popc(cm));
             }
             x0 = (float)(cx << depth);
             y0 = (float)(cy << depth);
@@ -583,6 +586,7 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
         }
         I.ckind[slot] = (uint8_t)kind;
     }
+    if (P.abytes) wave_add64(P.abytes + 0, nodeBytes);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -656,6 +660,7 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
     const uint32_t n = min(P.count[L], P.cap[L]);
     if (*P.errors) return;
     const Items I = P.items[L];
+    uint64_t leafBytes = 0;  // old and original leaves read (measurement)
     for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += gridDim.x * blockDim.x) {
         const uint32_t i = t0 + (threadIdx.x & 63u);
         const bool active = i < n;
@@ -663,6 +668,7 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
         if (active) {
             const uint32_t bx = I.px[i] << 2, by = I.py[i] << 2, bz = I.pz[i] << 2;
             const uint32_t old = I.old_[i], orig = I.orig[i];
+            leafBytes += (old != kInvalidNode ? 8u : 0u) + (orig != kInvalidNode && orig != old ? 8u : 0u);
             uint64_t v = old != kInvalidNode ? *reinterpret_cast<const uint64_t*>(P.vpool + old) : 0ull;
             const uint64_t origMask = orig != kInvalidNode ? *reinterpret_cast<const uint64_t*>(P.vpool + orig) : 0ull;
             const uint32_t es = I.estart[i], ec = I.ecount[i];
@@ -739,6 +745,7 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
         }
         if (P.single_edit) wave_add64(P.stat_voxels, (uint64_t)__popcll(touched));
     }
+    if (P.abytes) wave_add64(P.abytes + 0, leafBytes);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -879,6 +886,7 @@ __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
     const bool leafLevel = l == g.leaf_level();
     // the representatives listed by k_dedup
     const uint32_t nReps = min(P.find_count[l], n);
+    uint64_t scanBytes = 0;  // bucket bytes this lane read (measurement)
     for (uint32_t r = wave_index(); r < nReps; r += wave_count()) {
         const uint32_t i = I.nstart[r];
         const uint32_t bucket = g.bucket_from_hash(l, I.hash[i]);
@@ -897,10 +905,12 @@ __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
                 uint32_t hitSlot = ~0u;
                 if (sl < nslots) {
                     const uint32_t four = *reinterpret_cast<const uint32_t*>(tags + sl);
+                    scanBytes += min(4u, nslots - sl);
                     for (uint32_t k = 0; k < 4 && hitSlot == ~0u; ++k)
-                        if (sl + k < nslots && ((four >> (8 * k)) & 0xFFu) == want &&
-                            *reinterpret_cast<const uint64_t*>(P.vpool + base + 2 * (sl + k)) == key)
-                            hitSlot = sl + k;
+                        if (sl + k < nslots && ((four >> (8 * k)) & 0xFFu) == want) {
+                            scanBytes += 8;
+                            if (*reinterpret_cast<const uint64_t*>(P.vpool + base + 2 * (sl + k)) == key) hitSlot = sl + k;
+                        }
                 }
                 const uint64_t b = __ballot(hitSlot != ~0u);
                 if (b) found = 2u * (uint32_t)__shfl((int)hitSlot, __ffsll((long long)b) - 1, 64);
@@ -909,6 +919,7 @@ __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
             const uint64_t key = (uint64_t)I.cont[i * 9] | ((uint64_t)I.cont[i * 9 + 1] << 32);
             for (uint32_t p0 = 0; p0 < bs && found == ~0u; p0 += 128) {
                 const uint32_t pos = p0 + 2 * lane;
+                if (pos + 2 <= bs) scanBytes += 8;
                 const bool hit = pos + 2 <= bs && *reinterpret_cast<const uint64_t*>(P.vpool + base + pos) == key;
                 const uint64_t b = __ballot(hit);
                 if (b) found = p0 + 2 * (uint32_t)__ffsll((long long)b) - 2;
@@ -922,13 +933,16 @@ __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
             for (uint32_t p0 = 0; p0 + nw < bs && found == ~0u; p0 += 64) {
                 const uint32_t pos = p0 + lane;
                 const uint32_t pageEnd = min(bs, pos - pos % ps + ps);
+                if (pos < bs) scanBytes += 4;
                 bool hit = pos + nw < pageEnd && P.vpool[base + pos] == h0;
                 if (hit)
-                    for (uint32_t k = 1; k < nw; ++k)
+                    for (uint32_t k = 1; k < nw; ++k) {
+                        scanBytes += 4;
                         if (P.vpool[base + pos + k] != I.cont[i * 9 + k]) {
                             hit = false;
                             break;
                         }
+                    }
                 uint64_t b = __ballot(hit);
                 while (b) {
                     const uint32_t c = p0 + (uint32_t)__ffsll((long long)b) - 1;  // wave-uniform
@@ -949,6 +963,7 @@ __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
             }
         }
     }
+    if (P.abytes) wave_add64(P.abytes + 1, scanBytes);
 }
 
 __device__ bool alloc_page(EditPass& P, uint32_t page) {
@@ -2201,6 +2216,7 @@ static void fill_pass(hd_ctx* ctx, EditPass& P, const EditScratch& S) {
     P.pool_top = ctx->counters + 0;
     P.errors = ctx->counters + 1;
     P.leaf_tags = ctx->leaf_tags;
+    P.abytes = ctx->count_edit_bytes ? ctx->edit_bytes : nullptr;
     P.n_cnodes = ctx->counters + 2;
     P.n_leaves = ctx->counters + 3;
     P.undo_count = ctx->counters + 4;

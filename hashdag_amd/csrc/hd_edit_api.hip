@@ -384,6 +384,11 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
         }
         if (err & ERR_EMPTY) return fail(ctx, HD_ERR_INVALID, "edit would leave an empty scene (old root kept)");
         if (err & ERR_KIND) return fail(ctx, HD_ERR_INVALID, "copy or fill editor in a pass built without them");
+        if (ctx->count_edit_bytes) {
+            ctx->edit_bytes_host[0] += rb.stats[1] * 4;                  // node words written
+            ctx->edit_bytes_host[1] += rb.stats[4] * 16;                 // staged colours written and read
+            ctx->edit_bytes_host[2] += (rb.stats[3] - ctx->arena_top) * 8;  // colour leaf words written
+        }
         ctx->arena_top = rb.stats[3];
         ctx->n_cnodes = rb.counters[2];
         ctx->n_cleaves = rb.counters[3];
@@ -544,6 +549,30 @@ hd_status hd_undo_depth(hd_ctx* ctx, uint32_t* undo, uint32_t* redo) {
     if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
     if (undo) *undo = (uint32_t)ctx->undo.size();
     if (redo) *redo = (uint32_t)ctx->redo.size();
+    return HD_OK;
+}
+
+hd_status hd_edit_count_bytes(hd_ctx* ctx, uint32_t on, uint64_t out[5]) {
+    if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
+    HD_HIP(ctx, hipDeviceSynchronize());
+    if (on) {
+        HD_HIP(ctx, hipMemset(ctx->edit_bytes, 0, 8 * sizeof(unsigned long long)));
+        for (uint64_t& v : ctx->edit_bytes_host) v = 0;
+        ctx->count_edit_bytes = true;
+        ctx->ptr_gen++;  // captured passes bake the counter pointer in
+        return HD_OK;
+    }
+    unsigned long long dev[2] = {0, 0};
+    HD_HIP(ctx, hipMemcpy(dev, ctx->edit_bytes, sizeof(dev), hipMemcpyDeviceToHost));
+    if (ctx->count_edit_bytes) ctx->ptr_gen++;
+    ctx->count_edit_bytes = false;
+    if (out) {
+        out[0] = dev[0];
+        out[1] = dev[1];
+        out[2] = ctx->edit_bytes_host[0];
+        out[3] = ctx->edit_bytes_host[1];
+        out[4] = ctx->edit_bytes_host[2];
+    }
     return HD_OK;
 }
 
@@ -735,7 +764,7 @@ static hd_status graph_capture(hd_frame_graph* g) {
     EditScratch& S = ctx->edit;
     const bool colors = ctx->colors_loaded && ctx->g.leaf_level() > ctx->g.color_tree_levels;
     hipStream_t s = g->stream;
-    if (ctx->trace_persist) {
+    if (ctx->trace_persist || ctx->long_steps) {
         const hd_status rs = trace_reserve(ctx, s, g->width, g->height);
         if (rs != HD_OK) return rs;
     }

@@ -285,6 +285,40 @@ __device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, u
     return r == kWalkHit;
 }
 
+// descend() for at most `cap` steps: kWalkHit / kWalkMiss as descend, kWalkOn when the walk
+// was stopped at the cap (the ray is then traced by the wavefront-per-ray search).
+template <int kFetch>
+__device__ __forceinline__ uint32_t descend_capped(const DevDag& dag, F3 o, F3 d, F3 inv, uint32_t order, uint32_t& px,
+                                                   uint32_t& py, uint32_t& pz, Stack st, uint32_t cap) {
+    Walk w;
+    uint64_t unused = 0;
+    walk_begin<false, kFetch>(dag, w, o, d, inv, unused);
+    uint32_t r, steps = 0;
+    do {
+        r = walk_step<false, kFetch>(dag, w, o, d, inv, order, st, unused);
+    } while (r == kWalkOn && ++steps < cap);
+    px = w.px;
+    py = w.py;
+    pz = w.pz;
+    return r;
+}
+
+// Rays whose walk stopped at the cap: appended as pixel indices, one atomic per wavefront.
+struct LongRays {
+    uint32_t* items;  // pixel indices (x + y * W)
+    uint32_t* count;
+    uint32_t cap;     // walk steps before a ray is handed over; ~0u: never
+};
+__device__ __forceinline__ void long_append(bool give, uint32_t pix, const LongRays& L) {
+    const uint64_t m = __ballot(give);
+    if (!m) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0;
+    if (lane == (uint32_t)__ffsll((long long)m) - 1u) base = atomicAdd(L.count, (uint32_t)__popcll(m));
+    base = __shfl(base, __ffsll((long long)m) - 1, 64);
+    if (give) L.items[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = pix;
+}
+
 __device__ __forceinline__ void dnormalize(double& x, double& y, double& z) {
     const double len = __dsqrt_rn(x * x + y * y + z * z);
     const double s = 1.0 / len;
@@ -322,25 +356,33 @@ __device__ __forceinline__ void primary_ray(const RayP& p, uint32_t x, uint32_t 
 }
 
 // tracer.cu:145-252
+// L.cap < ~0u: walks stopped at L.cap steps leave their pixel to the wavefront-per-ray search
+// (appended to L) and write nothing.
 template <bool kCount, int kFetch>
 __global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W, uint32_t H,
                                                         uint4* __restrict__ out, uint32_t tileW, uint32_t tileH,
-                                                        unsigned long long* counter) {
+                                                        unsigned long long* counter, LongRays L) {
     const Stack st = stack_of(dag.levels);
     if (pmem) p = *pmem;
     if (dag.root_ptr) dag.root = *dag.root_ptr;
     uint32_t x, y;
     pixel_of(tileW, tileH, x, y);
     uint64_t bytes = 0;
+    bool give = false;
     if (x < W && y < H) {
         F3 o, d, inv;
         uint32_t order;
         primary_ray(p, x, y, o, d, inv, order);
         uint32_t vx, vy, vz;
-        descend<kCount, kFetch>(dag, o, d, inv, order, vx, vy, vz, st, bytes);
-        out[(uint64_t)(H - 1 - y) * W + x] = make_uint4(vx, vy, vz, 0u);
+        if (kCount || L.cap == ~0u) {
+            descend<kCount, kFetch>(dag, o, d, inv, order, vx, vy, vz, st, bytes);
+        } else {
+            give = descend_capped<kFetch>(dag, o, d, inv, order, vx, vy, vz, st, L.cap) == kWalkOn;
+        }
+        if (!give) out[(uint64_t)(H - 1 - y) * W + x] = make_uint4(vx, vy, vz, 0u);
         if (kCount) bytes += 16;
     }
+    if (!kCount && L.cap != ~0u) long_append(give, y * W + x, L);
     flush_count<kCount>(bytes, counter);
 }
 
@@ -628,13 +670,14 @@ template <bool kCount, int kFetch>
 __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W, uint32_t H,
                                                           const uint4* __restrict__ paths, uint32_t* __restrict__ colors,
                                                           float bias, float fog, uint32_t tileW, uint32_t tileH,
-                                                          unsigned long long* counter) {
+                                                          unsigned long long* counter, LongRays L) {
     const Stack st = stack_of(dag.levels);
     if (pmem) p = *pmem;
     if (dag.root_ptr) dag.root = *dag.root_ptr;
     uint32_t x, y;
     pixel_of(tileW, tileH, x, y);
     uint64_t bytes = 0;
+    bool give = false;
     if (x < W && y < H) {
         const uint64_t pix = (uint64_t)y * W + x;
         const uint4 pp = paths[pix];
@@ -660,8 +703,16 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, co
             const F3 inv{(float)(1.0 / (double)sun.x), (float)(1.0 / (double)sun.y), (float)(1.0 / (double)sun.z)};
             const uint32_t order = (sun.x < 0.f ? 4u : 0u) + (sun.y < 0.f ? 2u : 0u) + (sun.z < 0.f ? 1u : 0u);
             uint32_t vx, vy, vz;
-            const bool shadowed = descend<kCount, kFetch>(dag, o, sun, inv, order, vx, vy, vz, st, bytes);
-            if (noFog) {
+            bool shadowed;
+            if (kCount || L.cap == ~0u) {
+                shadowed = descend<kCount, kFetch>(dag, o, sun, inv, order, vx, vy, vz, st, bytes);
+            } else {
+                const uint32_t r = descend_capped<kFetch>(dag, o, sun, inv, order, vx, vy, vz, st, L.cap);
+                give = r == kWalkOn;
+                shadowed = r == kWalkHit;
+            }
+            if (give) {
+            } else if (noFog) {
                 colors[pix] = shade_clear(colorIn, shadowed ? 0.f : 1.f);
             } else {
                 double distance, nx, ny, nz;
@@ -670,6 +721,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, co
             }
         }
     }
+    if (!kCount && L.cap != ~0u) long_append(give, y * W + x, L);
     flush_count<kCount>(bytes, counter);
 }
 
@@ -892,6 +944,259 @@ __global__ __launch_bounds__(kBlock) void k_trace_persist(DevDag dag, RayP p, co
     }
 }
 
+// ---- wavefront per ray: ordered beam search for the rays the per-tile walk handed over ----
+//
+// A few rays per frame take hundreds of DFS steps (rays grazing the terrain), and the wavefront
+// holding one lives that long: the per-tile walks stop at a step cap and hand those rays here.
+// The 64 lanes search one ray together. The frontier holds the next (at most) 64 unexpanded
+// nodes of the ray's DFS in the reference's order, one per lane; each round every lane expands
+// its node into the children the node's intersection mask lets through, in the order
+// child ^ order, and the concatenation in lane order is again in DFS order (the subtrees of
+// frontier nodes are disjoint and ordered). The first 64 form the next frontier, the rest go
+// onto a per-wavefront ring whose top is the smallest; a short frontier takes entries back
+// from the top. The first hit is found when the head of the frontier is a voxel (nothing
+// before it in DFS order is left); an any-hit search stops at any voxel. When the ring would
+// overflow it drops its largest entries and a search that then ends without a hit is retried
+// by the plain walk (k_trace_rest); so is one that runs out of rounds.
+constexpr uint32_t kBeamRing = 512;  // ring entries per wavefront (a power of two)
+constexpr uint32_t kBeamRounds = 4096;
+
+// entry = node index (or leaf address), a = px | (py low 15 bits) << 17, b = py >> 15 | pz << 2
+// | level << 19 (paths have at most 17 bits per axis)
+__device__ __forceinline__ void beam_unpack(uint32_t a, uint32_t b, uint32_t& level, uint32_t& px, uint32_t& py,
+                                            uint32_t& pz) {
+    px = a & 0x1FFFFu;
+    py = (a >> 17) | ((b & 3u) << 15);
+    pz = (b >> 2) & 0x1FFFFu;
+    level = b >> 19;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct BeamLds {
+    uint32_t* f;  // 3 x 64: frontier staging (index, a, b)
+    uint32_t* r;  // 3 x kBeamRing: ring (index, a, b)
+};
+
+// Returns 1 on a hit (voxel path in hx, hy, hz), 0 on a miss, 2 when the walk must be retried.
+// Wave-uniform: every lane calls it with the same ray.
+template <bool kAnyHit>
+__device__ uint32_t beam_search(const DevDag& dag, F3 o, F3 d, F3 inv, uint32_t order, BeamLds lds, uint32_t& hx,
+                                uint32_t& hy, uint32_t& hz) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t levels = dag.levels, L = dag.leaf_level;
+    const uint32_t* __restrict__ vpool = dag.vpool;
+    constexpr uint32_t kMask = kBeamRing - 1u;
+    bool valid = lane == 0;
+    uint32_t level = 0, px = 0, py = 0, pz = 0, index = dag.root;
+    uint32_t top = 0, bottom = 0;  // ring holds [bottom, top); top - 1 is the smallest entry
+    bool truncated = false;
+    for (uint32_t round = 0; round < kBeamRounds; ++round) {
+        if (!__ballot(valid)) return truncated ? 2u : 0u;
+        if (kAnyHit) {
+            const uint64_t hm = __ballot(valid && level == levels);
+            if (hm) {
+                const int src = __ffsll((long long)hm) - 1;
+                hx = (uint32_t)__shfl((int)px, src, 64);
+                hy = (uint32_t)__shfl((int)py, src, 64);
+                hz = (uint32_t)__shfl((int)pz, src, 64);
+                return 1u;
+            }
+        } else if ((uint32_t)__shfl((int)level, 0, 64) == levels) {  // lane 0 holds the head
+            hx = (uint32_t)__shfl((int)px, 0, 64);
+            hy = (uint32_t)__shfl((int)py, 0, 64);
+            hz = (uint32_t)__shfl((int)pz, 0, 64);
+            return 1u;
+        }
+        // expand: the children the node's mask lets through, or the voxel entry itself
+        uint32_t cm = 0, visit = 0, cnt = 0;
+        uint32_t nd[9];
+        if (valid && level < levels) {
+            if (level < L) {
+                load_node(vpool, index, nd);
+                cm = nd[0] & 0xFFu;
+            } else {
+                const uint64_t leaf = *reinterpret_cast<const uint64_t*>(vpool + index);
+                cm = level == L ? leaf_first_mask(leaf)
+                                : leaf_second_mask(leaf, ((px & 1u) << 2) | ((py & 1u) << 1) | (pz & 1u));
+            }
+            const uint32_t im = level == 0 ? intersection_mask<true>(levels, 0, 0, 0, 0, o, d, inv)
+                                           : intersection_mask<false>(levels, level, px, py, pz, o, d, inv);
+            visit = cm & im;
+            cnt = (uint32_t)__popc(visit);
+        } else if (valid) {
+            cnt = 1;
+        }
+        uint32_t incl = cnt;
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t t = (uint32_t)__shfl_up((int)incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        const uint32_t T = (uint32_t)__shfl((int)incl, 63, 64);
+        const uint32_t spilled = T > 64u ? T - 64u : 0u;
+        if (top - bottom + spilled > kBeamRing) {
+            bottom = top + spilled - kBeamRing;  // the largest entries make room
+            truncated = true;
+        }
+        uint32_t pos = incl - cnt;
+        auto put = [&](uint32_t idx, uint32_t lv, uint32_t x, uint32_t y, uint32_t z) {
+            const uint32_t a = x | (y << 17), b = (y >> 15) | (z << 2) | (lv << 19);
+            if (pos < 64u) {
+                lds.f[pos] = idx;
+                lds.f[64 + pos] = a;
+                lds.f[128 + pos] = b;
+            } else {
+                const uint32_t q = (top + (T - 1u - pos)) & kMask;
+                lds.r[q] = idx;
+                lds.r[kBeamRing + q] = a;
+                lds.r[2 * kBeamRing + q] = b;
+            }
+            ++pos;
+        };
+        if (valid && level == levels) {
+            put(index, level, px, py, pz);
+        } else if (valid) {
+            uint32_t v = visit;
+            while (v) {
+                const uint32_t c = next_child(order, v);
+                v &= ~(1u << c);
+                const uint32_t ci = level < L ? pick(nd, child_off(cm, c)) : index;
+                put(ci, level + 1, (px << 1) | ((c >> 2) & 1u), (py << 1) | ((c >> 1) & 1u), (pz << 1) | (c & 1u));
+            }
+        }
+        top += spilled;
+        wave_lds_sync();
+        const uint32_t n = min(T, 64u);
+        valid = lane < n;
+        uint32_t ea = 0, eb = 0;
+        if (valid) {
+            index = lds.f[lane];
+            ea = lds.f[64 + lane];
+            eb = lds.f[128 + lane];
+        }
+        const uint32_t m = min(64u - n, top - bottom);
+        if (lane >= n && lane < n + m) {
+            const uint32_t q = (top - 1u - (lane - n)) & kMask;
+            index = lds.r[q];
+            ea = lds.r[kBeamRing + q];
+            eb = lds.r[2 * kBeamRing + q];
+            valid = true;
+        }
+        top -= m;
+        if (valid) beam_unpack(ea, eb, level, px, py, pz);
+        wave_lds_sync();
+    }
+    return 2u;
+}
+
+// The rays the per-tile walk handed over (L), one wavefront each; rays the search could not
+// finish go to `rest`.
+template <bool kShadow>
+__global__ __launch_bounds__(kBlock) void k_trace_beam(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W,
+                                                       uint32_t H, uint4* __restrict__ paths,
+                                                       uint32_t* __restrict__ colors, float bias, float fog,
+                                                       LongRays L, LongRays rest) {
+    __shared__ uint32_t sF[kBlock / 64][3 * 64];
+    __shared__ uint32_t sR[kBlock / 64][3 * kBeamRing];
+    if (pmem) p = *pmem;
+    if (dag.root_ptr) dag.root = *dag.root_ptr;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const BeamLds lds{sF[wave], sR[wave]};
+    const uint32_t n = *L.count;
+    for (uint32_t r = blockIdx.x * (kBlock / 64) + wave; r < n; r += gridDim.x * (kBlock / 64)) {
+        const uint32_t pix = __builtin_amdgcn_readfirstlane(L.items[r]);
+        const uint32_t x = pix % W, y = pix / W;
+        uint32_t hx = 0, hy = 0, hz = 0, res;
+        F3 sun{};
+        if (kShadow) {
+            sun_direction(sun.x, sun.y, sun.z);
+            const uint4 pp = paths[pix];
+            double cx, cy, cz;
+            camera_dir(p, x, H - 1 - y, cx, cy, cz);
+            const F3 o = shadow_origin(p, pp, cx, cy, cz, bias, sun);
+            const F3 inv{(float)(1.0 / (double)sun.x), (float)(1.0 / (double)sun.y), (float)(1.0 / (double)sun.z)};
+            const uint32_t order = (sun.x < 0.f ? 4u : 0u) + (sun.y < 0.f ? 2u : 0u) + (sun.z < 0.f ? 1u : 0u);
+            res = beam_search<true>(dag, o, sun, inv, order, lds, hx, hy, hz);
+        } else {
+            F3 o, d, inv;
+            uint32_t order;
+            primary_ray(p, x, y, o, d, inv, order);
+            res = beam_search<false>(dag, o, d, inv, order, lds, hx, hy, hz);
+        }
+        if (lane != 0) continue;
+        if (res == 2u) {
+            rest.items[atomicAdd(rest.count, 1u)] = pix;
+        } else if (kShadow) {
+            const uint32_t colorIn = colors[pix];
+            const float light = res == 1u ? 0.f : 1.f;
+            if (fog == 0.f) {
+                colors[pix] = shade_clear(colorIn, light);
+            } else {
+                const uint4 pp = paths[pix];
+                double distance, nx, ny, nz;
+                hit_distance(p, pp, distance, nx, ny, nz);
+                colors[pix] = shade(colorIn, light, distance, nx, ny, nz, fog, sun);
+            }
+        } else {
+            paths[(uint64_t)(H - 1 - y) * W + x] = res == 1u ? make_uint4(hx, hy, hz, 0u) : make_uint4(0u, 0u, 0u, 0u);
+        }
+    }
+}
+
+// The rays the beam search gave back: the plain walk without a cap, one lane each.
+template <bool kShadow, int kFetch>
+__global__ __launch_bounds__(kBlock) void k_trace_rest(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W,
+                                                       uint32_t H, uint4* __restrict__ paths,
+                                                       uint32_t* __restrict__ colors, float bias, float fog,
+                                                       LongRays rest) {
+    const Stack st = stack_of(dag.levels);
+    if (pmem) p = *pmem;
+    if (dag.root_ptr) dag.root = *dag.root_ptr;
+    const uint32_t n = *rest.count;
+    uint64_t unused = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t pix = rest.items[i], x = pix % W, y = pix / W;
+        uint32_t vx, vy, vz;
+        if (kShadow) {
+            F3 sun;
+            sun_direction(sun.x, sun.y, sun.z);
+            const uint4 pp = paths[pix];
+            double cx, cy, cz;
+            camera_dir(p, x, H - 1 - y, cx, cy, cz);
+            const F3 o = shadow_origin(p, pp, cx, cy, cz, bias, sun);
+            const F3 inv{(float)(1.0 / (double)sun.x), (float)(1.0 / (double)sun.y), (float)(1.0 / (double)sun.z)};
+            const uint32_t order = (sun.x < 0.f ? 4u : 0u) + (sun.y < 0.f ? 2u : 0u) + (sun.z < 0.f ? 1u : 0u);
+            const bool shadowed = descend<false, kFetch>(dag, o, sun, inv, order, vx, vy, vz, st, unused);
+            const uint32_t colorIn = colors[pix];
+            if (fog == 0.f) {
+                colors[pix] = shade_clear(colorIn, shadowed ? 0.f : 1.f);
+            } else {
+                double distance, nx, ny, nz;
+                hit_distance(p, pp, distance, nx, ny, nz);
+                colors[pix] = shade(colorIn, shadowed ? 0.f : 1.f, distance, nx, ny, nz, fog, sun);
+            }
+        } else {
+            F3 o, d, inv;
+            uint32_t order;
+            primary_ray(p, x, y, o, d, inv, order);
+            descend<false, kFetch>(dag, o, d, inv, order, vx, vy, vz, st, unused);
+            paths[(uint64_t)(H - 1 - y) * W + x] = make_uint4(vx, vy, vz, 0u);
+        }
+    }
+}
+
+__global__ void k_long_reset(uint32_t* a, uint32_t* b) {
+    if (threadIdx.x == 0) {
+        *a = 0;
+        *b = 0;
+    }
+}
+
 inline RayP to_rayp(const hd_ray_params* p) {
     RayP r;
     for (int i = 0; i < 3; ++i) {
@@ -955,13 +1260,46 @@ hd_status launch_persist(hd_ctx* ctx, const DevDag& dag, const RayP& p, const Ra
 
 // Primary paths with the fetch strategy and schedule of the context; pm (device) overrides p
 // when given.
+// The handed-over rays of a capped per-tile walk: the wavefront-per-ray search, then the plain
+// walk for the few it gives back.
+template <bool kShadow>
+hd_status launch_long(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP* pm, uint32_t W, uint32_t H,
+                      uint4* paths, uint32_t* colors, float bias, float fog, const LongRays& L, const LongRays& rest,
+                      hipStream_t s) {
+    k_trace_beam<kShadow><<<2048, kBlock, 0, s>>>(dag, p, pm, W, H, paths, colors, bias, fog, L, rest);
+    HD_HIP(ctx, hipGetLastError());
+    const auto k = ctx->fetch_mode == kFetchNode ? k_trace_rest<kShadow, kFetchNode> : k_trace_rest<kShadow, kFetchPointer>;
+    k<<<64, kBlock, stack_bytes(ctx->g.levels), s>>>(dag, p, pm, W, H, paths, colors, bias, fog, rest);
+    HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+
+// The per-tile walk's hand-over lists (cap ~0u when the context runs walks to the end).
+hd_status long_lists(hd_ctx* ctx, uint32_t W, uint32_t H, hipStream_t s, LongRays& L, LongRays& rest) {
+    L = LongRays{nullptr, nullptr, ~0u};
+    rest = L;
+    if (!ctx->long_steps) return HD_OK;
+    TraceScratch* ts = nullptr;
+    const hd_status rs = trace_reserve(ctx, s, W, H, &ts);
+    if (rs != HD_OK) return rs;
+    L = LongRays{ts->long_items, ts->long_count, ctx->long_steps};
+    rest = LongRays{ts->rest_items, ts->long_count + 1, ~0u};
+    k_long_reset<<<1, 64, 0, s>>>(L.count, rest.count);
+    HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+
 hd_status launch_paths(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP* pm, uint32_t W, uint32_t H,
                        uint4* out, hipStream_t s) {
     if (ctx->trace_persist) return launch_persist<false>(ctx, dag, p, pm, W, H, out, nullptr, 0.f, 0.f, s);
+    LongRays L, rest;
+    hd_status st = long_lists(ctx, W, H, s, L, rest);
+    if (st != HD_OK) return st;
     const auto k = ctx->fetch_mode == kFetchNode ? k_trace_paths<false, kFetchNode> : k_trace_paths<false, kFetchPointer>;
     k<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), s>>>(dag, p, pm, W, H, out, ctx->tile_w, ctx->tile_h,
-                                                                     nullptr);
+                                                                     nullptr, L);
     HD_HIP(ctx, hipGetLastError());
+    if (L.cap != ~0u) return launch_long<false>(ctx, dag, p, pm, W, H, out, nullptr, 0.f, 0.f, L, rest, s);
     return HD_OK;
 }
 
@@ -969,10 +1307,15 @@ hd_status launch_shadows(hd_ctx* ctx, const DevDag& dag, const RayP& p, const Ra
                          const uint4* paths, uint32_t* colors, float bias, float fog, hipStream_t s) {
     if (ctx->trace_persist)
         return launch_persist<true>(ctx, dag, p, pm, W, H, const_cast<uint4*>(paths), colors, bias, fog, s);
+    LongRays L, rest;
+    hd_status st = long_lists(ctx, W, H, s, L, rest);
+    if (st != HD_OK) return st;
     const auto k = ctx->fetch_mode == kFetchNode ? k_trace_shadows<false, kFetchNode> : k_trace_shadows<false, kFetchPointer>;
     k<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), s>>>(dag, p, pm, W, H, paths, colors, bias, fog,
-                                                                     ctx->tile_w, ctx->tile_h, nullptr);
+                                                                     ctx->tile_w, ctx->tile_h, nullptr, L);
     HD_HIP(ctx, hipGetLastError());
+    if (L.cap != ~0u)
+        return launch_long<true>(ctx, dag, p, pm, W, H, const_cast<uint4*>(paths), colors, bias, fog, L, rest, s);
     return HD_OK;
 }
 
@@ -985,10 +1328,11 @@ hd_status trace_reserve(hd_ctx* ctx, hipStream_t s, uint32_t W, uint32_t H, Trac
     uint64_t tiles = 0;
     for (uint32_t w = 1; w <= 64; w *= 2)
         tiles = std::max<uint64_t>(tiles, (uint64_t)((W + w - 1) / w) * ((H + 64 / w - 1) / (64 / w)));
+    const uint64_t pixels = (uint64_t)W * H;
     TraceScratch* ts = nullptr;
     for (TraceScratch& t : ctx->trace_scratch)
         if (t.stream == s) ts = &t;
-    if (ts && ts->tiles_cap >= tiles) {
+    if (ts && ts->tiles_cap >= tiles && ts->pixels_cap >= pixels) {
         if (out) *out = ts;
         return HD_OK;
     }
@@ -1001,20 +1345,27 @@ hd_status trace_reserve(hd_ctx* ctx, hipStream_t s, uint32_t W, uint32_t H, Trac
         ts = &ctx->trace_scratch.back();
         ts->stream = s;
         HD_HIP(ctx, hipMalloc(&ts->queues, 64 * 32 * 4));
+        HD_HIP(ctx, hipMalloc(&ts->long_count, 64));
     } else {
         HD_HIP(ctx, hipStreamSynchronize(s));
         hipFree(ts->rec);
         hipFree(ts->shades);
         hipFree(ts->cnt);
+        hipFree(ts->long_items);
+        hipFree(ts->rest_items);
         ts->rec = nullptr;
         ts->shades = nullptr;
         ts->cnt = nullptr;
-        ts->tiles_cap = 0;
+        ts->long_items = ts->rest_items = nullptr;
+        ts->tiles_cap = ts->pixels_cap = 0;
     }
     HD_HIP(ctx, hipMalloc(&ts->rec, tiles * 64 * sizeof(uint4)));
     HD_HIP(ctx, hipMalloc(&ts->shades, tiles * 64 * sizeof(uint2)));
     HD_HIP(ctx, hipMalloc(&ts->cnt, tiles * sizeof(uint32_t)));
+    HD_HIP(ctx, hipMalloc(&ts->long_items, pixels * sizeof(uint32_t)));
+    HD_HIP(ctx, hipMalloc(&ts->rest_items, pixels * sizeof(uint32_t)));
     ts->tiles_cap = tiles;
+    ts->pixels_cap = pixels;
     ctx->ptr_gen++;
     if (out) *out = ts;
     return HD_OK;
@@ -1026,6 +1377,9 @@ void trace_scratch_free(hd_ctx* ctx) {
         hipFree(t.rec);
         hipFree(t.shades);
         hipFree(t.cnt);
+        hipFree(t.long_items);
+        hipFree(t.rest_items);
+        hipFree(t.long_count);
     }
     ctx->trace_scratch.clear();
 }
@@ -1103,20 +1457,23 @@ hd_status hd_set_trace_fetch(hd_ctx* ctx, uint32_t fetch_mode) {
     return HD_OK;
 }
 
-hd_status hd_set_trace_schedule(hd_ctx* ctx, uint32_t persistent, uint32_t refill_at) {
+hd_status hd_set_trace_schedule(hd_ctx* ctx, uint32_t schedule, uint32_t param) {
     if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
-    if (persistent > 1 || (persistent && (refill_at == 0 || refill_at > 64)))
-        return fail(ctx, HD_ERR_INVALID, "schedule is 0 (per tile) or 1 (persistent, refill_at in 1..64)");
-    ctx->trace_persist = persistent;
-    if (persistent) ctx->refill_at = refill_at;
+    if (schedule > 2 || (schedule == 1 && (param == 0 || param > 64)) || (schedule == 2 && param == 0))
+        return fail(ctx, HD_ERR_INVALID,
+                    "schedule is 0 (per tile), 1 (persistent, refill at 1..64 idle lanes) or 2 (per tile, walks "
+                    "beyond param steps searched a wavefront per ray)");
+    ctx->trace_persist = schedule == 1 ? 1u : 0u;
+    if (schedule == 1) ctx->refill_at = param;
+    ctx->long_steps = schedule == 2 ? param : 0u;
     ctx->ptr_gen++;
     return HD_OK;
 }
 
 hd_status hd_get_trace_schedule(hd_ctx* ctx, uint32_t out[2]) {
     if (!ctx || !out) return fail(ctx, HD_ERR_INVALID, "bad arguments");
-    out[0] = ctx->trace_persist;
-    out[1] = ctx->refill_at;
+    out[0] = ctx->trace_persist ? 1u : (ctx->long_steps ? 2u : 0u);
+    out[1] = ctx->trace_persist ? ctx->refill_at : ctx->long_steps;
     return HD_OK;
 }
 
@@ -1136,22 +1493,26 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
     if (!ctx->loaded || !ctx->colors_loaded) return fail(ctx, HD_ERR_INVALID, "no DAG or colours loaded");
     static const uint32_t shapes[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}, {2, 32}, {64, 1}};
     static const uint32_t fetches[] = {kFetchPointer, kFetchNode};
-    // schedule: one wavefront per tile, or persistent wavefronts refilled at 8 / 24 idle lanes
-    static const uint32_t schedules[][2] = {{0, 16}, {1, 8}, {1, 24}};
-    constexpr uint32_t kTrials = 6 * 2 * 3;  // tile shape x fetch strategy x schedule
+    // schedule: one wavefront per tile; persistent wavefronts refilled at 8 / 24 idle lanes; one
+    // wavefront per tile with walks beyond 96 / 128 / 192 steps searched a wavefront per ray
+    static const uint32_t schedules[][2] = {{0, 0}, {1, 8}, {1, 24}, {2, 96}, {2, 128}, {2, 192}};
+    constexpr uint32_t kTrials = 6 * 2 * 6;  // tile shape x fetch strategy x schedule
     hipEvent_t e0, e1;
     HD_HIP(ctx, hipEventCreate(&e0));
     HD_HIP(ctx, hipEventCreate(&e1));
     float bestMs = 1e30f;
-    uint32_t bw = ctx->tile_w, bh = ctx->tile_h, bf = ctx->fetch_mode, bp = ctx->trace_persist, br = ctx->refill_at;
+    uint32_t bw = ctx->tile_w, bh = ctx->tile_h, bf = ctx->fetch_mode, bp = ctx->trace_persist, br = ctx->refill_at,
+             bl = ctx->long_steps;
     reps = std::max<uint32_t>(reps, 1);
     for (uint32_t trial = 0; trial < kTrials; ++trial) {
         const auto& sh = shapes[trial % 6];
         ctx->tile_w = sh[0];
         ctx->tile_h = sh[1];
         ctx->fetch_mode = fetches[(trial / 6) % 2];
-        ctx->trace_persist = schedules[trial / 12][0];
-        ctx->refill_at = schedules[trial / 12][1];
+        const uint32_t* sch = schedules[trial / 12];
+        ctx->trace_persist = sch[0] == 1 ? 1u : 0u;
+        if (sch[0] == 1) ctx->refill_at = sch[1];
+        ctx->long_steps = sch[0] == 2 ? sch[1] : 0u;
         float total = 0;
         for (uint32_t rep = 0; rep <= reps; ++rep) {  // rep 0 warms up
             HD_HIP(ctx, hipEventRecord(e0, ctx->stream));
@@ -1179,6 +1540,7 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
             bf = ctx->fetch_mode;
             bp = ctx->trace_persist;
             br = ctx->refill_at;
+            bl = ctx->long_steps;
         }
     }
     hipEventDestroy(e0);
@@ -1188,6 +1550,7 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
     ctx->fetch_mode = bf;
     ctx->trace_persist = bp;
     ctx->refill_at = br;
+    ctx->long_steps = bl;
     ctx->ptr_gen++;
     if (best_w) *best_w = bw;
     if (best_h) *best_h = bh;
@@ -1204,7 +1567,7 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
     HD_HIP(ctx, hipMemsetAsync(ctx->byte_counters, 0, 3 * sizeof(unsigned long long), ctx->stream));
     k_trace_paths<true, kFetchPointer><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), nullptr, W, H,
                                                           reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h,
-                                                          ctx->byte_counters + 0);
+                                                          ctx->byte_counters + 0, LongRays{nullptr, nullptr, ~0u});
     HD_HIP(ctx, hipGetLastError());
     if (ctx->colors_loaded) {
         k_trace_colors<true><<<grid, kBlock, 0, ctx->stream>>>(dev_dag(ctx), dev_colors(ctx), W, H,
@@ -1214,7 +1577,8 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
     }
     k_trace_shadows<true, kFetchPointer><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), nullptr, W, H,
                                                             reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
-                                                            ctx->tile_w, ctx->tile_h, ctx->byte_counters + 2);
+                                                            ctx->tile_w, ctx->tile_h, ctx->byte_counters + 2,
+                                                            LongRays{nullptr, nullptr, ~0u});
     HD_HIP(ctx, hipGetLastError());
     unsigned long long b[3];
     HD_HIP(ctx, hipMemcpyAsync(b, ctx->byte_counters, sizeof(b), hipMemcpyDeviceToHost, ctx->stream));

@@ -238,6 +238,16 @@ class HashDAG:
         check(lib.hd_edit_batch_static(self._h, arr, len(edits), C.byref(st), stream), self._h)
         return st
 
+    def count_edit_bytes(self, on: bool):
+        """Starts (on) or stops counting the edit passes' algorithmic bytes; stopping returns
+        {descent, find, nodes_written, colours_staged, leaves_written} in bytes."""
+        out = np.zeros(5, np.uint64)
+        check(lib.hd_edit_count_bytes(self._h, 1 if on else 0, ptr(out, C.c_uint64)), self._h)
+        if on:
+            return None
+        keys = ("descent", "find", "nodes_written", "colours_staged", "leaves_written")
+        return {k: int(v) for k, v in zip(keys, out)}
+
     def edit(self, kind: int, center, radius: float) -> hd_edit_stats:
         return self.edit_batch([(kind, center, radius)])
 

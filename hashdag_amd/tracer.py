@@ -102,10 +102,11 @@ class DAGTracer:
         check(lib.hd_set_trace_fetch(dag.handle, fetch), dag.handle)
 
 
-    def set_schedule(self, dag: HashDAG, persistent: bool, refill_at: int = 16) -> None:
-        """0: one wavefront per pixel tile; 1: persistent wavefronts fed from tile queues,
-        refilling idle lanes once refill_at of them are idle."""
-        check(lib.hd_set_trace_schedule(dag.handle, 1 if persistent else 0, refill_at), dag.handle)
+    def set_schedule(self, dag: HashDAG, schedule, param: int = 16) -> None:
+        """0 (or False): one wavefront per pixel tile; 1 (or True): persistent wavefronts fed
+        from tile queues, refilling idle lanes once `param` of them are idle; 2: one wavefront
+        per tile, walks beyond `param` steps searched a wavefront per ray."""
+        check(lib.hd_set_trace_schedule(dag.handle, int(schedule), param), dag.handle)
 
 
 def launch_config(dag: HashDAG) -> dict:
@@ -113,7 +114,8 @@ def launch_config(dag: HashDAG) -> dict:
     check(lib.hd_get_trace_launch(dag.handle, out), dag.handle)
     sch = np.zeros(2, np.uint32)
     check(lib.hd_get_trace_schedule(dag.handle, ptr(sch)), dag.handle)
-    schedule = f"persistent (refill at {int(sch[1])})" if sch[0] else "per tile"
+    schedule = {0: "per tile", 1: f"persistent (refill at {int(sch[1])})",
+                2: f"per tile, walks beyond {int(sch[1])} steps a wavefront per ray"}[int(sch[0])]
     return {"tile": f"{out[0]}x{out[1]}", "fetch": ["pointer", "node"][out[2]], "schedule": schedule}
 
 

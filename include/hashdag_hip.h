@@ -147,17 +147,21 @@ hd_status hd_set_trace_tile(hd_ctx* ctx, uint32_t tile_w, uint32_t tile_h);
 hd_status hd_set_trace_fetch(hd_ctx* ctx, uint32_t fetch_mode);
 /* out: tile_w, tile_h, fetch mode */
 hd_status hd_get_trace_launch(hd_ctx* ctx, uint32_t out[3]);
-/* Tracer schedule, not in the reference (one thread per pixel, dag_tracer.cu:121-122):
- * persistent = 0 runs one wavefront per pixel tile; 1 keeps every wavefront resident, taking
- * tiles from per-region queues and handing new pixels to its idle lanes once refill_at
- * (1..64) of them are idle. Paths and shadows; both give the same images. */
-hd_status hd_set_trace_schedule(hd_ctx* ctx, uint32_t persistent, uint32_t refill_at);
-/* out: persistent, refill_at */
+/* Tracer schedule, not in the reference (one thread per pixel, dag_tracer.cu:121-122), for
+ * paths and shadows; every schedule gives the same images:
+ *   0  one wavefront per pixel tile, each lane walking its ray to the end;
+ *   1  persistent wavefronts taking tiles from per-region queues, handing new pixels to their
+ *      idle lanes once `param` (1..64) of them are idle;
+ *   2  one wavefront per pixel tile, walks stopped after `param` steps; those rays are then
+ *      searched a wavefront per ray (64 lanes expanding the ray's next nodes in DFS order). */
+hd_status hd_set_trace_schedule(hd_ctx* ctx, uint32_t schedule, uint32_t param);
+/* out: schedule, param */
 hd_status hd_get_trace_schedule(hd_ctx* ctx, uint32_t out[2]);
 
 /* Autotuner for the tracer launch shape: times full frames (paths + colours + shadows) over
  * n_params cameras for each wavefront tile shape (8x8, 16x4, 4x16, 32x2, 2x32, 64x1), node
- * fetch strategy and schedule (per tile; persistent refilled at 8 or 24 idle lanes), keeps
+ * fetch strategy and schedule (per tile; persistent refilled at 8 or 24 idle lanes; per tile
+ * with walks beyond 96, 128 or 192 steps searched a wavefront per ray), keeps
  * the fastest in ctx and returns it with its ms per frame. No counterpart in the reference
  * (fixed (4,64) blocks, dag_tracer.cu:121-122). */
 hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n_params, uint32_t width,
@@ -232,6 +236,14 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
  * history, which it clears. Calls that overlap on one GPU share its array and race. */
 #define HD_STATIC_MAX_EDITS 256u
 hd_status hd_edit_batch_static(hd_ctx* ctx, const hd_edit* edits, uint32_t n, hd_edit_stats* stats, void* stream);
+/* Edit pass measurement, no counterpart in the reference (its "edits" stat is a time,
+ * hash_dag.h:262,368): on = 1 zeroes the counters and counts the algorithmic bytes of every
+ * following pass (the counting kernels are slower; time passes without it); on = 0 stops and
+ * returns out[0] node words read by the descent (header and child pointers of every node the
+ * passes visit, 8 B per leaf), out[1] bucket bytes the finds read (tag bytes and compared
+ * words), out[2] node words written, out[3] staged colours written and read by the encoder
+ * (16 B each), out[4] colour leaf words written. Old colours decoded are not counted. */
+hd_status hd_edit_count_bytes(hd_ctx* ctx, uint32_t on, uint64_t out[5]);
 /* HashDAGUndoRedo::undo / redo (hash_dag.h:68-120) and free (133-145). */
 hd_status hd_undo(hd_ctx* ctx);
 hd_status hd_redo(hd_ctx* ctx);

@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 
 def _frames_match(dag, od, oc, cams, levels, W, H, what):
     """Each camera's paths, colours and shaded colours against the oracle, traced with one
-    wavefront per tile and with persistent wavefronts."""
+    wavefront per tile, with persistent wavefronts, and per tile with walks beyond 128 (and 24)
+    steps searched a wavefront per ray."""
     import torch
 
     from hashdag_amd import DAGInfo, DAGTracer, ray_params
@@ -34,9 +35,9 @@ def _frames_match(dag, od, oc, cams, levels, W, H, what):
         assert (opaths[..., :3] != 0).any(-1).mean() > 0.05, f"{what} camera {k} sees almost nothing"
         ocols = Oracle.trace_colors(od, oc, opaths, W, H, threads=16)
         oshad = Oracle.trace_shadows(od, oparams, opaths, ocols, W, H, 1.0, 0.0, threads=16)
-        for persistent in (False, True):
-            print(f"[{what}] camera {k} {W}x{H} persistent={persistent}", flush=True)
-            tracer.set_schedule(dag, persistent, 16)
+        for schedule in ((0, 0), (1, 16), (2, 128), (2, 24)):
+            print(f"[{what}] camera {k} {W}x{H} schedule={schedule}", flush=True)
+            tracer.set_schedule(dag, *schedule)
             tracer.resolve_paths(cam, info, dag, params=prm)
             tracer.resolve_colors(dag)
             torch.cuda.synchronize()
@@ -45,7 +46,7 @@ def _frames_match(dag, od, oc, cams, levels, W, H, what):
             tracer.resolve_shadows(cam, info, dag, 1.0, 0.0, params=prm)
             torch.cuda.synchronize()
             gshad = tracer.colors.cpu().numpy().view(np.uint32)
-            sched = "persistent" if persistent else "per tile"
+            sched = f"schedule {schedule}"
             bad = (gpaths != opaths).any(-1)
             assert not bad.any(), f"{what} camera {k} {sched}: {int(bad.sum())} paths differ, first {np.argwhere(bad)[:3]}"
             assert np.array_equal(gcols, ocols), f"{what} camera {k} {sched}: {int((gcols != ocols).sum())} colours differ"
@@ -53,7 +54,7 @@ def _frames_match(dag, od, oc, cams, levels, W, H, what):
                 f"{what} camera {k} {sched}: {int((gshad != oshad).sum())} shaded pixels differ"
             del tracer
             tracer = DAGTracer(W, H)
-    tracer.set_schedule(dag, False)
+    tracer.set_schedule(dag, 0)
 
 
 def test_full_size_frames_bit_exact():

@@ -380,3 +380,27 @@ def test_noise_matches_oracle(fused):
     assert st.passes == (4 if fused else len(edits))
     assert st.num_voxels == total
     assert_same(dag, sc, od, oc, f"noise fused={fused}")
+
+
+def test_edit_byte_counters():
+    """Counting a pass's algorithmic bytes leaves its result unchanged (DAG, colours and leaf
+    words against the oracle); written node words and staged colours agree with the pass's own
+    statistics; an idle count reads zero."""
+    p, sc, g, dag, od, oc = make("d13")
+    edits = random_edits(p, 30, seed=41)
+    dag.count_edit_bytes(True)
+    st = dag.edit_batch(edits, fused=True)
+    b = dag.count_edit_bytes(False)
+    for kind, center, r in edits:
+        od.edit(kind, center, r, oc)
+    assert_same(dag, sc, od, oc, "counted batch")
+    assert b["nodes_written"] == 4 * st.new_words
+    assert b["colours_staged"] == 16 * st.colors_encoded
+    assert b["descent"] > b["nodes_written"] > 0
+    assert b["find"] > 0 and b["leaves_written"] > 0
+    dag.count_edit_bytes(True)
+    assert all(v == 0 for v in dag.count_edit_bytes(False).values())
+    st2 = dag.edit_batch(random_edits(p, 5, seed=43), fused=True)  # not counted
+    dag.count_edit_bytes(True)
+    assert all(v == 0 for v in dag.count_edit_bytes(False).values())
+    assert st2.num_voxels > 0

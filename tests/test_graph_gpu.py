@@ -16,8 +16,8 @@ def canonical(dag):
     return OracleDAG.from_image(dag.download(), dag.pool_pages).canonical()
 
 
-@pytest.mark.parametrize("tiny_scratch", [False, True])
-def test_graph_replays_match_eager(tiny_scratch):
+@pytest.mark.parametrize("tiny_scratch,schedule", [(False, (0, 0)), (True, (0, 0)), (False, (2, 4)), (False, (1, 16))])
+def test_graph_replays_match_eager(tiny_scratch, schedule):
     import torch
 
     from hashdag_amd import DAGInfo, DAGTracer, HashDAG, ray_params
@@ -37,6 +37,7 @@ def test_graph_replays_match_eager(tiny_scratch):
     eb = HashDAG(g, pool_pages=POOL_PAGES)
     eb.load_scene(sc)
     ta, tb = DAGTracer(W, H), DAGTracer(W, H)
+    ta.set_schedule(ga, *schedule)  # the graph traces with this schedule, the eager side per tile
     graph = FrameGraph(ga, ta, max_edits=32)
     cams = cameras(p, 6, seed=31)
     replays = 0
