@@ -51,6 +51,11 @@ def main() -> int:
     ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
     ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
     ap.add_argument("--no-edits", action="store_true")
+    ap.add_argument("--edit-forms", choices=("fast", "all", "level"), default="fast",
+                    help="edit launch forms compared: fast (a launch per step and level, a captured single-edit "
+                         "graph per plain edit, one cooperative launch for the fused pass, one workgroup on a "
+                         "sample by radius), all (also one workgroup per plain edit over the whole list) or "
+                         "level (the first only; for profilers that cannot follow a cooperative launch)")
     ap.add_argument("--edit-levels", type=int, default=16,
                     help="scene depth of the batched-vs-plain edit comparison (0 = the traced scene)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the served stream (0 = skip)")
@@ -209,7 +214,7 @@ def main() -> int:
                         f"cameras" + (f" (cycled {args.steps / len(cams):.2f} times)" if args.steps > len(cams) else ""),
             "levels": D, "width": W, "height": H, "cameras_timed": min(args.steps, len(cams)),
             "cameras_generated": len(cams), "seed": args.seed,
-            "voxels": int(scene.num_voxels), "tile": args.tile, "fetch": launch["fetch"],
+            "voxels": int(scene.num_voxels), "tile": launch["tile"], "fetch": launch["fetch"],
             "schedule": launch["schedule"],
             "bucket_geometry": {"num_top_levels": geom.num_top_levels,
                                 "top": f"{1 << geom.bucket_bits_top} x {geom.bucket_size_top} words",
@@ -226,7 +231,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(float(achieved) / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(names[dom], D, W, H),
+            "traffic": measured_traffic(names[dom], D, W, H, launch),
             "traffic_source": "profiles/trace_kernels_2p15_1080p.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "achievable_stream_gbs": round(stream_gbs, 1),
             "achievable_read_gbs": round(read_gbs, 1),
@@ -302,8 +307,12 @@ def edit_bench(scene, geom, params, W, H, args):
             out.append(tr.colors.cpu().numpy().copy())
         return out
 
-    def timed(d, fused, warm_edits):
-        """The edit list, then the GC pass that follows it (mark / compact of the stale nodes)."""
+    def timed(d, fused, warm_edits, launch=(0, 0)):
+        """The edit list on the DAG as it is, after an untimed run that sizes the scratch (undone,
+        and its nodes collected, so the timed run finds the table as loaded). launch: passes of
+        at most launch[0] edits run in one workgroup, else of at most launch[1] edits as one
+        cooperative launch, else a launch per step and level."""
+        d.set_edit_launch(*launch)
         d.edit_batch(warm_edits, fused=fused)
         for _ in range(d.undo_depth()[0]):
             d.undo()
@@ -312,59 +321,155 @@ def edit_bench(scene, geom, params, W, H, args):
         t0 = time.perf_counter()
         st = d.edit_batch(edits, fused=fused)
         torch.cuda.synchronize()
-        t1 = time.perf_counter()
+        return st, time.perf_counter() - t0
+
+    def gc_timed(d):
+        """The GC pass that follows an edit list (mark / compact of the stale nodes)."""
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         gc = d.remove_stale_nodes()
         torch.cuda.synchronize()
-        return st, t1 - t0, time.perf_counter() - t1, gc
+        return time.perf_counter() - t0, gc
 
+    def undo_all(d):
+        for _ in range(d.undo_depth()[0]):
+            d.undo()
+
+    def plain_by_radius(d, sample):
+        """Median milliseconds of one plain pass by editor radius, a launch per step and level
+        against one single-workgroup launch, on a sample of the edit list (each edit undone)."""
+        classes = ((2, 8), (8, 32), (32, 128), (128, 257))
+        out = {}
+        for name, launch in (("level_synchronous", (0, 0)), ("single_workgroup", (1, 0))):
+            d.set_edit_launch(*launch)
+            for e in sample[:8]:
+                d.edit(*e)
+            undo_all(d)
+            times = []
+            for kind, center, r in sample:
+                t0 = time.perf_counter()
+                d.edit(kind, center, r)
+                times.append((r, time.perf_counter() - t0))
+                d.undo()
+            out[name] = {f"r{lo}-{hi - 1}": round(float(np.median([t for r, t in times if lo <= r < hi])) * 1e3, 3)
+                         for lo, hi in classes if any(lo <= r < hi for r, _ in times)}
+        d.set_edit_launch(0, 0)
+        return out
+
+    # fused: one launch per step and level, then (after undo) one cooperative launch
     dag = HashDAG(geom)
     dag.load_scene(scene)
-    st_f, t_f, gc_f, gcs_f = timed(dag, True, edits)
-    log(f"fused: {st_f.num_voxels} voxels in {t_f * 1e3:.1f} ms ({st_f.passes} pass), then GC {gc_f * 1e3:.1f} ms "
-        f"({gcs_f['words_before']} -> {gcs_f['words_after']} words)")
+    st_f, t_f = timed(dag, True, edits)
     img_f = frames(dag)
-    # algorithmic bytes of the same fused pass, counted on a replay after undo + GC (the counting
-    # kernels are slower, so the timed pass above runs without them)
-    for _ in range(dag.undo_depth()[0]):
-        dag.undo()
+    # algorithmic bytes of the same pass, counted on a replay from the loaded state (the
+    # counting kernels are slower, so the timed runs do not count)
+    undo_all(dag)
     dag.remove_stale_nodes()
     dag.count_edit_bytes(True)
     dag.edit_batch(edits, fused=True)
     ab = dag.count_edit_bytes(False)
-    torch.cuda.synchronize()
+    undo_all(dag)
+    if args.edit_forms != "level":
+        st_fp, t_fp = timed(dag, True, edits, launch=(0, 1 << 30))
+        img_fp = frames(dag)
+    else:
+        st_fp, t_fp, img_fp = st_f, float("inf"), img_f
+    gc_f, gcs_f = gc_timed(dag)
     dag.close()
     abytes = sum(ab.values())
+    log(f"fused: {st_f.num_voxels} voxels in {t_f * 1e3:.1f} ms ({st_f.passes} pass; {t_fp * 1e3:.1f} ms as one "
+        f"cooperative launch), then GC {gc_f * 1e3:.1f} ms ({gcs_f['words_before']} -> {gcs_f['words_after']} words)")
     log(f"fused pass algorithmic bytes: {ab} = {abytes / 1e9:.3f} GB -> {abytes / t_f / 1e9:.1f} GB/s")
+    if st_fp.num_voxels != st_f.num_voxels or not all(np.array_equal(a, b) for a, b in zip(img_f, img_fp)):
+        raise RuntimeError("fused pass: cooperative and level-synchronous launches differ")
+    if t_fp < t_f:
+        st_b, t_b, batched_form = st_fp, t_fp, "one cooperative launch"
+    else:
+        st_b, t_b, batched_form = st_f, t_f, "a launch per step and level"
+
+    # plain: one pass per edit, in both launch forms; the faster is the baseline
     plain = HashDAG(geom)
     plain.load_scene(scene)
-    st_p, t_p, gc_p, _ = timed(plain, False, edits[:16])
-    log(f"plain: {st_p.num_voxels} voxels in {t_p * 1e3:.1f} ms ({st_p.passes} passes), then GC {gc_p * 1e3:.1f} ms")
-    img_p = frames(plain)
+    inf = float("inf")
+    nv_pp, t_pp, img_pp = None, inf, None
+    nv_pg, t_pg, img_pg, replays = None, inf, None, 0
+    by_radius = None
+    if args.edit_forms in ("all", "fast"):
+        # a captured single-edit pass replayed per edit (HIP graph, edit-only)
+        from hashdag_amd.graph import FrameGraph
+        plain.set_edit_launch(0, 0)
+        fg = FrameGraph(plain, None, 1)
+        for e in edits[:16]:
+            fg.launch([e])
+        undo_all(plain)
+        plain.remove_stale_nodes()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nv_pg = 0
+        for e in edits:
+            st, ran = fg.launch([e])
+            nv_pg += st.num_voxels
+            replays += ran
+        torch.cuda.synchronize()
+        t_pg = time.perf_counter() - t0
+        img_pg = frames(plain)
+        fg.close()
+        undo_all(plain)
+        by_radius = plain_by_radius(plain, edits[::16])
+    if args.edit_forms == "all":
+        st_pp, t_pp = timed(plain, False, edits[:16], launch=(1, 0))
+        nv_pp = st_pp.num_voxels
+        img_pp = frames(plain)
+        undo_all(plain)
+    # the level-synchronous form last, so the GC timed after it finds this list's stale nodes
+    st_pl, t_pl = timed(plain, False, edits[:16])
+    img_pl = frames(plain)
+    gc_p, _ = gc_timed(plain)
     plain.close()
-    same = st_p.num_voxels == st_f.num_voxels and all(np.array_equal(a, b) for a, b in zip(img_f, img_p))
+    log(f"plain: {st_pl.num_voxels} voxels in {t_pl * 1e3:.1f} ms ({st_pl.passes} passes, a launch per step and "
+        f"level), {t_pp * 1e3:.1f} ms (one single-workgroup launch per edit), {t_pg * 1e3:.1f} ms (a captured "
+        f"single-edit graph replayed per edit, {replays} replays), then GC {gc_p * 1e3:.1f} ms")
+    plain_forms = {"a launch per step and level": t_pl, "one single-workgroup launch per edit": t_pp,
+                   "a captured single-edit pass replayed per edit": t_pg}
+    plain_form = min(plain_forms, key=plain_forms.get)
+    t_p = plain_forms[plain_form]
+    st_p = st_pl
+    same = st_pl.num_voxels == st_f.num_voxels and all(np.array_equal(a, b) for a, b in zip(img_f, img_pl))
+    for nv, img in ((nv_pp, img_pp), (nv_pg, img_pg)):
+        if img is not None:
+            same = same and nv == st_f.num_voxels and all(np.array_equal(a, b) for a, b in zip(img_f, img))
     if not same:
         raise RuntimeError("fused and plain edit results differ")
     return {
         "workload": f"{len(edits)} mixed edits on the 2^{params.levels} coloured terrain (70% sphere add/carve, "
                     f"20% box add/carve, 10% sphere paint; radius log-uniform 2-256; seed {args.seed + 1})",
-        "edited_voxels": int(st_f.num_voxels),
+        "edited_voxels": int(st_b.num_voxels),
         "plain_voxels_per_s": round(st_p.num_voxels / t_p, 1),
-        "batched_voxels_per_s": round(st_f.num_voxels / t_f, 1),
-        "batched_over_plain": round(t_p / t_f, 2),
+        "batched_voxels_per_s": round(st_b.num_voxels / t_b, 1),
+        "batched_over_plain": round(t_p / t_b, 2),
         "plain_ms": round(t_p * 1e3, 2),
-        "batched_ms": round(t_f * 1e3, 2),
+        "batched_ms": round(t_b * 1e3, 2),
         "plain_ms_per_edit": round(t_p * 1e3 / len(edits), 3),
+        "plain_form": plain_form,
+        "batched_form": batched_form,
+        "plain_ms_by_form": {"level_synchronous": round(t_pl * 1e3, 2),
+                             "single_workgroup": round(t_pp * 1e3, 2) if t_pp < float("inf") else None,
+                             "graph_replay": round(t_pg * 1e3, 2) if t_pg < float("inf") else None},
+        "plain_pass_ms_by_radius": by_radius,
+        "batched_ms_by_form": {"level_synchronous": round(t_f * 1e3, 2),
+                               "cooperative": round(t_fp * 1e3, 2) if t_fp < float("inf") else None},
         "gc_after_batched_ms": round(gc_f * 1e3, 2),
         "gc_after_plain_ms": round(gc_p * 1e3, 2),
         "gc_words_before_after": [gcs_f["words_before"], gcs_f["words_after"]],
-        "batched_with_gc_voxels_per_s": round(st_f.num_voxels / (t_f + gc_f), 1),
-        "batched_over_plain_with_gc": round((t_p + gc_p) / (t_f + gc_f), 2),
+        "batched_with_gc_voxels_per_s": round(st_b.num_voxels / (t_b + gc_f), 1),
+        "batched_over_plain_with_gc": round((t_p + gc_p) / (t_b + gc_f), 2),
         "results_identical": same,
         "roofline": {"bound": "hbm", "achieved": round(abytes / t_f / 1e9, 2), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(abytes / t_f / 8e12, 5), "traffic": None,
                      "bytes": ab, "note": "algorithmic bytes of the fused pass (node words read by the descent, "
-                     "bucket bytes read by the finds, node words written, staged colours written and read, colour "
-                     "leaf words written) over its wall time including host steps"},
+                     "bucket bytes read by the finds, node words written, staged colour markers and explicit "
+                     "colours written once and read twice, colour leaf words written) over the wall time of its "
+                     "level-synchronous form including host steps"},
     }
 
 
@@ -571,16 +676,19 @@ def cpu_edit_baseline(od, oc, params, args):
                       f"in {v1[3]:.2f}s" + ("; both reach the same canonical DAG" if same else "")}
 
 
-def measured_traffic(kernel: str, levels: int, W: int, H: int):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary (scaled by the copy-kernel calibration), when it covers this workload."""
+def measured_traffic(kernel: str, levels: int, W: int, H: int, launch: dict):
+    """HBM bytes per call of `kernel` (every kernel the call launches) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration), when it
+    covers this workload, tile shape, fetch strategy and schedule."""
     path = os.path.join(ROOT, "profiles", "trace_kernels_2p15_1080p.md")
     if not os.path.exists(path):
         return None
+    want = {"levels": str(levels), "width": str(W), "height": str(H), "tile": launch["tile"],
+            "fetch": launch["fetch"], "schedule": launch["schedule_key"]}
     for line in open(path):
         if line.startswith("traffic_bytes "):
             kv = dict(tok.split("=") for tok in line.split()[1:])
-            if (int(kv.get("levels", -1)), int(kv.get("width", -1)), int(kv.get("height", -1))) == (levels, W, H):
+            if all(kv.get(k) == v for k, v in want.items()):
                 return int(kv[kernel]) if kernel in kv else None
     return None
 

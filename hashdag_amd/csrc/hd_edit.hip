@@ -26,6 +26,7 @@
 //
 // Kernels read item counts from device memory (grid-stride loops), so a pass needs no host
 // round trip and can be captured in a HIP graph.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <random>
@@ -45,10 +46,15 @@ constexpr uint32_t kGrid = 2048;  // blocks of grid-stride kernels
 
 // Wave-strided loops (one work item per wavefront): the index is made wave-uniform so the
 // compiler keeps it in a scalar register and loads the item's fields through the scalar cache.
-__device__ __forceinline__ uint32_t wave_index() {
-    return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+// The pass kernels' bodies take the grid as an argument (VGrid), so one persistent launch can run
+// them in sequence with the same work division.
+struct VGrid {
+    uint32_t block, nblocks;
+};
+__device__ __forceinline__ uint32_t wave_index(VGrid G) {
+    return __builtin_amdgcn_readfirstlane((G.block * blockDim.x + threadIdx.x) >> 6);
 }
-__device__ __forceinline__ uint32_t wave_count() { return (gridDim.x * blockDim.x) >> 6; }
+__device__ __forceinline__ uint32_t wave_count(VGrid G) { return (G.nblocks * blockDim.x) >> 6; }
 
 // Wave-aggregated counters (every lane of the wavefront must call these): one atomic per
 // wavefront instead of one per lane, so millions of items do not queue on one address.
@@ -423,7 +429,7 @@ __device__ __forceinline__ bool copy_source_empty(const EditPass& P, uint32_t ei
 // ---------------------------------------------------------------------------------------------
 // expand: items of level l -> child classification and items of level l+1
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
+__device__ __forceinline__ void d_expand(const EditPass& P, uint32_t l, VGrid G) {
     __shared__ uint32_t sAlloc[2 * (256 / 64 + 1)];
     __shared__ uint32_t sErr;
     const uint32_t n = min(P.count[l], P.cap[l]);
@@ -439,7 +445,7 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
     uint64_t nodeBytes = 0;  // header + child pointers of the items' nodes (measurement)
     // block-uniform loop: child items and edit-list words are allocated with one atomic each
     // per block
-    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < total; b0 += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t b0 = (uint64_t)G.block * blockDim.x; b0 < total; b0 += (uint64_t)G.nblocks * blockDim.x) {
         const uint64_t t = b0 + threadIdx.x;
         const bool active = t < total;
         const uint32_t i = (uint32_t)(t >> 3), c = (uint32_t)(t & 7u);
@@ -588,6 +594,9 @@ __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
     }
     if (P.abytes) wave_add64(P.abytes + 0, nodeBytes);
 }
+__global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
+    d_expand(P, l, VGrid{blockIdx.x, gridDim.x});
+}
 
 // ---------------------------------------------------------------------------------------------
 // leaf level: one thread per leaf item, its 64 voxels as bit masks (bit = child1 * 8 + child2).
@@ -655,13 +664,13 @@ __device__ __forceinline__ uint64_t leaf_touch_mask(const E& e, uint32_t bx, uin
 
 // kSpecial: the pass may hold copy / fill editors (their code is built in only then)
 template <bool kSpecial>
-__global__ __launch_bounds__(256) void k_leaf(EditPass P) {
+__device__ __forceinline__ void d_leaf(const EditPass& P, VGrid G) {
     const uint32_t L = P.levels - 2;
     const uint32_t n = min(P.count[L], P.cap[L]);
     if (*P.errors) return;
     const Items I = P.items[L];
     uint64_t leafBytes = 0;  // old and original leaves read (measurement)
-    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += gridDim.x * blockDim.x) {
+    for (uint32_t t0 = G.block * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += G.nblocks * blockDim.x) {
         const uint32_t i = t0 + (threadIdx.x & 63u);
         const bool active = i < n;
         uint64_t touched = 0;
@@ -747,17 +756,21 @@ __global__ __launch_bounds__(256) void k_leaf(EditPass P) {
     }
     if (P.abytes) wave_add64(P.abytes + 0, leafBytes);
 }
+template <bool kSpecial>
+__global__ __launch_bounds__(256) void k_leaf(EditPass P) {
+    d_leaf<kSpecial>(P, VGrid{blockIdx.x, gridDim.x});
+}
 
 // ---------------------------------------------------------------------------------------------
 // build interior items of level l from their children's results
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_build(EditPass P, uint32_t l) {
+__device__ __forceinline__ void d_build(const EditPass& P, uint32_t l, VGrid G) {
     const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
     const uint32_t D = P.levels, ctl = P.ctl;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = G.block * blockDim.x + threadIdx.x; i < n; i += G.nblocks * blockDim.x) {
         uint32_t node[9];
         uint32_t nn = 1, mask = 0;
         uint64_t count = 0;
@@ -802,6 +815,9 @@ __global__ __launch_bounds__(256) void k_build(EditPass P, uint32_t l) {
         I.state[i] = 1;
     }
 }
+__global__ __launch_bounds__(256) void k_build(EditPass P, uint32_t l) {
+    d_build(P, l, VGrid{blockIdx.x, gridDim.x});
+}
 
 // ---------------------------------------------------------------------------------------------
 // dedup / find / add / resolve for the pending items of one level
@@ -820,9 +836,12 @@ __device__ __forceinline__ uint32_t hset_slots(const EditPass& P, uint32_t l) {
 }
 
 // clears the slots of the dedup hash set that level l will use
-__global__ void k_hset_clear(EditPass P, uint32_t l) {
+__device__ __forceinline__ void d_hset_clear(const EditPass& P, uint32_t l, VGrid G) {
     const uint32_t m = hset_slots(P, l);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) P.hset[i] = 0u;
+    for (uint32_t i = G.block * blockDim.x + threadIdx.x; i < m; i += G.nblocks * blockDim.x) P.hset[i] = 0u;
+}
+__global__ void k_hset_clear(EditPass P, uint32_t l) {
+    d_hset_clear(P, l, VGrid{blockIdx.x, gridDim.x});
 }
 
 __device__ __forceinline__ bool same_content(const Items& I, uint32_t a, uint32_t b) {
@@ -835,7 +854,7 @@ __device__ __forceinline__ bool same_content(const Items& I, uint32_t a, uint32_
 
 // Representatives (the first item of each distinct content) are listed for k_find, in the
 // level's nstart words (free until the colour phase), with one atomic per block.
-__global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
+__device__ __forceinline__ void d_dedup(const EditPass& P, uint32_t l, VGrid G) {
     __shared__ uint32_t sAlloc[256 / 64 + 1];
     __shared__ uint32_t sErr;
     const uint32_t n = min(P.count[l], P.cap[l]);
@@ -844,7 +863,7 @@ __global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
     if (sErr) return;  // block-uniform: the loop below synchronises the block
     const Items I = P.items[l];
     const uint32_t hmask = hset_slots(P, l) - 1;
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+    for (uint32_t b0 = G.block * blockDim.x; b0 < n; b0 += G.nblocks * blockDim.x) {
         const uint32_t i = b0 + threadIdx.x;
         bool isRep = false;
         if (i < n && I.state[i] == 1) {
@@ -873,11 +892,14 @@ __global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
         if (isRep) I.nstart[at] = i;
     }
 }
+__global__ __launch_bounds__(256) void k_dedup(EditPass P, uint32_t l) {
+    d_dedup(P, l, VGrid{blockIdx.x, gridDim.x});
+}
 
 // One wavefront per representative: scan the bucket for an identical node (any word
 // position inside the bucket's used size whose words equal the node and that does not
 // cross a page boundary holds the same node content).
-__global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
+__device__ __forceinline__ void d_find(const EditPass& P, uint32_t l, VGrid G) {
     const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
@@ -887,7 +909,7 @@ __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
     // the representatives listed by k_dedup
     const uint32_t nReps = min(P.find_count[l], n);
     uint64_t scanBytes = 0;  // bucket bytes this lane read (measurement)
-    for (uint32_t r = wave_index(); r < nReps; r += wave_count()) {
+    for (uint32_t r = wave_index(G); r < nReps; r += wave_count(G)) {
         const uint32_t i = I.nstart[r];
         const uint32_t bucket = g.bucket_from_hash(l, I.hash[i]);
         const uint32_t base = g.make_ptr(l, bucket, 0);
@@ -965,8 +987,11 @@ __global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
     }
     if (P.abytes) wave_add64(P.abytes + 1, scanBytes);
 }
+__global__ __launch_bounds__(256) void k_find(EditPass P, uint32_t l) {
+    d_find(P, l, VGrid{blockIdx.x, gridDim.x});
+}
 
-__device__ bool alloc_page(EditPass& P, uint32_t page) {
+__device__ bool alloc_page(const EditPass& P, uint32_t page) {
     if (atomicCAS(&P.page_table[page], 0u, 0xFFFFFFFFu) != 0u) return true;  // already mapped (after GC)
     const uint32_t phys = atomicAdd(P.pool_top, 1u);
     if (phys >= P.pool_pages) {
@@ -978,13 +1003,13 @@ __device__ bool alloc_page(EditPass& P, uint32_t page) {
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_add(EditPass P, uint32_t l) {
+__device__ __forceinline__ void d_add(const EditPass& P, uint32_t l, VGrid G) {
     const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Geom g = P.g;
     const uint32_t ps = g.page_size;
-    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += gridDim.x * blockDim.x) {
+    for (uint32_t t0 = G.block * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += G.nblocks * blockDim.x) {
         const uint32_t i = t0 + (threadIdx.x & 63u);
         uint32_t added = 0;
         if (i < n && I.state[i] == 1 && I.rep[i] == i) {
@@ -1022,13 +1047,19 @@ __global__ __launch_bounds__(256) void k_add(EditPass P, uint32_t l) {
         wave_add64(P.stat_new_words, added);
     }
 }
+__global__ __launch_bounds__(256) void k_add(EditPass P, uint32_t l) {
+    d_add(P, l, VGrid{blockIdx.x, gridDim.x});
+}
 
-__global__ __launch_bounds__(256) void k_resolve(EditPass P, uint32_t l) {
+__device__ __forceinline__ void d_resolve(const EditPass& P, uint32_t l, VGrid G) {
     const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (uint32_t i = G.block * blockDim.x + threadIdx.x; i < n; i += G.nblocks * blockDim.x)
         if (I.state[i] == 1 && I.rep[i] != i) I.nidx[i] = I.nidx[I.rep[i]];
+}
+__global__ __launch_bounds__(256) void k_resolve(EditPass P, uint32_t l) {
+    d_resolve(P, l, VGrid{blockIdx.x, gridDim.x});
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1036,15 +1067,15 @@ __global__ __launch_bounds__(256) void k_resolve(EditPass P, uint32_t l) {
 // ---------------------------------------------------------------------------------------------
 // new / old colour starts of the child items (relative to the task's colour array): one lane per
 // child slot, the 8 slots of an item in 8 consecutive lanes (their starts by a scan over the 8)
-__global__ __launch_bounds__(256) void k_color_starts(EditPass P, uint32_t l) {
+__device__ __forceinline__ void d_color_starts(const EditPass& P, uint32_t l, VGrid G) {
     const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
     const uint64_t total = (uint64_t)n * 8;
     // the loop bound is a multiple of 8 per step, so the 8 lanes of an item step together
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ((total + 63) & ~63ull);
-         t += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t t = (uint64_t)G.block * blockDim.x + threadIdx.x; t < ((total + 63) & ~63ull);
+         t += (uint64_t)G.nblocks * blockDim.x) {
         const bool active = t < total;
         const uint32_t slot = (uint32_t)t, i = slot >> 3, c = slot & 7u;
         const uint32_t cn = active ? I.cnew[slot] : 0u, co = active ? I.cold[slot] : 0u;
@@ -1079,17 +1110,20 @@ __global__ __launch_bounds__(256) void k_color_starts(EditPass P, uint32_t l) {
         I.costart[slot] = os;
     }
 }
+__global__ __launch_bounds__(256) void k_color_starts(EditPass P, uint32_t l) {
+    d_color_starts(P, l, VGrid{blockIdx.x, gridDim.x});
+}
 
 // colour array of every task with colours: base offset in the colour scratch. Arrays start on
 // a 64-colour chunk; the chunk markers of the wave's range are cleared here (the colour
 // writers mark the chunks they fill with one colour).
-__global__ __launch_bounds__(256) void k_color_alloc(EditPass P) {
+__device__ __forceinline__ void d_color_alloc(const EditPass& P, VGrid G) {
     const uint32_t ctl = P.ctl;
     const uint32_t n = min(P.count[ctl], P.cap[ctl]);
     if (*P.errors) return;
     const Items T = P.items[ctl];
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += gridDim.x * blockDim.x) {
+    for (uint32_t t0 = G.block * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += G.nblocks * blockDim.x) {
         const uint32_t i = t0 + lane;
         const uint32_t cnt = i < n ? T.ncount[i] : 0u;
         const uint64_t acnt = ((uint64_t)cnt + 63u) & ~63ull;
@@ -1104,6 +1138,9 @@ __global__ __launch_bounds__(256) void k_color_alloc(EditPass P) {
         }
         for (uint64_t k = lo / 64 + lane; k < hi / 64; k += 64) P.cmarks[k] = 0;
     }
+}
+__global__ __launch_bounds__(256) void k_color_alloc(EditPass P) {
+    d_color_alloc(P, VGrid{blockIdx.x, gridDim.x});
 }
 
 __device__ __forceinline__ LeafDesc task_leaf(const EditPass& P, uint32_t handle) {
@@ -1172,13 +1209,13 @@ constexpr uint64_t kColorKey = 0x0000FF00FFFFFFFFull;
 
 // keep / full slots of interior items at levels >= ctl: one wavefront per slot. Whole chunks
 // of one colour become markers; the partial chunks at either end are written as colours.
-__global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
+__device__ __forceinline__ void d_color_copy(const EditPass& P, uint32_t l, VGrid G) {
     const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items T = P.items[P.ctl];
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t w = wave_index(); w < n * 8; w += wave_count()) {
+    for (uint32_t w = wave_index(G); w < n * 8; w += wave_count(G)) {
         const uint32_t i = w >> 3, slot = w;
         const uint32_t kind = I.ckind[slot];
         if (kind != KIND_KEEP && kind != KIND_FULL) continue;
@@ -1215,18 +1252,21 @@ __global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
         }
     }
 }
+__global__ __launch_bounds__(256) void k_color_copy(EditPass P, uint32_t l) {
+    d_color_copy(P, l, VGrid{blockIdx.x, gridDim.x});
+}
 
 // keep / full slots of the items at level L - 1 (their children are leaves: at most 64 colours
 // a slot): one wavefront per item, the colours of all its keep / full slots laid out over the
 // lanes (lane j of round r takes colour r * 64 + j of the item's slots in slot order).
-__global__ __launch_bounds__(256) void k_color_copy_leaves(EditPass P) {
+__device__ __forceinline__ void d_color_copy_leaves(const EditPass& P, VGrid G) {
     const uint32_t l = P.levels - 3;
     const uint32_t n = min(P.count[l], P.cap[l]);
     if (*P.errors) return;
     const Items I = P.items[l];
     const Items T = P.items[P.ctl];
     const uint32_t lane = threadIdx.x & 63u, s = lane & 7u;
-    for (uint32_t i = wave_index(); i < n; i += wave_count()) {
+    for (uint32_t i = wave_index(G); i < n; i += wave_count(G)) {
         const uint32_t slot = i * 8 + s;
         const uint32_t kind = I.ckind[slot];
         const uint32_t cnt = (kind == KIND_KEEP || kind == KIND_FULL) ? I.cnew[slot] : 0u;
@@ -1263,6 +1303,9 @@ __global__ __launch_bounds__(256) void k_color_copy_leaves(EditPass P) {
             P.colors[cb + csj + k] = col;
         }
     }
+}
+__global__ __launch_bounds__(256) void k_color_copy_leaves(EditPass P) {
+    d_color_copy_leaves(P, VGrid{blockIdx.x, gridDim.x});
 }
 
 // leaf_decode of ascending colour indices of one leaf. An index in the macro block of the
@@ -1325,13 +1368,13 @@ struct LeafCursor {
 // the unclaimed set voxels read their old colours in ascending order through a LeafCursor.
 // With a paint editor (whose colour depends on the colour before it) every set voxel follows
 // the edit list on its own, as a lane of k_color_leaf does.
-__global__ __launch_bounds__(256) void k_color_leaf_items(EditPass P) {
+__device__ __forceinline__ void d_color_leaf_items(const EditPass& P, VGrid G) {
     const uint32_t L = P.levels - 2;
     const uint32_t n = min(P.count[L], P.cap[L]);
     if (*P.errors) return;
     const Items I = P.items[L];
     const Items T = P.items[P.ctl];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = G.block * blockDim.x + threadIdx.x; i < n; i += G.nblocks * blockDim.x) {
         const uint32_t es = I.estart[i], ec = I.ecount[i];
         const uint32_t bx = I.px[i] << 2, by = I.py[i] << 2, bz = I.pz[i] << 2;
         const uint32_t old = I.old_[i];
@@ -1422,6 +1465,9 @@ __global__ __launch_bounds__(256) void k_color_leaf_items(EditPass P) {
         }
     }
 }
+__global__ __launch_bounds__(256) void k_color_leaf_items(EditPass P) {
+    d_color_leaf_items(P, VGrid{blockIdx.x, gridDim.x});
+}
 
 // A leaf item's fields that k_color_leaf reads first.
 struct LeafMeta {
@@ -1446,15 +1492,15 @@ __device__ __forceinline__ EditHead head_of_lane(const EditHead& mh, uint32_t k)
 // share edits, so their edit heads stay cached), reads the next item's fields while it works
 // on one, and loads an item's edit heads into the lanes at once (lane k: edit k), each edit
 // then read out of its lane.
-__global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
+__device__ __forceinline__ void d_color_leaf(const EditPass& P, VGrid G) {
     const uint32_t L = P.levels - 2;
     const uint32_t n = min(P.count[L], P.cap[L]);
     if (*P.errors) return;
     const Items I = P.items[L];
     const Items T = P.items[P.ctl];
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t run = max(1u, min(32u, n / (4u * wave_count())));
-    for (uint32_t i0 = wave_index() * run; i0 < n; i0 += wave_count() * run) {
+    const uint32_t run = max(1u, min(32u, n / (4u * wave_count(G))));
+    for (uint32_t i0 = wave_index(G) * run; i0 < n; i0 += wave_count(G) * run) {
         const uint32_t iEnd = min(i0 + run, n);
         LeafMeta next = load_meta(I, i0);
         for (uint32_t i = i0; i < iEnd; ++i) {
@@ -1559,6 +1605,9 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
         }
     }
 }
+__global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
+    d_color_leaf(P, VGrid{blockIdx.x, gridDim.x});
+}
 
 // colour tree: make sure every task with colours has colour-tree nodes down to its leaf slot
 // and a unique leaf handle (hash_dag_edits.h:225-249); one thread, tasks in index order.
@@ -1568,12 +1617,12 @@ __global__ __launch_bounds__(256) void k_color_leaf(EditPass P) {
 // for undo. Colour-node entries only ever change from 0 to a fixed value, so a stale read is
 // either that value or 0, and a 0 goes through the CAS. Leaf slots are per task (one task per
 // level-ctl node position), so they are written without atomics.
-__global__ __launch_bounds__(256) void k_color_tree(EditPass P) {
+__device__ __forceinline__ void d_color_tree(const EditPass& P, VGrid G) {
     if (*P.errors) return;
     const uint32_t ctl = P.ctl;
     const uint32_t n = min(P.count[ctl], P.cap[ctl]);
     const Items T = P.items[ctl];
-    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += gridDim.x * blockDim.x) {
+    for (uint32_t t0 = G.block * blockDim.x + (threadIdx.x & ~63u); t0 < n; t0 += G.nblocks * blockDim.x) {
         const uint32_t i = t0 + (threadIdx.x & 63u);
         bool ok = i < n && T.ncount[i] != 0;
         if (i < n && !ok) T.newhandle[i] = 0;
@@ -1626,6 +1675,9 @@ __global__ __launch_bounds__(256) void k_color_tree(EditPass P) {
             set_error(P.errors, ERR_UNDO);
         }
     }
+}
+__global__ __launch_bounds__(256) void k_color_tree(EditPass P) {
+    d_color_tree(P, VGrid{blockIdx.x, gridDim.x});
 }
 
 // A failed pass: every slot in the undo log gets back the leaf it resolved to before.
@@ -1801,7 +1853,7 @@ __device__ __forceinline__ void encode_colors(const uint64_t* __restrict__ cols,
 // tasks per wavefront would leave the kernel's time to the wavefront with the largest share.
 // Each task reserves its arena words with one atomic once its blocks and weight bits are
 // counted.
-__global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
+__device__ __forceinline__ void d_encode(const EditPass& P, VGrid G) {
     const uint32_t ctl = P.ctl;
     const uint32_t n = min(P.count[ctl], P.cap[ctl]);
     if (*P.errors) return;
@@ -1809,7 +1861,8 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
     __shared__ uint32_t sBits[kEncWaves][12];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     uint32_t* buf = sBits[wv];
-    for (uint32_t t = blockIdx.x * kEncWaves + wv; t < n; t += gridDim.x * kEncWaves) {
+    uint64_t stagedBytes = 0;  // chunk markers and explicit colours (measurement)
+    for (uint32_t t = G.block * kEncWaves + wv; t < n; t += G.nblocks * kEncWaves) {
         const uint32_t cnt = T.ncount[t];
         if (cnt == 0) continue;
         if ((cnt + 16383u) / 16384u > 160) {  // more than 160 * 16384 colours in one subtree
@@ -1817,6 +1870,15 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
             continue;
         }
         const uint64_t cb = T.cbase[t];
+        if (P.abytes) {
+            // each marker and explicit colour (8 B) is written once by the staging kernels and
+            // read twice here (the counting walk, then the writing walk)
+            const uint32_t nCh = (cnt + 63u) / 64u;
+            for (uint32_t c = lane; c < nCh; c += 64) {
+                const bool uniform = (P.cmarks[cb / 64 + c] & kUniformChunk) != 0;
+                stagedBytes += 24u * (1u + (uniform ? 0u : min(64u, cnt - c * 64u)));
+            }
+        }
         uint32_t nBlocks, nBits;
         encode_colors<false>(P.colors + cb, P.cmarks + cb / 64, cnt, lane, buf, nullptr, nullptr, nullptr, nBlocks,
                              nBits);
@@ -1849,6 +1911,10 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
             P.leaves[T.newhandle[t] & 0x7FFFFFFFu] = d;
         }
     }
+    if (P.abytes) wave_add64(P.abytes + 2, stagedBytes);
+}
+__global__ __launch_bounds__(kEncBlock) void k_encode(EditPass P) {
+    d_encode(P, VGrid{blockIdx.x, gridDim.x});
 }
 
 // ---- fill editor preparation (BufferEditor / FillEditorBase, hash_dag_editors.h:144-209, 629-671) ----
@@ -1978,8 +2044,8 @@ __global__ void k_init_pass(EditPass P, uint32_t root, uint32_t first, uint32_t 
     }
 }
 
-__global__ void k_finish(EditPass P, uint32_t* root_out) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && *P.errors == 0) {
+__device__ __forceinline__ void d_finish(const EditPass& P, uint32_t* root_out, VGrid G) {
+    if (G.block == 0 && threadIdx.x == 0 && *P.errors == 0) {
         const uint32_t r = P.items[0].nidx[0];
         if (r == kInvalidNode) {
             set_error(P.errors, ERR_EMPTY);
@@ -1989,6 +2055,9 @@ __global__ void k_finish(EditPass P, uint32_t* root_out) {
             *reinterpret_cast<unsigned long long*>(P.pool_top + kArenaTopWord) = *P.arena_top;
         }
     }
+}
+__global__ void k_finish(EditPass P, uint32_t* root_out) {
+    d_finish(P, root_out, VGrid{blockIdx.x, gridDim.x});
 }
 
 }  // namespace
@@ -2205,6 +2274,91 @@ hd_status leaf_tags_rebuild(hd_ctx* ctx, hipStream_t s) {
     return HD_OK;
 }
 
+namespace {
+// A whole pass in one launch: the pass kernels' bodies in the order the level-synchronous form
+// launches them, each over the same per-level item queues (the items k_expand appends for the
+// next level, the representatives k_dedup lists for k_find), with a grid-wide barrier between
+// steps. Launched cooperatively, so every block is resident and every barrier is reached by
+// every block: the step sequence depends only on the kernel's arguments, and a body that finds
+// an error flag set returns to the sequence, not out of the kernel.
+template <bool kSpecial, bool kColors, class Sync>
+__device__ __forceinline__ void pass_steps(const EditPass& P, uint32_t* root_out, VGrid G, Sync sync) {
+    const uint32_t L = P.levels - 2;
+    for (uint32_t l = 0; l < L; ++l) {
+        d_expand(P, l, G);
+        sync();
+    }
+    d_leaf<kSpecial>(P, G);
+    sync();
+    for (int l = (int)L; l >= 0; --l) {
+        if (l < (int)L) {
+            d_build(P, (uint32_t)l, G);
+            sync();
+        }
+        d_hset_clear(P, (uint32_t)l, G);
+        sync();
+        d_dedup(P, (uint32_t)l, G);
+        sync();
+        d_find(P, (uint32_t)l, G);
+        sync();
+        d_add(P, (uint32_t)l, G);
+        sync();
+        d_resolve(P, (uint32_t)l, G);
+        sync();
+    }
+    if constexpr (kColors) {
+        for (uint32_t l = P.ctl; l < L; ++l) {
+            d_color_starts(P, l, G);
+            sync();
+        }
+        d_color_alloc(P, G);
+        sync();
+        d_color_tree(P, G);
+        sync();
+        for (uint32_t l = P.ctl; l < L; ++l) {
+            if (l + 1 == L) d_color_copy_leaves(P, G);
+            else d_color_copy(P, l, G);
+            sync();
+        }
+        if constexpr (kSpecial) d_color_leaf(P, G);
+        else d_color_leaf_items(P, G);
+        sync();
+        d_encode(P, G);
+        sync();
+    }
+    d_finish(P, root_out, G);
+}
+
+template <bool kSpecial, bool kColors>
+__global__ __launch_bounds__(256) void k_pass_persistent(EditPass P, uint32_t* root_out) {
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    pass_steps<kSpecial, kColors>(P, root_out, VGrid{blockIdx.x, gridDim.x}, [&] { grid.sync(); });
+}
+
+// The same steps in one workgroup (a small pass: a few edits of small radius touch few nodes
+// per level), a block barrier between steps: one ordinary launch per pass.
+template <bool kSpecial, bool kColors>
+__global__ __launch_bounds__(256) void k_pass_block(EditPass P, uint32_t* root_out) {
+    pass_steps<kSpecial, kColors>(P, root_out, VGrid{0, 1}, [] { __syncthreads(); });
+}
+
+template <bool kSpecial, bool kColors>
+hd_status launch_persistent(hd_ctx* ctx, EditPass& P, hipStream_t s) {
+    auto* fn = k_pass_persistent<kSpecial, kColors>;
+    int perCU = 0;
+    HD_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, 256, 0));
+    int dev = 0, cus = 0;
+    HD_HIP(ctx, hipGetDevice(&dev));
+    HD_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (perCU < 1 || cus < 1) return fail(ctx, HD_ERR_HIP, "persistent edit pass: no resident block fits");
+    const uint32_t blocks = std::min<uint32_t>((uint32_t)perCU, ctx->edit_persist_per_cu) * (uint32_t)cus;
+    uint32_t* root_out = ctx->d_root;
+    void* args[] = {&P, &root_out};
+    HD_HIP(ctx, hipLaunchCooperativeKernel(reinterpret_cast<const void*>(fn), dim3(blocks), dim3(256), args, 0, s));
+    return HD_OK;
+}
+}  // namespace
+
 static void fill_pass(hd_ctx* ctx, EditPass& P, const EditScratch& S) {
     P.g = ctx->g;
     P.levels = ctx->g.levels;
@@ -2269,6 +2423,26 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
     // the root index lives in ctx->d_root (device) so passes chain without a host round trip
     hipMemcpyAsync(&P.items[0].old_[0], ctx->d_root, 4, hipMemcpyDeviceToDevice, s);
     hipMemcpyAsync(&P.items[0].orig[0], ctx->d_root, 4, hipMemcpyDeviceToDevice, s);
+    // small passes: one workgroup, one launch
+    if (n <= ctx->edit_block_max) {
+        if (special) {
+            if (colors) k_pass_block<true, true><<<1, 256, 0, s>>>(P, ctx->d_root);
+            else k_pass_block<true, false><<<1, 256, 0, s>>>(P, ctx->d_root);
+        } else {
+            if (colors) k_pass_block<false, true><<<1, 256, 0, s>>>(P, ctx->d_root);
+            else k_pass_block<false, false><<<1, 256, 0, s>>>(P, ctx->d_root);
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(ctx, e, "edit pass launch");
+        return HD_OK;
+    }
+    // one cooperative launch for passes of at most edit_persist_max edits (not under capture)
+    if (!dN && n <= ctx->edit_persist_max) {
+        hd_status st;
+        if (special) st = colors ? launch_persistent<true, true>(ctx, P, s) : launch_persistent<true, false>(ctx, P, s);
+        else st = colors ? launch_persistent<false, true>(ctx, P, s) : launch_persistent<false, false>(ctx, P, s);
+        return st;
+    }
     for (uint32_t l = 0; l < L; ++l) k_expand<<<kGrid, 256, 0, s>>>(P, l);
     if (special) k_leaf<true><<<kGrid, 256, 0, s>>>(P);
     else k_leaf<false><<<kGrid, 256, 0, s>>>(P);

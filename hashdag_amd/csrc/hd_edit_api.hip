@@ -386,7 +386,6 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
         if (err & ERR_KIND) return fail(ctx, HD_ERR_INVALID, "copy or fill editor in a pass built without them");
         if (ctx->count_edit_bytes) {
             ctx->edit_bytes_host[0] += rb.stats[1] * 4;                  // node words written
-            ctx->edit_bytes_host[1] += rb.stats[4] * 16;                 // staged colours written and read
             ctx->edit_bytes_host[2] += (rb.stats[3] - ctx->arena_top) * 8;  // colour leaf words written
         }
         ctx->arena_top = rb.stats[3];
@@ -552,6 +551,26 @@ hd_status hd_undo_depth(hd_ctx* ctx, uint32_t* undo, uint32_t* redo) {
     return HD_OK;
 }
 
+hd_status hd_set_edit_launch(hd_ctx* ctx, uint32_t workgroup_max_edits, uint32_t persist_max_edits,
+                             uint32_t blocks_per_cu) {
+    if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
+    if (blocks_per_cu < 1 || blocks_per_cu > 8) return fail(ctx, HD_ERR_INVALID, "blocks_per_cu must be 1..8");
+    ctx->edit_block_max = workgroup_max_edits;
+    ctx->edit_persist_max = persist_max_edits;
+    ctx->edit_persist_per_cu = blocks_per_cu;
+    ctx->ptr_gen++;  // a captured pass bakes its launch form in
+    return HD_OK;
+}
+
+hd_status hd_get_edit_launch(hd_ctx* ctx, uint32_t* workgroup_max_edits, uint32_t* persist_max_edits,
+                             uint32_t* blocks_per_cu) {
+    if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
+    if (workgroup_max_edits) *workgroup_max_edits = ctx->edit_block_max;
+    if (persist_max_edits) *persist_max_edits = ctx->edit_persist_max;
+    if (blocks_per_cu) *blocks_per_cu = ctx->edit_persist_per_cu;
+    return HD_OK;
+}
+
 hd_status hd_edit_count_bytes(hd_ctx* ctx, uint32_t on, uint64_t out[5]) {
     if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
     HD_HIP(ctx, hipDeviceSynchronize());
@@ -562,7 +581,7 @@ hd_status hd_edit_count_bytes(hd_ctx* ctx, uint32_t on, uint64_t out[5]) {
         ctx->ptr_gen++;  // captured passes bake the counter pointer in
         return HD_OK;
     }
-    unsigned long long dev[2] = {0, 0};
+    unsigned long long dev[3] = {0, 0, 0};
     HD_HIP(ctx, hipMemcpy(dev, ctx->edit_bytes, sizeof(dev), hipMemcpyDeviceToHost));
     if (ctx->count_edit_bytes) ctx->ptr_gen++;
     ctx->count_edit_bytes = false;
@@ -570,7 +589,7 @@ hd_status hd_edit_count_bytes(hd_ctx* ctx, uint32_t on, uint64_t out[5]) {
         out[0] = dev[0];
         out[1] = dev[1];
         out[2] = ctx->edit_bytes_host[0];
-        out[3] = ctx->edit_bytes_host[1];
+        out[3] = dev[2];
         out[4] = ctx->edit_bytes_host[2];
     }
     return HD_OK;
@@ -764,21 +783,22 @@ static hd_status graph_capture(hd_frame_graph* g) {
     EditScratch& S = ctx->edit;
     const bool colors = ctx->colors_loaded && ctx->g.leaf_level() > ctx->g.color_tree_levels;
     hipStream_t s = g->stream;
-    if (ctx->trace_persist || ctx->long_steps) {
+    const bool trace = g->width != 0;  // edit-only graphs hold no trace
+    if (trace && (ctx->trace_persist || ctx->long_steps)) {
         const hd_status rs = trace_reserve(ctx, s, g->width, g->height);
         if (rs != HD_OK) return rs;
     }
     HD_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipMemcpyAsync(g->d_n, g->h_n, 4, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(S.edits_dev, g->h_edits, (uint64_t)g->max_edits * sizeof(DevEdit), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(g->d_params, g->h_params, sizeof(hd_ray_params), hipMemcpyHostToDevice, s);
+    if (trace) hipMemcpyAsync(g->d_params, g->h_params, sizeof(hd_ray_params), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(ctx->bucket_snapshot, ctx->bucket_sizes, (uint64_t)ctx->g.total_buckets() * 4, hipMemcpyDeviceToDevice, s);
     k_reset<<<1, 1, 0, s>>>(ctx->counters, S.dstats);
     hd_status st = enqueue_pass(ctx, 0, g->max_edits, colors, s, g->d_n, false);  // kinds checked at capture
     k_count_voxels<<<std::min<uint32_t>(g->max_edits, 4096), 256, 0, s>>>(
         S.edits_dev, g->max_edits, g->d_n, 1u << ctx->g.levels, reinterpret_cast<unsigned long long*>(S.dstats + 0));
-    if (st == HD_OK) st = enqueue_frame_from_device(ctx, g->d_params, g->width, g->height, g->paths, g->colors, g->bias,
-                                                    g->fog, s);
+    if (st == HD_OK && trace)
+        st = enqueue_frame_from_device(ctx, g->d_params, g->width, g->height, g->paths, g->colors, g->bias, g->fog, s);
     hipMemcpyAsync(g->h_rb->counters, ctx->counters, sizeof(g->h_rb->counters), hipMemcpyDeviceToHost, s);
     hipMemcpyAsync(g->h_rb->stats, S.dstats, sizeof(g->h_rb->stats), hipMemcpyDeviceToHost, s);
     hipMemcpyAsync(&g->h_rb->root, ctx->d_root, 4, hipMemcpyDeviceToHost, s);
@@ -794,7 +814,8 @@ extern "C" {
 
 hd_status hd_frame_graph_create(hd_ctx* ctx, uint32_t max_edits, uint32_t width, uint32_t height, uint32_t* d_paths,
                                 uint32_t* d_colors, float shadow_bias, float fog_density, hd_frame_graph** out) {
-    if (!ctx || !out || !d_paths || !d_colors || !width || !height || !max_edits)
+    const bool editOnly = !width && !height && !d_paths && !d_colors;
+    if (!ctx || !out || !max_edits || (!editOnly && (!d_paths || !d_colors || !width || !height)))
         return fail(ctx, HD_ERR_INVALID, "bad arguments");
     if (!ctx->loaded || !ctx->colors_loaded) return fail(ctx, HD_ERR_INVALID, "no DAG or colours loaded");
     auto* g = new hd_frame_graph();
@@ -846,7 +867,7 @@ hd_status hd_frame_graph_destroy(hd_frame_graph* g) {
 
 hd_status hd_frame_graph_launch(hd_frame_graph* g, const hd_edit* edits, uint32_t n, const hd_ray_params* params,
                                 hd_edit_stats* stats, uint32_t* replayed) {
-    if (!g || !params || (n && !edits)) return fail(g ? g->ctx : nullptr, HD_ERR_INVALID, "bad arguments");
+    if (!g || (!params && g->width) || (n && !edits)) return fail(g ? g->ctx : nullptr, HD_ERR_INVALID, "bad arguments");
     hd_ctx* ctx = g->ctx;
     if (!ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
     if (n > g->max_edits) return fail(ctx, HD_ERR_INVALID, "more edits than the graph was built for");
@@ -863,7 +884,7 @@ hd_status hd_frame_graph_launch(hd_frame_graph* g, const hd_edit* edits, uint32_
     }
     for (uint32_t i = 0; i < n; ++i) g->h_edits[i] = make_edit(edits[i]);
     *g->h_n = n;
-    *g->h_params = *params;
+    if (params) *g->h_params = *params;
     HD_HIP(ctx, hipGraphLaunch(g->exec, g->stream));
     HD_HIP(ctx, hipStreamSynchronize(g->stream));
     const Readback& rb = *g->h_rb;
@@ -884,6 +905,10 @@ hd_status hd_frame_graph_launch(hd_frame_graph* g, const hd_edit* edits, uint32_
         }
         if (replayed) *replayed = 0;
         st = hd_edit_batch(ctx, edits, n, 1, stats, g->stream);
+        if (!g->width) {
+            if (st == HD_OK) HD_HIP(ctx, hipStreamSynchronize(g->stream));
+            return st;
+        }
         if (st == HD_OK) st = hd_trace_paths(ctx, params, g->width, g->height, g->paths, g->stream);
         if (st == HD_OK) st = hd_trace_colors(ctx, g->width, g->height, g->paths, g->colors, g->stream);
         if (st == HD_OK)

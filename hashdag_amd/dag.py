@@ -238,6 +238,18 @@ class HashDAG:
         check(lib.hd_edit_batch_static(self._h, arr, len(edits), C.byref(st), stream), self._h)
         return st
 
+    def set_edit_launch(self, workgroup_max_edits: int = 0, persist_max_edits: int = 0,
+                        blocks_per_cu: int = 2) -> None:
+        """Passes of at most workgroup_max_edits editors run in one workgroup (one launch), else
+        of at most persist_max_edits editors as one cooperative launch; larger passes launch per
+        step and level."""
+        check(lib.hd_set_edit_launch(self._h, workgroup_max_edits, persist_max_edits, blocks_per_cu), self._h)
+
+    def edit_launch(self) -> tuple[int, int, int]:
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib.hd_get_edit_launch(self._h, C.byref(a), C.byref(b), C.byref(c)), self._h)
+        return a.value, b.value, c.value
+
     def count_edit_bytes(self, on: bool):
         """Starts (on) or stops counting the edit passes' algorithmic bytes; stopping returns
         {descent, find, nodes_written, colours_staged, leaves_written} in bytes."""

@@ -116,7 +116,9 @@ def launch_config(dag: HashDAG) -> dict:
     check(lib.hd_get_trace_schedule(dag.handle, ptr(sch)), dag.handle)
     schedule = {0: "per tile", 1: f"persistent (refill at {int(sch[1])})",
                 2: f"per tile, walks beyond {int(sch[1])} steps a wavefront per ray"}[int(sch[0])]
-    return {"tile": f"{out[0]}x{out[1]}", "fetch": ["pointer", "node"][out[2]], "schedule": schedule}
+    key = {0: "tile", 1: f"persist:{int(sch[1])}", 2: f"beam:{int(sch[1])}"}[int(sch[0])]
+    return {"tile": f"{out[0]}x{out[1]}", "fetch": ["pointer", "node"][out[2]], "schedule": schedule,
+            "schedule_key": key}
 
 
 def autotune(tracer: "DAGTracer", dag: HashDAG, params_list, reps: int = 2) -> dict:

@@ -236,13 +236,29 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
  * history, which it clears. Calls that overlap on one GPU share its array and race. */
 #define HD_STATIC_MAX_EDITS 256u
 hd_status hd_edit_batch_static(hd_ctx* ctx, const hd_edit* edits, uint32_t n, hd_edit_stats* stats, void* stream);
+/* Launch form of edit passes, no counterpart in the reference (its edit_threads recurses on 8
+ * host threads, hash_dag.h:256-405). A pass runs every step (expand per level, leaves, build /
+ * dedup / find / add per level, colours, encode) as:
+ *   - one launch of a single workgroup, a block barrier between steps, when it holds at most
+ *     workgroup_max_edits editors (small passes; also inside a frame graph);
+ *   - else one cooperative launch of blocks_per_cu (1..8, capped at what fits) resident blocks
+ *     per CU, a grid-wide barrier between steps, when it holds at most persist_max_edits
+ *     editors (not inside a frame graph);
+ *   - else one launch per step and level.
+ * Both limits 0 (the default): every pass launches per step and level. Results are the same in
+ * every form. */
+hd_status hd_set_edit_launch(hd_ctx* ctx, uint32_t workgroup_max_edits, uint32_t persist_max_edits,
+                             uint32_t blocks_per_cu);
+hd_status hd_get_edit_launch(hd_ctx* ctx, uint32_t* workgroup_max_edits, uint32_t* persist_max_edits,
+                             uint32_t* blocks_per_cu);
 /* Edit pass measurement, no counterpart in the reference (its "edits" stat is a time,
  * hash_dag.h:262,368): on = 1 zeroes the counters and counts the algorithmic bytes of every
  * following pass (the counting kernels are slower; time passes without it); on = 0 stops and
  * returns out[0] node words read by the descent (header and child pointers of every node the
  * passes visit, 8 B per leaf), out[1] bucket bytes the finds read (tag bytes and compared
- * words), out[2] node words written, out[3] staged colours written and read by the encoder
- * (16 B each), out[4] colour leaf words written. Old colours decoded are not counted. */
+ * words), out[2] node words written, out[3] staged colours: every 64-colour chunk's marker and
+ * every colour of an explicit chunk (8 B each), written once and read twice by the encoder,
+ * out[4] colour leaf words written. Old colours decoded are not counted. */
 hd_status hd_edit_count_bytes(hd_ctx* ctx, uint32_t on, uint64_t out[5]);
 /* HashDAGUndoRedo::undo / redo (hash_dag.h:68-120) and free (133-145). */
 hd_status hd_undo(hd_ctx* ctx);
@@ -274,7 +290,8 @@ hd_status hd_gc_roots(hd_ctx* ctx, uint32_t max_level, uint32_t keep_undo, uint3
  * hd_edit_batch(fused = 1) + hd_trace_*; a replay that runs out of a capacity is rolled back
  * and run eagerly instead (*replayed = 0), and the graph is re-captured when a buffer it uses
  * has been reallocated. No counterpart in the reference (one edit per UI event, then a frame:
- * engine.cpp:575-718). */
+ * engine.cpp:575-718). width = height = 0 with null d_paths / d_colors builds an edit-only
+ * graph (the pass and its readback; params may then be null). */
 typedef struct hd_frame_graph hd_frame_graph;
 hd_status hd_frame_graph_create(hd_ctx* ctx, uint32_t max_edits, uint32_t width, uint32_t height, uint32_t* d_paths,
                                 uint32_t* d_colors, float shadow_bias, float fog_density, hd_frame_graph** out);

@@ -43,6 +43,13 @@
  *                     from hash_table.h:276-357                   tests/test_oracle_pins.py
  *   decompressing copy dense-grid brute force with the buffer mapping and the empty-source
  *                     checks                                     tests/test_oracle_copy_dec.py
+ *   noise editor      FastNoise's Perlin fractal: the seeded permutation against std::mt19937_64
+ *                     (pinned by the standard's 10000th value), zero at lattice points, the
+ *                     selected gradient next to them, continuity across cell faces, the bounded
+ *                     octave sum; the editor against a dense-grid brute force
+ *                                                                tests/test_oracle_noise.py
+ *   fog (density > 0) host expf / pow; the reference's device expf is not available: parity
+ *                     unpinned beyond +-1 per RGB8 channel        tests/test_trace_gpu.py
  */
 #pragma once
 #include <stdint.h>

@@ -16,8 +16,10 @@ def canonical(dag):
     return OracleDAG.from_image(dag.download(), dag.pool_pages).canonical()
 
 
-@pytest.mark.parametrize("tiny_scratch,schedule", [(False, (0, 0)), (True, (0, 0)), (False, (2, 4)), (False, (1, 16))])
-def test_graph_replays_match_eager(tiny_scratch, schedule):
+@pytest.mark.parametrize("tiny_scratch,schedule,workgroup", [(False, (0, 0), False), (True, (0, 0), False),
+                                                        (False, (2, 4), False), (False, (1, 16), False),
+                                                        (True, (0, 0), True)])
+def test_graph_replays_match_eager(tiny_scratch, schedule, workgroup):
     import torch
 
     from hashdag_amd import DAGInfo, DAGTracer, HashDAG, ray_params
@@ -34,6 +36,8 @@ def test_graph_replays_match_eager(tiny_scratch, schedule):
     ga.load_scene(sc)
     if tiny_scratch:  # the first replays overflow: eager fallback, growth, re-capture
         ga.reserve_edit_scratch(64, 256, 1024)
+    if workgroup:  # the captured pass (and the eager fallback) run in one workgroup
+        ga.set_edit_launch(1 << 30)
     eb = HashDAG(g, pool_pages=POOL_PAGES)
     eb.load_scene(sc)
     ta, tb = DAGTracer(W, H), DAGTracer(W, H)
@@ -69,3 +73,33 @@ def test_graph_replays_match_eager(tiny_scratch, schedule):
     ref.load_scene(sc)
     assert np.array_equal(canonical(ga), canonical(ref))
     graph.close()
+
+
+def test_edit_only_graph_matches_oracle():
+    """An edit-only graph (the pass without a trace) replayed once per edit, with an overflow
+    on its first replays (eager fallback, growth, re-capture): voxel counts, DAG, colours and
+    leaf words against the oracle; undo restores the loaded scene."""
+    from hashdag_amd import HashDAG
+    from hashdag_amd.graph import FrameGraph
+    from oracle import OracleColors, OracleDAG
+    from tests.test_edit_gpu import assert_same
+    p, sc, g = scene("d13")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    dag = HashDAG(g, pool_pages=POOL_PAGES)
+    dag.load_scene(sc)
+    dag.reserve_edit_scratch(64, 256, 1024)
+    before = canonical(dag)
+    graph = FrameGraph(dag, None, 1)
+    replays = 0
+    for k, e in enumerate(random_edits(p, 10, seed=57)):
+        st, ran = graph.launch([e])
+        replays += ran
+        assert st.num_voxels == od.edit(e[0], e[1], e[2], oc)[1], f"edit {k}"
+    assert replays >= 5
+    assert_same(dag, sc, od, oc, "edit-only graph")
+    for _ in range(dag.undo_depth()[0]):
+        dag.undo()
+    assert np.array_equal(canonical(dag), before)
+    graph.close()
+    dag.close()
