@@ -337,10 +337,12 @@ def edit_bench(scene, geom, params, W, H, args):
 
     def plain_by_radius(d, sample):
         """Median milliseconds of one plain pass by editor radius, a launch per step and level
-        against one single-workgroup launch, on a sample of the edit list (each edit undone)."""
+        against the steps of levels with at most 256 / 2048 work items in one workgroup and against
+        one single-workgroup launch, on a sample of the edit list (each edit undone)."""
         classes = ((2, 8), (8, 32), (32, 128), (128, 257))
         out = {}
-        for name, launch in (("level_synchronous", (0, 0)), ("single_workgroup", (1, 0))):
+        for name, launch in (("level_synchronous", (0, 0)), ("small_levels_in_a_workgroup_256", (256, 0)),
+                             ("small_levels_in_a_workgroup_2048", (2048, 0)), ("single_workgroup", (1 << 30, 0))):
             d.set_edit_launch(*launch)
             for e in sample[:8]:
                 d.edit(*e)
@@ -387,14 +389,23 @@ def edit_bench(scene, geom, params, W, H, args):
     else:
         st_b, t_b, batched_form = st_f, t_f, "a launch per step and level"
 
-    # plain: one pass per edit, in both launch forms; the faster is the baseline
+    # plain: one pass per edit, in every launch form; the fastest is the baseline
     plain = HashDAG(geom)
     plain.load_scene(scene)
     inf = float("inf")
     nv_pp, t_pp, img_pp = None, inf, None
     nv_pg, t_pg, img_pg, replays = None, inf, None, 0
+    nv_pt, t_pt, img_pt, tuned = None, inf, None, None
     by_radius = None
     if args.edit_forms in ("all", "fast"):
+        # the launch autotuner on a sample of the list, then the list with its pick
+        tuned = plain.autotune_edit_launch(edits[::64], reps=1)
+        plain.remove_stale_nodes()
+        log(f"edit launch autotuner: {tuned}")
+        st_pt, t_pt = timed(plain, False, edits[:16], launch=(tuned["workgroup_max_items"], 0))
+        nv_pt = st_pt.num_voxels
+        img_pt = frames(plain)
+        undo_all(plain)
         # a captured single-edit pass replayed per edit (HIP graph, edit-only)
         from hashdag_amd.graph import FrameGraph
         plain.set_edit_launch(0, 0)
@@ -417,7 +428,7 @@ def edit_bench(scene, geom, params, W, H, args):
         undo_all(plain)
         by_radius = plain_by_radius(plain, edits[::16])
     if args.edit_forms == "all":
-        st_pp, t_pp = timed(plain, False, edits[:16], launch=(1, 0))
+        st_pp, t_pp = timed(plain, False, edits[:16], launch=(1 << 30, 0))
         nv_pp = st_pp.num_voxels
         img_pp = frames(plain)
         undo_all(plain)
@@ -427,15 +438,17 @@ def edit_bench(scene, geom, params, W, H, args):
     gc_p, _ = gc_timed(plain)
     plain.close()
     log(f"plain: {st_pl.num_voxels} voxels in {t_pl * 1e3:.1f} ms ({st_pl.passes} passes, a launch per step and "
-        f"level), {t_pp * 1e3:.1f} ms (one single-workgroup launch per edit), {t_pg * 1e3:.1f} ms (a captured "
-        f"single-edit graph replayed per edit, {replays} replays), then GC {gc_p * 1e3:.1f} ms")
+        f"level), {t_pt * 1e3:.1f} ms (autotuned small levels in a workgroup), {t_pp * 1e3:.1f} ms (one "
+        f"single-workgroup launch per edit), {t_pg * 1e3:.1f} ms (a captured single-edit graph replayed per edit, "
+        f"{replays} replays), then GC {gc_p * 1e3:.1f} ms")
     plain_forms = {"a launch per step and level": t_pl, "one single-workgroup launch per edit": t_pp,
-                   "a captured single-edit pass replayed per edit": t_pg}
+                   "a captured single-edit pass replayed per edit": t_pg,
+                   "steps of small levels in one workgroup (autotuned), the rest a launch each": t_pt}
     plain_form = min(plain_forms, key=plain_forms.get)
     t_p = plain_forms[plain_form]
     st_p = st_pl
     same = st_pl.num_voxels == st_f.num_voxels and all(np.array_equal(a, b) for a, b in zip(img_f, img_pl))
-    for nv, img in ((nv_pp, img_pp), (nv_pg, img_pg)):
+    for nv, img in ((nv_pp, img_pp), (nv_pg, img_pg), (nv_pt, img_pt)):
         if img is not None:
             same = same and nv == st_f.num_voxels and all(np.array_equal(a, b) for a, b in zip(img_f, img))
     if not same:
@@ -454,7 +467,9 @@ def edit_bench(scene, geom, params, W, H, args):
         "batched_form": batched_form,
         "plain_ms_by_form": {"level_synchronous": round(t_pl * 1e3, 2),
                              "single_workgroup": round(t_pp * 1e3, 2) if t_pp < float("inf") else None,
-                             "graph_replay": round(t_pg * 1e3, 2) if t_pg < float("inf") else None},
+                             "graph_replay": round(t_pg * 1e3, 2) if t_pg < float("inf") else None,
+                             "small_levels_in_a_workgroup": round(t_pt * 1e3, 2) if t_pt < float("inf") else None},
+        "edit_launch_autotuner": tuned,
         "plain_pass_ms_by_radius": by_radius,
         "batched_ms_by_form": {"level_synchronous": round(t_f * 1e3, 2),
                                "cooperative": round(t_fp * 1e3, 2) if t_fp < float("inf") else None},

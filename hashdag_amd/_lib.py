@@ -152,6 +152,8 @@ _SIGS = {
     "hd_edit_count_bytes": (C.c_int32, [C.c_void_p, C.c_uint32, u64p]),
     "hd_set_edit_launch": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     "hd_get_edit_launch": (C.c_int32, [C.c_void_p, u32p, u32p, u32p]),
+    "hd_autotune_edit_launch": (C.c_int32, [C.c_void_p, C.POINTER(hd_edit), C.c_uint32, C.c_uint32, u32p,
+                                            C.POINTER(C.c_float)]),
     "hd_gc": (C.c_int32, [C.c_void_p, C.c_uint32, u64p, C.c_void_p]),
     "hd_write_basic_dag_file": (C.c_int32, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint32, u32p,
                                             C.c_uint64]),

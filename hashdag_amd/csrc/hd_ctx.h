@@ -59,8 +59,9 @@ struct hd_ctx {
     // the passes' own statistics
     unsigned long long* edit_bytes = nullptr;
     bool count_edit_bytes = false;
-    // passes of at most edit_block_max edits run in one workgroup, else of at most
-    // edit_persist_max edits as one cooperative launch (0: never)
+    // pass steps on levels with at most edit_block_max work items run together in one
+    // single-workgroup launch; passes of at most edit_persist_max edits as one cooperative
+    // launch (0: never)
     uint32_t edit_block_max = 0;
     uint32_t edit_persist_max = 0;
     uint32_t edit_persist_per_cu = 2;  // resident blocks per CU of that launch (at most what fits)

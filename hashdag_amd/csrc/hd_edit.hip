@@ -2281,70 +2281,99 @@ namespace {
 // steps. Launched cooperatively, so every block is resident and every barrier is reached by
 // every block: the step sequence depends only on the kernel's arguments, and a body that finds
 // an error flag set returns to the sequence, not out of the kernel.
-template <bool kSpecial, bool kColors, class Sync>
-__device__ __forceinline__ void pass_steps(const EditPass& P, uint32_t* root_out, VGrid G, Sync sync) {
-    const uint32_t L = P.levels - 2;
-    for (uint32_t l = 0; l < L; ++l) {
-        d_expand(P, l, G);
-        sync();
-    }
-    d_leaf<kSpecial>(P, G);
-    sync();
+// The steps of a pass, in order, as (step << 8 | level): expand per level top-down, the
+// leaves, then per level bottom-up build / clear / dedup / find / add / resolve, the colour
+// steps, and the commit. Every launch form runs this sequence.
+enum PassStep : uint32_t {
+    ST_EXPAND, ST_LEAF, ST_BUILD, ST_HCLEAR, ST_DEDUP, ST_FIND, ST_ADD, ST_RESOLVE, ST_CSTARTS, ST_CALLOC,
+    ST_CTREE, ST_CCOPY, ST_CCOPY_LEAVES, ST_CLEAF, ST_ENCODE, ST_FINISH
+};
+constexpr uint32_t kMaxSteps = 8 * kMaxEditLevels + 8;
+struct StepList {
+    uint16_t step[kMaxSteps];
+    uint32_t n;
+};
+
+static StepList pass_step_list(uint32_t levels, uint32_t ctl, bool colors) {
+    StepList sl{};
+    auto add = [&](uint32_t st, uint32_t l) { sl.step[sl.n++] = (uint16_t)(st << 8 | l); };
+    const uint32_t L = levels - 2;
+    for (uint32_t l = 0; l < L; ++l) add(ST_EXPAND, l);
+    add(ST_LEAF, L);
     for (int l = (int)L; l >= 0; --l) {
-        if (l < (int)L) {
-            d_build(P, (uint32_t)l, G);
-            sync();
-        }
-        d_hset_clear(P, (uint32_t)l, G);
-        sync();
-        d_dedup(P, (uint32_t)l, G);
-        sync();
-        d_find(P, (uint32_t)l, G);
-        sync();
-        d_add(P, (uint32_t)l, G);
-        sync();
-        d_resolve(P, (uint32_t)l, G);
-        sync();
+        if (l < (int)L) add(ST_BUILD, (uint32_t)l);
+        add(ST_HCLEAR, (uint32_t)l);
+        add(ST_DEDUP, (uint32_t)l);
+        add(ST_FIND, (uint32_t)l);
+        add(ST_ADD, (uint32_t)l);
+        add(ST_RESOLVE, (uint32_t)l);
     }
-    if constexpr (kColors) {
-        for (uint32_t l = P.ctl; l < L; ++l) {
-            d_color_starts(P, l, G);
-            sync();
-        }
-        d_color_alloc(P, G);
-        sync();
-        d_color_tree(P, G);
-        sync();
-        for (uint32_t l = P.ctl; l < L; ++l) {
-            if (l + 1 == L) d_color_copy_leaves(P, G);
-            else d_color_copy(P, l, G);
-            sync();
-        }
-        if constexpr (kSpecial) d_color_leaf(P, G);
-        else d_color_leaf_items(P, G);
-        sync();
-        d_encode(P, G);
-        sync();
+    if (colors) {
+        for (uint32_t l = ctl; l < L; ++l) add(ST_CSTARTS, l);
+        add(ST_CALLOC, ctl);
+        add(ST_CTREE, ctl);
+        for (uint32_t l = ctl; l < L; ++l) add(l + 1 == L ? ST_CCOPY_LEAVES : ST_CCOPY, l);
+        add(ST_CLEAF, L);
+        add(ST_ENCODE, ctl);
     }
-    d_finish(P, root_out, G);
+    add(ST_FINISH, 0);
+    return sl;
 }
 
-template <bool kSpecial, bool kColors>
-__global__ __launch_bounds__(256) void k_pass_persistent(EditPass P, uint32_t* root_out) {
+template <bool kSpecial>
+__device__ __forceinline__ void run_step(const EditPass& P, uint32_t code, uint32_t* root_out, VGrid G) {
+    const uint32_t l = code & 0xFFu;
+    switch (code >> 8) {
+        case ST_EXPAND: d_expand(P, l, G); break;
+        case ST_LEAF: d_leaf<kSpecial>(P, G); break;
+        case ST_BUILD: d_build(P, l, G); break;
+        case ST_HCLEAR: d_hset_clear(P, l, G); break;
+        case ST_DEDUP: d_dedup(P, l, G); break;
+        case ST_FIND: d_find(P, l, G); break;
+        case ST_ADD: d_add(P, l, G); break;
+        case ST_RESOLVE: d_resolve(P, l, G); break;
+        case ST_CSTARTS: d_color_starts(P, l, G); break;
+        case ST_CALLOC: d_color_alloc(P, G); break;
+        case ST_CTREE: d_color_tree(P, G); break;
+        case ST_CCOPY: d_color_copy(P, l, G); break;
+        case ST_CCOPY_LEAVES: d_color_copy_leaves(P, G); break;
+        case ST_CLEAF:
+            if constexpr (kSpecial) d_color_leaf(P, G);
+            else d_color_leaf_items(P, G);
+            break;
+        case ST_ENCODE: d_encode(P, G); break;
+        default: d_finish(P, root_out, G); break;
+    }
+}
+
+// The whole pass in one cooperative launch: every block resident, a grid-wide barrier between
+// steps; the step sequence depends only on the kernel's arguments, and a step that finds an
+// error flag set returns to the sequence, not out of the kernel, so every block reaches every
+// barrier.
+template <bool kSpecial>
+__global__ __launch_bounds__(256) void k_pass_persistent(EditPass P, StepList sl, uint32_t* root_out) {
     cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-    pass_steps<kSpecial, kColors>(P, root_out, VGrid{blockIdx.x, gridDim.x}, [&] { grid.sync(); });
+    const VGrid G{blockIdx.x, gridDim.x};
+    for (uint32_t i = 0; i < sl.n; ++i) {
+        run_step<kSpecial>(P, sl.step[i], root_out, G);
+        if (i + 1 < sl.n) grid.sync();
+    }
 }
 
-// The same steps in one workgroup (a small pass: a few edits of small radius touch few nodes
-// per level), a block barrier between steps: one ordinary launch per pass.
-template <bool kSpecial, bool kColors>
-__global__ __launch_bounds__(256) void k_pass_block(EditPass P, uint32_t* root_out) {
-    pass_steps<kSpecial, kColors>(P, root_out, VGrid{0, 1}, [] { __syncthreads(); });
+// Steps [first, last) of a pass in one workgroup, a block barrier between steps: the steps of
+// levels with few items (all of a small pass) in one ordinary launch.
+template <bool kSpecial>
+__global__ __launch_bounds__(256) void k_pass_block(EditPass P, StepList sl, uint32_t first, uint32_t last,
+                                                    uint32_t* root_out) {
+    for (uint32_t i = first; i < last; ++i) {
+        run_step<kSpecial>(P, sl.step[i], root_out, VGrid{0, 1});
+        if (i + 1 < last) __syncthreads();
+    }
 }
 
-template <bool kSpecial, bool kColors>
-hd_status launch_persistent(hd_ctx* ctx, EditPass& P, hipStream_t s) {
-    auto* fn = k_pass_persistent<kSpecial, kColors>;
+template <bool kSpecial>
+hd_status launch_persistent(hd_ctx* ctx, EditPass& P, StepList& sl, hipStream_t s) {
+    auto* fn = k_pass_persistent<kSpecial>;
     int perCU = 0;
     HD_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, 256, 0));
     int dev = 0, cus = 0;
@@ -2353,7 +2382,7 @@ hd_status launch_persistent(hd_ctx* ctx, EditPass& P, hipStream_t s) {
     if (perCU < 1 || cus < 1) return fail(ctx, HD_ERR_HIP, "persistent edit pass: no resident block fits");
     const uint32_t blocks = std::min<uint32_t>((uint32_t)perCU, ctx->edit_persist_per_cu) * (uint32_t)cus;
     uint32_t* root_out = ctx->d_root;
-    void* args[] = {&P, &root_out};
+    void* args[] = {&P, &sl, &root_out};
     HD_HIP(ctx, hipLaunchCooperativeKernel(reinterpret_cast<const void*>(fn), dim3(blocks), dim3(256), args, 0, s));
     return HD_OK;
 }
@@ -2413,63 +2442,103 @@ static void fill_pass(hd_ctx* ctx, EditPass& P, const EditScratch& S) {
 // Enqueues one pass over edits [first, first + n) of the uploaded edit array.
 // dN (optional): the edit count is read from device memory (at most n), for graph replays.
 // scratch (optional): the scratch the pass runs in (default: the context's own).
+// Upper bound of the work items of each level a pass over these editors creates: an item is a
+// node some editor's box overlaps, so per editor and level the nodes of that level its box
+// overlaps, capped by the level's node count and item capacity.
+static void level_bounds(const EditPass& P, const DevEdit* e, uint32_t n, uint64_t out[kMaxEditLevels]) {
+    const uint32_t D = P.levels, L = D - 2;
+    const double world = (double)(1ull << D);
+    for (uint32_t l = 0; l <= L && l < kMaxEditLevels; ++l) {
+        const double side = (double)(1ull << (D - l));
+        const uint64_t perAxis = 1ull << l;
+        uint64_t sum = 0;
+        for (uint32_t i = 0; i < n && sum < P.cap[l]; ++i) {
+            uint64_t cnt = 1;
+            for (int a = 0; a < 3; ++a) {
+                const double lo = std::max(0.0, (double)e[i].bmin[a]), hi = std::min(world, (double)e[i].bmax[a]);
+                if (!(hi >= lo)) {
+                    cnt = 0;
+                    break;
+                }
+                const uint64_t c0 = (uint64_t)(lo / side), c1 = std::min<uint64_t>((uint64_t)(hi / side), perAxis - 1);
+                cnt *= c1 >= c0 ? c1 - c0 + 1 : 1;
+            }
+            sum += cnt;
+        }
+        out[l] = std::min<uint64_t>(sum, P.cap[l]);
+    }
+}
+
 hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s, const uint32_t* dN,
-                       bool special, const EditScratch* scratch) {
+                       bool special, const EditScratch* scratch, const DevEdit* hostEdits) {
     EditPass P{};
     fill_pass(ctx, P, scratch ? *scratch : ctx->edit);
+    // grid per level: two blocks per item the level can hold (a wavefront per child slot is the
+    // most any step uses), at most kGrid; kGrid everywhere without the host's copy of the edits
+    uint32_t grid[kMaxEditLevels];
+    uint32_t gridMax = 1;
+    uint64_t bound[kMaxEditLevels];
+    for (uint32_t l = 0; l < kMaxEditLevels; ++l) bound[l] = P.cap[l];
+    if (hostEdits && !dN) level_bounds(P, hostEdits, n, bound);
+    for (uint32_t l = 0; l < kMaxEditLevels; ++l) {
+        grid[l] = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(2 * bound[l], 1), kGrid);
+        gridMax = std::max(gridMax, grid[l]);
+    }
     P.single_edit = (n == 1 && !dN) ? 1u : 0u;
     const uint32_t D = P.levels, L = D - 2;
     k_init_pass<<<(std::max<uint32_t>(n, 1) + 255) / 256, 256, 0, s>>>(P, 0xFFFFFFFFu, first, n, dN);
     // the root index lives in ctx->d_root (device) so passes chain without a host round trip
     hipMemcpyAsync(&P.items[0].old_[0], ctx->d_root, 4, hipMemcpyDeviceToDevice, s);
     hipMemcpyAsync(&P.items[0].orig[0], ctx->d_root, 4, hipMemcpyDeviceToDevice, s);
-    // small passes: one workgroup, one launch
-    if (n <= ctx->edit_block_max) {
-        if (special) {
-            if (colors) k_pass_block<true, true><<<1, 256, 0, s>>>(P, ctx->d_root);
-            else k_pass_block<true, false><<<1, 256, 0, s>>>(P, ctx->d_root);
-        } else {
-            if (colors) k_pass_block<false, true><<<1, 256, 0, s>>>(P, ctx->d_root);
-            else k_pass_block<false, false><<<1, 256, 0, s>>>(P, ctx->d_root);
-        }
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return hip_fail(ctx, e, "edit pass launch");
-        return HD_OK;
-    }
+    StepList sl = pass_step_list(D, P.ctl, colors);
     // one cooperative launch for passes of at most edit_persist_max edits (not under capture)
-    if (!dN && n <= ctx->edit_persist_max) {
-        hd_status st;
-        if (special) st = colors ? launch_persistent<true, true>(ctx, P, s) : launch_persistent<true, false>(ctx, P, s);
-        else st = colors ? launch_persistent<false, true>(ctx, P, s) : launch_persistent<false, false>(ctx, P, s);
-        return st;
-    }
-    for (uint32_t l = 0; l < L; ++l) k_expand<<<kGrid, 256, 0, s>>>(P, l);
-    if (special) k_leaf<true><<<kGrid, 256, 0, s>>>(P);
-    else k_leaf<false><<<kGrid, 256, 0, s>>>(P);
-    for (int l = (int)L; l >= 0; --l) {
-        if (l < (int)L) k_build<<<kGrid, 256, 0, s>>>(P, (uint32_t)l);
-        k_hset_clear<<<256, 256, 0, s>>>(P, (uint32_t)l);
-        k_dedup<<<kGrid, 256, 0, s>>>(P, (uint32_t)l);
-        k_find<<<kGrid, 256, 0, s>>>(P, (uint32_t)l);
-        k_add<<<kGrid, 256, 0, s>>>(P, (uint32_t)l);
-        k_resolve<<<kGrid, 256, 0, s>>>(P, (uint32_t)l);
-    }
-    if (colors) {
-        for (uint32_t l = P.ctl; l < L; ++l) k_color_starts<<<kGrid, 256, 0, s>>>(P, l);
-        k_color_alloc<<<kGrid, 256, 0, s>>>(P);
-        k_color_tree<<<kGrid, 256, 0, s>>>(P);
-        for (uint32_t l = P.ctl; l < L; ++l) {
-            if (l + 1 == L) k_color_copy_leaves<<<kGrid, 256, 0, s>>>(P);
-            else k_color_copy<<<kGrid, 256, 0, s>>>(P, l);
+    if (!dN && n <= ctx->edit_persist_max)
+        return special ? launch_persistent<true>(ctx, P, sl, s) : launch_persistent<false>(ctx, P, sl, s);
+    // otherwise every run of consecutive steps on levels with at most edit_block_max items (by
+    // the editors' boxes; all of a small pass) is one single-workgroup launch, and every other
+    // step is a launch of its own over a grid sized for its level
+    const auto small = [&](uint32_t code) {
+        return (code >> 8) == ST_FINISH || bound[code & 0xFFu] <= (uint64_t)ctx->edit_block_max;
+    };
+    // a wavefront per task slot (the task count is on the device), at most 2^18 of them
+    const uint32_t encBlocks =
+        (uint32_t)std::min<uint64_t>(std::max<uint64_t>((bound[P.ctl] + kEncWaves - 1) / kEncWaves, 1u), 1u << 16);
+    for (uint32_t i = 0; i < sl.n;) {
+        if (small(sl.step[i])) {
+            uint32_t j = i + 1;
+            while (j < sl.n && small(sl.step[j])) ++j;
+            if (special) k_pass_block<true><<<1, 256, 0, s>>>(P, sl, i, j, ctx->d_root);
+            else k_pass_block<false><<<1, 256, 0, s>>>(P, sl, i, j, ctx->d_root);
+            i = j;
+            continue;
         }
-        if (special) k_color_leaf<<<kGrid, 256, 0, s>>>(P);
-        else k_color_leaf_items<<<kGrid, 256, 0, s>>>(P);
-        // a wavefront per task slot (the task count is on the device), at most 2^18 of them
-        const uint32_t encBlocks = std::min<uint32_t>(std::max<uint32_t>((P.cap[P.ctl] + kEncWaves - 1) / kEncWaves, 1u),
-                                                      1u << 16);
-        k_encode<<<encBlocks, kEncBlock, 0, s>>>(P);
+        const uint32_t l = sl.step[i] & 0xFFu, gl = grid[l];
+        switch (sl.step[i] >> 8) {
+            case ST_EXPAND: k_expand<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_LEAF:
+                if (special) k_leaf<true><<<gl, 256, 0, s>>>(P);
+                else k_leaf<false><<<gl, 256, 0, s>>>(P);
+                break;
+            case ST_BUILD: k_build<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_HCLEAR: k_hset_clear<<<std::min<uint32_t>(gl, 256), 256, 0, s>>>(P, l); break;
+            case ST_DEDUP: k_dedup<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_FIND: k_find<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_ADD: k_add<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_RESOLVE: k_resolve<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_CSTARTS: k_color_starts<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_CALLOC: k_color_alloc<<<gl, 256, 0, s>>>(P); break;
+            case ST_CTREE: k_color_tree<<<gridMax, 256, 0, s>>>(P); break;
+            case ST_CCOPY: k_color_copy<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_CCOPY_LEAVES: k_color_copy_leaves<<<gl, 256, 0, s>>>(P); break;
+            case ST_CLEAF:
+                if (special) k_color_leaf<<<gl, 256, 0, s>>>(P);
+                else k_color_leaf_items<<<gl, 256, 0, s>>>(P);
+                break;
+            case ST_ENCODE: k_encode<<<encBlocks, kEncBlock, 0, s>>>(P); break;
+            default: k_finish<<<1, 64, 0, s>>>(P, ctx->d_root); break;
+        }
+        ++i;
     }
-    k_finish<<<1, 64, 0, s>>>(P, ctx->d_root);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "edit pass launch");
     return HD_OK;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -16,7 +17,8 @@ namespace hd {
 hd_status edit_reserve(hd_ctx* ctx, const uint32_t* items, const uint32_t* elist, uint64_t colors);
 void edit_free(hd_ctx* ctx);
 hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s,
-                       const uint32_t* dN = nullptr, bool special = true, const EditScratch* scratch = nullptr);
+                       const uint32_t* dN = nullptr, bool special = true, const EditScratch* scratch = nullptr,
+                       const DevEdit* hostEdits = nullptr);
 hd_status edit_ensure_edits(hd_ctx* ctx, uint32_t n);
 hd_status enqueue_frame_from_device(hd_ctx* ctx, const hd_ray_params* d_params, uint32_t W, uint32_t H,
                                     uint32_t* d_paths, uint32_t* d_colors, float bias, float fog, hipStream_t s);
@@ -286,7 +288,7 @@ static hd_status edit_prepare(hd_ctx* ctx) {
 // `fixed`: run in this scratch and grow nothing (the colour arena, tables and undo log
 // included); a pass that needs more fails with the previous state kept.
 static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s, Readback& rb, UndoFrame* frame,
-                          bool special, const EditScratch* fixed = nullptr) {
+                          bool special, const EditScratch* fixed = nullptr, const DevEdit* hostEdits = nullptr) {
     const bool colors = ctx->colors_loaded && ctx->g.leaf_level() > ctx->g.color_tree_levels;
     for (int attempt = 0; attempt < 40; ++attempt) {
         hd_status st = HD_OK;
@@ -296,7 +298,7 @@ static hd_status run_pass(hd_ctx* ctx, uint32_t first, uint32_t n, hipStream_t s
         const uint64_t bucketBytes = (uint64_t)ctx->g.total_buckets() * 4;
         HD_HIP(ctx, hipMemcpyAsync(ctx->bucket_snapshot, ctx->bucket_sizes, bucketBytes, hipMemcpyDeviceToDevice, s));
         k_reset<<<1, 1, 0, s>>>(ctx->counters, S.dstats);
-        st = enqueue_pass(ctx, first, n, colors, s, nullptr, special, fixed);
+        st = enqueue_pass(ctx, first, n, colors, s, nullptr, special, fixed, hostEdits);
         if (st != HD_OK) return st;
         if (n > 1) {
             k_count_voxels<<<std::min<uint32_t>(n, 4096), 256, 0, s>>>(S.edits_dev + first, n, nullptr,
@@ -466,7 +468,8 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             st = prepare_fill(first);
             if (st != HD_OK) return st;
             st = run_pass(ctx, first, last - first, s, rb, &frame,
-                          is_copy_kind(edits[first].kind) || is_buffer_kind(edits[first].kind));
+                          is_copy_kind(edits[first].kind) || is_buffer_kind(edits[first].kind), nullptr,
+                          dev.data() + first);
             if (st != HD_OK) return st;
             ctx->root = rb.root;
             frame.new_root = rb.root;
@@ -488,7 +491,8 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
             Readback rb{};
             st = prepare_fill(i);
             if (st != HD_OK) return st;
-            st = run_pass(ctx, i, 1, s, rb, &frame, is_copy_kind(edits[i].kind) || is_buffer_kind(edits[i].kind));
+            st = run_pass(ctx, i, 1, s, rb, &frame, is_copy_kind(edits[i].kind) || is_buffer_kind(edits[i].kind),
+                          nullptr, dev.data() + i);
             if (st != HD_OK) return st;
             ctx->root = rb.root;
             frame.new_root = rb.root;
@@ -551,21 +555,62 @@ hd_status hd_undo_depth(hd_ctx* ctx, uint32_t* undo, uint32_t* redo) {
     return HD_OK;
 }
 
-hd_status hd_set_edit_launch(hd_ctx* ctx, uint32_t workgroup_max_edits, uint32_t persist_max_edits,
+hd_status hd_set_edit_launch(hd_ctx* ctx, uint32_t workgroup_max_items, uint32_t persist_max_edits,
                              uint32_t blocks_per_cu) {
     if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
     if (blocks_per_cu < 1 || blocks_per_cu > 8) return fail(ctx, HD_ERR_INVALID, "blocks_per_cu must be 1..8");
-    ctx->edit_block_max = workgroup_max_edits;
+    ctx->edit_block_max = workgroup_max_items;
     ctx->edit_persist_max = persist_max_edits;
     ctx->edit_persist_per_cu = blocks_per_cu;
     ctx->ptr_gen++;  // a captured pass bakes its launch form in
     return HD_OK;
 }
 
-hd_status hd_get_edit_launch(hd_ctx* ctx, uint32_t* workgroup_max_edits, uint32_t* persist_max_edits,
+hd_status hd_autotune_edit_launch(hd_ctx* ctx, const hd_edit* sample, uint32_t n, uint32_t reps,
+                                  uint32_t* workgroup_max_items, float ms_per_edit[5]) {
+    static const uint32_t kCandidates[5] = {0u, 64u, 256u, 1024u, 4096u};
+    if (!ctx || !sample || !n) return fail(ctx, HD_ERR_INVALID, "bad arguments");
+    for (uint32_t i = 0; i < n; ++i)
+        if (sample[i].kind > EDIT_SPHERE_PAINT) return fail(ctx, HD_ERR_INVALID, "sample editors must be sphere / box / paint");
+    const uint32_t keepBlock = ctx->edit_block_max, keepPersist = ctx->edit_persist_max;
+    ctx->edit_persist_max = 0;
+    double best = 1e300;
+    uint32_t pick = keepBlock;
+    for (uint32_t c = 0; c < 5; ++c) {
+        ctx->edit_block_max = kCandidates[c];
+        double total = 0;
+        for (uint32_t r = 0; r <= std::max(reps, 1u); ++r) {  // round 0 warms up (scratch sizes)
+            for (uint32_t i = 0; i < n; ++i) {
+                const auto t0 = std::chrono::steady_clock::now();
+                hd_status st = hd_edit_batch(ctx, sample + i, 1, 0, nullptr, nullptr);
+                const auto t1 = std::chrono::steady_clock::now();
+                if (st == HD_OK) st = hd_undo(ctx);
+                if (st != HD_OK) {
+                    ctx->edit_block_max = keepBlock;
+                    ctx->edit_persist_max = keepPersist;
+                    return st;
+                }
+                if (r > 0) total += std::chrono::duration<double, std::milli>(t1 - t0).count();
+            }
+        }
+        const double per = total / ((double)n * std::max(reps, 1u));
+        if (ms_per_edit) ms_per_edit[c] = (float)per;
+        if (per < best) {
+            best = per;
+            pick = kCandidates[c];
+        }
+    }
+    ctx->edit_block_max = pick;
+    ctx->edit_persist_max = keepPersist;
+    ctx->ptr_gen++;
+    if (workgroup_max_items) *workgroup_max_items = pick;
+    return HD_OK;
+}
+
+hd_status hd_get_edit_launch(hd_ctx* ctx, uint32_t* workgroup_max_items, uint32_t* persist_max_edits,
                              uint32_t* blocks_per_cu) {
     if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
-    if (workgroup_max_edits) *workgroup_max_edits = ctx->edit_block_max;
+    if (workgroup_max_items) *workgroup_max_items = ctx->edit_block_max;
     if (persist_max_edits) *persist_max_edits = ctx->edit_persist_max;
     if (blocks_per_cu) *blocks_per_cu = ctx->edit_persist_per_cu;
     return HD_OK;

@@ -237,20 +237,30 @@ hd_status hd_edit_batch(hd_ctx* ctx, const hd_edit* edits, uint32_t n, uint32_t 
 #define HD_STATIC_MAX_EDITS 256u
 hd_status hd_edit_batch_static(hd_ctx* ctx, const hd_edit* edits, uint32_t n, hd_edit_stats* stats, void* stream);
 /* Launch form of edit passes, no counterpart in the reference (its edit_threads recurses on 8
- * host threads, hash_dag.h:256-405). A pass runs every step (expand per level, leaves, build /
- * dedup / find / add per level, colours, encode) as:
- *   - one launch of a single workgroup, a block barrier between steps, when it holds at most
- *     workgroup_max_edits editors (small passes; also inside a frame graph);
- *   - else one cooperative launch of blocks_per_cu (1..8, capped at what fits) resident blocks
- *     per CU, a grid-wide barrier between steps, when it holds at most persist_max_edits
+ * host threads, hash_dag.h:256-405). A pass is a fixed sequence of steps (expand per level, the
+ * leaves, build / dedup / find / add per level, the colour steps, the commit), run as:
+ *   - one cooperative launch of blocks_per_cu (1..8, capped at what fits) resident blocks per
+ *     CU, a grid-wide barrier between steps, when the pass holds at most persist_max_edits
  *     editors (not inside a frame graph);
- *   - else one launch per step and level.
- * Both limits 0 (the default): every pass launches per step and level. Results are the same in
- * every form. */
-hd_status hd_set_edit_launch(hd_ctx* ctx, uint32_t workgroup_max_edits, uint32_t persist_max_edits,
+ *   - else: every run of consecutive steps on levels where the pass has at most
+ *     workgroup_max_items work items (bounded on the host by the nodes of that level the
+ *     editors' boxes overlap; inside a frame graph by the scratch capacity) as one launch of a
+ *     single workgroup with a block barrier between steps, and every other step as a launch of
+ *     its own over a grid sized by that bound.
+ * workgroup_max_items = 0 and persist_max_edits = 0 (the defaults after hd_create, unless
+ * hd_autotune_edit_launch picked others): a launch per step. Results are the same in every
+ * form. */
+hd_status hd_set_edit_launch(hd_ctx* ctx, uint32_t workgroup_max_items, uint32_t persist_max_edits,
                              uint32_t blocks_per_cu);
-hd_status hd_get_edit_launch(hd_ctx* ctx, uint32_t* workgroup_max_edits, uint32_t* persist_max_edits,
+hd_status hd_get_edit_launch(hd_ctx* ctx, uint32_t* workgroup_max_items, uint32_t* persist_max_edits,
                              uint32_t* blocks_per_cu);
+/* Edit launch autotuner, no counterpart in the reference: times one pass per sample editor
+ * (sphere / box / paint; each applied, timed, undone; reps rounds after one warm-up round) with
+ * workgroup_max_items = 0, 64, 256, 1024 and 4096, keeps the fastest in the context (out:
+ * the pick and the mean milliseconds per edit of each candidate). The trial passes leave their
+ * nodes for the next GC and clear the redo history. */
+hd_status hd_autotune_edit_launch(hd_ctx* ctx, const hd_edit* sample, uint32_t n, uint32_t reps,
+                                  uint32_t* workgroup_max_items, float ms_per_edit[5]);
 /* Edit pass measurement, no counterpart in the reference (its "edits" stat is a time,
  * hash_dag.h:262,368): on = 1 zeroes the counters and counts the algorithmic bytes of every
  * following pass (the counting kernels are slower; time passes without it); on = 0 stops and

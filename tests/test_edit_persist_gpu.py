@@ -1,6 +1,6 @@
 """Edit passes run as one launch (every step of the pass in sequence: in one workgroup with a
-block barrier between steps, or cooperatively over every CU with a grid-wide barrier) against
-the oracle, as the level-synchronous form is tested: plain edits one by one and fused batches on
+block barrier between steps, or cooperatively over every CU with a grid-wide barrier), or with
+the steps of small levels in one workgroup and the others launched alone, against the oracle, as the level-synchronous form is tested: plain edits one by one and fused batches on
 scenes with one and several macro blocks per colour subtree, passes with copy / fill / noise
 editors (the per-voxel colour kernel), the binary depth-12 configuration, overflow errors, and
 undo / redo."""
@@ -13,11 +13,13 @@ from tests.test_edit_gpu import COPY, assert_same, copy_edits, fill_edits, make,
 pytestmark = pytest.mark.gpu
 
 ALL = 1 << 30  # every pass
-WORKGROUP = (ALL, 0, 2)
+WORKGROUP = (ALL, 0, 2)     # every step in one workgroup
+MIXED = (256, 0, 2)         # steps on levels of at most 256 items in one workgroup, the rest alone
 COOPERATIVE = (0, ALL, 2)
 
 
 @pytest.mark.parametrize("name,launch", [("d13", WORKGROUP), ("d10w", WORKGROUP), ("d8", WORKGROUP),
+                                         ("d13", MIXED), ("d10w", (64, 0, 2)), ("d8", MIXED),
                                          ("d13", COOPERATIVE), ("d10w", (0, ALL, 1)), ("d8", (0, ALL, 4))])
 def test_persistent_plain_and_fused_match_oracle(name, launch):
     p, sc, g, dag, od, oc = make(name)
@@ -36,7 +38,7 @@ def test_persistent_plain_and_fused_match_oracle(name, launch):
     assert_same(dag, sc, od, oc, "persistent fused batch")
 
 
-@pytest.mark.parametrize("launch", [WORKGROUP, COOPERATIVE])
+@pytest.mark.parametrize("launch", [WORKGROUP, MIXED, COOPERATIVE])
 def test_persistent_copy_fill_noise_passes(launch):
     from hashdag_amd import terrain_height
     p, sc, g, dag, od, oc = make("d13")
@@ -59,7 +61,7 @@ def test_persistent_copy_fill_noise_passes(launch):
     assert_same(dag, sc, od, oc, "persistent copy / fill / noise passes")
 
 
-@pytest.mark.parametrize("launch", [WORKGROUP, COOPERATIVE])
+@pytest.mark.parametrize("launch", [WORKGROUP, MIXED, COOPERATIVE])
 def test_persistent_binary_depth12(launch):
     from hashdag_amd import Geometry, HashDAG, TerrainParams, generate_terrain
     from hashdag_amd.workload import sphere_edits
@@ -112,3 +114,19 @@ def test_persistent_overflow_and_undo(launch):
     dag.redo()
     assert np.array_equal(_frame(dag, p), after)
     dag.close()
+
+
+def test_edit_launch_autotune_keeps_results():
+    """The edit launch autotuner times its candidates on applied-and-undone sample edits, keeps
+    one of them, and leaves the DAG as it was; edits after it match the oracle."""
+    from tests.test_graph_gpu import canonical
+    p, sc, g, dag, od, oc = make("d13")
+    before = canonical(dag)
+    got = dag.autotune_edit_launch(random_edits(p, 6, seed=79), reps=1)
+    assert got["workgroup_max_items"] in (0, 64, 256, 1024, 4096)
+    assert dag.edit_launch()[0] == got["workgroup_max_items"]
+    assert all(v > 0 for v in got["ms_per_edit"].values())
+    assert np.array_equal(canonical(dag), before)
+    for kind, center, r in random_edits(p, 6, seed=81):
+        assert dag.edit(kind, center, r).num_voxels == od.edit(kind, center, r, oc)[1]
+    assert_same(dag, sc, od, oc, "edits after the launch autotuner")
