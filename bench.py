@@ -337,12 +337,10 @@ def edit_bench(scene, geom, params, W, H, args):
 
     def plain_by_radius(d, sample):
         """Median milliseconds of one plain pass by editor radius, a launch per step and level
-        against the steps of levels with at most 256 / 2048 work items in one workgroup and against
-        one single-workgroup launch, on a sample of the edit list (each edit undone)."""
+        against one single-workgroup launch, on a sample of the edit list (each edit undone)."""
         classes = ((2, 8), (8, 32), (32, 128), (128, 257))
         out = {}
-        for name, launch in (("level_synchronous", (0, 0)), ("small_levels_in_a_workgroup_256", (256, 0)),
-                             ("small_levels_in_a_workgroup_2048", (2048, 0)), ("single_workgroup", (1 << 30, 0))):
+        for name, launch in (("level_synchronous", (0, 0)), ("single_workgroup", (1 << 30, 0))):
             d.set_edit_launch(*launch)
             for e in sample[:8]:
                 d.edit(*e)
@@ -438,12 +436,12 @@ def edit_bench(scene, geom, params, W, H, args):
     gc_p, _ = gc_timed(plain)
     plain.close()
     log(f"plain: {st_pl.num_voxels} voxels in {t_pl * 1e3:.1f} ms ({st_pl.passes} passes, a launch per step and "
-        f"level), {t_pt * 1e3:.1f} ms (autotuned small levels in a workgroup), {t_pp * 1e3:.1f} ms (one "
+        f"level), {t_pt * 1e3:.1f} ms (autotuned: small passes in one workgroup), {t_pp * 1e3:.1f} ms (one "
         f"single-workgroup launch per edit), {t_pg * 1e3:.1f} ms (a captured single-edit graph replayed per edit, "
         f"{replays} replays), then GC {gc_p * 1e3:.1f} ms")
     plain_forms = {"a launch per step and level": t_pl, "one single-workgroup launch per edit": t_pp,
                    "a captured single-edit pass replayed per edit": t_pg,
-                   "steps of small levels in one workgroup (autotuned), the rest a launch each": t_pt}
+                   "small passes in one workgroup (autotuned), the rest a launch per step": t_pt}
     plain_form = min(plain_forms, key=plain_forms.get)
     t_p = plain_forms[plain_form]
     st_p = st_pl
@@ -468,7 +466,7 @@ def edit_bench(scene, geom, params, W, H, args):
         "plain_ms_by_form": {"level_synchronous": round(t_pl * 1e3, 2),
                              "single_workgroup": round(t_pp * 1e3, 2) if t_pp < float("inf") else None,
                              "graph_replay": round(t_pg * 1e3, 2) if t_pg < float("inf") else None,
-                             "small_levels_in_a_workgroup": round(t_pt * 1e3, 2) if t_pt < float("inf") else None},
+                             "small_passes_in_a_workgroup": round(t_pt * 1e3, 2) if t_pt < float("inf") else None},
         "edit_launch_autotuner": tuned,
         "plain_pass_ms_by_radius": by_radius,
         "batched_ms_by_form": {"level_synchronous": round(t_f * 1e3, 2),

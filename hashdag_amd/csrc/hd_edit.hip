@@ -2494,12 +2494,15 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
     // one cooperative launch for passes of at most edit_persist_max edits (not under capture)
     if (!dN && n <= ctx->edit_persist_max)
         return special ? launch_persistent<true>(ctx, P, sl, s) : launch_persistent<false>(ctx, P, sl, s);
-    // otherwise every run of consecutive steps on levels with at most edit_block_max items (by
-    // the editors' boxes; all of a small pass) is one single-workgroup launch, and every other
-    // step is a launch of its own over a grid sized for its level
-    const auto small = [&](uint32_t code) {
-        return (code >> 8) == ST_FINISH || bound[code & 0xFFu] <= (uint64_t)ctx->edit_block_max;
-    };
+    // otherwise a pass with at most edit_block_max work items on every level (by the editors'
+    // boxes) is one single-workgroup launch, and any other pass a launch per step over grids
+    // sized for each level. (Running only a bigger pass's small levels in a workgroup was
+    // slower: a workgroup's four wavefronts scan the buckets of a level's finds one after
+    // another, where a launch gives each its own wavefront.)
+    uint64_t largest = 0;
+    for (uint32_t l = 0; l <= L && l < kMaxEditLevels; ++l) largest = std::max(largest, bound[l]);
+    const bool smallPass = largest <= (uint64_t)ctx->edit_block_max;
+    const auto small = [&](uint32_t) { return smallPass; };
     // a wavefront per task slot (the task count is on the device), at most 2^18 of them
     const uint32_t encBlocks =
         (uint32_t)std::min<uint64_t>(std::max<uint64_t>((bound[P.ctl] + kEncWaves - 1) / kEncWaves, 1u), 1u << 16);

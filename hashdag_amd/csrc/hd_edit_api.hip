@@ -568,7 +568,7 @@ hd_status hd_set_edit_launch(hd_ctx* ctx, uint32_t workgroup_max_items, uint32_t
 
 hd_status hd_autotune_edit_launch(hd_ctx* ctx, const hd_edit* sample, uint32_t n, uint32_t reps,
                                   uint32_t* workgroup_max_items, float ms_per_edit[5]) {
-    static const uint32_t kCandidates[5] = {0u, 64u, 256u, 1024u, 4096u};
+    static const uint32_t kCandidates[5] = {0u, 32u, 128u, 512u, 2048u};
     if (!ctx || !sample || !n) return fail(ctx, HD_ERR_INVALID, "bad arguments");
     for (uint32_t i = 0; i < n; ++i)
         if (sample[i].kind > EDIT_SPHERE_PAINT) return fail(ctx, HD_ERR_INVALID, "sample editors must be sphere / box / paint");

@@ -32,29 +32,34 @@ __device__ __forceinline__ uint32_t child_off(uint32_t mask, uint32_t child) {
     return __popc(mask & ((1u << child) - 1u)) + 1u;
 }
 
-// base_dag.h:16-58
+// base_dag.h:16-58: bit i of the mask = byte i of the leaf is non-zero (the bytes' bits are
+// folded onto their bit 0, then the four bit-0s of each word gathered)
+__device__ __forceinline__ uint32_t byte_nonzero4(uint32_t w) {
+    w |= w >> 4;
+    w |= w >> 2;
+    w |= w >> 1;
+    w &= 0x01010101u;
+    return (w | (w >> 7) | (w >> 14) | (w >> 21)) & 0xFu;
+}
 __device__ __forceinline__ uint32_t leaf_first_mask(uint64_t leaf) {
-    const uint32_t lo = (uint32_t)leaf, hi = (uint32_t)(leaf >> 32);
-    uint32_t m = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        m |= ((lo >> (8 * i)) & 0xFFu) ? (1u << i) : 0u;
-        m |= ((hi >> (8 * i)) & 0xFFu) ? (1u << (4 + i)) : 0u;
-    }
-    return m;
+    return byte_nonzero4((uint32_t)leaf) | (byte_nonzero4((uint32_t)(leaf >> 32)) << 4);
 }
 __device__ __forceinline__ uint32_t leaf_second_mask(uint64_t leaf, uint32_t c) {
     const uint32_t w = (c & 4) ? (uint32_t)(leaf >> 32) : (uint32_t)leaf;
     return (w >> ((c & 3) * 8)) & 0xFFu;
 }
 
-// tracer.cu:7-17: first child c ^ order with its bit set, via a bit permutation
-__device__ __forceinline__ uint32_t next_child(uint32_t order, uint32_t mask) {
+// tracer.cu:7-17: children are visited as c = k ^ order for k = 0..7. Bit k of the permuted
+// mask is bit k ^ order of the mask, so its lowest set bit is the first child in that order.
+__device__ __forceinline__ uint32_t permute_order(uint32_t order, uint32_t mask) {
     uint32_t m = mask;
     if (order & 4) m = ((m & 0x0Fu) << 4) | ((m & 0xF0u) >> 4);
     if (order & 2) m = ((m & 0x33u) << 2) | ((m & 0xCCu) >> 2);
     if (order & 1) m = ((m & 0x55u) << 1) | ((m & 0xAAu) >> 1);
-    return (uint32_t)__builtin_ctz(m) ^ order;
+    return m;
+}
+__device__ __forceinline__ uint32_t next_child(uint32_t order, uint32_t mask) {
+    return (uint32_t)__builtin_ctz(permute_order(order, mask)) ^ order;
 }
 
 __device__ __forceinline__ float tmax3(float x, float y, float z) {
@@ -1061,10 +1066,10 @@ __device__ uint32_t beam_search(const DevDag& dag, F3 o, F3 d, F3 inv, uint32_t 
         if (valid && level == levels) {
             put(index, level, px, py, pz);
         } else if (valid) {
-            uint32_t v = visit;
+            uint32_t v = permute_order(order, visit);
             while (v) {
-                const uint32_t c = next_child(order, v);
-                v &= ~(1u << c);
+                const uint32_t c = (uint32_t)__builtin_ctz(v) ^ order;
+                v &= v - 1u;
                 const uint32_t ci = level < L ? pick(nd, child_off(cm, c)) : index;
                 put(ci, level + 1, (px << 1) | ((c >> 2) & 1u), (py << 1) | ((c >> 1) & 1u), (pz << 1) | (c & 1u));
             }

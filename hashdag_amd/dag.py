@@ -241,13 +241,13 @@ class HashDAG:
     def set_edit_launch(self, workgroup_max_items: int = 0, persist_max_edits: int = 0,
                         blocks_per_cu: int = 2) -> None:
         """Passes of at most persist_max_edits editors run as one cooperative launch; otherwise
-        the steps on levels with at most workgroup_max_items work items run together in one
-        single-workgroup launch and every other step launches on its own."""
+        a pass with at most workgroup_max_items work items on every level runs in one
+        single-workgroup launch, and any other pass launches each step on its own."""
         check(lib.hd_set_edit_launch(self._h, workgroup_max_items, persist_max_edits, blocks_per_cu), self._h)
 
     def autotune_edit_launch(self, sample, reps: int = 1) -> dict:
-        """Times one pass per sample editor with each small-level threshold (0, 64, 256, 1024,
-        4096 work items) and keeps the fastest (hd_autotune_edit_launch)."""
+        """Times one pass per sample editor with each small-pass threshold (0, 32, 128, 512, 2048
+        work items per level) and keeps the fastest (hd_autotune_edit_launch)."""
         arr = (hd_edit * max(len(sample), 1))()
         for i, ed in enumerate(sample):
             fill_edit(arr[i], ed)
@@ -255,7 +255,7 @@ class HashDAG:
         ms = (C.c_float * 5)()
         check(lib.hd_autotune_edit_launch(self._h, arr, len(sample), reps, C.byref(pick), ms), self._h)
         return {"workgroup_max_items": pick.value,
-                "ms_per_edit": {str(c): round(float(v), 4) for c, v in zip((0, 64, 256, 1024, 4096), ms)}}
+                "ms_per_edit": {str(c): round(float(v), 4) for c, v in zip((0, 32, 128, 512, 2048), ms)}}
 
     def edit_launch(self) -> tuple[int, int, int]:
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()

@@ -242,11 +242,11 @@ hd_status hd_edit_batch_static(hd_ctx* ctx, const hd_edit* edits, uint32_t n, hd
  *   - one cooperative launch of blocks_per_cu (1..8, capped at what fits) resident blocks per
  *     CU, a grid-wide barrier between steps, when the pass holds at most persist_max_edits
  *     editors (not inside a frame graph);
- *   - else: every run of consecutive steps on levels where the pass has at most
- *     workgroup_max_items work items (bounded on the host by the nodes of that level the
- *     editors' boxes overlap; inside a frame graph by the scratch capacity) as one launch of a
- *     single workgroup with a block barrier between steps, and every other step as a launch of
- *     its own over a grid sized by that bound.
+ *   - else, when the pass has at most workgroup_max_items work items on every level (bounded on
+ *     the host by the nodes of each level the editors' boxes overlap; inside a frame graph by
+ *     the scratch capacity), as one launch of a single workgroup with a block barrier between
+ *     steps;
+ *   - else as a launch per step, over a grid sized by that bound for the step's level.
  * workgroup_max_items = 0 and persist_max_edits = 0 (the defaults after hd_create, unless
  * hd_autotune_edit_launch picked others): a launch per step. Results are the same in every
  * form. */
@@ -256,7 +256,7 @@ hd_status hd_get_edit_launch(hd_ctx* ctx, uint32_t* workgroup_max_items, uint32_
                              uint32_t* blocks_per_cu);
 /* Edit launch autotuner, no counterpart in the reference: times one pass per sample editor
  * (sphere / box / paint; each applied, timed, undone; reps rounds after one warm-up round) with
- * workgroup_max_items = 0, 64, 256, 1024 and 4096, keeps the fastest in the context (out:
+ * workgroup_max_items = 0, 32, 128, 512 and 2048, keeps the fastest in the context (out:
  * the pick and the mean milliseconds per edit of each candidate). The trial passes leave their
  * nodes for the next GC and clear the redo history. */
 hd_status hd_autotune_edit_launch(hd_ctx* ctx, const hd_edit* sample, uint32_t n, uint32_t reps,
