@@ -47,7 +47,8 @@ def main() -> int:
     ap.add_argument("--schedule", type=str, default="auto",
                     help="tracer schedule: tile (one wavefront per tile), persist[:N] (persistent wavefronts "
                          "refilled at N idle lanes), beam[:N] (per tile, walks beyond N steps searched a wavefront "
-                         "per ray), or auto (the autotuner's pick with --tile auto, else tile)")
+                         "per ray), resume[:A[:B]] (per tile to A steps, resumed by compacted wavefronts to B, the "
+                         "rest searched per ray), or auto (the autotuner's pick with --tile auto, else tile)")
     ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
     ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
     ap.add_argument("--no-edits", action="store_true")
@@ -112,6 +113,11 @@ def main() -> int:
     elif args.schedule.startswith("beam"):
         cap = int(args.schedule.split(":")[1]) if ":" in args.schedule else 128
         tracer.set_schedule(dag, 2, cap)
+    elif args.schedule.startswith("resume"):
+        parts = args.schedule.split(":")
+        c1 = int(parts[1]) if len(parts) > 1 else 32
+        c2 = int(parts[2]) if len(parts) > 2 else 128
+        tracer.set_schedule(dag, 3, c1, resume_to=c2)
     elif args.schedule == "tile":
         tracer.set_schedule(dag, 0)
     from hashdag_amd.tracer import launch_config

@@ -26,6 +26,9 @@ struct TraceScratch {
     uint32_t* long_items = nullptr;
     uint32_t* rest_items = nullptr;
     uint32_t* long_count = nullptr;
+    // walks stopped at the per-tile cap and saved for the resume kernel (schedule 3)
+    uint32_t* resume_items = nullptr;
+    uint32_t* resume_state = nullptr;
     uint64_t pixels_cap = 0;
 };
 
@@ -92,6 +95,8 @@ struct hd_ctx {
     // per-tile schedule: walks longer than long_steps steps (0: no cap) are finished by the
     // wavefront-per-ray search
     uint32_t long_steps = 0;
+    // schedule 3: the per-tile walk stops at resume_steps and compacted wavefronts continue it
+    uint32_t resume_steps = 0;
     uint32_t persist_blocks[4] = {};   // resident blocks per persistent kernel (0: not measured)
     // scratch of the persistent schedule, one set per stream traced on (traces on different
     // streams of one context may overlap)

@@ -294,8 +294,7 @@ __device__ __forceinline__ bool descend(const DevDag& dag, F3 o, F3 d, F3 inv, u
 // was stopped at the cap (the ray is then traced by the wavefront-per-ray search).
 template <int kFetch>
 __device__ __forceinline__ uint32_t descend_capped(const DevDag& dag, F3 o, F3 d, F3 inv, uint32_t order, uint32_t& px,
-                                                   uint32_t& py, uint32_t& pz, Stack st, uint32_t cap) {
-    Walk w;
+                                                   uint32_t& py, uint32_t& pz, Stack st, uint32_t cap, Walk& w) {
     uint64_t unused = 0;
     walk_begin<false, kFetch>(dag, w, o, d, inv, unused);
     uint32_t r, steps = 0;
@@ -313,15 +312,68 @@ struct LongRays {
     uint32_t* items;  // pixel indices (x + y * W)
     uint32_t* count;
     uint32_t cap;     // walk steps before a ray is handed over; ~0u: never
+    uint32_t* state;  // with a state: the walk of entry i is saved at state[word * stride + i]
+    uint32_t stride;
 };
-__device__ __forceinline__ void long_append(bool give, uint32_t pix, const LongRays& L) {
+// slot of this lane's entry (lanes with give = false get ~0u); every lane of the wavefront calls it
+__device__ __forceinline__ uint32_t long_slot(bool give, const LongRays& L) {
     const uint64_t m = __ballot(give);
-    if (!m) return;
+    if (!m) return ~0u;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t base = 0;
     if (lane == (uint32_t)__ffsll((long long)m) - 1u) base = atomicAdd(L.count, (uint32_t)__popcll(m));
     base = __shfl(base, __ffsll((long long)m) - 1, 64);
-    if (give) L.items[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = pix;
+    return give ? base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : ~0u;
+}
+__device__ __forceinline__ void long_append(bool give, uint32_t pix, const LongRays& L) {
+    const uint32_t slot = long_slot(give, L);
+    if (give) L.items[slot] = pix;
+}
+
+// A walk stopped at the cap, saved so another launch continues it (pointer fetch): the walk's
+// registers, then the LDS stack's node indices (levels 1 .. leafLevel - 1) and child masks
+// (two levels per word). Only levels below the walk's level with children left are read back.
+constexpr uint32_t kWalkWords = 10;
+constexpr uint32_t kStateWords = kWalkWords + (kMaxLevels - 3) + (kMaxLevels + 1) / 2;
+__device__ __forceinline__ void save_walk(const Walk& w, Stack st, uint32_t levels, uint32_t leafLevel,
+                                          uint32_t* __restrict__ state, uint32_t stride, uint32_t slot) {
+    uint32_t* q = state + slot;
+    q[0 * (uint64_t)stride] = w.level;
+    q[1 * (uint64_t)stride] = w.px;
+    q[2 * (uint64_t)stride] = w.py;
+    q[3 * (uint64_t)stride] = w.pz;
+    q[4 * (uint64_t)stride] = w.index;
+    q[5 * (uint64_t)stride] = w.childMask;
+    q[6 * (uint64_t)stride] = w.visit;
+    q[7 * (uint64_t)stride] = w.alive;
+    q[8 * (uint64_t)stride] = (uint32_t)w.leaf;
+    q[9 * (uint64_t)stride] = (uint32_t)(w.leaf >> 32);
+    for (uint32_t l = 1; l < leafLevel && l < w.level; ++l) q[(uint64_t)(kWalkWords + l - 1) * stride] = st.at_idx(l);
+    const uint32_t m0 = kWalkWords + (kMaxLevels - 3);
+    for (uint32_t l = 0; l < levels && l < w.level; l += 2) {
+        const uint32_t hi = l + 1 < levels ? st.at_mask(l + 1) : 0u;
+        q[(uint64_t)(m0 + l / 2) * stride] = st.at_mask(l) | (hi << 16);
+    }
+}
+__device__ __forceinline__ void restore_walk(Walk& w, Stack st, uint32_t levels, uint32_t leafLevel,
+                                             const uint32_t* __restrict__ state, uint32_t stride, uint32_t slot) {
+    const uint32_t* q = state + slot;
+    w.level = q[0 * (uint64_t)stride];
+    w.px = q[1 * (uint64_t)stride];
+    w.py = q[2 * (uint64_t)stride];
+    w.pz = q[3 * (uint64_t)stride];
+    w.index = q[4 * (uint64_t)stride];
+    w.childMask = q[5 * (uint64_t)stride];
+    w.visit = q[6 * (uint64_t)stride];
+    w.alive = q[7 * (uint64_t)stride];
+    w.leaf = (uint64_t)q[8 * (uint64_t)stride] | ((uint64_t)q[9 * (uint64_t)stride] << 32);
+    for (uint32_t l = 1; l < leafLevel && l < w.level; ++l) st.at_idx(l) = q[(uint64_t)(kWalkWords + l - 1) * stride];
+    const uint32_t m0 = kWalkWords + (kMaxLevels - 3);
+    for (uint32_t l = 0; l < levels && l < w.level; l += 2) {
+        const uint32_t v = q[(uint64_t)(m0 + l / 2) * stride];
+        st.at_mask(l) = (uint16_t)v;
+        if (l + 1 < levels) st.at_mask(l + 1) = (uint16_t)(v >> 16);
+    }
 }
 
 __device__ __forceinline__ void dnormalize(double& x, double& y, double& z) {
@@ -374,6 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, cons
     pixel_of(tileW, tileH, x, y);
     uint64_t bytes = 0;
     bool give = false;
+    Walk w;
     if (x < W && y < H) {
         F3 o, d, inv;
         uint32_t order;
@@ -382,12 +435,18 @@ __global__ __launch_bounds__(kBlock) void k_trace_paths(DevDag dag, RayP p, cons
         if (kCount || L.cap == ~0u) {
             descend<kCount, kFetch>(dag, o, d, inv, order, vx, vy, vz, st, bytes);
         } else {
-            give = descend_capped<kFetch>(dag, o, d, inv, order, vx, vy, vz, st, L.cap) == kWalkOn;
+            give = descend_capped<kFetch>(dag, o, d, inv, order, vx, vy, vz, st, L.cap, w) == kWalkOn;
         }
         if (!give) out[(uint64_t)(H - 1 - y) * W + x] = make_uint4(vx, vy, vz, 0u);
         if (kCount) bytes += 16;
     }
-    if (!kCount && L.cap != ~0u) long_append(give, y * W + x, L);
+    if (!kCount && L.cap != ~0u) {
+        const uint32_t slot = long_slot(give, L);
+        if (give) {
+            L.items[slot] = y * W + x;
+            if (L.state) save_walk(w, st, dag.levels, dag.leaf_level, L.state, L.stride, slot);
+        }
+    }
     flush_count<kCount>(bytes, counter);
 }
 
@@ -683,6 +742,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, co
     pixel_of(tileW, tileH, x, y);
     uint64_t bytes = 0;
     bool give = false;
+    Walk w;
     if (x < W && y < H) {
         const uint64_t pix = (uint64_t)y * W + x;
         const uint4 pp = paths[pix];
@@ -712,7 +772,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, co
             if (kCount || L.cap == ~0u) {
                 shadowed = descend<kCount, kFetch>(dag, o, sun, inv, order, vx, vy, vz, st, bytes);
             } else {
-                const uint32_t r = descend_capped<kFetch>(dag, o, sun, inv, order, vx, vy, vz, st, L.cap);
+                const uint32_t r = descend_capped<kFetch>(dag, o, sun, inv, order, vx, vy, vz, st, L.cap, w);
                 give = r == kWalkOn;
                 shadowed = r == kWalkHit;
             }
@@ -726,7 +786,13 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadows(DevDag dag, RayP p, co
             }
         }
     }
-    if (!kCount && L.cap != ~0u) long_append(give, y * W + x, L);
+    if (!kCount && L.cap != ~0u) {
+        const uint32_t slot = long_slot(give, L);
+        if (give) {
+            L.items[slot] = y * W + x;
+            if (L.state) save_walk(w, st, dag.levels, dag.leaf_level, L.state, L.stride, slot);
+        }
+    }
     flush_count<kCount>(bytes, counter);
 }
 
@@ -1195,11 +1261,70 @@ __global__ __launch_bounds__(kBlock) void k_trace_rest(DevDag dag, RayP p, const
     }
 }
 
-__global__ void k_long_reset(uint32_t* a, uint32_t* b) {
-    if (threadIdx.x == 0) {
-        *a = 0;
-        *b = 0;
+// Walks stopped at the per-tile cap, continued by wavefronts over the compacted list (64
+// consecutive entries per wavefront, so each wavefront holds only long walks): each lane
+// restores its walk and steps on for at most `steps` steps; walks still unfinished are handed
+// to the wavefront-per-ray search (list B). Pointer fetch only.
+template <bool kShadow>
+__global__ __launch_bounds__(kBlock) void k_trace_resume(DevDag dag, RayP p, const RayP* __restrict__ pmem, uint32_t W,
+                                                         uint32_t H, uint4* __restrict__ paths,
+                                                         uint32_t* __restrict__ colors, float bias, float fog,
+                                                         LongRays R, LongRays B, uint32_t steps) {
+    const Stack st = stack_of(dag.levels);
+    if (pmem) p = *pmem;
+    if (dag.root_ptr) dag.root = *dag.root_ptr;
+    const uint32_t n = *R.count;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, waves = (gridDim.x * blockDim.x) >> 6;
+    uint64_t unused = 0;
+    for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
+        const uint32_t i = base + lane;
+        bool give = false;
+        uint32_t pix = 0;
+        if (i < n) {
+            pix = R.items[i];
+            const uint32_t x = pix % W, y = pix / W;
+            F3 o, d, inv;
+            uint32_t order;
+            uint4 pp{};
+            if (kShadow) {
+                sun_direction(d.x, d.y, d.z);
+                pp = paths[pix];
+                double cx, cy, cz;
+                camera_dir(p, x, H - 1 - y, cx, cy, cz);
+                o = shadow_origin(p, pp, cx, cy, cz, bias, d);
+                inv = F3{(float)(1.0 / (double)d.x), (float)(1.0 / (double)d.y), (float)(1.0 / (double)d.z)};
+                order = (d.x < 0.f ? 4u : 0u) + (d.y < 0.f ? 2u : 0u) + (d.z < 0.f ? 1u : 0u);
+            } else {
+                primary_ray(p, x, y, o, d, inv, order);
+            }
+            Walk w;
+            restore_walk(w, st, dag.levels, dag.leaf_level, R.state, R.stride, i);
+            uint32_t r = kWalkOn, k = 0;
+            while (r == kWalkOn && k++ < steps) r = walk_step<false, kFetchPointer>(dag, w, o, d, inv, order, st, unused);
+            give = r == kWalkOn;
+            if (!give) {
+                if (kShadow) {
+                    const uint32_t colorIn = colors[pix];
+                    const bool shadowed = r == kWalkHit;
+                    if (fog == 0.f) {
+                        colors[pix] = shade_clear(colorIn, shadowed ? 0.f : 1.f);
+                    } else {
+                        double distance, nx, ny, nz;
+                        hit_distance(p, pp, distance, nx, ny, nz);
+                        colors[pix] = shade(colorIn, shadowed ? 0.f : 1.f, distance, nx, ny, nz, fog, d);
+                    }
+                } else {
+                    paths[(uint64_t)(H - 1 - y) * W + x] = make_uint4(w.px, w.py, w.pz, 0u);
+                }
+            }
+        }
+        long_append(give, pix, B);
     }
+}
+
+__global__ void k_resume_reset(uint32_t* c) {
+    if (threadIdx.x < 3) c[threadIdx.x] = 0;
 }
 
 inline RayP to_rayp(const hd_ray_params* p) {
@@ -1279,17 +1404,35 @@ hd_status launch_long(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP*
     return HD_OK;
 }
 
-// The per-tile walk's hand-over lists (cap ~0u when the context runs walks to the end).
-hd_status long_lists(hd_ctx* ctx, uint32_t W, uint32_t H, hipStream_t s, LongRays& L, LongRays& rest) {
-    L = LongRays{nullptr, nullptr, ~0u};
+// The per-tile walk's hand-over lists (cap ~0u when the context runs walks to the end): with a
+// resume step (schedule 3, pointer fetch) the per-tile walk stops at resume_steps and saves its
+// walk in R; the resume kernel continues to long_steps and hands what is left to L; the
+// wavefront-per-ray search takes L and leaves what it could not hold in `rest`.
+hd_status long_lists(hd_ctx* ctx, uint32_t W, uint32_t H, hipStream_t s, LongRays& L, LongRays& rest, LongRays& R) {
+    L = LongRays{nullptr, nullptr, ~0u, nullptr, 0};
     rest = L;
+    R = L;
     if (!ctx->long_steps) return HD_OK;
     TraceScratch* ts = nullptr;
     const hd_status rs = trace_reserve(ctx, s, W, H, &ts);
     if (rs != HD_OK) return rs;
-    L = LongRays{ts->long_items, ts->long_count, ctx->long_steps};
-    rest = LongRays{ts->rest_items, ts->long_count + 1, ~0u};
-    k_long_reset<<<1, 64, 0, s>>>(L.count, rest.count);
+    L = LongRays{ts->long_items, ts->long_count, ctx->long_steps, nullptr, 0};
+    rest = LongRays{ts->rest_items, ts->long_count + 1, ~0u, nullptr, 0};
+    if (ctx->resume_steps && ctx->fetch_mode == kFetchPointer && ts->resume_state)
+        R = LongRays{ts->resume_items, ts->long_count + 2, ctx->resume_steps, ts->resume_state,
+                     (uint32_t)ts->pixels_cap};
+    k_resume_reset<<<1, 64, 0, s>>>(ts->long_count);
+    HD_HIP(ctx, hipGetLastError());
+    return HD_OK;
+}
+
+// Continues the walks saved in R (schedule 3) before the hand-over to the beam search.
+template <bool kShadow>
+hd_status launch_resume(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP* pm, uint32_t W, uint32_t H,
+                        uint4* paths, uint32_t* colors, float bias, float fog, const LongRays& R, const LongRays& L,
+                        hipStream_t s) {
+    k_trace_resume<kShadow><<<4096, kBlock, stack_bytes(ctx->g.levels), s>>>(dag, p, pm, W, H, paths, colors, bias, fog,
+                                                                            R, L, ctx->long_steps - ctx->resume_steps);
     HD_HIP(ctx, hipGetLastError());
     return HD_OK;
 }
@@ -1297,13 +1440,17 @@ hd_status long_lists(hd_ctx* ctx, uint32_t W, uint32_t H, hipStream_t s, LongRay
 hd_status launch_paths(hd_ctx* ctx, const DevDag& dag, const RayP& p, const RayP* pm, uint32_t W, uint32_t H,
                        uint4* out, hipStream_t s) {
     if (ctx->trace_persist) return launch_persist<false>(ctx, dag, p, pm, W, H, out, nullptr, 0.f, 0.f, s);
-    LongRays L, rest;
-    hd_status st = long_lists(ctx, W, H, s, L, rest);
+    LongRays L, rest, R;
+    hd_status st = long_lists(ctx, W, H, s, L, rest, R);
     if (st != HD_OK) return st;
     const auto k = ctx->fetch_mode == kFetchNode ? k_trace_paths<false, kFetchNode> : k_trace_paths<false, kFetchPointer>;
     k<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), s>>>(dag, p, pm, W, H, out, ctx->tile_w, ctx->tile_h,
-                                                                     nullptr, L);
+                                                                     nullptr, R.state ? R : L);
     HD_HIP(ctx, hipGetLastError());
+    if (R.state) {
+        st = launch_resume<false>(ctx, dag, p, pm, W, H, out, nullptr, 0.f, 0.f, R, L, s);
+        if (st != HD_OK) return st;
+    }
     if (L.cap != ~0u) return launch_long<false>(ctx, dag, p, pm, W, H, out, nullptr, 0.f, 0.f, L, rest, s);
     return HD_OK;
 }
@@ -1312,13 +1459,17 @@ hd_status launch_shadows(hd_ctx* ctx, const DevDag& dag, const RayP& p, const Ra
                          const uint4* paths, uint32_t* colors, float bias, float fog, hipStream_t s) {
     if (ctx->trace_persist)
         return launch_persist<true>(ctx, dag, p, pm, W, H, const_cast<uint4*>(paths), colors, bias, fog, s);
-    LongRays L, rest;
-    hd_status st = long_lists(ctx, W, H, s, L, rest);
+    LongRays L, rest, R;
+    hd_status st = long_lists(ctx, W, H, s, L, rest, R);
     if (st != HD_OK) return st;
     const auto k = ctx->fetch_mode == kFetchNode ? k_trace_shadows<false, kFetchNode> : k_trace_shadows<false, kFetchPointer>;
     k<<<grid_of(ctx, W, H), kBlock, stack_bytes(ctx->g.levels), s>>>(dag, p, pm, W, H, paths, colors, bias, fog,
-                                                                     ctx->tile_w, ctx->tile_h, nullptr, L);
+                                                                     ctx->tile_w, ctx->tile_h, nullptr, R.state ? R : L);
     HD_HIP(ctx, hipGetLastError());
+    if (R.state) {
+        st = launch_resume<true>(ctx, dag, p, pm, W, H, const_cast<uint4*>(paths), colors, bias, fog, R, L, s);
+        if (st != HD_OK) return st;
+    }
     if (L.cap != ~0u)
         return launch_long<true>(ctx, dag, p, pm, W, H, const_cast<uint4*>(paths), colors, bias, fog, L, rest, s);
     return HD_OK;
@@ -1337,7 +1488,8 @@ hd_status trace_reserve(hd_ctx* ctx, hipStream_t s, uint32_t W, uint32_t H, Trac
     TraceScratch* ts = nullptr;
     for (TraceScratch& t : ctx->trace_scratch)
         if (t.stream == s) ts = &t;
-    if (ts && ts->tiles_cap >= tiles && ts->pixels_cap >= pixels) {
+    const bool resume = ctx->resume_steps != 0;
+    if (ts && ts->tiles_cap >= tiles && ts->pixels_cap >= pixels && (!resume || ts->resume_state)) {
         if (out) *out = ts;
         return HD_OK;
     }
@@ -1358,10 +1510,12 @@ hd_status trace_reserve(hd_ctx* ctx, hipStream_t s, uint32_t W, uint32_t H, Trac
         hipFree(ts->cnt);
         hipFree(ts->long_items);
         hipFree(ts->rest_items);
+        hipFree(ts->resume_items);
+        hipFree(ts->resume_state);
         ts->rec = nullptr;
         ts->shades = nullptr;
         ts->cnt = nullptr;
-        ts->long_items = ts->rest_items = nullptr;
+        ts->long_items = ts->rest_items = ts->resume_items = ts->resume_state = nullptr;
         ts->tiles_cap = ts->pixels_cap = 0;
     }
     HD_HIP(ctx, hipMalloc(&ts->rec, tiles * 64 * sizeof(uint4)));
@@ -1369,6 +1523,11 @@ hd_status trace_reserve(hd_ctx* ctx, hipStream_t s, uint32_t W, uint32_t H, Trac
     HD_HIP(ctx, hipMalloc(&ts->cnt, tiles * sizeof(uint32_t)));
     HD_HIP(ctx, hipMalloc(&ts->long_items, pixels * sizeof(uint32_t)));
     HD_HIP(ctx, hipMalloc(&ts->rest_items, pixels * sizeof(uint32_t)));
+    if (resume) {
+        if (pixels * kStateWords >= (1ull << 32)) return fail(ctx, HD_ERR_INVALID, "frame too large for resumed walks");
+        HD_HIP(ctx, hipMalloc(&ts->resume_items, pixels * sizeof(uint32_t)));
+        HD_HIP(ctx, hipMalloc(&ts->resume_state, pixels * kStateWords * sizeof(uint32_t)));
+    }
     ts->tiles_cap = tiles;
     ts->pixels_cap = pixels;
     ctx->ptr_gen++;
@@ -1384,6 +1543,8 @@ void trace_scratch_free(hd_ctx* ctx) {
         hipFree(t.cnt);
         hipFree(t.long_items);
         hipFree(t.rest_items);
+        hipFree(t.resume_items);
+        hipFree(t.resume_state);
         hipFree(t.long_count);
     }
     ctx->trace_scratch.clear();
@@ -1464,21 +1625,26 @@ hd_status hd_set_trace_fetch(hd_ctx* ctx, uint32_t fetch_mode) {
 
 hd_status hd_set_trace_schedule(hd_ctx* ctx, uint32_t schedule, uint32_t param) {
     if (!ctx) return fail(ctx, HD_ERR_INVALID, "null context");
-    if (schedule > 2 || (schedule == 1 && (param == 0 || param > 64)) || (schedule == 2 && param == 0))
+    const uint32_t c1 = param & 0xFFFFu, c2 = param >> 16;
+    if (schedule > 3 || (schedule == 1 && (param == 0 || param > 64)) || (schedule == 2 && param == 0) ||
+        (schedule == 3 && (c1 == 0 || c2 <= c1)))
         return fail(ctx, HD_ERR_INVALID,
-                    "schedule is 0 (per tile), 1 (persistent, refill at 1..64 idle lanes) or 2 (per tile, walks "
-                    "beyond param steps searched a wavefront per ray)");
+                    "schedule is 0 (per tile), 1 (persistent, refill at 1..64 idle lanes), 2 (per tile, walks "
+                    "beyond param steps searched a wavefront per ray) or 3 (per tile to param & 0xFFFF steps, "
+                    "resumed by compacted wavefronts to param >> 16 steps, the rest searched per ray)");
     ctx->trace_persist = schedule == 1 ? 1u : 0u;
     if (schedule == 1) ctx->refill_at = param;
-    ctx->long_steps = schedule == 2 ? param : 0u;
+    ctx->long_steps = schedule == 2 ? param : (schedule == 3 ? c2 : 0u);
+    ctx->resume_steps = schedule == 3 ? c1 : 0u;
     ctx->ptr_gen++;
     return HD_OK;
 }
 
 hd_status hd_get_trace_schedule(hd_ctx* ctx, uint32_t out[2]) {
     if (!ctx || !out) return fail(ctx, HD_ERR_INVALID, "bad arguments");
-    out[0] = ctx->trace_persist ? 1u : (ctx->long_steps ? 2u : 0u);
-    out[1] = ctx->trace_persist ? ctx->refill_at : ctx->long_steps;
+    out[0] = ctx->trace_persist ? 1u : (ctx->resume_steps ? 3u : (ctx->long_steps ? 2u : 0u));
+    out[1] = ctx->trace_persist ? ctx->refill_at
+                                : (ctx->resume_steps ? (ctx->resume_steps | (ctx->long_steps << 16)) : ctx->long_steps);
     return HD_OK;
 }
 
@@ -1500,14 +1666,18 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
     static const uint32_t fetches[] = {kFetchPointer, kFetchNode};
     // schedule: one wavefront per tile; persistent wavefronts refilled at 8 / 24 idle lanes; one
     // wavefront per tile with walks beyond 96 / 128 / 192 steps searched a wavefront per ray
-    static const uint32_t schedules[][2] = {{0, 0}, {1, 8}, {1, 24}, {2, 96}, {2, 128}, {2, 192}};
-    constexpr uint32_t kTrials = 6 * 2 * 6;  // tile shape x fetch strategy x schedule
+    // (three trials with resumed walks: per tile to 24 / 32 / 48 steps, resumed to 128 / 128 / 160)
+    static const uint32_t schedules[][2] = {{0, 0},   {1, 8},   {1, 24},  {2, 96},           {2, 128},
+                                            {2, 192}, {3, 24 | (128u << 16)}, {3, 32 | (128u << 16)},
+                                            {3, 48 | (160u << 16)}};
+    constexpr uint32_t kSchedules = sizeof(schedules) / sizeof(schedules[0]);
+    constexpr uint32_t kTrials = 6 * 2 * kSchedules;  // tile shape x fetch strategy x schedule
     hipEvent_t e0, e1;
     HD_HIP(ctx, hipEventCreate(&e0));
     HD_HIP(ctx, hipEventCreate(&e1));
     float bestMs = 1e30f;
     uint32_t bw = ctx->tile_w, bh = ctx->tile_h, bf = ctx->fetch_mode, bp = ctx->trace_persist, br = ctx->refill_at,
-             bl = ctx->long_steps;
+             bl = ctx->long_steps, bs = ctx->resume_steps;
     reps = std::max<uint32_t>(reps, 1);
     for (uint32_t trial = 0; trial < kTrials; ++trial) {
         const auto& sh = shapes[trial % 6];
@@ -1515,9 +1685,11 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
         ctx->tile_h = sh[1];
         ctx->fetch_mode = fetches[(trial / 6) % 2];
         const uint32_t* sch = schedules[trial / 12];
+        if (sch[0] == 3 && ctx->fetch_mode != kFetchPointer) continue;  // resumed walks: pointer fetch only
         ctx->trace_persist = sch[0] == 1 ? 1u : 0u;
         if (sch[0] == 1) ctx->refill_at = sch[1];
-        ctx->long_steps = sch[0] == 2 ? sch[1] : 0u;
+        ctx->long_steps = sch[0] == 2 ? sch[1] : (sch[0] == 3 ? sch[1] >> 16 : 0u);
+        ctx->resume_steps = sch[0] == 3 ? (sch[1] & 0xFFFFu) : 0u;
         float total = 0;
         for (uint32_t rep = 0; rep <= reps; ++rep) {  // rep 0 warms up
             HD_HIP(ctx, hipEventRecord(e0, ctx->stream));
@@ -1546,6 +1718,7 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
             bp = ctx->trace_persist;
             br = ctx->refill_at;
             bl = ctx->long_steps;
+            bs = ctx->resume_steps;
         }
     }
     hipEventDestroy(e0);
@@ -1556,6 +1729,7 @@ hd_status hd_autotune_trace(hd_ctx* ctx, const hd_ray_params* params, uint32_t n
     ctx->trace_persist = bp;
     ctx->refill_at = br;
     ctx->long_steps = bl;
+    ctx->resume_steps = bs;
     ctx->ptr_gen++;
     if (best_w) *best_w = bw;
     if (best_h) *best_h = bh;
@@ -1572,7 +1746,7 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
     HD_HIP(ctx, hipMemsetAsync(ctx->byte_counters, 0, 3 * sizeof(unsigned long long), ctx->stream));
     k_trace_paths<true, kFetchPointer><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), nullptr, W, H,
                                                           reinterpret_cast<uint4*>(d_paths), ctx->tile_w, ctx->tile_h,
-                                                          ctx->byte_counters + 0, LongRays{nullptr, nullptr, ~0u});
+                                                          ctx->byte_counters + 0, LongRays{nullptr, nullptr, ~0u, nullptr, 0});
     HD_HIP(ctx, hipGetLastError());
     if (ctx->colors_loaded) {
         k_trace_colors<true><<<grid, kBlock, 0, ctx->stream>>>(dev_dag(ctx), dev_colors(ctx), W, H,
@@ -1583,7 +1757,7 @@ hd_status hd_trace_count_bytes(hd_ctx* ctx, const hd_ray_params* params, uint32_
     k_trace_shadows<true, kFetchPointer><<<grid, kBlock, stack_bytes(ctx->g.levels), ctx->stream>>>(dev_dag(ctx), to_rayp(params), nullptr, W, H,
                                                             reinterpret_cast<const uint4*>(d_paths), d_colors, bias, fog,
                                                             ctx->tile_w, ctx->tile_h, ctx->byte_counters + 2,
-                                                            LongRays{nullptr, nullptr, ~0u});
+                                                            LongRays{nullptr, nullptr, ~0u, nullptr, 0});
     HD_HIP(ctx, hipGetLastError());
     unsigned long long b[3];
     HD_HIP(ctx, hipMemcpyAsync(b, ctx->byte_counters, sizeof(b), hipMemcpyDeviceToHost, ctx->stream));

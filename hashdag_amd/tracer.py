@@ -102,10 +102,15 @@ class DAGTracer:
         check(lib.hd_set_trace_fetch(dag.handle, fetch), dag.handle)
 
 
-    def set_schedule(self, dag: HashDAG, schedule, param: int = 16) -> None:
+    def set_schedule(self, dag: HashDAG, schedule, param: int = 16, resume_to: int = 0) -> None:
         """0 (or False): one wavefront per pixel tile; 1 (or True): persistent wavefronts fed
         from tile queues, refilling idle lanes once `param` of them are idle; 2: one wavefront
-        per tile, walks beyond `param` steps searched a wavefront per ray."""
+        per tile, walks beyond `param` steps searched a wavefront per ray; 3: one wavefront per
+        tile to `param` steps, the unfinished walks continued by compacted wavefronts to
+        `resume_to` steps, the rest searched a wavefront per ray (or `param` already packed as
+        A | B << 16 when `resume_to` is 0)."""
+        if int(schedule) == 3 and resume_to:
+            param = (param & 0xFFFF) | (resume_to << 16)
         check(lib.hd_set_trace_schedule(dag.handle, int(schedule), param), dag.handle)
 
 
@@ -114,9 +119,12 @@ def launch_config(dag: HashDAG) -> dict:
     check(lib.hd_get_trace_launch(dag.handle, out), dag.handle)
     sch = np.zeros(2, np.uint32)
     check(lib.hd_get_trace_schedule(dag.handle, ptr(sch)), dag.handle)
+    c1, c2 = int(sch[1]) & 0xFFFF, int(sch[1]) >> 16
     schedule = {0: "per tile", 1: f"persistent (refill at {int(sch[1])})",
-                2: f"per tile, walks beyond {int(sch[1])} steps a wavefront per ray"}[int(sch[0])]
-    key = {0: "tile", 1: f"persist:{int(sch[1])}", 2: f"beam:{int(sch[1])}"}[int(sch[0])]
+                2: f"per tile, walks beyond {int(sch[1])} steps a wavefront per ray",
+                3: f"per tile to {c1} steps, resumed by compacted wavefronts to {c2}, the rest a wavefront per "
+                   f"ray"}[int(sch[0])]
+    key = {0: "tile", 1: f"persist:{int(sch[1])}", 2: f"beam:{int(sch[1])}", 3: f"resume:{c1}:{c2}"}[int(sch[0])]
     return {"tile": f"{out[0]}x{out[1]}", "fetch": ["pointer", "node"][out[2]], "schedule": schedule,
             "schedule_key": key}
 

@@ -153,7 +153,10 @@ hd_status hd_get_trace_launch(hd_ctx* ctx, uint32_t out[3]);
  *   1  persistent wavefronts taking tiles from per-region queues, handing new pixels to their
  *      idle lanes once `param` (1..64) of them are idle;
  *   2  one wavefront per pixel tile, walks stopped after `param` steps; those rays are then
- *      searched a wavefront per ray (64 lanes expanding the ray's next nodes in DFS order). */
+ *      searched a wavefront per ray (64 lanes expanding the ray's next nodes in DFS order);
+ *   3  as 2 with a resume step (node fetch 0 only; with fetch 1 it runs as 2): walks stopped
+ *      after param & 0xFFFF steps save their state, wavefronts of 64 such walks continue them
+ *      up to param >> 16 steps in all, and the rest are searched a wavefront per ray. */
 hd_status hd_set_trace_schedule(hd_ctx* ctx, uint32_t schedule, uint32_t param);
 /* out: schedule, param */
 hd_status hd_get_trace_schedule(hd_ctx* ctx, uint32_t out[2]);

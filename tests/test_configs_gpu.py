@@ -35,7 +35,7 @@ def _frames_match(dag, od, oc, cams, levels, W, H, what):
         assert (opaths[..., :3] != 0).any(-1).mean() > 0.05, f"{what} camera {k} sees almost nothing"
         ocols = Oracle.trace_colors(od, oc, opaths, W, H, threads=16)
         oshad = Oracle.trace_shadows(od, oparams, opaths, ocols, W, H, 1.0, 0.0, threads=16)
-        for schedule in ((0, 0), (1, 16), (2, 128), (2, 24)):
+        for schedule in ((0, 0), (1, 16), (2, 128), (2, 24), (3, 32 | (128 << 16))):
             print(f"[{what}] camera {k} {W}x{H} schedule={schedule}", flush=True)
             tracer.set_schedule(dag, *schedule)
             tracer.resolve_paths(cam, info, dag, params=prm)

@@ -18,7 +18,7 @@ def canonical(dag):
 
 @pytest.mark.parametrize("tiny_scratch,schedule,workgroup", [(False, (0, 0), False), (True, (0, 0), False),
                                                         (False, (2, 4), False), (False, (1, 16), False),
-                                                        (True, (0, 0), True)])
+                                                        (True, (0, 0), True), (False, (3, 4 | (20 << 16)), False)])
 def test_graph_replays_match_eager(tiny_scratch, schedule, workgroup):
     import torch
 

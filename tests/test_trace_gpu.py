@@ -98,7 +98,7 @@ def test_byte_counters_match_oracle():
     assert b[1] > 0
 
 
-@pytest.mark.parametrize("schedule", [(0, 0), (1, 16), (2, 3)])
+@pytest.mark.parametrize("schedule", [(0, 0), (1, 16), (2, 3), (3, 2 | (9 << 16))])
 def test_fog_close_to_oracle(schedule):
     """With fog the shading uses exp/pow; device and host libm may differ in the last ulp,
     so RGB8 bytes are allowed to differ by at most 1 (both tracer schedules)."""
@@ -129,12 +129,14 @@ def test_fog_close_to_oracle(schedule):
 @pytest.mark.parametrize("tile", [(16, 4, 0, 0, 0), (4, 16, 1, 0, 0), (64, 1, 0, 0, 0), (8, 8, 1, 0, 0),
                                   (2, 32, 1, 0, 0), (8, 8, 0, 1, 16), (16, 4, 1, 1, 8), (64, 1, 0, 1, 1),
                                   (32, 2, 1, 1, 64), (2, 32, 0, 1, 24), (8, 8, 0, 2, 1), (8, 8, 1, 2, 6),
-                                  (32, 2, 0, 2, 20), (16, 4, 1, 2, 128), "auto"])
+                                  (32, 2, 0, 2, 20), (16, 4, 1, 2, 128), (8, 8, 0, 3, (24, 128)),
+                                  (16, 4, 0, 3, (1, 3)), (64, 1, 0, 3, (5, 400)), (8, 8, 1, 3, (8, 40)), "auto"])
 def test_tile_shapes_bit_exact(tile):
     """Every wavefront tile shape, node fetch strategy and schedule (per tile; persistent
     wavefronts refilled at the given idle-lane count; per tile with walks beyond the given step
-    count searched a wavefront per ray, down to every ray at 1 step), and the autotuner's
-    pick, traces the same frame."""
+    count searched a wavefront per ray, down to every ray at 1 step; per tile to A steps with the
+    saved walks resumed to B, down to 1 and 3), and the autotuner's pick, traces the same
+    frame."""
     import torch
     from hashdag_amd import DAGInfo, DAGTracer, ray_params
     from hashdag_amd.tracer import autotune
@@ -153,7 +155,10 @@ def test_tile_shapes_bit_exact(tile):
     else:
         tracer.set_tile(dag, tile[0], tile[1])
         tracer.set_fetch(dag, tile[2])
-        tracer.set_schedule(dag, tile[3], tile[4] if tile[3] else 16)
+        if tile[3] == 3:
+            tracer.set_schedule(dag, 3, tile[4][0], resume_to=tile[4][1])
+        else:
+            tracer.set_schedule(dag, tile[3], tile[4] if tile[3] else 16)
     for cam, prm in zip(cams, prms):
         tracer.resolve_paths(cam, info, dag, params=prm)
         tracer.resolve_colors(dag)
