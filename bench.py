@@ -52,6 +52,8 @@ def main() -> int:
     ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
     ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
     ap.add_argument("--no-edits", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain leg of the edit comparison (for counter runs of the fused pass)")
     ap.add_argument("--edit-forms", choices=("fast", "all", "level"), default="fast",
                     help="edit launch forms compared: fast (a launch per step and level, a captured single-edit "
                          "graph per plain edit, one cooperative launch for the fused pass, one workgroup on a "
@@ -393,6 +395,23 @@ def edit_bench(scene, geom, params, W, H, args):
     else:
         st_b, t_b, batched_form = st_f, t_f, "a launch per step and level"
 
+    roofline = {"bound": "hbm", "achieved": round(abytes / t_f / 1e9, 2), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(abytes / t_f / 8e12, 5),
+                "traffic": measured_edit_traffic(params.levels, len(edits), args.seed + 1),
+                "traffic_source": "profiles/edit_kernels_2p16_4096edits.md (rocprofv3 FETCH_SIZE + WRITE_SIZE of the "
+                                  "fused pass, copy-calibrated)",
+                "bytes": ab, "note": "algorithmic bytes of the fused pass (node words read by the descent, "
+                "bucket bytes read by the finds, node words written, staged colour markers and explicit "
+                "colours written once and read twice, colour leaf words written) over the wall time of its "
+                "level-synchronous form including host steps"}
+    workload = (f"{len(edits)} mixed edits on the 2^{params.levels} coloured terrain (70% sphere add/carve, "
+                f"20% box add/carve, 10% sphere paint; radius log-uniform 2-256; seed {args.seed + 1})")
+    if args.no_plain:
+        return {"workload": workload, "edited_voxels": int(st_b.num_voxels),
+                "batched_voxels_per_s": round(st_b.num_voxels / t_b, 1), "batched_ms": round(t_b * 1e3, 2),
+                "batched_form": batched_form, "gc_after_batched_ms": round(gc_f * 1e3, 2),
+                "plain": "not run (--no-plain)", "roofline": roofline}
+
     # plain: one pass per edit, in every launch form; the fastest is the baseline
     plain = HashDAG(geom)
     plain.load_scene(scene)
@@ -458,8 +477,7 @@ def edit_bench(scene, geom, params, W, H, args):
     if not same:
         raise RuntimeError("fused and plain edit results differ")
     return {
-        "workload": f"{len(edits)} mixed edits on the 2^{params.levels} coloured terrain (70% sphere add/carve, "
-                    f"20% box add/carve, 10% sphere paint; radius log-uniform 2-256; seed {args.seed + 1})",
+        "workload": workload,
         "edited_voxels": int(st_b.num_voxels),
         "plain_voxels_per_s": round(st_p.num_voxels / t_p, 1),
         "batched_voxels_per_s": round(st_b.num_voxels / t_b, 1),
@@ -483,12 +501,7 @@ def edit_bench(scene, geom, params, W, H, args):
         "batched_with_gc_voxels_per_s": round(st_b.num_voxels / (t_b + gc_f), 1),
         "batched_over_plain_with_gc": round((t_p + gc_p) / (t_b + gc_f), 2),
         "results_identical": same,
-        "roofline": {"bound": "hbm", "achieved": round(abytes / t_f / 1e9, 2), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(abytes / t_f / 8e12, 5), "traffic": None,
-                     "bytes": ab, "note": "algorithmic bytes of the fused pass (node words read by the descent, "
-                     "bucket bytes read by the finds, node words written, staged colour markers and explicit "
-                     "colours written once and read twice, colour leaf words written) over the wall time of its "
-                     "level-synchronous form including host steps"},
+        "roofline": roofline,
     }
 
 
@@ -693,6 +706,21 @@ def cpu_edit_baseline(od, oc, params, args):
             "sample": f"C1: 2^12 binary terrain, {vn[1]} of 100 sphere edits (50 add / 50 carve, radius 8-64), "
                       f"{vn[2]} voxels in {vn[3]:.2f}s on {threads} threads; 1 thread: {v1[1]} edits, {v1[2]} voxels "
                       f"in {v1[3]:.2f}s" + ("; both reach the same canonical DAG" if same else "")}
+
+
+def measured_edit_traffic(levels: int, n_edits: int, seed: int):
+    """HBM bytes of the fused edit pass from the committed rocprofv3 summary, when it covers
+    this workload (scene depth, edit count and seed of the edit list)."""
+    path = os.path.join(ROOT, "profiles", "edit_kernels_2p16_4096edits.md")
+    if not os.path.exists(path):
+        return None
+    want = {"levels": str(levels), "edits": str(n_edits), "seed": str(seed)}
+    for line in open(path):
+        if line.startswith("traffic_bytes "):
+            kv = dict(tok.split("=") for tok in line.split()[1:])
+            if all(kv.get(k) == v for k, v in want.items()):
+                return int(kv["fused_pass"])
+    return None
 
 
 def measured_traffic(kernel: str, levels: int, W: int, H: int, launch: dict):
