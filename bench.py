@@ -52,6 +52,8 @@ def main() -> int:
     ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
     ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
     ap.add_argument("--no-edits", action="store_true")
+    ap.add_argument("--no-bucket-tune", action="store_true",
+                    help="keep the reference's bucket counts for the edit scene (no bucket-count trials)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain leg of the edit comparison (for counter runs of the fused pass)")
     ap.add_argument("--edit-forms", choices=("fast", "all", "level"), default="fast",
@@ -265,8 +267,15 @@ def main() -> int:
                 f"buckets top {egeom.bucket_size_top} low {egeom.bucket_size_low} words")
         else:
             eparams, escene, egeom = params, scene, geom
+        bucket_trials = None
+        if not args.no_edits and not args.no_bucket_tune:
+            egeom, bucket_trials = tune_buckets(escene, eparams, args)
         if not args.no_edits:
             result["edits"] = edit_bench(escene, egeom, eparams, W, H, args)
+            result["edits"]["bucket_geometry"] = {"low": f"{1 << egeom.bucket_bits_low} x {egeom.bucket_size_low} words",
+                                                  "top": f"{1 << egeom.bucket_bits_top} x {egeom.bucket_size_top} words",
+                                                  "top_levels": egeom.num_top_levels,
+                                                  "fused_ms_by_geometry": bucket_trials}
             if not args.no_cpu_baseline:
                 result["edits"]["cpu_baseline"] = edit_cpu
         if args.serve_requests > 0:
@@ -279,13 +288,56 @@ def main() -> int:
     return 0
 
 
-def fit_geometry(levels, level_words):
+def fit_geometry(levels, level_words, bucket_scale=1):
     """Bucket geometry with 50% headroom for edits when it fits, else the tightest that does."""
     from hashdag_amd import Geometry
     try:
-        return Geometry.fit(levels, level_words, headroom=1.5)
+        return Geometry.fit(levels, level_words, headroom=1.5, bucket_scale=bucket_scale)
     except ValueError:
-        return Geometry.fit(levels, level_words, headroom=1.0)
+        return Geometry.fit(levels, level_words, headroom=1.0, bucket_scale=bucket_scale)
+
+
+def tune_buckets(scene, params, args):
+    """The bucket-count axis: the edit list's fused pass timed on the scene loaded with the
+    reference's bucket counts and with 2x / 4x / 8x as many buckets of 1/2 / 1/4 / 1/8 the size
+    (the same capacity; a find scans a shorter bucket); a geometry whose buckets overflow drops
+    out. Returns the fastest geometry and the trials."""
+    from hashdag_amd._lib import HashDAGError
+    import torch
+
+    from hashdag_amd import HashDAG
+    from hashdag_amd.workload import mixed_edits
+    edits = mixed_edits(params, args.edits, seed=args.seed + 1)
+    trials, best = {}, None
+    for scale in (1, 2, 4, 8):
+        try:
+            g = fit_geometry(params.levels, scene.level_words, bucket_scale=scale)
+        except ValueError:
+            continue
+        d = HashDAG(g)
+        try:
+            d.load_scene(scene)
+            d.edit_batch(edits, fused=True)  # sizes the scratch
+            for _ in range(d.undo_depth()[0]):
+                d.undo()
+            d.remove_stale_nodes()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            d.edit_batch(edits, fused=True)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+        except HashDAGError as e:
+            log(f"bucket geometry x{scale}: {e}")
+            continue
+        finally:
+            d.close()
+        key = f"{1 << g.bucket_bits_low} x {g.bucket_size_low} words (low levels)"
+        trials[key] = round(ms, 2)
+        log(f"bucket geometry x{scale}: {key}, top {1 << g.bucket_bits_top} x {g.bucket_size_top}: "
+            f"fused pass {ms:.1f} ms")
+        if best is None or ms < best[0]:
+            best = (ms, g)
+    return best[1], trials
 
 
 def edit_bench(scene, geom, params, W, H, args):

@@ -56,19 +56,28 @@ class Geometry:
 
     @staticmethod
     def fit(levels: int, level_words, color_tree_levels: int = 10, top_fill: float = 0.4,
-            low_fill: float = 0.6, headroom: float = 1.0) -> "Geometry":
+            low_fill: float = 0.6, headroom: float = 1.0, bucket_scale: int = 1) -> "Geometry":
         """Bucket geometry for measured per-level node words: the reference's bucket counts,
         with the reference's bucket sizes when they hold the scene and larger buckets (in that
         order of preference) when they do not. A level is 'top' while its words fit `top_fill`
         of the top capacity (the reference's 9 when all do); every level must fit its fill
         limit, and the virtual space must stay below 2^32 words. More than the reference's 9
         top levels are used only when no geometry with 9 fits (deep scenes whose uniform low
-        levels would not fit 32-bit addresses). `headroom` scales the words for later growth."""
+        levels would not fit 32-bit addresses). `headroom` scales the words for later growth.
+        `bucket_scale` (a power of two) multiplies the bucket counts and divides the bucket
+        sizes by it (down to one page): the same capacity in more, shorter buckets, so a find
+        scans fewer words."""
         last = None
+        if bucket_scale < 1 or bucket_scale & (bucket_scale - 1):
+            raise ValueError("bucket_scale must be a power of two")
+        shift = bucket_scale.bit_length() - 1
         sizes = ((1024, 4096), (2048, 4096), (4096, 4096), (1024, 8192), (2048, 8192), (4096, 8192))
         for max_top, (size_top, size_low) in [(9, sz) for sz in sizes] + [(levels - 1, sz) for sz in sizes]:
-            g = Geometry(levels=levels, color_tree_levels=color_tree_levels, bucket_size_top=size_top,
-                         bucket_size_low=size_low)
+            st = max(size_top >> shift, 512)
+            sl = max(size_low >> shift, 512)
+            g = Geometry(levels=levels, color_tree_levels=color_tree_levels, bucket_size_top=st, bucket_size_low=sl,
+                         bucket_bits_top=10 + (size_top // st).bit_length() - 1,
+                         bucket_bits_low=16 + (size_low // sl).bit_length() - 1)
             top_cap = (1 << g.bucket_bits_top) * g.bucket_size_top
             n_top = 0
             while n_top < min(max_top, levels) and n_top < len(level_words) and \

@@ -406,3 +406,25 @@ def test_edit_byte_counters():
     dag.count_edit_bytes(True)
     assert all(v == 0 for v in dag.count_edit_bytes(False).values())
     assert st2.num_voxels > 0
+
+
+def test_fused_batch_with_more_shorter_buckets():
+    """Twice the buckets at half the size (the bench's bucket-count axis): the same DAG,
+    colours and leaf words as the oracle on the same geometry."""
+    from hashdag_amd import Geometry, HashDAG
+    from oracle import OracleColors, OracleDAG
+    p, sc, g0 = scene("d13")
+    g = Geometry(levels=g0.levels, num_top_levels=g0.num_top_levels, bucket_bits_top=g0.bucket_bits_top + 1,
+                 bucket_bits_low=g0.bucket_bits_low + 1, bucket_size_top=g0.bucket_size_top // 2,
+                 bucket_size_low=g0.bucket_size_low // 2, color_tree_levels=g0.color_tree_levels)
+    dag = HashDAG(g, pool_pages=POOL_PAGES)
+    dag.load_scene(sc)
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    edits = random_edits(p, 40, seed=13)
+    st = dag.edit_batch(edits, fused=True)
+    assert st.num_voxels == sum(od.edit(kind, center, r, oc)[1] for kind, center, r in edits)
+    assert_same(dag, sc, od, oc, "fused batch, 2x buckets")
+    dag.remove_stale_nodes()
+    od.gc(g.levels - 2)
+    assert_same(dag, sc, od, oc, "after GC, 2x buckets")
