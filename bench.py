@@ -52,8 +52,9 @@ def main() -> int:
     ap.add_argument("--edits", type=int, default=4096, help="mixed edits of the batched-vs-plain comparison")
     ap.add_argument("--edit-cpu-seconds", type=float, default=10.0, help="budget of the CPU edit baseline sample")
     ap.add_argument("--no-edits", action="store_true")
-    ap.add_argument("--no-bucket-tune", action="store_true",
-                    help="keep the reference's bucket counts for the edit scene (no bucket-count trials)")
+    ap.add_argument("--bucket-scale", type=int, default=0,
+                    help="edit scene buckets: N times the reference's bucket counts at 1/N the size (1 = the "
+                         "reference's), or 0 to time 1, 2, 4, 8 and keep the fastest")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain leg of the edit comparison (for counter runs of the fused pass)")
     ap.add_argument("--edit-forms", choices=("fast", "all", "level"), default="fast",
@@ -268,7 +269,9 @@ def main() -> int:
         else:
             eparams, escene, egeom = params, scene, geom
         bucket_trials = None
-        if not args.no_edits and not args.no_bucket_tune:
+        if args.bucket_scale > 0:
+            egeom = fit_geometry(eparams.levels, escene.level_words, bucket_scale=args.bucket_scale)
+        elif not args.no_edits:
             egeom, bucket_trials = tune_buckets(escene, eparams, args)
         if not args.no_edits:
             result["edits"] = edit_bench(escene, egeom, eparams, W, H, args)
@@ -449,7 +452,7 @@ def edit_bench(scene, geom, params, W, H, args):
 
     roofline = {"bound": "hbm", "achieved": round(abytes / t_f / 1e9, 2), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(abytes / t_f / 8e12, 5),
-                "traffic": measured_edit_traffic(params.levels, len(edits), args.seed + 1),
+                "traffic": measured_edit_traffic(params.levels, len(edits), args.seed + 1, geom),
                 "traffic_source": "profiles/edit_kernels_2p16_4096edits.md (rocprofv3 FETCH_SIZE + WRITE_SIZE of the "
                                   "fused pass, copy-calibrated)",
                 "bytes": ab, "note": "algorithmic bytes of the fused pass (node words read by the descent, "
@@ -760,13 +763,14 @@ def cpu_edit_baseline(od, oc, params, args):
                       f"in {v1[3]:.2f}s" + ("; both reach the same canonical DAG" if same else "")}
 
 
-def measured_edit_traffic(levels: int, n_edits: int, seed: int):
+def measured_edit_traffic(levels: int, n_edits: int, seed: int, geom):
     """HBM bytes of the fused edit pass from the committed rocprofv3 summary, when it covers
-    this workload (scene depth, edit count and seed of the edit list)."""
+    this workload (scene depth, edit count and seed of the edit list, low-level buckets)."""
     path = os.path.join(ROOT, "profiles", "edit_kernels_2p16_4096edits.md")
     if not os.path.exists(path):
         return None
-    want = {"levels": str(levels), "edits": str(n_edits), "seed": str(seed)}
+    want = {"levels": str(levels), "edits": str(n_edits), "seed": str(seed),
+            "buckets_low": f"{1 << geom.bucket_bits_low}x{geom.bucket_size_low}"}
     for line in open(path):
         if line.startswith("traffic_bytes "):
             kv = dict(tok.split("=") for tok in line.split()[1:])
