@@ -34,8 +34,9 @@
  *                     axes exactly)                              tests/test_oracle_edits.py
  *   fill              dense-grid flood fill with the buffer's loaded set, bounds and colour
  *                     scan written out independently                tests/test_oracle_edits.py
- *   copy swirl        host cosf / sinf as the reference's host edit path calls them: parity
- *                     unpinned (read from hash_dag_editors.h:364-379 only)
+ *   copy swirl        dense-grid brute force turning each offset about y in FP64, on every
+ *                     voxel whose source coordinate is 1e-3 or more from an integer
+ *                                                                tests/test_oracle_pins.py
  *   GC                canonical form unchanged, idempotent, used words == reachable words +
  *                     full node + padding        tests/test_oracle_table.py, test_oracle_edits.py
  *   interior find     the walk rule (start + n < min(bucket size, page end)) on synthetic
@@ -48,8 +49,9 @@
  *                     selected gradient next to them, continuity across cell faces, the bounded
  *                     octave sum; the editor against a dense-grid brute force
  *                                                                tests/test_oracle_noise.py
- *   fog (density > 0) host expf / pow; the reference's device expf is not available: parity
- *                     unpinned beyond +-1 per RGB8 channel        tests/test_trace_gpu.py
+ *   fog (density > 0) a density that saturates the fog: every pixel its fog colour, the sky
+ *                     colour exactly away from the sun; a density with exp(..) = 1/2 at a
+ *                     pixel: the midpoint of its fog-free and fog colours  tests/test_oracle_pins.py
  */
 #pragma once
 #include <stdint.h>

@@ -20,6 +20,10 @@ or a round trip leaves open:
   exactly at page ends and at the bucket end, against a walk written here from the
   reference text: per page, compare nodes while start < min(bucket size, page end) - n,
   stepping by each node's size; a page where pageStart + n >= that end ends the search.
+* the copy swirl (hash_dag_editors.h:364-379): a dense-grid brute force turning each offset
+  about y in FP64, away from the voxels whose source coordinate is within 1e-3 of an integer.
+* applyFog (tracer.cu:551-572): a saturating density (every pixel its fog colour) and, per
+  pixel, the density at which exp gives 1/2 (the midpoint of the fog-free and fog colours).
 """
 import ctypes as C
 import math
@@ -27,7 +31,7 @@ import math
 import numpy as np
 import pytest
 
-from hashdag_amd import DAGInfo, Geometry
+from hashdag_amd import CameraView, DAGInfo, Geometry
 from hashdag_amd.dag import HostImage
 from hashdag_amd.workload import cameras
 from oracle import Oracle, OracleColors, OracleDAG, orc
@@ -414,3 +418,135 @@ def test_paint_quantisation():
                 assert (bits1, wb1) == (bits0, wb0), v
             checked += 1
     assert checked > 500 and blended > 200
+
+
+# ---------------------------------------------------------------------------------------------
+# copy swirl: dense-grid brute force with the rotation in FP64
+# ---------------------------------------------------------------------------------------------
+
+SWIRLS = [
+    # center, radius, source, dest, swirl period, transform (None = identity)
+    ((40.0, 60.0, 40.0), 10.0, (40.0, 60.0, 40.0), (90.0, 80.0, 90.0), 30.0, None),
+    ((64.0, 64.0, 64.0), 12.0, (60.0, 50.0, 64.0), (30.0, 60.0, 90.0), 6.5, None),
+    ((50.0, 58.0, 70.0), 10.0, (50.0, 58.0, 70.0), (90.0, 40.0, 30.0), 20.0, [[0, 0, 1], [0, 1, 0], [1, 0, 0]]),
+]
+
+
+@pytest.mark.parametrize("center,radius,source,dest,period,transform", SWIRLS)
+def test_swirl_copy_brute_force(center, radius, source, dest, period, transform):
+    """The copy swirl (hash_dag_editors.h:364-379) against a brute force that turns each
+    destination voxel's offset from dest about the y axis by angle y / period in FP64
+    (x' = x cos - z sin, z' = x sin + z cos), applies T^-1, adds source and floors. The oracle
+    turns in float32 with cosf / sinf; the two agree on every voxel whose source coordinate
+    lies more than 1e-3 from an integer (a decision boundary), and those are checked: voxel
+    set and the colours of newly set voxels."""
+    from tests.test_oracle_edits import BOX_ADD, per_voxel_colours, touched
+    p, sc, g = scene("d7")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    before, cols_before = per_voxel_colours(od, oc, g)
+    T = np.eye(3) if transform is None else np.array(transform, np.float64)
+    od.edit_copy(center, radius, source, dest, oc, transform=None if transform is None else T,
+                 swirl_period=period)
+    after, cols_after = per_voxel_colours(od, oc, g)
+    E = 1 << g.levels
+    box_c = np.array(dest) + T @ (np.array(center) - np.array(source))
+    box_r = radius * np.abs(T @ np.ones(3)).max()
+    t = touched(BOX_ADD, tuple(box_c), box_r, E)
+    Tinv = np.linalg.inv(T)
+    src, boundary = {}, np.zeros_like(t)
+    for v in zip(*np.nonzero(t)):
+        x, y, z = (float(a) for a in np.array(v, np.float64) - np.array(dest))
+        a = y / period
+        rel = np.array([x * math.cos(a) - z * math.sin(a), y, x * math.sin(a) + z * math.cos(a)])
+        s_ = np.array(source) + Tinv @ rel
+        assert (s_ >= 0).all() and (s_ < E).all()
+        near = np.abs(s_ - np.round(s_))
+        if ((near < 1e-3) & (near != 0)).any():
+            boundary[v] = 1
+            continue
+        src[v] = tuple(int(c) for c in np.floor(s_))
+    expect = before.copy()
+    for v, s_ in src.items():
+        if before[s_]:
+            expect[v] = 1
+    keep = boundary == 0
+    assert np.array_equal(after[keep], expect[keep])
+    assert boundary.sum() <= 0.01 * t.sum()
+    turned = sum(1 for v, s_ in src.items() if before[s_] and not before[v])
+    assert turned > 50
+    for v, col in cols_after.items():
+        if v in src and not before[v]:
+            assert col == cols_before[src[v]], v
+        elif not t[v]:
+            assert col == cols_before[v], v
+
+
+# ---------------------------------------------------------------------------------------------
+# fog: cases whose result the mathematics fixes
+# ---------------------------------------------------------------------------------------------
+
+
+def test_fog_special_cases():
+    """applyFog (tracer.cu:551-572): fogAmount = 1 - exp(-distance * density * 1e-5), the fog
+    colour lerp((187, 242, 250) / 255, white, (1.01 max(dir . sun, 0))^30), the pixel
+    lerp(colour, fog colour, clamp(fogAmount)). Pinned by two cases:
+    * a density so high that fogAmount is 1 for every pixel: each pixel is its fog colour,
+      exactly the sky colour (187, 242, 250) where the direction faces away from the sun;
+    * for a chosen hit pixel, the density with distance * density * 1e-5 = ln 2, so that
+      exp gives 1/2: the pixel is the midpoint of its fog-free colour and its fog colour (to
+      the RGB8 truncation of both)."""
+    f = np.float32
+    p, sc, g = scene("d7")
+    od = OracleDAG.from_basic(g, sc.dag, POOL_PAGES)
+    oc = OracleColors.from_scene(sc)
+    W, H = 48, 32
+    # looking slightly down along the sun's azimuth: terrain below, sky toward the sun above
+    cam = CameraView.look((10.0, 90.0, 10.0), math.atan2(0.3, 0.5), -0.25)
+    prm = Oracle.ray_params(cam, DAGInfo.unit(g.levels), g.levels, W, H)
+    paths = Oracle.trace_paths(od, prm, W, H)
+    cols = Oracle.trace_colors(od, oc, paths, W, H)
+    clear = Oracle.trace_shadows(od, prm, paths, cols, W, H, 1.0, 0.0)
+    thick = Oracle.trace_shadows(od, prm, paths, cols, W, H, 1.0, 1e12)
+    s = 1.0 / math.sqrt(0.3 * 0.3 + 1.0 + 0.25)
+    sun = np.array([0.3 * s, s, 0.5 * s])
+    camp, rmin, ddx, ddy = prm[0:3], prm[3:6], prm[6:9], prm[9:12]
+
+    def direction(px, py):  # toward the hit voxel's corner, or the camera ray for the sky
+        pp = paths[py, px, :3]
+        if not pp.any():
+            v = rmin + px * ddx + (H - 1 - py) * ddy - camp
+        else:
+            v = pp.astype(np.float64) - camp
+        return v / np.linalg.norm(v), float(np.linalg.norm(v))
+
+    def fog_colour(dirn):
+        sa = (1.01 * max(float(dirn @ sun), 0.0)) ** 30
+        return np.array([187.0, 242.0, 250.0]) / 255.0 * (1 - sa) + sa, sa
+
+    def rgb(c):
+        return np.array([(int(c) >> (8 * k)) & 0xFF for k in range(3)])
+
+    away = sunward = 0
+    hits = []
+    for py in range(H):
+        for px in range(W):
+            dirn, dist = direction(px, py)
+            fc, sa = fog_colour(dirn)
+            got = rgb(thick[py, px])
+            if sa == 0.0:
+                assert tuple(got) == (187, 242, 250), (px, py)
+                away += 1
+            else:
+                assert np.abs(got - np.minimum(fc, 1.0) * 255).max() <= 1.0, (px, py)
+                sunward += 1
+            if paths[py, px, :3].any():
+                hits.append((px, py, dist, fc))
+    assert away > 0 and sunward > 0 and len(hits) > 100
+    rng = np.random.default_rng(0)
+    for k in rng.choice(len(hits), 6, replace=False):
+        px, py, dist, fc = hits[int(k)]
+        density = math.log(2.0) / (dist * 1e-5)
+        half = Oracle.trace_shadows(od, prm, paths, cols, W, H, 1.0, density)
+        want = (rgb(clear[py, px]) + np.minimum(fc, 1.0) * 255) / 2
+        assert np.abs(rgb(half[py, px]) - want).max() <= 1.0, (px, py, rgb(half[py, px]), want)
