@@ -179,7 +179,7 @@ __global__ void k_gc_full(TableRef T, uint32_t level, uint32_t* full_nodes, uint
 
 // The live nodes of a level are placed bucket by bucket: each node takes a rank within its
 // bucket (the bucket from k_gc_gather, the rank from k_gc_bucket), the buckets' ranges of a level-wide order follow from a scan of the
-// bucket counts (k_gc_scan, k_gc_order), one lane per bucket lays its nodes out in rank order
+// bucket counts (k_gc_tile_sums, k_gc_tile_scan, k_gc_tile_apply, k_gc_order), one lane per bucket lays its nodes out in rank order
 // with the append rule of table_append (a node that does not fit in the rest of its page starts
 // the next page; k_gc_layout), and every node is then written where it was placed
 // (k_gc_write). This is what appending the nodes one at a time in rank order gives, without a
@@ -192,30 +192,63 @@ __global__ __launch_bounds__(256) void k_gc_bucket(uint32_t level, const uint32_
         rank[i] = atomicAdd(&bcount[bkt[i]], 1u);
 }
 
-// exclusive scan of the bucket counts of one level (one block)
-__global__ __launch_bounds__(1024) void k_gc_scan(const uint32_t* __restrict__ bcount, uint32_t* __restrict__ bstart,
-                                                  uint32_t nb) {
-    __shared__ uint32_t sums[1024];
-    const uint32_t per = (nb + 1023u) / 1024u;
-    const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
-    uint32_t acc = 0;
-    for (uint32_t b = b0; b < b1; ++b) acc += bcount[b];
-    sums[threadIdx.x] = acc;
+// exclusive scan of the bucket counts of one level, in tiles of 1024 buckets: the tiles' sums
+// (k_gc_tile_sums), their exclusive scan (k_gc_tile_scan, one block), then each tile scanned
+// from its offset (k_gc_tile_apply)
+constexpr uint32_t kScanTile = 1024;
+__device__ __forceinline__ uint32_t block_incl_scan1024(uint32_t v, uint32_t* lds, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    if (lane == 63) lds[wv] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t run = 0;
-        for (uint32_t t = 0; t < 1024; ++t) {
-            const uint32_t x = sums[t];
-            sums[t] = run;
+        for (uint32_t w = 0; w < kScanTile / 64; ++w) {
+            const uint32_t x = lds[w];
+            lds[w] = run;
             run += x;
         }
+        lds[kScanTile / 64] = run;
     }
     __syncthreads();
-    acc = sums[threadIdx.x];
-    for (uint32_t b = b0; b < b1; ++b) {
-        bstart[b] = acc;
-        acc += bcount[b];
+    v += lds[wv];
+    total = lds[kScanTile / 64];
+    __syncthreads();  // lds is reused by the caller's next call
+    return v;
+}
+__global__ __launch_bounds__(kScanTile) void k_gc_tile_sums(const uint32_t* __restrict__ bcount, uint32_t nb,
+                                                           uint32_t* __restrict__ tileSum) {
+    __shared__ uint32_t lds[kScanTile / 64 + 1];
+    const uint32_t i = blockIdx.x * kScanTile + threadIdx.x;
+    uint32_t total;
+    block_incl_scan1024(i < nb ? bcount[i] : 0u, lds, total);
+    if (threadIdx.x == 0) tileSum[blockIdx.x] = total;
+}
+__global__ __launch_bounds__(kScanTile) void k_gc_tile_scan(uint32_t* __restrict__ tileSum, uint32_t nTiles) {
+    __shared__ uint32_t lds[kScanTile / 64 + 1];
+    uint32_t carry = 0;
+    for (uint32_t t0 = 0; t0 < nTiles; t0 += kScanTile) {  // block-uniform
+        const uint32_t t = t0 + threadIdx.x;
+        const uint32_t v = t < nTiles ? tileSum[t] : 0u;
+        uint32_t total;
+        const uint32_t incl = block_incl_scan1024(v, lds, total);
+        if (t < nTiles) tileSum[t] = carry + incl - v;
+        carry += total;
     }
+}
+__global__ __launch_bounds__(kScanTile) void k_gc_tile_apply(const uint32_t* __restrict__ bcount, uint32_t nb,
+                                                            const uint32_t* __restrict__ tileSum,
+                                                            uint32_t* __restrict__ bstart) {
+    __shared__ uint32_t lds[kScanTile / 64 + 1];
+    const uint32_t i = blockIdx.x * kScanTile + threadIdx.x;
+    const uint32_t v = i < nb ? bcount[i] : 0u;
+    uint32_t total;
+    const uint32_t incl = block_incl_scan1024(v, lds, total);
+    if (i < nb) bstart[i] = tileSum[blockIdx.x] + incl - v;
 }
 
 // node lengths in level-wide (bucket, rank) order
@@ -368,13 +401,14 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
     // count and start per bucket of a level
     uint32_t maxBuckets = 1;
     for (uint32_t l = 0; l <= max_level; ++l) maxBuckets = std::max(maxBuckets, g.buckets_per_level(l));
-    uint32_t *bkt, *rank, *olen, *opos, *bcount, *bstart;
+    uint32_t *bkt, *rank, *olen, *opos, *bcount, *bstart, *tileSum;
     if (e == hipSuccess) e = alloc((void**)&bkt, (uint64_t)cap * 4);
     if (e == hipSuccess) e = alloc((void**)&rank, (uint64_t)cap * 4);
     if (e == hipSuccess) e = alloc((void**)&olen, (uint64_t)cap * 4);
     if (e == hipSuccess) e = alloc((void**)&opos, (uint64_t)cap * 4);
     if (e == hipSuccess) e = alloc((void**)&bcount, (uint64_t)maxBuckets * 4);
     if (e == hipSuccess) e = alloc((void**)&bstart, (uint64_t)maxBuckets * 4);
+    if (e == hipSuccess) e = alloc((void**)&tileSum, (uint64_t)(maxBuckets / kScanTile + 1) * 4);
     // old address -> new address of every live node: one word per virtual address (written
     // for live nodes only, read only for them)
     if (e == hipSuccess) e = alloc((void**)&fwd, (uint64_t)g.total_virtual() * 4);
@@ -413,7 +447,10 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
         const uint32_t nb = g.buckets_per_level(l);
         k_gc_fill<<<256, 256, 0, s>>>(bcount, nb, 0u);
         k_gc_bucket<<<kGcGrid, 256, 0, s>>>(l, counts, cap, bkt, rank, bcount);
-        k_gc_scan<<<1, 1024, 0, s>>>(bcount, bstart, nb);
+        const uint32_t nTiles = (nb + kScanTile - 1) / kScanTile;
+        k_gc_tile_sums<<<nTiles, kScanTile, 0, s>>>(bcount, nb, tileSum);
+        k_gc_tile_scan<<<1, kScanTile, 0, s>>>(tileSum, nTiles);
+        k_gc_tile_apply<<<nTiles, kScanTile, 0, s>>>(bcount, nb, tileSum, bstart);
         k_gc_order<<<kGcGrid, 256, 0, s>>>(l, counts, cap, tlen, bkt, rank, bstart, olen);
         k_gc_layout<<<(nb + 255) / 256, 256, 0, s>>>(g, l, ctx->bucket_sizes, bcount, bstart, olen, opos, errors);
         k_gc_write<<<kGcGrid, 256, 0, s>>>(T, l, counts, cap, frontier[l], tmp, tlen, bkt, rank, bstart, opos, fwd,
