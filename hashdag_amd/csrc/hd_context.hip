@@ -187,6 +187,7 @@ hd_status hd_destroy(hd_ctx* ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     hipDeviceSynchronize();
     edit_free(ctx);
+    gc_scratch_free(ctx);
     hipFree(ctx->d_root);
     hipFree(ctx->vpool);
     hipFree(ctx->page_table);

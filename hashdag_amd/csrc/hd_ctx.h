@@ -5,6 +5,7 @@
 
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/hashdag_hip.h"
@@ -106,6 +107,9 @@ struct hd_ctx {
     uint32_t* d_root = nullptr;  // device copy of the root (edit passes chain through it)
     hd::EditScratch edit;
     std::vector<hd::UndoFrame> undo, redo;
+    // GC scratch, kept between collections and grown when a call needs more (freed with the
+    // context): slot k holds (buffer, bytes)
+    std::vector<std::pair<void*, uint64_t>> gc_slots;
     // bumped whenever a device buffer or launch shape that a captured graph bakes in changes
     uint64_t ptr_gen = 0;
 };
@@ -115,6 +119,8 @@ constexpr uint64_t kVpoolPad = 64;  // words after the virtual space (vector rea
 constexpr uint32_t kArenaTopWord = 8;  // counters[8..9]: committed colour arena top (64-bit)
 
 void edit_free(hd_ctx* ctx);
+// releases the GC scratch kept in the context
+void gc_scratch_free(hd_ctx* ctx);
 // device address of the static edit scratch on ctx's device, looked up once per device
 hd_status static_scratch_setup(hd_ctx* ctx);
 // the persistent tracer's scratch for stream s, grown to hold a W x H frame (allocates: not

@@ -31,6 +31,19 @@ __global__ void k_gc_fill(uint32_t* p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// words used per level (sum of the level's bucket sizes): blockIdx.y is the level
+__global__ __launch_bounds__(256) void k_gc_level_used(Geom g, const uint32_t* __restrict__ bucket_sizes,
+                                                       unsigned long long* used) {
+    const uint32_t l = blockIdx.y;
+    const uint32_t nb = g.buckets_per_level(l);
+    const uint32_t* __restrict__ bs = bucket_sizes + g.bucket_global(l, 0);
+    unsigned long long sum = 0;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) sum += bs[b];
+#pragma unroll
+    for (uint32_t off = 32; off; off >>= 1) sum += __shfl_down(sum, off, 64);
+    if ((threadIdx.x & 63u) == 0 && sum) atomicAdd(&used[l], sum);
+}
+
 // level 0 frontier = the (distinct) roots kept alive
 __global__ void k_gc_seed(uint32_t* frontier0, uint32_t* counts, const uint32_t* roots, uint32_t n_roots,
                           uint32_t levels) {
@@ -349,6 +362,47 @@ static uint32_t remap(const std::vector<uint32_t>& from, const std::vector<uint3
     return to[std::lower_bound(from.begin(), from.end(), v) - from.begin()];
 }
 
+// GC scratch slots (fixed indices; frontier l at kSlotFrontier + l)
+enum : uint32_t {
+    kSlotBitmap, kSlotCounts, kSlotErrors, kSlotRoots, kSlotTmp, kSlotTlen, kSlotBkt, kSlotRank, kSlotOlen,
+    kSlotOpos, kSlotBcount, kSlotBstart, kSlotTileSum, kSlotFwd, kSlotUsed, kSlotFrontier
+};
+
+// slot k of the context's GC scratch with at least `bytes` (grown by a quarter beyond the
+// request, so tables that grow between collections reallocate rarely)
+static hipError_t gc_slot(hd_ctx* ctx, uint32_t k, uint64_t bytes, void** p) {
+    if (ctx->gc_slots.size() <= k) ctx->gc_slots.resize(k + 1, {nullptr, 0});
+    auto& sl = ctx->gc_slots[k];
+    if (sl.second < bytes) {
+        if (sl.first) {
+            const hipError_t e = hipFree(sl.first);
+            sl = {nullptr, 0};
+            if (e != hipSuccess) return e;
+        }
+        const uint64_t want = std::max<uint64_t>(bytes + bytes / 4, 256);
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, want);
+        if (e != hipSuccess) return e;
+        sl = {q, want};
+    }
+    *p = sl.first;
+    return hipSuccess;
+}
+
+// words used per level 0..levels-1 (one small readback in place of every bucket size)
+static hd_status level_used(hd_ctx* ctx, hipStream_t s, std::vector<uint64_t>& used) {
+    const Geom& g = ctx->g;
+    unsigned long long* d = nullptr;
+    hipError_t e = gc_slot(ctx, kSlotUsed, (uint64_t)g.levels * 8, (void**)&d);
+    if (e != hipSuccess) return hip_fail(ctx, e, "GC scratch");
+    k_gc_fill<<<1, 64, 0, s>>>((uint32_t*)d, (uint64_t)g.levels * 2, 0u);
+    k_gc_level_used<<<dim3(64, g.levels), 256, 0, s>>>(g, ctx->bucket_sizes, d);
+    used.assign(g.levels, 0);
+    HD_HIP(ctx, hipMemcpyAsync(used.data(), d, (uint64_t)g.levels * 8, hipMemcpyDeviceToHost, s));
+    HD_HIP(ctx, hipStreamSynchronize(s));
+    return HD_OK;
+}
+
 static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32_t n_extra, bool keep_undo,
                         uint64_t stats[4], void* stream) {
     if (!ctx || !ctx->loaded) return fail(ctx, HD_ERR_INVALID, "no DAG loaded");
@@ -362,12 +416,11 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
     const uint32_t nRoots = (uint32_t)roots.size();
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     // capacities: used words per level bound the number of nodes
-    std::vector<uint32_t> bs(g.total_buckets());
-    HD_HIP(ctx, hipMemcpyAsync(bs.data(), ctx->bucket_sizes, bs.size() * 4, hipMemcpyDeviceToHost, s));
-    HD_HIP(ctx, hipStreamSynchronize(s));
-    std::vector<uint64_t> used(g.levels, 0);
-    for (uint32_t l = 0; l < g.levels; ++l)
-        for (uint32_t b = 0; b < g.buckets_per_level(l); ++b) used[l] += bs[g.bucket_global(l, b)];
+    std::vector<uint64_t> used;
+    {
+        const hd_status st = level_used(ctx, s, used);
+        if (st != HD_OK) return st;
+    }
     // every node takes at least 2 words, so a level holds at most used / 2 nodes
     auto fcap = [&](uint32_t l) {
         return (uint32_t)std::max<uint64_t>(std::max<uint64_t>(used[l] / 2 + 16, 16), l == 0 ? nRoots : 0);
@@ -377,43 +430,38 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
     if (capMax >= (1u << 30)) return fail(ctx, HD_ERR_INVALID, "level too large for GC scratch");
     const uint32_t cap = (uint32_t)capMax;
 
-    std::vector<void*> allocs;
-    auto alloc = [&](void** p, uint64_t bytes) -> hipError_t {
-        const hipError_t e = hipMalloc(p, std::max<uint64_t>(bytes, 16));
-        if (e == hipSuccess) allocs.push_back(*p);
-        return e;
-    };
-    auto release = [&]() {
-        hipStreamSynchronize(s);
-        for (void* p : allocs) hipFree(p);
-    };
+    // scratch from the context's slots (kept between calls; every kernel is given its sizes)
+    auto alloc = [&](uint32_t slot, void** p, uint64_t bytes) -> hipError_t { return gc_slot(ctx, slot, bytes, p); };
+    auto release = [&]() { hipStreamSynchronize(s); };
     const uint64_t bitmapWords = (g.total_virtual() + 31) / 32;
     uint32_t *bitmap, *counts, *errors, *tmp, *tlen, *fwd, *droots;
     std::vector<uint32_t*> frontier(max_level + 1, nullptr);
-    hipError_t e = alloc((void**)&bitmap, bitmapWords * 4);
-    if (e == hipSuccess) e = alloc((void**)&counts, 64 * 4);
-    if (e == hipSuccess) e = alloc((void**)&errors, 4);
-    if (e == hipSuccess) e = alloc((void**)&droots, (uint64_t)nRoots * 4);
-    for (uint32_t l = 0; l <= max_level && e == hipSuccess; ++l) e = alloc((void**)&frontier[l], (uint64_t)fcap(l) * 4);
-    if (e == hipSuccess) e = alloc((void**)&tmp, (uint64_t)cap * 36);
-    if (e == hipSuccess) e = alloc((void**)&tlen, (uint64_t)cap * 4);
+    hipError_t e = alloc(kSlotBitmap, (void**)&bitmap, bitmapWords * 4);
+    if (e == hipSuccess) e = alloc(kSlotCounts, (void**)&counts, 64 * 4);
+    if (e == hipSuccess) e = alloc(kSlotErrors, (void**)&errors, 4);
+    if (e == hipSuccess) e = alloc(kSlotRoots, (void**)&droots, (uint64_t)nRoots * 4);
+    for (uint32_t l = 0; l <= max_level && e == hipSuccess; ++l)
+        e = alloc(kSlotFrontier + l, (void**)&frontier[l], (uint64_t)fcap(l) * 4);
+    if (e == hipSuccess) e = alloc(kSlotTmp, (void**)&tmp, (uint64_t)cap * 36);
+    if (e == hipSuccess) e = alloc(kSlotTlen, (void**)&tlen, (uint64_t)cap * 4);
     // placement scratch: bucket and rank per node, lengths and positions in placement order,
     // count and start per bucket of a level
     uint32_t maxBuckets = 1;
     for (uint32_t l = 0; l <= max_level; ++l) maxBuckets = std::max(maxBuckets, g.buckets_per_level(l));
     uint32_t *bkt, *rank, *olen, *opos, *bcount, *bstart, *tileSum;
-    if (e == hipSuccess) e = alloc((void**)&bkt, (uint64_t)cap * 4);
-    if (e == hipSuccess) e = alloc((void**)&rank, (uint64_t)cap * 4);
-    if (e == hipSuccess) e = alloc((void**)&olen, (uint64_t)cap * 4);
-    if (e == hipSuccess) e = alloc((void**)&opos, (uint64_t)cap * 4);
-    if (e == hipSuccess) e = alloc((void**)&bcount, (uint64_t)maxBuckets * 4);
-    if (e == hipSuccess) e = alloc((void**)&bstart, (uint64_t)maxBuckets * 4);
-    if (e == hipSuccess) e = alloc((void**)&tileSum, (uint64_t)(maxBuckets / kScanTile + 1) * 4);
+    if (e == hipSuccess) e = alloc(kSlotBkt, (void**)&bkt, (uint64_t)cap * 4);
+    if (e == hipSuccess) e = alloc(kSlotRank, (void**)&rank, (uint64_t)cap * 4);
+    if (e == hipSuccess) e = alloc(kSlotOlen, (void**)&olen, (uint64_t)cap * 4);
+    if (e == hipSuccess) e = alloc(kSlotOpos, (void**)&opos, (uint64_t)cap * 4);
+    if (e == hipSuccess) e = alloc(kSlotBcount, (void**)&bcount, (uint64_t)maxBuckets * 4);
+    if (e == hipSuccess) e = alloc(kSlotBstart, (void**)&bstart, (uint64_t)maxBuckets * 4);
+    if (e == hipSuccess) e = alloc(kSlotTileSum, (void**)&tileSum, (uint64_t)(maxBuckets / kScanTile + 1) * 4);
     // old address -> new address of every live node: one word per virtual address (written
     // for live nodes only, read only for them)
-    if (e == hipSuccess) e = alloc((void**)&fwd, (uint64_t)g.total_virtual() * 4);
+    if (e == hipSuccess) e = alloc(kSlotFwd, (void**)&fwd, (uint64_t)g.total_virtual() * 4);
     if (e != hipSuccess) {
         release();
+        gc_scratch_free(ctx);  // give back what a smaller call could not reuse
         return hip_fail(ctx, e, "GC scratch");
     }
     k_gc_fill<<<1024, 256, 0, s>>>(bitmap, bitmapWords, 0u);
@@ -499,16 +547,25 @@ static hd_status gc_run(hd_ctx* ctx, uint32_t max_level, uint32_t* extra, uint32
         for (uint32_t l = 0; l <= max_level; ++l) before += used[l];
         stats[0] = live;
         stats[1] = before;
-        std::vector<uint32_t> bs2(g.total_buckets());
-        HD_HIP(ctx, hipMemcpy(bs2.data(), ctx->bucket_sizes, bs2.size() * 4, hipMemcpyDeviceToHost));
+        std::vector<uint64_t> used2;
+        const hd_status st = level_used(ctx, s, used2);
+        if (st != HD_OK) return st;
         uint64_t after = 0;
-        for (uint32_t l = 0; l <= max_level; ++l)
-            for (uint32_t b = 0; b < g.buckets_per_level(l); ++b) after += bs2[g.bucket_global(l, b)];
+        for (uint32_t l = 0; l <= max_level; ++l) after += used2[l];
         stats[2] = after;
         stats[3] = 0;
     }
     return HD_OK;
 }
+
+namespace hd {
+void gc_scratch_free(hd_ctx* ctx) {
+    if (!ctx) return;
+    for (auto& sl : ctx->gc_slots)
+        if (sl.first) hipFree(sl.first);
+    ctx->gc_slots.clear();
+}
+}  // namespace hd
 
 extern "C" hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], void* stream) {
     return gc_run(ctx, max_level, nullptr, 0, false, stats, stream);

@@ -286,8 +286,9 @@ hd_status hd_reserve_edit_scratch(hd_ctx* ctx, uint32_t items_per_level, uint32_
  * engine (engine.cpp:178-185): keeps only nodes reachable from the root at levels 0..max_level,
  * re-appended level by level bottom-up with their children remapped, full nodes first.
  * Clears the undo history. stats (may be NULL): live nodes, words before, words after, 0.
- * Device scratch for the call: 4 B + 1 bit per virtual word of the table, plus about 64 B per
- * node of the largest level (allocated and freed by the call). */
+ * Device scratch: 4 B + 1 bit per virtual word of the table, plus about 64 B per node of the
+ * largest level; allocated by the first call, kept in the context for later calls (grown when
+ * one needs more) and freed by hd_destroy. */
 hd_status hd_gc(hd_ctx* ctx, uint32_t max_level, uint64_t stats[4], void* stream);
 /* GC that also keeps alive the nodes reachable from extra level-0 roots (rewritten in place to
  * their addresses after GC) and, with keep_undo = 1, from the roots of every undo / redo

@@ -49,6 +49,9 @@ def test_gc_after_edits_matches_oracle(name):
     assert st["words_after"] < st["words_before"]
     after = device_words(dag)
     assert sum(after) < sum(before)
+    # the call's word counts are the table's bucket sizes summed over levels 0..max_level
+    assert st["words_before"] == sum(before[:g.leaf_level + 1])
+    assert st["words_after"] == sum(after[:g.leaf_level + 1])
     assert reachable_on_device(dag, g.leaf_level) == live
     assert dag.undo_depth() == (0, 0)
     assert_same(dag, sc, od, oc, "after GC")
@@ -57,6 +60,13 @@ def test_gc_after_edits_matches_oracle(name):
         dag.edit(kind, center, r)
         od.edit(kind, center, r, oc)
     assert_same(dag, sc, od, oc, "edits after GC")
+    # a second GC on the same context (its scratch kept from the first call)
+    live = reachable_on_device(dag, g.leaf_level)
+    st = dag.remove_stale_nodes()
+    od.gc(g.leaf_level)
+    assert st["live_nodes"] == live
+    assert reachable_on_device(dag, g.leaf_level) == live
+    assert_same(dag, sc, od, oc, "second GC")
 
 
 def test_gc_partial_max_level_matches_oracle():
@@ -74,6 +84,15 @@ def test_gc_partial_max_level_matches_oracle():
     assert after[lvl + 1:] == before[lvl + 1:]  # levels below max_level untouched
     assert sum(after[:lvl + 1]) < sum(before[:lvl + 1])
     assert_same(dag, sc, od, oc, "after partial GC")
+    # then a full GC on the same context: more levels than the scratch was first sized for
+    for kind, center, r in random_edits(p, 3, seed=32):
+        dag.edit(kind, center, r)
+        od.edit(kind, center, r, oc)
+    live = reachable_on_device(dag, g.leaf_level)
+    st = dag.remove_stale_nodes()
+    od.gc(g.leaf_level)
+    assert st["live_nodes"] == live
+    assert_same(dag, sc, od, oc, "full GC after a partial one")
 
 
 def test_gc_keeps_trace_bit_exact():
