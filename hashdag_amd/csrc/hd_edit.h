@@ -144,6 +144,7 @@ struct EditPass {
     uint32_t hcap;
     const DevEdit* edits;
     uint32_t single_edit;
+    uint32_t wide_levels;  // bit l: level l's expand runs a wavefront per child slot
     unsigned long long* stat_voxels;
     unsigned long long* stat_new_words;
     unsigned long long* color_top;

@@ -429,6 +429,137 @@ __device__ __forceinline__ bool copy_source_empty(const EditPass& P, uint32_t ei
 // ---------------------------------------------------------------------------------------------
 // expand: items of level l -> child classification and items of level l+1
 // ---------------------------------------------------------------------------------------------
+// does edit ei (head h) take part in the child box [x0, x1) at level l + 1: its box overlaps,
+// and a decompressing copy's source region there is not empty
+__device__ __forceinline__ bool expand_touch(const EditPass& P, const EditHead& h, uint32_t ei, uint32_t l, float x0,
+                                             float y0, float z0, float x1, float y1, float z1) {
+    if (!e_touch(h, x0, y0, z0, x1, y1, z1)) return false;
+    return !(h.kind == EDIT_COPY && l + 1 + 4 < P.levels && copy_source_empty(P, ei, x0, y0, z0, l + 1));
+}
+
+// child slot c of item i: its position, box, and its node under the item's effective old node
+// and under its original node
+struct SlotRefs {
+    uint32_t cx, cy, cz, childOld, childOrig;
+    float x0, y0, z0, x1, y1, z1;
+};
+__device__ __forceinline__ SlotRefs slot_refs(const EditPass& P, const Items& I, uint32_t i, uint32_t c,
+                                              uint32_t depth, uint64_t& nodeBytes) {
+    SlotRefs r;
+    r.cx = (I.px[i] << 1) | ((c >> 2) & 1u);
+    r.cy = (I.py[i] << 1) | ((c >> 1) & 1u);
+    r.cz = (I.pz[i] << 1) | (c & 1u);
+    r.childOld = kInvalidNode;
+    r.childOrig = kInvalidNode;
+    const uint32_t old = I.old_[i], orig = I.orig[i];
+    if (old != kInvalidNode) {
+        const uint32_t cm = P.vpool[old] & 0xFFu;
+        if (cm & (1u << c)) r.childOld = P.vpool[old + child_off(cm, c)];
+        if (c == 0) nodeBytes += 4u * (1u + (uint32_t)__popc(cm));
+    }
+    if (orig == old) {
+        r.childOrig = r.childOld;
+    } else if (orig != kInvalidNode) {
+        const uint32_t cm = P.vpool[orig] & 0xFFu;
+        if (cm & (1u << c)) r.childOrig = P.vpool[orig + child_off(cm, c)];
+        if (c == 0) nodeBytes += 4u * (1u + (uint32_t)__popc(cm));
+    }
+    const float side = (float)(1u << depth);
+    r.x0 = (float)(r.cx << depth);
+    r.y0 = (float)(r.cy << depth);
+    r.z0 = (float)(r.cz << depth);
+    r.x1 = r.x0 + side;
+    r.y1 = r.y0 + side;
+    r.z1 = r.z0 + side;
+    return r;
+}
+
+// Classifies a child slot from its filtered edits (m edits touch it; the last full / empty
+// cover among them is the lastCover-th, ~0 for none): keep, full, empty or a child item.
+// Writes the slot's corig / cold / cval / cnew / ccolor; coveredVoxels is the voxel count of a
+// full / empty early exit.
+__device__ __forceinline__ uint32_t slot_classify(const EditPass& P, const Items& I, uint32_t l, uint32_t slot,
+                                                  uint32_t depth, const SlotRefs& r, uint32_t m, uint32_t lastCover,
+                                                  uint32_t coverKind, uint32_t coverColor,
+                                                  unsigned long long& coveredVoxels) {
+    const uint32_t L = P.levels - 2, ctl = P.ctl;
+    // old voxel count of the effective old child (colour offsets, "keep" counts)
+    uint32_t oldCount = 0;
+    if (l + 1 >= ctl && r.childOld != kInvalidNode) {
+        if (l + 1 == L) {
+            const uint64_t lf = *reinterpret_cast<const uint64_t*>(P.vpool + r.childOld);
+            oldCount = __popcll(lf);
+        } else {
+            oldCount = P.vpool[r.childOld] >> 8;
+        }
+    }
+    I.corig[slot] = r.childOrig;
+    I.cold[slot] = oldCount;
+    const bool covered = lastCover != ~0u;
+    uint32_t kind;
+    if (m == 0) {
+        kind = KIND_KEEP;
+        I.cval[slot] = r.childOld;
+        I.cnew[slot] = oldCount;
+    } else {
+        const uint32_t rest = covered ? m - lastCover - 1 : m;
+        // a covered colour subtree root stays an item (its colour leaf is rebuilt: the
+        // reference's task runs edit_leaf<color_tree_levels>, which takes the early exit)
+        if (covered && rest == 0 && l + 1 > ctl) {
+            kind = coverKind == 1 ? KIND_FULL : KIND_EMPTY;
+            I.cval[slot] = coverKind == 1 ? P.full_nodes[l + 1] : kInvalidNode;
+            I.cnew[slot] = coverKind == 1 ? (1u << (3 * depth)) : 0u;
+        } else {
+            kind = KIND_ITEM;
+        }
+        // numVoxels of a plain edit: a full / empty early exit counts the whole node
+        if (covered && rest == 0) coveredVoxels = 1ull << (3 * depth);
+        // colour of a full cover (add_large_single_color of the covering editor)
+        if (covered) I.ccolor[slot] = coverColor;
+    }
+    return kind;
+}
+
+// child item ci of a slot, all but its edit list (rest words from eo)
+__device__ __forceinline__ void slot_child_item(const EditPass& P, const Items& I, const Items& C, uint32_t l,
+                                                uint32_t i, uint32_t c, uint32_t slot, uint32_t ci, uint32_t eo,
+                                                uint32_t rest, const SlotRefs& r, uint32_t lastCover,
+                                                uint32_t coverKind) {
+    const uint32_t ctl = P.ctl;
+    C.px[ci] = r.cx;
+    C.py[ci] = r.cy;
+    C.pz[ci] = r.cz;
+    C.orig[ci] = r.childOrig;
+    C.parent[ci] = i;
+    C.slot[ci] = (uint8_t)c;
+    C.estart[ci] = eo;
+    C.ecount[ci] = rest;
+    if (lastCover != ~0u) {
+        C.old_[ci] = coverKind == 1 ? P.full_nodes[l + 1] : kInvalidNode;
+        C.cmode[ci] = coverKind == 1 ? 1u : 2u;  // old colours: single colour / none
+        C.cmodeval[ci] = I.ccolor[slot];
+    } else {
+        C.old_[ci] = r.childOld;
+        C.cmode[ci] = I.cmode[i];
+        C.cmodeval[ci] = I.cmodeval[i];
+    }
+    // colour tree: node of the child (levels < ctl), or the subtree's leaf handle / task
+    const uint32_t cn = I.aux[i];
+    if (l + 1 < ctl) {
+        C.aux[ci] = (l == 0 || cn > 0) ? P.cnodes[cn + c] : 0u;
+    } else if (l + 1 == ctl) {
+        C.aux[ci] = ci;  // task id = own index at level ctl
+        const uint32_t h = (l == 0 || cn > 0) ? P.cnodes[cn + c] : 0u;
+        C.handle[ci] = h;
+    } else {
+        C.aux[ci] = cn;  // task of the ancestor at level ctl
+    }
+    I.cval[slot] = ci;
+}
+
+// ---------------------------------------------------------------------------------------------
+// expand: items of level l -> child classification and items of level l+1
+// ---------------------------------------------------------------------------------------------
 __device__ __forceinline__ void d_expand(const EditPass& P, uint32_t l, VGrid G) {
     __shared__ uint32_t sAlloc[2 * (256 / 64 + 1)];
     __shared__ uint32_t sErr;
@@ -438,9 +569,8 @@ __device__ __forceinline__ void d_expand(const EditPass& P, uint32_t l, VGrid G)
     if (sErr) return;  // block-uniform: the loop below synchronises the block
     const Items I = P.items[l];
     const Items C = P.items[l + 1];
-    const uint32_t D = P.levels, L = D - 2, ctl = P.ctl;
+    const uint32_t D = P.levels, ctl = P.ctl;
     const uint32_t depth = D - (l + 1);  // child cube side = 2^depth
-    const float side = (float)(1u << depth);
     const uint64_t total = (uint64_t)n * 8;
     uint64_t nodeBytes = 0;  // header + child pointers of the items' nodes (measurement)
     // block-uniform loop: child items and edit-list words are allocated with one atomic each
@@ -450,39 +580,17 @@ __device__ __forceinline__ void d_expand(const EditPass& P, uint32_t l, VGrid G)
         const bool active = t < total;
         const uint32_t i = (uint32_t)(t >> 3), c = (uint32_t)(t & 7u);
         const uint32_t slot = i * 8 + c;
-        uint32_t kind = KIND_KEEP, m = 0, lastCover = ~0u, coverKind = 0, coverColor = 0, es = 0, ec = 0;
+        uint32_t kind = KIND_KEEP, m = 0, lastCover = ~0u, coverKind = 0, es = 0, ec = 0;
         uint64_t touch = 0;
-        uint32_t cx = 0, cy = 0, cz = 0, childOld = kInvalidNode, childOrig = kInvalidNode;
-        float x0 = 0, y0 = 0, z0 = 0, x1 = 0, y1 = 0, z1 = 0;
+        SlotRefs r{};
         unsigned long long coveredVoxels = 0;
         if (active) {
-            cx = (I.px[i] << 1) | ((c >> 2) & 1u);
-            cy = (I.py[i] << 1) | ((c >> 1) & 1u);
-            cz = (I.pz[i] << 1) | (c & 1u);
-            // child of the effective old node, and of the original node
-            const uint32_t old = I.old_[i], orig = I.orig[i];
-            if (old != kInvalidNode) {
-                const uint32_t cm = P.vpool[old] & 0xFFu;
-                if (cm & (1u << c)) childOld = P.vpool[old + child_off(cm, c)];
-                if (c == 0) nodeBytes += 4u * (1u + (uint32_t)__popc(cm));
-            }
-            if (orig == old) {
-                childOrig = childOld;
-            } else if (orig != kInvalidNode) {
-                const uint32_t cm = P.vpool[orig] & 0xFFu;
-                if (cm & (1u << c)) childOrig = P.vpool[orig + child_off(cm, c)];
-                if (c == 0) nodeBytes += 4u * (1u + (uint32_t)__popc(cm));
-            }
-            x0 = (float)(cx << depth);
-            y0 = (float)(cy << depth);
-            z0 = (float)(cz << depth);
-            x1 = x0 + side;
-            y1 = y0 + side;
-            z1 = z0 + side;
+            r = slot_refs(P, I, i, c, depth, nodeBytes);
             es = I.estart[i];
             ec = I.ecount[i];
             // filter the edit list, two edit heads in flight; remember the last cover at levels
             // >= ctl (and its colour) and which of the first 64 edits touch
+            uint32_t coverColor = 0;
             for (uint32_t k = 0; k < ec; k += 2) {
                 const bool two = k + 1 < ec;
                 const uint32_t e0 = P.elist[l][es + k];
@@ -492,12 +600,10 @@ __device__ __forceinline__ void d_expand(const EditPass& P, uint32_t l, VGrid G)
                 for (uint32_t u = 0; u < 2; ++u) {
                     if (u == 1 && !two) break;
                     const EditHead& h = u ? h1 : h0;
-                    if (!e_touch(h, x0, y0, z0, x1, y1, z1)) continue;
-                    if (h.kind == EDIT_COPY && l + 1 + 4 < D && copy_source_empty(P, u ? e1 : e0, x0, y0, z0, l + 1))
-                        continue;
+                    if (!expand_touch(P, h, u ? e1 : e0, l, r.x0, r.y0, r.z0, r.x1, r.y1, r.z1)) continue;
                     if (k + u < 64) touch |= 1ull << (k + u);
                     if (l + 1 >= ctl) {
-                        const uint32_t cv = e_cover(h, x0, y0, z0, x1, y1, z1);
+                        const uint32_t cv = e_cover(h, r.x0, r.y0, r.z0, r.x1, r.y1, r.z1);
                         if (cv) {
                             lastCover = m;
                             coverKind = cv;
@@ -507,39 +613,7 @@ __device__ __forceinline__ void d_expand(const EditPass& P, uint32_t l, VGrid G)
                     m++;
                 }
             }
-            // old voxel count of the effective old child (colour offsets, "keep" counts)
-            uint32_t oldCount = 0;
-            if (l + 1 >= ctl && childOld != kInvalidNode) {
-                if (l + 1 == L) {
-                    const uint64_t lf = *reinterpret_cast<const uint64_t*>(P.vpool + childOld);
-                    oldCount = __popcll(lf);
-                } else {
-                    oldCount = P.vpool[childOld] >> 8;
-                }
-            }
-            I.corig[slot] = childOrig;
-            I.cold[slot] = oldCount;
-            const bool covered = lastCover != ~0u;
-            if (m == 0) {
-                kind = KIND_KEEP;
-                I.cval[slot] = childOld;
-                I.cnew[slot] = oldCount;
-            } else {
-                const uint32_t rest = covered ? m - lastCover - 1 : m;
-                // a covered colour subtree root stays an item (its colour leaf is rebuilt: the
-                // reference's task runs edit_leaf<color_tree_levels>, which takes the early exit)
-                if (covered && rest == 0 && l + 1 > ctl) {
-                    kind = coverKind == 1 ? KIND_FULL : KIND_EMPTY;
-                    I.cval[slot] = coverKind == 1 ? P.full_nodes[l + 1] : kInvalidNode;
-                    I.cnew[slot] = coverKind == 1 ? (1u << (3 * depth)) : 0u;
-                } else {
-                    kind = KIND_ITEM;
-                }
-                // numVoxels of a plain edit: a full / empty early exit counts the whole node
-                if (covered && rest == 0) coveredVoxels = 1ull << (3 * depth);
-                // colour of a full cover (add_large_single_color of the covering editor)
-                if (covered) I.ccolor[slot] = coverColor;
-            }
+            kind = slot_classify(P, I, l, slot, depth, r, m, lastCover, coverKind, coverColor, coveredVoxels);
         }
         const bool isItem = active && kind == KIND_ITEM;
         const uint32_t rest = isItem ? (lastCover == ~0u ? m : m - lastCover - 1) : 0u;
@@ -553,49 +627,122 @@ __device__ __forceinline__ void d_expand(const EditPass& P, uint32_t l, VGrid G)
             continue;
         }
         if (isItem) {
-            C.px[ci] = cx;
-            C.py[ci] = cy;
-            C.pz[ci] = cz;
-            C.orig[ci] = childOrig;
-            C.parent[ci] = i;
-            C.slot[ci] = (uint8_t)c;
-            C.estart[ci] = eo;
-            C.ecount[ci] = rest;
+            // the child's edit list: the touching edits after the last cover, in list order
             uint32_t q = 0, w = 0;
             for (uint32_t k = 0; k < ec; ++k) {
                 const uint32_t ei = P.elist[l][es + k];
-                if (k < 64 ? !((touch >> k) & 1ull) : !e_touch(load_head(P.edits, ei), x0, y0, z0, x1, y1, z1)) continue;
+                if (k < 64 ? !((touch >> k) & 1ull)
+                           : !expand_touch(P, load_head(P.edits, ei), ei, l, r.x0, r.y0, r.z0, r.x1, r.y1, r.z1))
+                    continue;
                 if (lastCover == ~0u || q > lastCover) P.elist[l + 1][eo + w++] = ei;
                 q++;
             }
-            if (lastCover != ~0u) {
-                C.old_[ci] = coverKind == 1 ? P.full_nodes[l + 1] : kInvalidNode;
-                C.cmode[ci] = coverKind == 1 ? 1u : 2u;  // old colours: single colour / none
-                C.cmodeval[ci] = I.ccolor[slot];
-            } else {
-                C.old_[ci] = childOld;
-                C.cmode[ci] = I.cmode[i];
-                C.cmodeval[ci] = I.cmodeval[i];
-            }
-            // colour tree: node of the child (levels < ctl), or the subtree's leaf handle / task
-            const uint32_t cn = I.aux[i];
-            if (l + 1 < ctl) {
-                C.aux[ci] = (l == 0 || cn > 0) ? P.cnodes[cn + c] : 0u;
-            } else if (l + 1 == ctl) {
-                C.aux[ci] = ci;  // task id = own index at level ctl
-                const uint32_t h = (l == 0 || cn > 0) ? P.cnodes[cn + c] : 0u;
-                C.handle[ci] = h;
-            } else {
-                C.aux[ci] = cn;  // task of the ancestor at level ctl
-            }
-            I.cval[slot] = ci;
+            slot_child_item(P, I, C, l, i, c, slot, ci, eo, rest, r, lastCover, coverKind);
         }
         I.ckind[slot] = (uint8_t)kind;
     }
     if (P.abytes) wave_add64(P.abytes + 0, nodeBytes);
 }
+
+// expand with a wavefront per child slot, for levels whose items hold long edit lists (the top
+// levels of a pass over many editors): the lanes test 64 edits of the list at once, two ballots
+// give the touching edits and the covers, and the child's list is written in list order by a
+// second sweep from the chunk of the last cover. Same classification and items as d_expand.
+__device__ __forceinline__ void d_expand_wide(const EditPass& P, uint32_t l, VGrid G) {
+    __shared__ uint32_t sErr;
+    if (threadIdx.x == 0) sErr = *P.errors;
+    __syncthreads();
+    if (sErr) return;
+    const uint32_t n = min(P.count[l], P.cap[l]);
+    const Items I = P.items[l];
+    const Items C = P.items[l + 1];
+    const uint32_t ctl = P.ctl, depth = P.levels - (l + 1);
+    const uint32_t lane = threadIdx.x & 63u, wpb = blockDim.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t nodeBytes = 0;
+    // wavefront-uniform loop
+    for (uint64_t slot = (uint64_t)G.block * wpb + (threadIdx.x >> 6); slot < (uint64_t)n * 8;
+         slot += (uint64_t)G.nblocks * wpb) {
+        const uint32_t i = (uint32_t)(slot >> 3), c = (uint32_t)(slot & 7u);
+        uint64_t nb = 0;
+        const SlotRefs r = slot_refs(P, I, i, c, depth, nb);
+        if (lane == 0) nodeBytes += nb;
+        const uint32_t es = I.estart[i], ec = I.ecount[i];
+        uint32_t m = 0, lastCover = ~0u, coverKind = 0, coverColor = 0, coverBase = 0, coverQ = 0;
+        for (uint32_t b = 0; b < ec; b += 64) {
+            const uint32_t k = b + lane;
+            bool t = false;
+            uint32_t cv = 0, col = 0;
+            if (k < ec) {
+                const uint32_t ei = P.elist[l][es + k];
+                const EditHead h = load_head(P.edits, ei);
+                t = expand_touch(P, h, ei, l, r.x0, r.y0, r.z0, r.x1, r.y1, r.z1);
+                if (t && l + 1 >= ctl) {
+                    cv = e_cover(h, r.x0, r.y0, r.z0, r.x1, r.y1, r.z1);
+                    col = h.single;
+                }
+            }
+            const uint64_t T = __ballot(t), V = __ballot(cv != 0);
+            if (V) {
+                const uint32_t hb = 63u - (uint32_t)__builtin_clzll(V);
+                lastCover = m + (uint32_t)__popcll(T & ((1ull << hb) - 1ull));
+                coverKind = __shfl(cv, hb, 64);
+                coverColor = __shfl(col, hb, 64);
+                coverBase = b;
+                coverQ = m;
+            }
+            m += (uint32_t)__popcll(T);
+        }
+        unsigned long long coveredVoxels = 0;
+        uint32_t kind = 0;
+        if (lane == 0)
+            kind = slot_classify(P, I, l, (uint32_t)slot, depth, r, m, lastCover, coverKind, coverColor, coveredVoxels);
+        kind = __shfl(kind, 0, 64);
+        if (P.single_edit && lane == 0 && coveredVoxels) atomicAdd(P.stat_voxels, coveredVoxels);
+        if (kind == KIND_ITEM) {
+            const uint32_t rest = lastCover == ~0u ? m : m - lastCover - 1;
+            uint32_t ci = 0, eo = 0;
+            if (lane == 0) {
+                ci = atomicAdd(&P.count[l + 1], 1u);
+                eo = rest ? atomicAdd(&P.ecount[l + 1], rest) : 0u;
+            }
+            ci = __shfl(ci, 0, 64);
+            eo = __shfl(eo, 0, 64);
+            if (ci >= P.cap[l + 1] || eo + rest > P.ecap[l + 1]) {
+                if (lane == 0) {
+                    set_error(P.errors, ERR_SCRATCH);
+                    I.ckind[slot] = (uint8_t)kind;
+                }
+                continue;
+            }
+            // the touching edits after the last cover, in list order
+            uint32_t q = coverQ, w = 0;
+            for (uint32_t b = coverBase; b < ec; b += 64) {
+                const uint32_t k = b + lane;
+                bool t = false;
+                uint32_t ei = 0;
+                if (k < ec) {
+                    ei = P.elist[l][es + k];
+                    t = expand_touch(P, load_head(P.edits, ei), ei, l, r.x0, r.y0, r.z0, r.x1, r.y1, r.z1);
+                }
+                const uint64_t T = __ballot(t);
+                const bool emit = t && (lastCover == ~0u || q + (uint32_t)__popcll(T & below) > lastCover);
+                const uint64_t E = __ballot(emit);
+                if (emit) P.elist[l + 1][eo + w + (uint32_t)__popcll(E & below)] = ei;
+                w += (uint32_t)__popcll(E);
+                q += (uint32_t)__popcll(T);
+            }
+            if (lane == 0) slot_child_item(P, I, C, l, i, c, (uint32_t)slot, ci, eo, rest, r, lastCover, coverKind);
+        }
+        if (lane == 0) I.ckind[slot] = (uint8_t)kind;
+    }
+    if (P.abytes) wave_add64(P.abytes + 0, nodeBytes);
+}
 __global__ __launch_bounds__(256) void k_expand(EditPass P, uint32_t l) {
     d_expand(P, l, VGrid{blockIdx.x, gridDim.x});
+}
+__global__ __launch_bounds__(256) void k_expand_wide(EditPass P, uint32_t l) {
+    d_expand_wide(P, l, VGrid{blockIdx.x, gridDim.x});
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2324,7 +2471,10 @@ template <bool kSpecial>
 __device__ __forceinline__ void run_step(const EditPass& P, uint32_t code, uint32_t* root_out, VGrid G) {
     const uint32_t l = code & 0xFFu;
     switch (code >> 8) {
-        case ST_EXPAND: d_expand(P, l, G); break;
+        case ST_EXPAND:
+            if ((P.wide_levels >> l) & 1u) d_expand_wide(P, l, G);
+            else d_expand(P, l, G);
+            break;
         case ST_LEAF: d_leaf<kSpecial>(P, G); break;
         case ST_BUILD: d_build(P, l, G); break;
         case ST_HCLEAR: d_hset_clear(P, l, G); break;
@@ -2469,6 +2619,8 @@ static void level_bounds(const EditPass& P, const DevEdit* e, uint32_t n, uint64
     }
 }
 
+constexpr uint32_t kWideEdits = 64, kWideItems = 512;
+
 hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hipStream_t s, const uint32_t* dN,
                        bool special, const EditScratch* scratch, const DevEdit* hostEdits) {
     EditPass P{};
@@ -2485,6 +2637,13 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
         gridMax = std::max(gridMax, grid[l]);
     }
     P.single_edit = (n == 1 && !dN) ? 1u : 0u;
+    // a wavefront per child slot where a level's items hold long edit lists: passes over at
+    // least kWideEdits editors, on levels with at most kWideItems items (the grid's two blocks
+    // per item hold a wavefront per slot)
+    P.wide_levels = 0;
+    if (!dN && n >= kWideEdits)
+        for (uint32_t l = 0; l < kMaxEditLevels && l < 32; ++l)
+            if (bound[l] <= kWideItems) P.wide_levels |= 1u << l;
     const uint32_t D = P.levels, L = D - 2;
     k_init_pass<<<(std::max<uint32_t>(n, 1) + 255) / 256, 256, 0, s>>>(P, 0xFFFFFFFFu, first, n, dN);
     // the root index lives in ctx->d_root (device) so passes chain without a host round trip
@@ -2517,7 +2676,10 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
         }
         const uint32_t l = sl.step[i] & 0xFFu, gl = grid[l];
         switch (sl.step[i] >> 8) {
-            case ST_EXPAND: k_expand<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_EXPAND:
+                if ((P.wide_levels >> l) & 1u) k_expand_wide<<<gl, 256, 0, s>>>(P, l);
+                else k_expand<<<gl, 256, 0, s>>>(P, l);
+                break;
             case ST_LEAF:
                 if (special) k_leaf<true><<<gl, 256, 0, s>>>(P);
                 else k_leaf<false><<<gl, 256, 0, s>>>(P);

@@ -97,6 +97,27 @@ def test_fused_batch_matches_sequential_oracle(name):
     assert_same(dag, sc, od, oc, "fused batch")
 
 
+@pytest.mark.parametrize("name", ["d8", "d13"])
+def test_fused_batch_of_many_editors_matches_sequential_oracle(name):
+    """A pass over 200 editors: the top levels' items hold edit lists longer than one
+    wavefront (their expand runs a wavefront per child slot), with large boxes that cover whole
+    nodes at list positions past the first 64 (the last cover decides which edits the child
+    keeps)."""
+    p, sc, g, dag, od, oc = make(name)
+    F = p.footprint or (1 << p.levels)
+    edits = random_edits(p, 200, seed=29, rmax=max(4, F // 32))
+    (_, c1, _), (_, c2, _) = edits[100], edits[170]
+    edits[100] = (BOX_ADD, c1, float(F // 4))
+    edits[170] = (BOX_CARVE, c2, float(F // 8))
+    st = dag.edit_batch(edits, fused=True)
+    total = 0
+    for kind, center, r in edits:
+        total += od.edit(kind, center, r, oc)[1]
+    assert st.passes == 1
+    assert st.num_voxels == total
+    assert_same(dag, sc, od, oc, "fused batch of 200 editors")
+
+
 def test_undo_redo_restore_roots_and_colours():
     p, sc, g, dag, od, oc = make("d13")
     from oracle import OracleColors, OracleDAG
