@@ -272,6 +272,77 @@ __device__ uint64_t leaf_decode(const uint64_t* __restrict__ arena, const LeafDe
     const uint32_t wq = (dst >> (16u - bpw - (uint32_t)(bit & 7u))) & ((1u << bpw) - 1u);
     return pack_color(bits, wq, bpw);
 }
+// Runs of consecutive colours decoded by a wavefront: lane r (0 <= r < span) takes colour
+// first + r. `pos` is a block whose start is at most `first` and after which no block starts
+// before `first` (~0u: search it); the 64 block headers after it are read at once, the block
+// starts among them that fall in [first, first + 64) are gathered into a bit mask, and lane r's
+// block is pos plus the number of those starts at or before r: no search per colour. pos
+// moves to the block of colour first + span - 1, the hint for a run that starts right after.
+// Lanes outside the run (active false) or past the leaf's valid colours return 0. Called by
+// every lane of the wavefront with the same first, span and pos.
+__device__ uint64_t leaf_decode_run(const uint64_t* __restrict__ arena, const LeafDesc& d, uint64_t first,
+                                    uint32_t span, int32_t r, bool active, uint32_t& pos) {
+    const uint64_t limit = (d.n_macro / 2) * 16384ull;  // leaf_valid_index: 2 i / 16384 < n_macro
+    const uint64_t x0 = first + (d.offset != ~0ull ? d.offset : 0ull);
+    if (d.n_blocks == 0 || x0 >= limit) return 0;
+    const uint64_t* blocks = arena + d.blocks;
+    const uint64_t* macro = arena + d.macro;
+    const uint64_t m0 = x0 >> 14;
+    if (pos == ~0u) {  // the block holding colour x0 (the search of leaf_decode)
+        const uint32_t local = (uint32_t)(x0 & 16383u);
+        uint32_t lo = (uint32_t)macro[2 * m0];
+        uint32_t hi = 2 * (m0 + 1) < d.n_macro ? (uint32_t)macro[2 * (m0 + 1)] - 1u : (uint32_t)d.n_blocks - 1u;
+        uint32_t p = (lo + hi) / 2;
+        uint64_t blk = blocks[p];
+        while (hdr_index((uint32_t)blk) != local && lo <= hi) {
+            if (hdr_index((uint32_t)blk) > local) hi = p - 1;
+            else lo = p + 1;
+            p = (lo + hi) / 2;
+            blk = blocks[p];
+        }
+        pos = p;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    // block pos + 1 + lane: its macro block (a run of 64 colours crosses at most one macro
+    // block start, and pos may end the macro block before x0's), its start, and its bit.
+    // Blocks from macro block m0 + 2 on start past the run (a window can reach them when
+    // macro block m0 + 1 holds few blocks) and are left out.
+    const uint32_t mFirst = (uint32_t)macro[2 * m0];
+    const uint32_t mNext = 2 * (m0 + 1) < d.n_macro ? (uint32_t)macro[2 * (m0 + 1)] : (uint32_t)d.n_blocks;
+    const uint32_t mNext2 = 2 * (m0 + 2) < d.n_macro ? (uint32_t)macro[2 * (m0 + 2)] : (uint32_t)d.n_blocks;
+    const uint64_t wb = (uint64_t)pos + 1 + lane;
+    uint64_t wblk = 0, bit = 0;
+    if (wb < mNext2) {
+        wblk = blocks[wb];
+        const uint64_t wm = wb < mFirst ? m0 - 1 : (wb < mNext ? m0 : m0 + 1);
+        const uint64_t st = (wm << 14) + hdr_index((uint32_t)wblk);
+        if (st >= x0 && st < x0 + 64) bit = 1ull << (st - x0);
+    }
+#pragma unroll
+    for (uint32_t off = 32; off > 0; off >>= 1) bit |= shfl64(bit, lane ^ off);
+    const uint64_t starts = bit;  // the same in every lane
+    const uint64_t b0 = blocks[pos];
+    const uint32_t rr = r < 0 ? 0u : (uint32_t)r;
+    const uint32_t nb = (uint32_t)__popcll(starts & (rr >= 63 ? ~0ull : (2ull << rr) - 1ull));
+    const uint64_t wsel = shfl64(wblk, nb == 0 ? 0u : nb - 1);  // every lane takes part
+    const uint64_t blk = nb == 0 ? b0 : wsel;
+    pos += (uint32_t)__popcll(starts & (span >= 64 ? ~0ull : (1ull << span) - 1ull));
+    const uint64_t x = x0 + rr;
+    if (!active || r < 0 || x >= limit) return 0;
+    const uint32_t h = (uint32_t)blk, bits = (uint32_t)(blk >> 32), bpw = hdr_bpw(h);
+    if (bpw == 0) return pack_color(bits, 0, 0);
+    const uint32_t local = (uint32_t)(x & 16383u);
+    const uint64_t m = x >> 14;
+    const uint64_t wbit = macro[2 * m + 1] + (h >> 16) + (uint64_t)((local - hdr_index(h)) * bpw);
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(arena) + d.weights32;
+    const uint64_t byte = wbit >> 3;
+    const uint32_t w0 = w[byte >> 2], sh = (uint32_t)(byte & 3u) * 8u;
+    const uint32_t c0 = (w0 >> sh) & 0xFFu;
+    const uint32_t c1 = (byte & 3u) == 3u ? (w[(byte >> 2) + 1] & 0xFFu) : ((w0 >> (sh + 8)) & 0xFFu);
+    const uint32_t dst = (c0 << 8) | c1;
+    const uint32_t wq = (dst >> (16u - bpw - (uint32_t)(wbit & 7u))) & ((1u << bpw) - 1u);
+    return pack_color(bits, wq, bpw);
+}
 // is_valid() && is_valid_index(i) (variable_weight_size_colors.h:247-262)
 __device__ __forceinline__ bool leaf_valid_index(const LeafDesc& d, uint64_t i) {
     if (d.n_blocks == 0) return false;
@@ -1383,10 +1454,17 @@ __device__ __forceinline__ void d_color_copy(const EditPass& P, uint32_t l, VGri
             const LeafDesc d = task_leaf(P, T.handle[task]);
             const uint32_t mode = I.cmode[i], mv = I.cmodeval[i];
             const uint64_t os = (uint64_t)I.costart[slot] - cs;
+            uint32_t pos = ~0u;  // block of the colour before the chunk (decoded runs)
             for (uint32_t k0 = cs & ~63u; k0 < ce; k0 += 64) {
                 const uint32_t k = k0 + lane;
                 const bool in = k >= cs && k < ce;
-                const uint64_t col = in ? old_color(P, mode, mv, d, os + k) : 0ull;
+                uint64_t col;
+                if (mode == 1) {
+                    col = in ? pack_color(mv, 0, 0) : 0ull;
+                } else {  // the chunk's old colours decoded as one run
+                    const uint32_t b0 = k0 < cs ? cs - k0 : 0u, e0 = min(ce - k0, 64u);
+                    col = leaf_decode_run(P.arena, d, os + k0 + b0, e0 - b0, (int32_t)lane - (int32_t)b0, in, pos);
+                }
                 if (k0 >= cs && k0 + 64 <= ce) {
                     const uint64_t c0 = shfl64(col, 0);
                     if (__ballot(((col ^ c0) & kColorKey) != 0 || (col >> 40) != 0) == 0) {
