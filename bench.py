@@ -54,7 +54,7 @@ def main() -> int:
     ap.add_argument("--no-edits", action="store_true")
     ap.add_argument("--bucket-scale", type=int, default=0,
                     help="edit scene buckets: N times the reference's bucket counts at 1/N the size (1 = the "
-                         "reference's), or 0 to time 1, 2, 4, 8 and keep the fastest")
+                         "reference's), or 0 to time 1, 4, 8 and keep the fastest")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain leg of the edit comparison (for counter runs of the fused pass)")
     ap.add_argument("--edit-forms", choices=("fast", "all", "level"), default="fast",
@@ -302,7 +302,7 @@ def fit_geometry(levels, level_words, bucket_scale=1):
 
 def tune_buckets(scene, params, args):
     """The bucket-count axis: the edit list's fused pass timed on the scene loaded with the
-    reference's bucket counts and with 2x / 4x / 8x as many buckets of 1/2 / 1/4 / 1/8 the size
+    reference's bucket counts and with 4x / 8x as many buckets of 1/4 / 1/8 the size
     (the same capacity; a find scans a shorter bucket); a geometry whose buckets overflow drops
     out. Returns the fastest geometry and the trials."""
     from hashdag_amd._lib import HashDAGError
@@ -312,7 +312,7 @@ def tune_buckets(scene, params, args):
     from hashdag_amd.workload import mixed_edits
     edits = mixed_edits(params, args.edits, seed=args.seed + 1)
     trials, best = {}, None
-    for scale in (1, 2, 4, 8):
+    for scale in (1, 4, 8):
         try:
             g = fit_geometry(params.levels, scene.level_words, bucket_scale=scale)
         except ValueError:
