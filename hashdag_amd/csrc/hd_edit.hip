@@ -2741,6 +2741,10 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
     const bool smallPass = largest <= (uint64_t)ctx->edit_block_max;
     const auto small = [&](uint32_t) { return smallPass; };
     // a wavefront per task slot (the task count is on the device), at most 2^18 of them
+    const bool bounded = hostEdits && !dN;  // bound[] from the editors' boxes
+    const auto wideGrid = [](uint64_t blocks) {
+        return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(blocks, 1u), 1u << 16);
+    };
     const uint32_t encBlocks =
         (uint32_t)std::min<uint64_t>(std::max<uint64_t>((bound[P.ctl] + kEncWaves - 1) / kEncWaves, 1u), 1u << 16);
     for (uint32_t i = 0; i < sl.n;) {
@@ -2753,29 +2757,39 @@ hd_status enqueue_pass(hd_ctx* ctx, uint32_t first, uint32_t n, bool colors, hip
             continue;
         }
         const uint32_t l = sl.step[i] & 0xFFu, gl = grid[l];
+        // Grids from the level's item bound by each kernel's unit of work (a thread, or a
+        // wavefront, per item or per child slot), up to 2^16 blocks: free wavefronts then take
+        // the next unit as the hardware hands out blocks, where a fixed stride of units per
+        // thread leaves the launch to the one with the largest share (item costs vary by orders
+        // of magnitude). Without the editors' boxes (graph replays) the grid is gl.
+        const auto units = [&](uint64_t n, uint32_t perBlock) {
+            return bounded ? wideGrid((n + perBlock - 1) / perBlock) : gl;
+        };
+        const uint64_t b = bound[l];
         switch (sl.step[i] >> 8) {
-            case ST_EXPAND:
+            case ST_EXPAND:  // wide: a wavefront per child slot; otherwise a thread per slot
                 if ((P.wide_levels >> l) & 1u) k_expand_wide<<<gl, 256, 0, s>>>(P, l);
-                else k_expand<<<gl, 256, 0, s>>>(P, l);
+                else k_expand<<<units(8 * b, 256), 256, 0, s>>>(P, l);
                 break;
-            case ST_LEAF:
+            case ST_LEAF:  // a thread per item
                 if (special) k_leaf<true><<<gl, 256, 0, s>>>(P);
-                else k_leaf<false><<<gl, 256, 0, s>>>(P);
+                else k_leaf<false><<<units(b, 256), 256, 0, s>>>(P);
                 break;
-            case ST_BUILD: k_build<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_BUILD: k_build<<<units(b, 256), 256, 0, s>>>(P, l); break;  // a thread per item
             case ST_HCLEAR: k_hset_clear<<<std::min<uint32_t>(gl, 256), 256, 0, s>>>(P, l); break;
-            case ST_DEDUP: k_dedup<<<gl, 256, 0, s>>>(P, l); break;
-            case ST_FIND: k_find<<<gl, 256, 0, s>>>(P, l); break;
-            case ST_ADD: k_add<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_DEDUP: k_dedup<<<units(b, 256), 256, 0, s>>>(P, l); break;  // a thread per item
+            case ST_FIND: k_find<<<units(b, 4), 256, 0, s>>>(P, l); break;      // a wavefront per representative
+            case ST_ADD: k_add<<<units(b, 256), 256, 0, s>>>(P, l); break;      // a thread per item
             case ST_RESOLVE: k_resolve<<<gl, 256, 0, s>>>(P, l); break;
-            case ST_CSTARTS: k_color_starts<<<gl, 256, 0, s>>>(P, l); break;
+            case ST_CSTARTS: k_color_starts<<<units(8 * b, 256), 256, 0, s>>>(P, l); break;  // a thread per slot
             case ST_CALLOC: k_color_alloc<<<gl, 256, 0, s>>>(P); break;
             case ST_CTREE: k_color_tree<<<gridMax, 256, 0, s>>>(P); break;
-            case ST_CCOPY: k_color_copy<<<gl, 256, 0, s>>>(P, l); break;
-            case ST_CCOPY_LEAVES: k_color_copy_leaves<<<gl, 256, 0, s>>>(P); break;
-            case ST_CLEAF:
+            // colour copies: a wavefront per slot (per item at the leaf parents)
+            case ST_CCOPY: k_color_copy<<<units(8 * b, 4), 256, 0, s>>>(P, l); break;
+            case ST_CCOPY_LEAVES: k_color_copy_leaves<<<units(b, 4), 256, 0, s>>>(P); break;
+            case ST_CLEAF:  // special: a wavefront per item; otherwise a thread per item
                 if (special) k_color_leaf<<<gl, 256, 0, s>>>(P);
-                else k_color_leaf_items<<<gl, 256, 0, s>>>(P);
+                else k_color_leaf_items<<<units(b, 256), 256, 0, s>>>(P);
                 break;
             case ST_ENCODE: k_encode<<<encBlocks, kEncBlock, 0, s>>>(P); break;
             default: k_finish<<<1, 64, 0, s>>>(P, ctx->d_root); break;
